@@ -1,0 +1,128 @@
+#!/usr/bin/env python3
+"""Throughput of the shared_cfd solver hot path on one MI355X.
+
+Workload: the Mach-3 forward-facing step, 2 structured blocks
+(192x320 + 768x256 = 258,048 cells in a 960x320 index space, h = 1/320),
+ideal gas, MUSCL (order 3, reference limiter) + SLAU, preconditioned
+LU-SGS in dual time with "InternalLoop Max Number : 21" (20 inner
+iterations per physical step), FP64.  One bench "step" = one physical time
+step = 20 inner iterations over all cells; a cell-update is one inner
+iteration of one cell.  Inputs are resident in HBM before timing starts.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line (metric, value = Mcell-updates/s, roofline of the
+dominant kernel from HIP events over the timed region, cpu_baseline of the
+CPU oracle on the host cores over a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 Mcell-updates/s vs plain GPU & CPU; adjoint/primal cost at 1e-10 tol"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic HBM bytes per cell for one launch of each kernel class
+# (minimum-pass schedule, dimq = 4, 13 inviscid geometry doubles)
+def alg_bytes_per_cell(kclass: str, dimq: int = 4, geo: int = 13) -> float:
+    d = 8.0
+    return {
+        "SWEEP_FWD": 3 * dimq * d,      # read R, q; write dq*
+        "SWEEP_BWD": 3 * dimq * d,      # read dq*, q; write dq
+        "UPDATE": 3 * dimq * d,         # read dq, q; write q
+        "RHS": (3 * dimq + geo + dimq) * d,       # read q, qp, qpp, G; write R
+        "RESIDUAL": (3 * dimq + geo + dimq) * d,
+    }.get(kclass, 0.0)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-per-unit", type=int, default=320)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "blocks would shard across GPUs as described in DESIGN.md, but no multi-GPU path is built.",
+              file=sys.stderr)
+        return 2
+
+    from shared_cfd_amd import cases as C
+    from shared_cfd_amd.native import Solver
+
+    case = C.forward_step(n_per_unit=a.n_per_unit, num_internal_loop=21)
+    inner = case.params.num_internal_loop - 1
+    cells = case.ncells
+    s = Solver(case)
+    s.prepare()
+    step = 0
+    for _ in range(a.warmup):
+        step += 1
+        s.step(step, 0)
+    # short all-class profile to find the dominant kernel
+    s.profile(None)
+    step += 1
+    s.step(step, 0)
+    prof_all = s.profile_read()
+    dominant = max((k for k in prof_all if alg_bytes_per_cell(k) > 0), key=lambda k: prof_all[k][0])
+    s.profile([dominant])
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step += 1
+        s.step(step, 0)
+    s.synchronize()
+    t1 = time.perf_counter()
+    prof = s.profile_read()
+    s.profile([])
+    dt = t1 - t0
+    value = cells * inner * a.steps / dt / 1e6
+    ms_dom, n_dom = prof[dominant]
+    avg_s = ms_dom / n_dom / 1e3
+    bytes_launch = alg_bytes_per_cell(dominant) * cells
+    achieved = bytes_launch / avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dominant,
+                "avg_launch_us": round(avg_s * 1e6, 3),
+                "alg_bytes_per_launch": bytes_launch,
+                "share_of_step": {k: round(v[0] / sum(x[0] for x in prof_all.values()), 4) for k, v in prof_all.items()}}
+
+    cpu = None
+    if not a.no_cpu:
+        from oracle.oracle import Oracle
+
+        o = Oracle(case)
+        o.prepare()
+        tc0 = time.perf_counter()
+        for _ in range(a.cpu_steps):
+            err, _, _ = o.step()
+        tc1 = time.perf_counter()
+        cpu_val = cells * inner * a.cpu_steps / (tc1 - tc0) / 1e6
+        cpu = {"value": round(cpu_val, 3), "unit": "Mcell-updates/s", "cores": o.L.or_omp_threads(), "kind": "port",
+               "sample": f"{a.cpu_steps} physical step(s) x {inner} inner iterations of the same "
+                         f"{cells}-cell forward-step workload, C oracle (OpenMP), {tc1 - tc0:.1f} s"}
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(dt / a.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": case.description + ", dual time 20 inner iterations/step, MUSCL3 + SLAU, FP64",
+                   "cells": cells, "inner_iterations_per_step": inner, "blocks": case.nblocks},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

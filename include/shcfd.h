@@ -1,0 +1,127 @@
+/*
+ * shcfd.h -- C ABI of libshcfd, the MI355X (gfx950) implementation of the
+ * shared_cfd per-iteration solver hot path.
+ *
+ * Every entry point returns an int status (SHCFD_OK or an SHCFD_E_* code,
+ * see shcfd_types.h; shcfd_strerror gives the text).  Arguments are plain
+ * pointers and sizes.  Host arrays are borrowed only for the duration of a
+ * call.  All device work of one context is queued on one HIP stream; calls
+ * that return a scalar synchronise that stream, all others are asynchronous.
+ *
+ * The reference exposes no library API: its main program calls external
+ * Fortran subroutines by name (generated main.f90, e.g.
+ * store/time_scheme/dual/2d/main.main.raw.f90).  Each per-routine entry
+ * point below replaces one of those subroutines; the citation gives the
+ * reference routine it stands in for.  The Fortran host module
+ * shared_cfd_amd/fortran/shcfd_f.f90 binds them with ISO_C_BINDING under the
+ * reference's names.
+ */
+#ifndef SHCFD_H
+#define SHCFD_H
+
+#include "shcfd_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct shcfd_ctx shcfd_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the compile-time sizes of store/core/n_grid.raw.f90:1-20 (Nplane,
+ * ni(:), nj(:), nY) and the module state of store/core/variable.f90:1-49:
+ * allocates all blocks' state in device memory (packed, no nimax padding). */
+int shcfd_create(int nblocks, const int *ni, const int *nj, int ny, const shcfd_params *params,
+                 shcfd_ctx **out);
+int shcfd_destroy(shcfd_ctx *ctx);
+/* run-time control.inp keys: store/core/read_control.f90:61-92,
+ * store/high_order/muscl/muscl.f90:76-116, store/time_scheme/dual/2d/sch_dual.raw.f90:301-340 */
+int shcfd_set_params(shcfd_ctx *ctx, const shcfd_params *params);
+const char *shcfd_strerror(int code);
+int shcfd_version(void);
+
+/* ---- data movement (reference layout, see shcfd_types.h) ----------------- */
+int shcfd_upload_field(shcfd_ctx *ctx, int field, int block, const double *host);
+int shcfd_download_field(shcfd_ctx *ctx, int field, int block, double *host);
+/* block-interface table, the cut_copro000.inp rows read by set_cut_copro
+ * (store/architecture/PC/mod_mpi_dummy.f90:59-79) */
+int shcfd_set_cut_table(shcfd_ctx *ctx, const int *rows, int nrows);
+/* physical boundary segments replacing the user set_BC loops
+ * (e.g. sample/no06.step_flow/condition.f90:31-157) */
+int shcfd_set_bc_table(shcfd_ctx *ctx, const int *rows, int nrows, const double *pool, int npool);
+/* geometry from the uploaded nodes (fields XH, RH):
+ * init_geometry, store/dim/2d/geometry.f90:6-25; set_vnc_dsc,
+ * store/time_scheme/LU-SGS/2d/sch_lusgs.raw.f90:143-181; set_geojac,
+ * store/viscosity/viscous/no-nV/2d/sch_viscous.f90:1-54 */
+int shcfd_init_geometry(shcfd_ctx *ctx);
+/* time and the dual-time BDF1 flag (tt, first_dual of the reference) */
+int shcfd_set_time(shcfd_ctx *ctx, double tt, int first_dual);
+int shcfd_get_time(shcfd_ctx *ctx, double *dt_grbl, double *tt);
+int shcfd_synchronize(shcfd_ctx *ctx);
+
+/* ---- per-routine drop-ins ------------------------------------------------ */
+/* set_w: store/core/scheme.f90:1-29 */
+int shcfd_set_w(shcfd_ctx *ctx);
+/* set_thermo_prop (ideal gas): store/therm_lib/ideal/thermal_model.raw.f90:7-33 */
+int shcfd_set_thermo_prop(shcfd_ctx *ctx, int step);
+/* set_BC: section_exchange + BCs + set_corners,
+ * store/cond/no-nV/condition.tail.f90:2-70, :331-356 */
+int shcfd_set_bc(shcfd_ctx *ctx, int step);
+/* set_HV: store/high_order/muscl/muscl.f90:7-73 (upwind: upwind.f90:6-36) */
+int shcfd_set_hv(shcfd_ctx *ctx);
+/* set_TG (SLAU): store/core/scheme.f90:31-259 */
+int shcfd_set_tg(shcfd_ctx *ctx);
+/* set_TGv: store/viscosity/viscous/no-nV/2d/sch_viscous.f90:56-272
+ * (inviscid: store/viscosity/non-viscous/sch_viscous.f90, TGv = 0) */
+int shcfd_set_tgv(shcfd_ctx *ctx);
+/* set_dt: store/time_step/global/set_dt.f90:1-43 / local/set_dt.f90:1-36;
+ * dt_grbl_out (may be NULL) receives the global step (synchronises) */
+int shcfd_set_dt(shcfd_ctx *ctx, int step, double *dt_grbl_out);
+/* set_ABpm: store/time_scheme/LU-SGS/2d/sch_lusgs.raw.f90:1-141,
+ * preconLU-SGS/2d/sch_precon.raw.f90:1-202, dual/2d/sch_dual.raw.f90:1-209 */
+int shcfd_set_abpm(shcfd_ctx *ctx);
+/* set_dqdt: store/time_scheme/dual/2d/sch_dual.raw.f90:211-240 */
+int shcfd_set_dqdt(shcfd_ctx *ctx);
+/* calc_next_step_implicit: LU-SGS/2d/time_lusgs_viscous.f90:1-65,
+ * preconLU-SGS/2d/time_precon_viscous.f90:1-66, dual/2d/time_dual_viscous.f90:1-125.
+ * res_out/omega_out (dual only, may be NULL) synchronise. */
+int shcfd_calc_next_step_implicit(shcfd_ctx *ctx, int step_internal, double *res_out, double *omega_out);
+/* calc_next_step (explicit Euler): store/time_scheme/euler/time_euler_viscous.raw.f90:1-23 */
+int shcfd_calc_next_step(shcfd_ctx *ctx);
+/* calc_next_step_RK2_first/second: store/time_scheme/RK2/time_RK2_viscous.raw.f90:1-59 */
+int shcfd_calc_next_step_rk2(shcfd_ctx *ctx, int stage);
+/* qp = q / qpp = qp copies of the main loops (interior cells) */
+int shcfd_copy_state(shcfd_ctx *ctx, int dst_field, int src_field);
+/* energy-change RMS of check_convergence: store/core/check_convergence.f90:17-28 */
+int shcfd_energy_rms(shcfd_ctx *ctx, double *rms_out);
+
+/* ---- whole steps --------------------------------------------------------- */
+/* main.body.raw.f90:14-15: part_primitive + set_BC before the main loop */
+int shcfd_prepare(shcfd_ctx *ctx);
+/* one physical step of store/time_scheme/<scheme>/main.main.raw.f90 with
+ * check_internal_loop_convergence (sch_dual.raw.f90:284-299) for dual
+ * time.  res_first/res_last: dual-time inner residuals of the first and
+ * last inner iteration (may be NULL). */
+int shcfd_step(shcfd_ctx *ctx, int step, int step_res, double *res_first, double *res_last);
+/* pending asynchronous error (omega, NaN, sweep flag timeout); clears it */
+int shcfd_check_error(shcfd_ctx *ctx);
+
+/* ---- per-kernel timing (no counterpart in the reference, whose timers
+ * are commented-out stubs: store/debug/time_barrier.f90:1-37) ------------- */
+/* kernel classes */
+enum {
+    SHCFD_K_PRIMITIVE = 0, SHCFD_K_BC, SHCFD_K_HV, SHCFD_K_TG, SHCFD_K_TGV, SHCFD_K_DT, SHCFD_K_ABPM,
+    SHCFD_K_DQDT, SHCFD_K_RHS, SHCFD_K_SWEEP_FWD, SHCFD_K_SWEEP_BWD, SHCFD_K_REDUCE, SHCFD_K_UPDATE,
+    SHCFD_K_EXPLICIT, SHCFD_K_RESIDUAL, SHCFD_K_PREP, SHCFD_K_COUNT
+};
+/* mask: bit k set = record HIP events (on the context stream) around every
+ * launch of kernel class k; -1 = all classes, 0 = off */
+int shcfd_profile_enable(shcfd_ctx *ctx, int mask);
+/* synchronise, then total milliseconds and launch count of one class since
+ * the last reset; reset != 0 clears the class afterwards */
+int shcfd_profile_read(shcfd_ctx *ctx, int kernel_class, double *total_ms, long long *launches, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHCFD_H */

@@ -1,0 +1,158 @@
+/*
+ * shcfd_types.h -- plain-C vocabulary shared by the shcfd C-ABI (shcfd.h):
+ * field identifiers, boundary-condition kinds, scheme codes and the
+ * per-case parameter block.  No functions are declared here.
+ *
+ * Layout convention for every field that crosses the ABI ("reference
+ * layout"): one block at a time, Fortran order with the component index
+ * fastest and the reference's two ghost layers,
+ *     f(1:K, -1:ni+2, -1:nj+2)
+ * i.e. element (k,i,j) (k 0-based, i/j Fortran cell indices) lives at
+ *     k + K*((i+1) + (ni+4)*(j+1)).
+ * This is the layout of the reference's module arrays
+ * (store/core/variable.f90:3-49) restricted to one plane.  Face arrays
+ * (TGi(0:ni,1:nj), wHlj(1:ni,0:nj), ...), node arrays (xh(0:ni,0:nj)) and
+ * cell-centre arrays use the same ghosted index space; entries outside the
+ * reference's declared range are zero.
+ */
+#ifndef SHCFD_TYPES_H
+#define SHCFD_TYPES_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    SHCFD_OK = 0,
+    SHCFD_E_ARG = 1,           /* bad argument / size / table row           */
+    SHCFD_E_HIP = 2,           /* HIP runtime error                          */
+    SHCFD_E_NOMEM = 3,         /* device allocation failed                   */
+    SHCFD_E_NAN = 4,           /* NaN in RMS or inner residual               */
+    SHCFD_E_OMEGA = 5,         /* dual time: omega < omega_min               */
+    SHCFD_E_RESRATE = 6,       /* dual time: res/res1 > ResRateErr           */
+    SHCFD_E_TNEWTON = 7,       /* T-from-e Newton iteration limit            */
+    SHCFD_E_CHEM = 8,          /* chemistry integrator failure               */
+    SHCFD_E_SPIN = 9,          /* sweep wavefront flag wait exceeded bound   */
+    SHCFD_E_STATE = 10         /* call made in the wrong state / scheme      */
+};
+
+/* ---- time schemes (checkout.inp "time scheme", checkout_flow.py:134-185) -- */
+enum {
+    SHCFD_SCHEME_EULER = 0,
+    SHCFD_SCHEME_RK2 = 1,
+    SHCFD_SCHEME_LUSGS = 2,
+    SHCFD_SCHEME_DUAL = 5,
+    SHCFD_SCHEME_PRECON_LUSGS = 6
+};
+
+/* ---- MUSCL limiter ------------------------------------------------------- */
+enum {
+    SHCFD_LIMITER_REFERENCE = 0,  /* s=2DuDl/(Du^2+Dl^2+1e-300), muscl.f90:32 */
+    SHCFD_LIMITER_MINMOD = 1      /* addition: minmod slopes                 */
+};
+
+/* ---- boundary-condition segment kinds ------------------------------------ */
+/* A segment covers cells start..end (Fortran, 1-based, inclusive) along one
+ * side of one block and fills the two ghost layers of w and vhi, as the
+ * user set_BC loops of the reference samples do
+ * (sample/no06.step_flow/condition.f90:31-157,
+ *  sample/no05.boundary_layer_ideal/condition.f90). */
+enum {
+    SHCFD_BC_EXTRAP = 0,     /* g1 = interior, g2 = g1                          */
+    SHCFD_BC_REFLECT = 1,    /* as EXTRAP, normal velocity (u on i-sides, v on
+                                j-sides) negated in g1; g2 = g1               */
+    SHCFD_BC_NOSLIP = 2,     /* as EXTRAP, u and v negated in g1; g2 = g1      */
+    SHCFD_BC_FIXED = 3,      /* g1 = g2 = given w(1:dimw) and vhi(1:nY)        */
+    SHCFD_BC_INFLOW_TUV = 4, /* T,u,v given, p from interior (ideal gas)        */
+    SHCFD_BC_OUTFLOW_P = 5,  /* as EXTRAP, p replaced by a given value in g1    */
+    SHCFD_BC_SLIP_NORMAL = 6 /* slip wall for a curved side: velocity mirrored
+                                about the face normal (addition)             */
+};
+/* sides */
+enum { SHCFD_SIDE_IMIN = 0, SHCFD_SIDE_IMAX = 1, SHCFD_SIDE_JMIN = 2, SHCFD_SIDE_JMAX = 3 };
+/* one BC row = 6 ints: block(1-based), side, start, end, kind, pool offset   */
+#define SHCFD_BC_ROW 6
+/* one cut row = 11 ints, the cut_copro000.inp format
+ * (geo,width,order,p1,c1,s1,pm1,p2,c2,s2,pm2), store/cond/core/mod_cond.py:86-91 */
+#define SHCFD_CUT_ROW 11
+
+/* ---- field identifiers (upload/download in the reference layout) --------- */
+enum {
+    SHCFD_F_Q = 0,      /* q(1:dimq)        conservative                     */
+    SHCFD_F_QP,         /* qp(1:dimq)                                        */
+    SHCFD_F_QPP,        /* qpp(1:dimq)                                       */
+    SHCFD_F_W,          /* w(1:dimw)        rho,u,v,p,Y(nY),gamma,H,R,mu     */
+    SHCFD_F_VHI,        /* vhi(1:nY)                                         */
+    SHCFD_F_DHI,        /* DHi(1:nY)                                         */
+    SHCFD_F_WHLI,       /* wHli(1:dimw)  i-face left states                  */
+    SHCFD_F_WHRI,
+    SHCFD_F_WHLJ,
+    SHCFD_F_WHRJ,
+    SHCFD_F_TGI,        /* TGi(1:dimq)   inviscid i-face flux                */
+    SHCFD_F_TGJ,
+    SHCFD_F_TGVI,       /* TGvi(1:dimq)  viscous i-face flux                 */
+    SHCFD_F_TGVJ,
+    SHCFD_F_VOL,        /* Vol                                               */
+    SHCFD_F_DSI,        /* dsi(0:ni,1:nj)                                    */
+    SHCFD_F_DSJ,
+    SHCFD_F_VNI,        /* vni(1:2)                                          */
+    SHCFD_F_VNJ,
+    SHCFD_F_DSCI,       /* dsci (cell centre, 0 on ghosts)                   */
+    SHCFD_F_DSCJ,
+    SHCFD_F_VNCI,       /* vnci(1:2)                                         */
+    SHCFD_F_VNCJ,
+    SHCFD_F_GEOJACI,    /* geojaci(1:2,1:2)                                  */
+    SHCFD_F_GEOJACJ,
+    SHCFD_F_X,          /* cell centre x(0:ni+1,0:nj+1)                      */
+    SHCFD_F_R,
+    SHCFD_F_XH,         /* node xh(0:ni,0:nj)                                */
+    SHCFD_F_RH,
+    SHCFD_F_DT,         /* dt_mat                                            */
+    SHCFD_F_ALPHA,      /* alpha                                             */
+    SHCFD_F_PHI,        /* phi (precon / dual)                               */
+    SHCFD_F_PRESCA,     /* pre_sca(1:4) (dual)                               */
+    SHCFD_F_PRE1,       /* pre1(1:dimq)                                      */
+    SHCFD_F_PRE2,       /* pre2(1:dimq)                                      */
+    SHCFD_F_DQDT,       /* dqdt(1:dimq) (dual)                               */
+    SHCFD_F_AP,         /* Ap(1:dimq,1:dimq)                                 */
+    SHCFD_F_AM,
+    SHCFD_F_BP,
+    SHCFD_F_BM,
+    SHCFD_F_DQ,         /* Dq(1:dimq) of the last implicit update            */
+    SHCFD_F_COUNT
+};
+
+/* ---- per-case parameters --------------------------------------------------
+ * Build-time choices of the reference's checkout.inp (checkout_flow.py:66-318)
+ * and the run-time keys of control.inp (read_control.f90:61-92,
+ * muscl.f90:76-116, sch_dual.raw.f90:301-340), with the ideal-gas constants
+ * (compile-time KAPPA/RGAS/NU in the reference, checkout_flow.py:217-221)
+ * made run-time values. */
+typedef struct shcfd_params {
+    int scheme;            /* SHCFD_SCHEME_*                                  */
+    int time_step;         /* 0 global (dt_grbl), 1 local (dt_mat)            */
+    int high_order;        /* 0 upwind, 1 MUSCL                               */
+    int limiter;           /* SHCFD_LIMITER_*                                 */
+    int viscous;           /* 0 inviscid, 1 viscous no-nV/2d                  */
+    int thermal;           /* 0 ideal gas                                     */
+    int num_internal_loop; /* "InternalLoop Max Number" (N-1 inner iterations)*/
+    int pad0;
+    double phi_mus;        /* "MUSCL ON(1)/OFF(0)"                            */
+    double k_mus;          /* -1 (order 2) or 1/3 (order 3)                   */
+    double cfl;            /* "CFL Number"                                    */
+    double cfl_tau;        /* "InternalLoop CFL tau"                          */
+    double omega_max;      /* "InternalLoop Omega Max"                        */
+    double omega_min;      /* "InternalLoop Omega Min"                        */
+    double dqmax;          /* "InternalLoop Dqrate Max"                       */
+    double res_rate_warning;
+    double res_rate_error;
+    double kappa;          /* ratio of specific heats                         */
+    double rgas;           /* gas constant                                    */
+    double nu;             /* kinematic viscosity (mu = rho*nu)               */
+} shcfd_params;
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHCFD_TYPES_H */

@@ -1,0 +1,70 @@
+/*
+ * shcfd_oracle.h -- CPU restatement of the reference solver's per-iteration
+ * hot path (ISASShimadaLab/shared_cfd, Fortran 90), written in plain C.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * implementation in shared_cfd_amd/ and the timed CPU baseline of bench.py.
+ * It is never linked into, loaded by or called from the product path.
+ *
+ * Data live in the reference layout: one AoS array per field and block,
+ * component fastest, ghosted index space (K,-1:ni+2,-1:nj+2) -- see
+ * include/shcfd_types.h.  Each routine cites the reference file:line it
+ * restates; arithmetic is written in the reference's evaluation order and the
+ * library is compiled with -ffp-contract=off.
+ */
+#ifndef SHCFD_ORACLE_H
+#define SHCFD_ORACLE_H
+
+#include <stddef.h>
+#include "../include/shcfd_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_ctx or_ctx;
+
+or_ctx *or_create(int nb, const int *ni, const int *nj, int ny, const shcfd_params *p);
+void or_destroy(or_ctx *c);
+int or_set_params(or_ctx *c, const shcfd_params *p);
+/* pointer to the AoS array of one field/block (size or_field_size doubles) */
+double *or_field(or_ctx *c, int field, int block);
+int or_field_ncomp(or_ctx *c, int field);
+size_t or_field_size(or_ctx *c, int field, int block);
+int or_set_cut_table(or_ctx *c, const int *rows, int nrows);
+int or_set_bc_table(or_ctx *c, const int *rows, int nrows, const double *pool, int npool);
+
+/* geometry: nodes xh/rh must be filled (fields SHCFD_F_XH/RH) */
+void or_init_geometry(or_ctx *c);     /* dim/2d/geometry.f90:6-169           */
+void or_set_vnc_dsc(or_ctx *c);       /* LU-SGS/2d/sch_lusgs.raw.f90:143-181 */
+void or_set_geojac(or_ctx *c);        /* viscous/no-nV/2d/sch_viscous.f90:1-54 */
+
+/* per-routine restatements */
+void or_set_w(or_ctx *c);             /* core/scheme.f90:1-29                */
+void or_set_thermo_prop(or_ctx *c);   /* therm_lib/ideal/thermal_model.raw.f90:7-33 */
+void or_set_bc(or_ctx *c);            /* cond/no-nV/condition.tail.f90 + user set_BC */
+void or_set_hv(or_ctx *c);            /* high_order/muscl/muscl.f90:7-73     */
+void or_set_tg(or_ctx *c);            /* core/scheme.f90:31-259              */
+void or_set_tgv(or_ctx *c);           /* viscous/no-nV/2d/sch_viscous.f90:56-272 */
+double or_set_dt(or_ctx *c);          /* time_step/{global,local}/set_dt.f90 */
+void or_set_abpm(or_ctx *c);          /* LU-SGS / preconLU-SGS / dual set_ABpm */
+void or_set_dqdt(or_ctx *c);          /* dual/2d/sch_dual.raw.f90:211-240    */
+int or_calc_next_step_implicit(or_ctx *c, int step_internal, double *res, double *omega);
+void or_calc_next_step(or_ctx *c);    /* euler/time_euler_viscous.raw.f90    */
+void or_calc_next_step_rk2(or_ctx *c, int stage); /* RK2/time_RK2_viscous.raw.f90 */
+double or_energy_rms(or_ctx *c);      /* core/check_convergence.f90:17-28    */
+void or_copy_field(or_ctx *c, int dst, int src);
+double or_dt_grbl(or_ctx *c);
+double or_tt(or_ctx *c);
+void or_set_first_dual(or_ctx *c, int first);
+
+/* whole steps, mirroring store/time_scheme/<scheme>/main.main.raw.f90.
+ * Return 0 or an SHCFD_E_* code.  res_last/res_first: dual only. */
+int or_prepare(or_ctx *c);            /* main.body.raw.f90: part_primitive + set_BC */
+int or_step(or_ctx *c, double *res_first, double *res_last);
+int or_omp_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,293 @@
+"""Case set-up for the shcfd solver: grids, boundary/cut tables, initial states.
+
+The reference builds a case from a Plot3D grid, a generated cut table
+(``cut_copro000.inp``, store/cond/core/mod_cond.py:86-91) and a user
+``condition.f90`` holding ``set_IC``/``set_BC`` (e.g.
+sample/no06.step_flow/condition.f90).  Here the same information is data:
+
+* node coordinates ``xh``/``rh`` per block, shape ``(ni+1, nj+1)`` indexed
+  ``[i, j]`` (the reference's ``xh(0:ni,0:nj)``),
+* BC rows (6 ints: block, side, start, end, kind, pool offset) plus a
+  double pool, replacing the user ``set_BC`` loops,
+* cut rows (11 ints, the ``cut_copro`` format),
+* initial conservative state ``q`` per block in the reference layout.
+
+Arrays in the *reference layout* are numpy arrays of shape
+``(nj+4, ni+4, K)`` in C order, i.e. Fortran ``f(1:K,-1:ni+2,-1:nj+2)``;
+element ``(k, i, j)`` is ``a[j+1, i+1, k]``.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+from typing import List, Optional
+
+import numpy as np
+
+# ---- mirrors of include/shcfd_types.h ----------------------------------------
+SCHEME_EULER, SCHEME_RK2, SCHEME_LUSGS, SCHEME_DUAL, SCHEME_PRECON_LUSGS = 0, 1, 2, 5, 6
+LIMITER_REFERENCE, LIMITER_MINMOD = 0, 1
+BC_EXTRAP, BC_REFLECT, BC_NOSLIP, BC_FIXED, BC_INFLOW_TUV, BC_OUTFLOW_P, BC_SLIP_NORMAL = range(7)
+SIDE_IMIN, SIDE_IMAX, SIDE_JMIN, SIDE_JMAX = range(4)
+
+FIELDS = [
+    "Q", "QP", "QPP", "W", "VHI", "DHI", "WHLI", "WHRI", "WHLJ", "WHRJ",
+    "TGI", "TGJ", "TGVI", "TGVJ", "VOL", "DSI", "DSJ", "VNI", "VNJ", "DSCI", "DSCJ",
+    "VNCI", "VNCJ", "GEOJACI", "GEOJACJ", "X", "R", "XH", "RH", "DT", "ALPHA", "PHI",
+    "PRESCA", "PRE1", "PRE2", "DQDT", "AP", "AM", "BP", "BM", "DQ",
+]
+F = {name: idx for idx, name in enumerate(FIELDS)}
+
+
+def field_ncomp(name: str, ny: int) -> int:
+    dimq = ny + 3
+    dimw = dimq + 5
+    if name in ("Q", "QP", "QPP", "TGI", "TGJ", "TGVI", "TGVJ", "PRE1", "PRE2", "DQDT", "DQ"):
+        return dimq
+    if name in ("W", "WHLI", "WHRI", "WHLJ", "WHRJ"):
+        return dimw
+    if name in ("VHI", "DHI"):
+        return ny
+    if name in ("VNI", "VNJ", "VNCI", "VNCJ"):
+        return 2
+    if name in ("GEOJACI", "GEOJACJ", "PRESCA"):
+        return 4
+    if name in ("AP", "AM", "BP", "BM"):
+        return dimq * dimq
+    return 1
+
+
+class Params(ctypes.Structure):
+    """ctypes mirror of ``shcfd_params`` (include/shcfd_types.h)."""
+
+    _fields_ = [
+        ("scheme", ctypes.c_int), ("time_step", ctypes.c_int), ("high_order", ctypes.c_int),
+        ("limiter", ctypes.c_int), ("viscous", ctypes.c_int), ("thermal", ctypes.c_int),
+        ("num_internal_loop", ctypes.c_int), ("pad0", ctypes.c_int),
+        ("phi_mus", ctypes.c_double), ("k_mus", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("cfl_tau", ctypes.c_double), ("omega_max", ctypes.c_double), ("omega_min", ctypes.c_double),
+        ("dqmax", ctypes.c_double), ("res_rate_warning", ctypes.c_double),
+        ("res_rate_error", ctypes.c_double), ("kappa", ctypes.c_double), ("rgas", ctypes.c_double),
+        ("nu", ctypes.c_double),
+    ]
+
+
+def make_params(**kw) -> Params:
+    d = dict(scheme=SCHEME_LUSGS, time_step=0, high_order=1, limiter=LIMITER_REFERENCE, viscous=0,
+             thermal=0, num_internal_loop=21, pad0=0, phi_mus=1.0, k_mus=-1.0, cfl=0.5, cfl_tau=1e-2,
+             omega_max=0.1, omega_min=1e-3, dqmax=1e-2, res_rate_warning=1e-5, res_rate_error=1e-3,
+             kappa=1.4, rgas=287.0, nu=1.6e-5)
+    d.update(kw)
+    p = Params()
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
+def ref_zeros(k: int, ni: int, nj: int) -> np.ndarray:
+    return np.zeros((nj + 4, ni + 4, k), dtype=np.float64)
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    ni: List[int]
+    nj: List[int]
+    ny: int
+    params: Params
+    xh: List[np.ndarray]
+    rh: List[np.ndarray]
+    q0: List[np.ndarray]
+    bc_rows: np.ndarray
+    bc_pool: np.ndarray
+    cut_rows: np.ndarray
+    description: str = ""
+
+    @property
+    def nblocks(self) -> int:
+        return len(self.ni)
+
+    @property
+    def ncells(self) -> int:
+        return int(sum(a * b for a, b in zip(self.ni, self.nj)))
+
+    @property
+    def dimq(self) -> int:
+        return self.ny + 3
+
+    @property
+    def dimw(self) -> int:
+        return self.ny + 8
+
+    def nodes_ref(self, b: int):
+        """xh/rh of block b as reference-layout arrays (K=1)."""
+        out = []
+        for arr in (self.xh[b], self.rh[b]):
+            a = ref_zeros(1, self.ni[b], self.nj[b])
+            a[1:self.nj[b] + 2, 1:self.ni[b] + 2, 0] = arr.T
+            out.append(a)
+        return out
+
+
+# ---- helpers --------------------------------------------------------------------
+def cons_from_prim(rho, u, v, p, kappa):
+    """q = (rho, rho u, rho v, rho E) for a single-species ideal gas."""
+    return np.stack([rho, rho * u, rho * v, p / (kappa - 1.0) + 0.5 * rho * (u * u + v * v)], axis=-1)
+
+
+def _fill_q(ni, nj, qcell):
+    """qcell: (nj, ni, dimq) interior values -> reference-layout q."""
+    q = ref_zeros(qcell.shape[-1], ni, nj)
+    q[2:nj + 2, 2:ni + 2, :] = qcell
+    return q
+
+
+def rect_nodes(ni, nj, x0, x1, y0, y1):
+    xs = np.linspace(x0, x1, ni + 1)
+    ys = np.linspace(y0, y1, nj + 1)
+    xh = np.repeat(xs[:, None], nj + 1, axis=1)
+    rh = np.repeat(ys[None, :], ni + 1, axis=0)
+    return xh.copy(), rh.copy()
+
+
+def fixed_state_pool(rho, u, v, p, kappa, rgas, mu=0.0, ny=1):
+    """w(1:dimw) + vhi(1:nY) for a single-species ideal-gas FIXED BC."""
+    T = p / (rho * rgas)
+    H = kappa / (kappa - 1.0) * rgas * T + 0.5 * (u * u + v * v)
+    w = [rho, u, v, p] + [1.0] * ny + [kappa, H, rgas, mu]
+    vhi = [kappa / (kappa - 1.0) * rgas * T] * ny
+    return w + vhi
+
+
+# ---- C1: Sod shock tube ---------------------------------------------------------
+def sod(ni=400, nj=4, limiter=LIMITER_REFERENCE, scheme=SCHEME_LUSGS, cfl=0.5, order=2,
+        high_order=1) -> Case:
+    """Sod tube on x in [0,1], uniform dx, nj cells of height dx; zero-gradient
+    BCs on all sides as in sample/no00.shock_tube/condition.f90."""
+    kappa, rgas = 1.4, 287.0
+    dx = 1.0 / ni
+    xh, rh = rect_nodes(ni, nj, 0.0, 1.0, 0.0, nj * dx)
+    xc = (np.arange(ni) + 0.5) * dx
+    left = xc < 0.5
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1e5, 1e4)
+    z = np.zeros_like(rho)
+    qrow = cons_from_prim(rho, z, z, p, kappa)
+    qcell = np.repeat(qrow[None, :, :], nj, axis=0)
+    rows = []
+    for side, n in ((SIDE_IMIN, nj), (SIDE_IMAX, nj), (SIDE_JMIN, ni), (SIDE_JMAX, ni)):
+        rows.append([1, side, 1, n, BC_EXTRAP, 0])
+    params = make_params(scheme=scheme, time_step=0, high_order=high_order, limiter=limiter, cfl=cfl,
+                         k_mus=-1.0 if order == 2 else 1.0 / 3.0, kappa=kappa, rgas=rgas, nu=1.6e-5,
+                         num_internal_loop=21, cfl_tau=5.0, omega_max=1.0, dqmax=0.5,
+                         res_rate_error=1e300, res_rate_warning=1e300)
+    return Case("sod", [ni], [nj], 1, params, [xh], [rh], [_fill_q(ni, nj, qcell)],
+                np.array(rows, dtype=np.int32), np.zeros(1), np.zeros((0, 11), dtype=np.int32),
+                description=f"Sod shock tube {ni}x{nj}")
+
+
+# ---- C2: Mach-3 forward-facing step (2 blocks) ------------------------------------
+def forward_step(n_per_unit=320, scheme=SCHEME_DUAL, num_internal_loop=21, cfl=0.5, cfl_tau=5.0,
+                 omega_max=1.0, dqmax=0.5, k_mus=1.0 / 3.0) -> Case:
+    """Woodward-Colella step: domain [0,3]x[0,1], step x>=0.6, y<0.2.
+
+    Block 1: x in [0,0.6], y in [0,1]  -> (0.6 h^-1) x (h^-1) cells.
+    Block 2: x in [0.6,3], y in [0.2,1] -> (2.4 h^-1) x (0.8 h^-1) cells.
+    One i-i cut joins block 1's i-max side above the step to block 2's i-min
+    side (the topology of sample/no06.step_flow/forward_step.x).  Inflow
+    rho=1.4, p=1, u=3 (gamma=1.4, so c=1 and M=3); slip walls; zero-gradient
+    outflow; the BC kinds of sample/no06.step_flow/condition.f90."""
+    h = 1.0 / n_per_unit
+    n1i, n1j = int(round(0.6 * n_per_unit)), n_per_unit
+    n2i, n2j = int(round(2.4 * n_per_unit)), int(round(0.8 * n_per_unit))
+    nstep = n1j - n2j
+    kappa, rgas = 1.4, 1.0
+    rho0, u0, p0 = 1.4, 3.0, 1.0
+    xh1, rh1 = rect_nodes(n1i, n1j, 0.0, 0.6, 0.0, 1.0)
+    xh2, rh2 = rect_nodes(n2i, n2j, 0.6, 3.0, 0.2, 1.0)
+    q = []
+    for ni, nj in ((n1i, n1j), (n2i, n2j)):
+        qc = cons_from_prim(np.full((nj, ni), rho0), np.full((nj, ni), u0), np.zeros((nj, ni)),
+                            np.full((nj, ni), p0), kappa)
+        q.append(_fill_q(ni, nj, qc))
+    pool = np.array(fixed_state_pool(rho0, u0, 0.0, p0, kappa, rgas, mu=0.0), dtype=np.float64)
+    rows = [
+        [1, SIDE_IMIN, 1, n1j, BC_FIXED, 0],
+        [1, SIDE_IMAX, 1, nstep, BC_REFLECT, 0],
+        [1, SIDE_JMIN, 1, n1i, BC_REFLECT, 0],
+        [1, SIDE_JMAX, 1, n1i, BC_REFLECT, 0],
+        [2, SIDE_IMAX, 1, n2j, BC_EXTRAP, 0],
+        [2, SIDE_JMIN, 1, n2i, BC_REFLECT, 0],
+        [2, SIDE_JMAX, 1, n2i, BC_REFLECT, 0],
+    ]
+    cut = [[1, n2j, 1, 1, n1i, nstep + 1, 1, 2, 1, 1, -1]]
+    params = make_params(scheme=scheme, time_step=0, high_order=1, limiter=LIMITER_REFERENCE, cfl=cfl,
+                         k_mus=k_mus, kappa=kappa, rgas=rgas, nu=1.6e-5, num_internal_loop=num_internal_loop,
+                         cfl_tau=cfl_tau, omega_max=omega_max, omega_min=1e-3, dqmax=dqmax,
+                         res_rate_warning=1e300, res_rate_error=1e300)
+    return Case("forward_step", [n1i, n2i], [n1j, n2j], 1, params, [xh1, xh2], [rh1, rh2], q,
+                np.array(rows, dtype=np.int32), pool, np.array(cut, dtype=np.int32),
+                description=f"Mach-3 forward step, h=1/{n_per_unit}, {n1i}x{n1j}+{n2i}x{n2j} cells")
+
+
+# ---- random smooth states on skewed grids (parity tests) -----------------------------
+def random_case(seed=0, ni=(13,), nj=(9,), ny=1, scheme=SCHEME_LUSGS, viscous=0, limiter=0,
+                time_step=0, k_mus=1.0 / 3.0, high_order=1, mach=0.5, with_cut=False) -> Case:
+    """Smooth random perturbations of a uniform flow on skewed curvilinear
+    blocks; all sides zero-gradient except a FIXED inflow and a slip wall."""
+    rng = np.random.default_rng(seed)
+    kappa, rgas = 1.4, 287.0
+    xhs, rhs, qs = [], [], []
+    x_off = 0.0
+    for b, (n_i, n_j) in enumerate(zip(ni, nj)):
+        s = np.linspace(0, 1, n_i + 1)[:, None]
+        t = np.linspace(0, 1, n_j + 1)[None, :]
+        xh = x_off + s * 1.0 + 0.05 * np.sin(2 * np.pi * t) * (1 - s) * 0.5 + 0.02 * rng.standard_normal((n_i + 1, n_j + 1)) / max(n_i, n_j)
+        rh = t * 0.8 + 0.04 * np.sin(np.pi * s) + 0.02 * rng.standard_normal((n_i + 1, n_j + 1)) / max(n_i, n_j)
+        if with_cut and b > 0:
+            xh[0, :] = xhs[-1][-1, :]
+            rh[0, :] = rhs[-1][-1, :]
+        xhs.append(xh)
+        rhs.append(rh)
+        x_off = float(xh[-1, :].mean())
+        T = 300.0 * (1 + 0.1 * rng.random((n_j, n_i)))
+        rho = 1.2 * (1 + 0.2 * rng.random((n_j, n_i)))
+        c0 = np.sqrt(kappa * rgas * 300.0)
+        u = mach * c0 * (1 + 0.3 * rng.standard_normal((n_j, n_i)))
+        v = 0.3 * mach * c0 * rng.standard_normal((n_j, n_i))
+        p = rho * rgas * T
+        if ny == 1:
+            qc = cons_from_prim(rho, u, v, p, kappa)
+        else:
+            Y = rng.random((n_j, n_i, ny)) + 0.1
+            Y /= Y.sum(axis=-1, keepdims=True)
+            qc = np.concatenate([rho[..., None] * Y, (rho * u)[..., None], (rho * v)[..., None],
+                                 (p / (kappa - 1) + 0.5 * rho * (u * u + v * v))[..., None]], axis=-1)
+        qs.append(_fill_q(n_i, n_j, qc))
+    pool = np.array(fixed_state_pool(1.2, 0.5 * np.sqrt(kappa * rgas * 300.0), 0.0, 1.2 * rgas * 300.0,
+                                     kappa, rgas, mu=1.2 * 1.6e-5, ny=ny) + [300.0, 100.0, 0.0, 1.1e5],
+                    dtype=np.float64)
+    rows = []
+    cut = []
+    nb = len(ni)
+    for b in range(nb):
+        first, last = b == 0, b == nb - 1
+        if first:
+            rows.append([b + 1, SIDE_IMIN, 1, nj[b], BC_FIXED, 0])
+        if last:
+            rows.append([b + 1, SIDE_IMAX, 1, nj[b], BC_EXTRAP, 0])
+        rows.append([b + 1, SIDE_JMIN, 1, ni[b], BC_NOSLIP if viscous else BC_REFLECT, 0])
+        rows.append([b + 1, SIDE_JMAX, 1, ni[b], BC_EXTRAP, 0])
+        if not last:
+            if with_cut:
+                cut.append([1, nj[b], 1, b + 1, ni[b], 1, 1, b + 2, 1, 1, -1])
+            else:
+                rows.append([b + 1, SIDE_IMAX, 1, nj[b], BC_EXTRAP, 0])
+                rows.append([b + 2, SIDE_IMIN, 1, nj[b + 1], BC_EXTRAP, 0])
+    params = make_params(scheme=scheme, time_step=time_step, high_order=high_order, limiter=limiter,
+                         viscous=viscous, cfl=0.8, k_mus=k_mus, kappa=kappa, rgas=rgas, nu=1.6e-5 * 1e3,
+                         num_internal_loop=4, cfl_tau=2.0, omega_max=1.0, omega_min=1e-6, dqmax=0.5,
+                         res_rate_warning=1e300, res_rate_error=1e300)
+    return Case(f"random{seed}", list(ni), list(nj), ny, params, xhs, rhs, qs,
+                np.array(rows, dtype=np.int32), pool,
+                np.array(cut, dtype=np.int32).reshape(-1, 11), description="random smooth test state")

@@ -1,0 +1,703 @@
+// shcfd_api.hip -- C ABI of libshcfd (include/shcfd.h): context set-up, data
+// movement, tables, and the per-routine / per-step drivers that enqueue the
+// HIP kernels on the context's stream.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "../../include/shcfd.h"
+#include "shcfd_kernels.h"
+
+using namespace shcfd;
+
+struct shcfd_ctx : public Ctx {};
+
+namespace shcfd {
+int field_ncomp(int fld, int ny) {
+    int dimq = ny + 3, dimw = dimq + 5;
+    switch (fld) {
+    case SHCFD_F_Q: case SHCFD_F_QP: case SHCFD_F_QPP: case SHCFD_F_TGI: case SHCFD_F_TGJ:
+    case SHCFD_F_TGVI: case SHCFD_F_TGVJ: case SHCFD_F_PRE1: case SHCFD_F_PRE2: case SHCFD_F_DQDT:
+    case SHCFD_F_DQ:
+        return dimq;
+    case SHCFD_F_W: case SHCFD_F_WHLI: case SHCFD_F_WHRI: case SHCFD_F_WHLJ: case SHCFD_F_WHRJ:
+        return dimw;
+    case SHCFD_F_VHI: case SHCFD_F_DHI:
+        return ny;
+    case SHCFD_F_VNI: case SHCFD_F_VNJ: case SHCFD_F_VNCI: case SHCFD_F_VNCJ:
+        return 2;
+    case SHCFD_F_GEOJACI: case SHCFD_F_GEOJACJ: case SHCFD_F_PRESCA:
+        return 4;
+    case SHCFD_F_AP: case SHCFD_F_AM: case SHCFD_F_BP: case SHCFD_F_BM:
+        return dimq * dimq;
+    default:
+        return 1;
+    }
+}
+}  // namespace shcfd
+
+#define HIPCHK(x)                                                                             \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess) {                                                               \
+            if (ctx) ctx->last_error = std::string(#x) + ": " + hipGetErrorString(e_);        \
+            return SHCFD_E_HIP;                                                               \
+        }                                                                                     \
+    } while (0)
+#define LAUNCHCHK()                                                                           \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess) {                                                               \
+            ctx->last_error = std::string("launch: ") + hipGetErrorString(e_);                \
+            return SHCFD_E_HIP;                                                               \
+        }                                                                                     \
+    } while (0)
+
+#define DISPATCH_NY(NYV, CALL)           \
+    switch (NYV) {                       \
+    case 1: { constexpr int NY = 1; CALL; } break; \
+    case 2: { constexpr int NY = 2; CALL; } break; \
+    case 9: { constexpr int NY = 9; CALL; } break; \
+    default: return SHCFD_E_ARG;         \
+    }
+
+static bool ny_supported(int ny) { return ny == 1 || ny == 2 || ny == 9; }
+
+extern "C" {
+
+const char* shcfd_strerror(int code) {
+    switch (code) {
+    case SHCFD_OK: return "ok";
+    case SHCFD_E_ARG: return "invalid argument";
+    case SHCFD_E_HIP: return "HIP runtime error";
+    case SHCFD_E_NOMEM: return "device allocation failed";
+    case SHCFD_E_NAN: return "NaN in residual";
+    case SHCFD_E_OMEGA: return "Omega becomes too small";
+    case SHCFD_E_RESRATE: return "ERROR. EXCEED MAXIMUM RESIDUAL.";
+    case SHCFD_E_TNEWTON: return "temperature Newton iteration did not converge";
+    case SHCFD_E_CHEM: return "chemistry integrator failure";
+    case SHCFD_E_SPIN: return "sweep wavefront wait exceeded its bound";
+    case SHCFD_E_STATE: return "call not valid for this scheme/state";
+    default: return "unknown error";
+    }
+}
+
+int shcfd_version(void) { return 100; }
+
+int shcfd_create(int nblocks, const int* ni, const int* nj, int ny, const shcfd_params* params, shcfd_ctx** out) {
+    if (!out || nblocks <= 0 || !ni || !nj || !params || !ny_supported(ny)) return SHCFD_E_ARG;
+    *out = nullptr;
+    for (int b = 0; b < nblocks; b++)
+        if (ni[b] < 2 || nj[b] < 2) return SHCFD_E_ARG;
+    shcfd_ctx* ctx = new (std::nothrow) shcfd_ctx();
+    if (!ctx) return SHCFD_E_NOMEM;
+    ctx->nb = nblocks;
+    ctx->ny = ny;
+    ctx->dimq = ny + 3;
+    ctx->dimw = ny + 8;
+    ctx->p = *params;
+    long long off = 0;
+    int maxni = 0;
+    for (int b = 0; b < nblocks; b++) {
+        DevBlock B{};
+        B.ni = ni[b];
+        B.nj = nj[b];
+        B.pitch = ((ni[b] + 4) + 7) / 8 * 8;
+        B.off = off;
+        off += (long long)B.pitch * (nj[b] + 4);
+        off = (off + 63) / 64 * 64;
+        ctx->hblk.push_back(B);
+        maxni = std::max(maxni, ni[b]);
+        // tiles covering i in [0, ni+1], j in [0, nj+1]
+        for (int j0 = 0; j0 <= nj[b] + 1; j0 += TJ)
+            for (int i0 = 0; i0 <= ni[b] + 1; i0 += TI) ctx->htiles.push_back(Tile{b, i0, j0, 0});
+    }
+    ctx->NC = off;
+    // sweep strips, ordered by block then ascending j0
+    for (int b = 0; b < nblocks; b++) {
+        int first = (int)ctx->hstrips.size();
+        int ns = (nj[b] + 63) / 64;
+        for (int k = 0; k < ns; k++) {
+            Strip S{};
+            S.b = b;
+            S.j0 = 1 + 64 * k;
+            S.prev = k > 0 ? first + k - 1 : -1;
+            S.next = k < ns - 1 ? first + k + 1 : -1;
+            ctx->hstrips.push_back(S);
+        }
+    }
+    ctx->bnd_stride = (long long)(maxni + 2) * ctx->dimq;
+    ctx->npart = (int)std::max(ctx->htiles.size(), ctx->hstrips.size());
+    hipError_t e;
+    e = hipStreamCreate(&ctx->stream);
+    if (e != hipSuccess) { delete ctx; return SHCFD_E_HIP; }
+    auto al = [&](void** p, size_t bytes) -> bool {
+        if (hipMalloc(p, bytes) != hipSuccess) return false;
+        return hipMemset(*p, 0, bytes) == hipSuccess;
+    };
+    bool ok = al((void**)&ctx->dblk, sizeof(DevBlock) * nblocks) &&
+              al((void**)&ctx->dtiles, sizeof(Tile) * ctx->htiles.size()) &&
+              al((void**)&ctx->dstrips, sizeof(Strip) * ctx->hstrips.size()) &&
+              al((void**)&ctx->dprog, sizeof(int) * 2 * ctx->hstrips.size()) &&
+              al((void**)&ctx->dbnd, sizeof(double) * ctx->bnd_stride * ctx->hstrips.size()) &&
+              al((void**)&ctx->dpart, sizeof(double) * 2 * ctx->npart) &&
+              al((void**)&ctx->scal, sizeof(double) * SCAL_COUNT) &&
+              al((void**)&ctx->iflag, sizeof(int) * IFLAG_COUNT);
+    for (int fld = 0; ok && fld < SHCFD_F_COUNT; fld++)
+        ok = al((void**)&ctx->f[fld], sizeof(double) * (size_t)field_ncomp(fld, ny) * (size_t)ctx->NC);
+    if (!ok) { shcfd_destroy(ctx); return SHCFD_E_NOMEM; }
+    hipMemcpy(ctx->dblk, ctx->hblk.data(), sizeof(DevBlock) * nblocks, hipMemcpyHostToDevice);
+    hipMemcpy(ctx->dtiles, ctx->htiles.data(), sizeof(Tile) * ctx->htiles.size(), hipMemcpyHostToDevice);
+    hipMemcpy(ctx->dstrips, ctx->hstrips.data(), sizeof(Strip) * ctx->hstrips.size(), hipMemcpyHostToDevice);
+    double one = 1.0;
+    hipMemcpy(ctx->scal + SCAL_FIRST_DUAL, &one, sizeof(double), hipMemcpyHostToDevice);
+    if (hipDeviceSynchronize() != hipSuccess) { shcfd_destroy(ctx); return SHCFD_E_HIP; }
+    *out = ctx;
+    return SHCFD_OK;
+}
+
+int shcfd_destroy(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (int fld = 0; fld < SHCFD_F_COUNT; fld++) if (ctx->f[fld]) hipFree(ctx->f[fld]);
+    void* ptrs[] = {ctx->dblk, ctx->dtiles, ctx->dstrips, ctx->dprog, ctx->dbnd, ctx->dpart, ctx->scal,
+                    ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool};
+    for (void* p : ptrs) if (p) hipFree(p);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SHCFD_OK;
+}
+
+int shcfd_set_params(shcfd_ctx* ctx, const shcfd_params* params) {
+    if (!ctx || !params) return SHCFD_E_ARG;
+    ctx->p = *params;
+    return SHCFD_OK;
+}
+
+static int check_field(shcfd_ctx* ctx, int field, int block) {
+    if (!ctx || field < 0 || field >= SHCFD_F_COUNT || block < 0 || block >= ctx->nb) return SHCFD_E_ARG;
+    return SHCFD_OK;
+}
+
+int shcfd_upload_field(shcfd_ctx* ctx, int field, int block, const double* host) {
+    int rc = check_field(ctx, field, block);
+    if (rc || !host) return SHCFD_E_ARG;
+    const DevBlock& B = ctx->hblk[block];
+    int K = field_ncomp(field, ctx->ny);
+    size_t rows = (size_t)B.nj + 4, w = (size_t)B.ni + 4;
+    std::vector<double> plane(rows * B.pitch, 0.0);
+    for (int k = 0; k < K; k++) {
+        for (size_t j = 0; j < rows; j++)
+            for (size_t i = 0; i < w; i++) plane[j * B.pitch + i] = host[k + (size_t)K * (i + w * j)];
+        HIPCHK(hipMemcpyAsync(ctx->f[field] + (long long)k * ctx->NC + B.off, plane.data(),
+                              sizeof(double) * plane.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return SHCFD_OK;
+}
+
+int shcfd_download_field(shcfd_ctx* ctx, int field, int block, double* host) {
+    int rc = check_field(ctx, field, block);
+    if (rc || !host) return SHCFD_E_ARG;
+    const DevBlock& B = ctx->hblk[block];
+    int K = field_ncomp(field, ctx->ny);
+    size_t rows = (size_t)B.nj + 4, w = (size_t)B.ni + 4;
+    std::vector<double> plane(rows * B.pitch, 0.0);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < K; k++) {
+        HIPCHK(hipMemcpy(plane.data(), ctx->f[field] + (long long)k * ctx->NC + B.off, sizeof(double) * plane.size(),
+                         hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < rows; j++)
+            for (size_t i = 0; i < w; i++) host[k + (size_t)K * (i + w * j)] = plane[j * B.pitch + i];
+    }
+    return SHCFD_OK;
+}
+
+int shcfd_set_cut_table(shcfd_ctx* ctx, const int* rows, int nrows) {
+    if (!ctx || nrows < 0 || (nrows > 0 && !rows)) return SHCFD_E_ARG;
+    for (int r = 0; r < nrows; r++) {
+        const int* t = rows + SHCFD_CUT_ROW * r;
+        int geo = t[0], width = t[1], order = t[2], p1 = t[3], p2 = t[7];
+        if (geo < 1 || geo > 4 || width < 0 || (order != 1 && order != -1) || p1 < 1 || p1 > ctx->nb || p2 < 1 ||
+            p2 > ctx->nb || std::abs(t[6]) != 1 || std::abs(t[10]) != 1)
+            return SHCFD_E_ARG;
+        // every cell a copy touches must lie inside its block's ghosted extents
+        auto inb = [&](int b, int i, int j) {
+            const DevBlock& B = ctx->hblk[b];
+            return i >= -1 && i <= B.ni + 2 && j >= -1 && j <= B.nj + 2;
+        };
+        int incs[2] = {0, width > 0 ? width - 1 : 0};
+        for (int q = 0; q < 2 && width > 0; q++)
+            for (int copy = 0; copy < 4; copy++) {
+                int bd, di, dj, bs, si, sj;
+                if (!cut_map(t, incs[q], copy, bd, di, dj, bs, si, sj)) return SHCFD_E_ARG;
+                if (!inb(bd, di, dj) || !inb(bs, si, sj)) return SHCFD_E_ARG;
+            }
+    }
+    if (ctx->dcut) { hipFree(ctx->dcut); ctx->dcut = nullptr; }
+    ctx->hcut.assign(rows, rows + SHCFD_CUT_ROW * nrows);
+    ctx->ncut = nrows;
+    if (nrows > 0) {
+        HIPCHK(hipMalloc((void**)&ctx->dcut, sizeof(int) * SHCFD_CUT_ROW * nrows));
+        HIPCHK(hipMemcpy(ctx->dcut, rows, sizeof(int) * SHCFD_CUT_ROW * nrows, hipMemcpyHostToDevice));
+    }
+    return SHCFD_OK;
+}
+
+int shcfd_set_bc_table(shcfd_ctx* ctx, const int* rows, int nrows, const double* pool, int npool) {
+    if (!ctx || nrows < 0 || npool < 0 || (nrows > 0 && !rows) || (npool > 0 && !pool)) return SHCFD_E_ARG;
+    int maxlen = 1;
+    // per block side: which positions are covered (segments may not overlap)
+    std::vector<std::vector<char>> cover(ctx->nb * 4);
+    for (int b = 0; b < ctx->nb; b++)
+        for (int s = 0; s < 4; s++) cover[b * 4 + s].assign((s <= 1 ? ctx->hblk[b].nj : ctx->hblk[b].ni) + 2, 0);
+    for (int r = 0; r < nrows; r++) {
+        const int* t = rows + SHCFD_BC_ROW * r;
+        int b = t[0], side = t[1], s = t[2], e = t[3], kind = t[4], off = t[5];
+        if (b < 1 || b > ctx->nb || side < 0 || side > 3 || kind < 0 || kind > SHCFD_BC_SLIP_NORMAL || off < 0)
+            return SHCFD_E_ARG;
+        int need = kind == SHCFD_BC_FIXED ? ctx->dimw + ctx->ny : kind == SHCFD_BC_INFLOW_TUV ? 3 : kind == SHCFD_BC_OUTFLOW_P ? 1 : 0;
+        if (need > 0 && off + need > npool) return SHCFD_E_ARG;
+        const DevBlock& B = ctx->hblk[b - 1];
+        int n = side <= 1 ? B.nj : B.ni;
+        s = std::max(s, 1);
+        e = std::min(e, n);
+        auto& cv = cover[(b - 1) * 4 + side];
+        for (int k = s; k <= e; k++) {
+            if (cv[k]) return SHCFD_E_ARG;
+            cv[k] = 1;
+        }
+        maxlen = std::max(maxlen, e - s + 1);
+    }
+    if (ctx->dbc) { hipFree(ctx->dbc); ctx->dbc = nullptr; }
+    if (ctx->dpool) { hipFree(ctx->dpool); ctx->dpool = nullptr; }
+    ctx->hbc.assign(rows, rows + SHCFD_BC_ROW * nrows);
+    ctx->nbc = nrows;
+    ctx->npool = npool;
+    ctx->bc_maxlen = maxlen;
+    if (nrows > 0) {
+        HIPCHK(hipMalloc((void**)&ctx->dbc, sizeof(int) * SHCFD_BC_ROW * nrows));
+        HIPCHK(hipMemcpy(ctx->dbc, rows, sizeof(int) * SHCFD_BC_ROW * nrows, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc((void**)&ctx->dpool, sizeof(double) * std::max(npool, 1)));
+    if (npool > 0) HIPCHK(hipMemcpy(ctx->dpool, pool, sizeof(double) * npool, hipMemcpyHostToDevice));
+    return SHCFD_OK;
+}
+
+static int ntiles(shcfd_ctx* ctx) { return (int)ctx->htiles.size(); }
+
+int shcfd_init_geometry(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    DevCtx d = ctx->dev();
+    hipLaunchKernelGGL(k_geom_a, dim3(ntiles(ctx)), dim3(256), 0, ctx->stream, d, ctx->dtiles);
+    int maxn = 0;
+    for (auto& B : ctx->hblk) maxn = std::max(maxn, std::max(B.ni, B.nj) + 2);
+    hipLaunchKernelGGL(k_geom_ghost, dim3((maxn + 255) / 256, ctx->nb), dim3(256), 0, ctx->stream, d, 1);
+    hipLaunchKernelGGL(k_geom_ghost, dim3((maxn + 255) / 256, ctx->nb), dim3(256), 0, ctx->stream, d, 2);
+    hipLaunchKernelGGL(k_geojac, dim3(ntiles(ctx)), dim3(256), 0, ctx->stream, d, ctx->dtiles);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SHCFD_OK;
+}
+
+__global__ void k_set_scal(DevCtx d, int slot, double v) { d.scal[slot] = v; }
+
+int shcfd_set_time(shcfd_ctx* ctx, double tt, int first_dual) {
+    if (!ctx) return SHCFD_E_ARG;
+    DevCtx d = ctx->dev();
+    hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(1), 0, ctx->stream, d, (int)SCAL_TT, tt);
+    hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(1), 0, ctx->stream, d, (int)SCAL_FIRST_DUAL, first_dual ? 1.0 : 0.0);
+    LAUNCHCHK();
+    ctx->first_dual = first_dual;
+    return SHCFD_OK;
+}
+
+int shcfd_get_time(shcfd_ctx* ctx, double* dt_grbl, double* tt) {
+    if (!ctx) return SHCFD_E_ARG;
+    double s[SCAL_COUNT];
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(s, ctx->scal, sizeof(s), hipMemcpyDeviceToHost));
+    if (dt_grbl) *dt_grbl = s[SCAL_DT_GRBL];
+    if (tt) *tt = s[SCAL_TT];
+    return SHCFD_OK;
+}
+
+int shcfd_synchronize(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SHCFD_OK;
+}
+
+int shcfd_check_error(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    int flags[IFLAG_COUNT];
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(flags, ctx->iflag, sizeof(flags), hipMemcpyDeviceToHost));
+    if (flags[IFLAG_ERR]) {
+        int z = 0;
+        HIPCHK(hipMemcpy(ctx->iflag + IFLAG_ERR, &z, sizeof(int), hipMemcpyHostToDevice));
+        return flags[IFLAG_ERR];
+    }
+    return SHCFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// profiler
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+struct ProfScope {
+    Ctx* c;
+    int id;
+    EventPair ev{};
+    ProfScope(Ctx* c_, int id_) : c(c_), id(id_) {
+        if (!((c->prof >> id) & 1u)) return;
+        if (c->prof_free.empty()) {
+            EventPair e{};
+            hipEventCreate(&e.a);
+            hipEventCreate(&e.b);
+            c->prof_free.push_back(e);
+        }
+        ev = c->prof_free.back();
+        c->prof_free.pop_back();
+        hipEventRecord(ev.a, c->stream);
+    }
+    ~ProfScope() {
+        if (!((c->prof >> id) & 1u)) return;
+        hipEventRecord(ev.b, c->stream);
+        c->prof_pending[id].push_back(ev);
+    }
+};
+void prof_collect(Ctx* c) {
+    for (int k = 0; k < PROF_COUNT; k++) {
+        for (auto& e : c->prof_pending[k]) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e.a, e.b);
+            c->prof_ms[k] += ms;
+            c->prof_n[k] += 1;
+            c->prof_free.push_back(e);
+        }
+        c->prof_pending[k].clear();
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int shcfd_profile_enable(shcfd_ctx* ctx, int on) {
+    if (!ctx) return SHCFD_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    ctx->prof = (unsigned)on;
+    return SHCFD_OK;
+}
+
+int shcfd_profile_read(shcfd_ctx* ctx, int kernel_class, double* total_ms, long long* launches, int reset) {
+    if (!ctx || kernel_class < 0 || kernel_class >= PROF_COUNT) return SHCFD_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    if (total_ms) *total_ms = ctx->prof_ms[kernel_class];
+    if (launches) *launches = ctx->prof_n[kernel_class];
+    if (reset) {
+        ctx->prof_ms[kernel_class] = 0.0;
+        ctx->prof_n[kernel_class] = 0;
+    }
+    return SHCFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-routine drop-ins
+// ---------------------------------------------------------------------------
+#define TILE_LAUNCH(KERNEL, ...)                                                                              \
+    hipLaunchKernelGGL(KERNEL, dim3(ntiles(ctx)), dim3(256), 0, ctx->stream, ctx->dev(), ctx->dtiles, ##__VA_ARGS__)
+
+int shcfd_set_w(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    ProfScope ps(ctx, PROF_PRIMITIVE);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_w<NY>));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_set_thermo_prop(shcfd_ctx* ctx, int step) {
+    (void)step;
+    if (!ctx) return SHCFD_E_ARG;
+    if (ctx->p.thermal != 0) return SHCFD_E_STATE;
+    ProfScope ps(ctx, PROF_PRIMITIVE);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_thermo_ideal<NY>));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_set_bc(shcfd_ctx* ctx, int step) {
+    (void)step;
+    if (!ctx) return SHCFD_E_ARG;
+    DevCtx d = ctx->dev();
+    ProfScope ps(ctx, PROF_BC);
+    if (ctx->ncut > 0) {
+        int maxw = 1;
+        for (int r = 0; r < ctx->ncut; r++) maxw = std::max(maxw, ctx->hcut[SHCFD_CUT_ROW * r + 1]);
+        long long n = 4LL * maxw * ctx->ncut;
+        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_bc_cut<NY>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                                                ctx->stream, d, ctx->dcut, ctx->ncut, maxw));
+    }
+    if (ctx->nbc > 0) {
+        long long n = (long long)ctx->bc_maxlen * ctx->nbc;
+        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_bc_side<NY>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                                                ctx->stream, d, ctx->dbc, ctx->nbc, ctx->dpool, ctx->bc_maxlen));
+    }
+    DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_bc_corner<NY>, dim3((4 * ctx->nb + 255) / 256), dim3(256), 0,
+                                            ctx->stream, d));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_set_hv(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    ProfScope ps(ctx, PROF_HV);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_hv<NY>));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_set_tg(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    ProfScope ps(ctx, PROF_TG);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_tg<NY>));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_set_tgv(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    ProfScope ps(ctx, PROF_TGV);
+    if (!ctx->p.viscous) {
+        // init_TGv: viscosity/non-viscous/sch_viscous.f90 (TGv = 0)
+        long long n = (long long)ctx->dimq * ctx->NC;
+        hipLaunchKernelGGL(k_zero_field, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[SHCFD_F_TGVI], n);
+        hipLaunchKernelGGL(k_zero_field, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[SHCFD_F_TGVJ], n);
+    } else {
+        DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_tgv<NY>));
+    }
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_set_dt(shcfd_ctx* ctx, int step, double* dt_grbl_out) {
+    (void)step;
+    if (!ctx) return SHCFD_E_ARG;
+    {
+    ProfScope ps(ctx, PROF_DT);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_dt<NY>, ctx->dpart));
+    if (ctx->p.time_step == 0)
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, ctx->stream, ctx->dev(), ctx->dpart, ntiles(ctx), 0,
+                           (int)SCAL_DT_GRBL);
+    }
+    LAUNCHCHK();
+    if (dt_grbl_out) {
+        double s;
+        HIPCHK(hipMemcpyAsync(&s, ctx->scal + SCAL_DT_GRBL, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *dt_grbl_out = s;
+    }
+    return SHCFD_OK;
+}
+
+int shcfd_set_abpm(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    ProfScope ps(ctx, PROF_ABPM);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_abpm<NY>));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_set_dqdt(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    ProfScope ps(ctx, PROF_DQDT);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_dqdt<NY>));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+static int implicit_async(shcfd_ctx* ctx, int step_internal) {
+    const int ns = (int)ctx->hstrips.size();
+    DevCtx d = ctx->dev();
+    int* prog_f = ctx->dprog;
+    int* prog_b = ctx->dprog + ns;
+    { ProfScope ps(ctx, PROF_RHS);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_rhs<NY>, ctx->dprog, 2 * ns)); }
+    { ProfScope ps(ctx, PROF_SWEEP_FWD);
+    DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep_fwd_plain<NY>, dim3(ns), dim3(64), 0, ctx->stream, d,
+                                            ctx->dstrips, prog_f, ctx->dbnd, ctx->bnd_stride, ctx->spin_limit)); }
+    { ProfScope ps(ctx, PROF_SWEEP_BWD);
+    DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep_bwd_plain<NY>, dim3(ns), dim3(64), 0, ctx->stream, d,
+                                            ctx->dstrips, ns, prog_b, ctx->dbnd, ctx->bnd_stride, ctx->dpart,
+                                            ctx->spin_limit)); }
+    if (ctx->p.scheme == SHCFD_SCHEME_DUAL) {
+        { ProfScope ps(ctx, PROF_REDUCE);
+        hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, ctx->stream, d, ctx->dpart, ns); }
+        if (step_internal == 1)
+            hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, ctx->stream, d, (int)SCAL_RES1, (int)SCAL_RES);
+        { ProfScope ps(ctx, PROF_UPDATE);
+        DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_update<NY>)); }
+    }
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_calc_next_step_implicit(shcfd_ctx* ctx, int step_internal, double* res_out, double* omega_out) {
+    if (!ctx) return SHCFD_E_ARG;
+    int s = ctx->p.scheme;
+    if (s != SHCFD_SCHEME_LUSGS && s != SHCFD_SCHEME_PRECON_LUSGS && s != SHCFD_SCHEME_DUAL) return SHCFD_E_STATE;
+    int rc = implicit_async(ctx, step_internal);
+    if (rc) return rc;
+    if (res_out || omega_out) {
+        double sc[SCAL_COUNT];
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(sc, ctx->scal, sizeof(sc), hipMemcpyDeviceToHost));
+        if (res_out) *res_out = sc[SCAL_RES];
+        if (omega_out) *omega_out = sc[SCAL_OMEGA];
+        return shcfd_check_error(ctx);
+    }
+    return SHCFD_OK;
+}
+
+int shcfd_calc_next_step(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    ProfScope ps(ctx, PROF_EXPLICIT);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_explicit<NY>, 0));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_calc_next_step_rk2(shcfd_ctx* ctx, int stage) {
+    if (!ctx || (stage != 1 && stage != 2)) return SHCFD_E_ARG;
+    ProfScope ps(ctx, PROF_EXPLICIT);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_explicit<NY>, stage));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_copy_state(shcfd_ctx* ctx, int dst, int src) {
+    if (!ctx) return SHCFD_E_ARG;
+    auto okf = [](int f) { return f == SHCFD_F_Q || f == SHCFD_F_QP || f == SHCFD_F_QPP; };
+    if (!okf(dst) || !okf(src) || dst == src) return SHCFD_E_ARG;
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_copy_q<NY>, dst, src));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_energy_rms(shcfd_ctx* ctx, double* rms_out) {
+    if (!ctx || !rms_out) return SHCFD_E_ARG;
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_rms_part<NY>, ctx->dpart));
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, ctx->stream, ctx->dev(), ctx->dpart, ntiles(ctx), 1,
+                       (int)SCAL_RMS);
+    LAUNCHCHK();
+    double s;
+    HIPCHK(hipMemcpyAsync(&s, ctx->scal + SCAL_RMS, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double cells = 0.0;
+    for (auto& B : ctx->hblk) cells = cells + (double)(B.ni * B.nj);
+    *rms_out = sqrt(s) / cells;
+    return SHCFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// whole steps
+// ---------------------------------------------------------------------------
+static int part_primitive(shcfd_ctx* ctx) {
+    int rc = shcfd_set_w(ctx);
+    if (!rc) rc = shcfd_set_thermo_prop(ctx, 0);
+    return rc;
+}
+static int part_main(shcfd_ctx* ctx) {
+    int rc = shcfd_set_bc(ctx, 0);
+    if (!rc) rc = shcfd_set_hv(ctx);
+    if (!rc) rc = shcfd_set_tg(ctx);
+    if (!rc) rc = shcfd_set_tgv(ctx);
+    return rc;
+}
+
+int shcfd_prepare(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    int rc = part_primitive(ctx);
+    if (!rc) rc = shcfd_set_bc(ctx, 0);
+    return rc;
+}
+
+int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double* res_last) {
+    if (!ctx) return SHCFD_E_ARG;
+    int rc = SHCFD_OK;
+#define TRY(x) do { rc = (x); if (rc) return rc; } while (0)
+    switch (ctx->p.scheme) {
+    case SHCFD_SCHEME_LUSGS:
+        TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
+        TRY(part_primitive(ctx));
+        TRY(shcfd_set_dt(ctx, step, nullptr));
+        TRY(part_main(ctx));
+        TRY(shcfd_set_abpm(ctx));
+        TRY(implicit_async(ctx, 0));
+        break;
+    case SHCFD_SCHEME_PRECON_LUSGS:
+        TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
+        TRY(part_primitive(ctx));
+        TRY(part_main(ctx));
+        TRY(shcfd_set_abpm(ctx));
+        TRY(implicit_async(ctx, 0));
+        break;
+    case SHCFD_SCHEME_EULER:
+        TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
+        TRY(part_primitive(ctx));
+        TRY(shcfd_set_dt(ctx, step, nullptr));
+        TRY(part_main(ctx));
+        TRY(shcfd_calc_next_step(ctx));
+        break;
+    case SHCFD_SCHEME_RK2:
+        TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
+        TRY(part_primitive(ctx));
+        TRY(shcfd_set_dt(ctx, step, nullptr));
+        TRY(part_main(ctx));
+        TRY(shcfd_calc_next_step_rk2(ctx, 1));
+        TRY(part_primitive(ctx));
+        TRY(part_main(ctx));
+        TRY(shcfd_calc_next_step_rk2(ctx, 2));
+        break;
+    case SHCFD_SCHEME_DUAL: {
+        TRY(part_primitive(ctx));
+        TRY(shcfd_set_dt(ctx, step, nullptr));
+        TRY(shcfd_copy_state(ctx, SHCFD_F_QPP, SHCFD_F_QP));
+        TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
+        for (int j = 1; j < ctx->p.num_internal_loop; j++) {
+            TRY(part_primitive(ctx));
+            TRY(part_main(ctx));
+            TRY(shcfd_set_abpm(ctx));
+            TRY(shcfd_set_dqdt(ctx));
+            TRY(implicit_async(ctx, j));
+        }
+        double sc[SCAL_COUNT];
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(sc, ctx->scal, sizeof(sc), hipMemcpyDeviceToHost));
+        if (res_first) *res_first = sc[SCAL_RES1];
+        if (res_last) *res_last = sc[SCAL_RES];
+        TRY(shcfd_check_error(ctx));
+        // check_internal_loop_convergence: sch_dual.raw.f90:284-299
+        double rate = sc[SCAL_RES] / sc[SCAL_RES1];
+        if (rate > ctx->p.res_rate_warning) {
+            fprintf(stderr, "WARNING:NOT CONVERGED AT INTERNAL LOOP (step,res_rate) = %12d %11.3E\n", step, rate);
+            if (rate > ctx->p.res_rate_error && step != step_res + 1) return SHCFD_E_RESRATE;
+        }
+        hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(1), 0, ctx->stream, ctx->dev(), (int)SCAL_FIRST_DUAL, 0.0);
+        ctx->first_dual = 0;
+        break;
+    }
+    default:
+        return SHCFD_E_STATE;
+    }
+#undef TRY
+    return shcfd_check_error(ctx);
+}
+
+}  // extern "C"

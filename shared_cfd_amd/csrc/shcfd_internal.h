@@ -1,0 +1,146 @@
+// shcfd_internal.h -- device data layout and shared helpers of libshcfd.
+//
+// Device layout (all blocks resident in one GPU's HBM):
+//   * one global "cell index space": block b owns the (ni+4) x (nj+4)
+//     ghosted cells i in [-1, ni+2], j in [-1, nj+2], row pitch `pitch`
+//     (>= ni+4, multiple of 8 doubles), starting at `off` (multiple of 64);
+//     NC is the total size.  Cell, i-face, j-face and node quantities all
+//     use this index space (face (i,j) = the face between cell i and i+1).
+//   * every field is structure-of-arrays: component k of field F lives at
+//     f[F][k*NC + cell], so consecutive i are consecutive addresses.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <vector>
+#include <string>
+
+#include "../../include/shcfd_types.h"
+
+namespace shcfd {
+
+struct DevBlock {
+    int ni, nj, pitch, pad;
+    long long off;  // index of cell (-1,-1)
+};
+
+// Device-side view passed by value to every kernel.
+struct DevCtx {
+    int nb, ny, dimq, dimw;
+    long long NC;
+    const DevBlock* blk;
+    double* f[SHCFD_F_COUNT];
+    double* scal;     // device scalars, see SCAL_* below
+    int* iflag;       // device int flags (error codes, counters)
+    shcfd_params p;
+};
+
+enum {
+    SCAL_DT_GRBL = 0,
+    SCAL_TT = 1,
+    SCAL_OMEGA = 2,     // min omega of the last implicit update
+    SCAL_RES = 3,       // res of the last implicit update
+    SCAL_RES1 = 4,
+    SCAL_RMS = 5,
+    SCAL_FIRST_DUAL = 6,
+    SCAL_COUNT = 16
+};
+enum { IFLAG_ERR = 0, IFLAG_COUNT = 8 };
+
+__host__ __device__ inline long long cidx(const DevBlock& B, int i, int j) {
+    return B.off + (long long)(j + 1) * B.pitch + (i + 1);
+}
+
+// 0-based component indices of w
+#define IW_RHO 0
+#define IW_U 1
+#define IW_V 2
+#define IW_P 3
+#define IW_Y(k) (4 + (k))
+
+// Tiles of 64 x 4 cells covering i in [0, ni+1], j in [0, nj+1] of each
+// block; one 256-thread workgroup per tile.
+constexpr int TI = 64;
+constexpr int TJ = 4;
+
+struct Tile {
+    int b, i0, j0, pad;
+};
+
+// One wavefront-strip of the LU-SGS sweeps: 64 consecutive rows of a block.
+struct Strip {
+    int b;       // block
+    int j0;      // first row (1-based)
+    int prev;    // strip index of the row range below (forward) or -1
+    int next;    // strip index of the row range above (backward) or -1
+};
+
+// kernel classes timed by the optional HIP-event profiler
+enum {
+    PROF_PRIMITIVE = 0, PROF_BC, PROF_HV, PROF_TG, PROF_TGV, PROF_DT, PROF_ABPM, PROF_DQDT, PROF_RHS,
+    PROF_SWEEP_FWD, PROF_SWEEP_BWD, PROF_REDUCE, PROF_UPDATE, PROF_EXPLICIT, PROF_RESIDUAL, PROF_PREP,
+    PROF_COUNT
+};
+
+struct EventPair {
+    hipEvent_t a, b;
+};
+
+// ---------------------------------------------------------------------
+// host-side context
+// ---------------------------------------------------------------------
+struct Ctx {
+    // profiler: events recorded around each launch of a kernel class
+    unsigned prof = 0;             // bit k set: class k is timed
+    std::vector<EventPair> prof_pending[PROF_COUNT];
+    std::vector<EventPair> prof_free;
+    double prof_ms[PROF_COUNT] = {};
+    long long prof_n[PROF_COUNT] = {};
+    int nb = 0, ny = 0, dimq = 0, dimw = 0;
+    shcfd_params p{};
+    std::vector<DevBlock> hblk;
+    long long NC = 0;
+    DevBlock* dblk = nullptr;
+    double* f[SHCFD_F_COUNT] = {};
+    double* scal = nullptr;
+    int* iflag = nullptr;
+    // tables
+    std::vector<Tile> htiles;
+    Tile* dtiles = nullptr;
+    std::vector<Strip> hstrips;
+    Strip* dstrips = nullptr;
+    int* dprog = nullptr;          // per-strip progress counters (fwd, bwd)
+    double* dbnd = nullptr;        // per-strip boundary rows (dimq per column)
+    long long bnd_stride = 0;      // doubles per strip in dbnd
+    int* dcut = nullptr;
+    int ncut = 0;
+    int* dbc = nullptr;
+    int nbc = 0;
+    double* dpool = nullptr;
+    int npool = 0;
+    int bc_maxlen = 0;
+    std::vector<int> hbc, hcut;
+    // reduction scratch
+    double* dpart = nullptr;       // per-tile / per-strip partials (2 per entry)
+    int npart = 0;
+    // host mirrors of device scalars
+    double dt_grbl = 0.0, tt = 0.0;
+    int first_dual = 1;
+    double res1 = 0.0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    int sweep_impl = 0;            // 0 = plain (stored Jacobians)
+    long long spin_limit = 1LL << 23;
+
+    DevCtx dev() const {
+        DevCtx d{};
+        d.nb = nb; d.ny = ny; d.dimq = dimq; d.dimw = dimw; d.NC = NC;
+        d.blk = dblk;
+        for (int i = 0; i < SHCFD_F_COUNT; i++) d.f[i] = f[i];
+        d.scal = scal; d.iflag = iflag; d.p = p;
+        return d;
+    }
+};
+
+int field_ncomp(int fld, int ny);
+
+}  // namespace shcfd

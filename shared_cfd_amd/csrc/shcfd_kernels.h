@@ -1,0 +1,415 @@
+// shcfd_kernels.h -- per-cell / per-face device functions shared by the plain
+// per-routine kernels and the fused kernels, plus kernel declarations.
+#pragma once
+#include "shcfd_device.h"
+
+// species counts with compiled kernels (1 = ideal gas)
+#define SHCFD_FOR_EACH_NY(X) X(1) X(2) X(9)
+
+namespace shcfd {
+
+// MUSCL slope pair of one cell: high_order/muscl/muscl.f90:28-35
+// (limiter 1 = minmod, an addition)
+__device__ inline void muscl_delta(double Du, double Dl, double k_mus, int limiter, double& du, double& dl) {
+    if (limiter == SHCFD_LIMITER_MINMOD) {
+        double m;
+        if (Du * Dl <= 0.0) m = 0.0;
+        else m = (fabs(Du) < fabs(Dl)) ? Du : Dl;
+        du = m;
+        dl = m;
+        return;
+    }
+    double s = ((2.0 * Du) * Dl) / (((Du * Du) + (Dl * Dl)) + 1e-300);
+    du = (0.5 * s) * (((1.0 + k_mus * s) * Du) + ((1.0 - k_mus * s) * Dl));
+    dl = (0.5 * s) * (((1.0 + k_mus * s) * Dl) + ((1.0 - k_mus * s) * Du));
+}
+
+// SLAU flux of one face: core/scheme.f90:71-153.  wl/wr are full w
+// vectors (rho,u,v,p,Y(NY),gamma,H,...), n the unit face normal.
+template <int NY>
+__device__ inline void slau_face(const double* wl, const double* wr, double n1, double n2, double* tg) {
+    constexpr int dq = NY + 3;
+    double rhol = wl[0], ul = wl[1], vl = wl[2], pl = wl[3], gl = wl[dq + 1];
+    double rhor = wr[0], ur = wr[1], vr = wr[2], pr = wr[3], gr = wr[dq + 1];
+    double al = sqrt((gl * pl) / rhol);
+    double unl = (n1 * ul) + (n2 * vl);
+    double ar = sqrt((gr * pr) / rhor);
+    double unr = (n1 * ur) + (n2 * vr);
+    double dp = pr - pl;
+    double a_bar = 0.5 * (ar + al);
+    double p_bar = 0.5 * (pr + pl);
+    double ml = unl / a_bar;
+    double mr = unr / a_bar;
+    double mt = (1.0 / a_bar) * sqrt(0.5 * ((((ul * ul) + (vl * vl)) + (ur * ur)) + (vr * vr)));
+    double m_tilde = (1.0 < mt) ? 1.0 : mt;
+    double chi = (1.0 - m_tilde) * (1.0 - m_tilde);
+    double tt = (ml < 0.0) ? ml : 0.0;
+    double gp = -((tt > -1.0) ? tt : -1.0);
+    tt = (mr > 0.0) ? mr : 0.0;
+    double gm = (tt < 1.0) ? tt : 1.0;
+    double g = gp * gm;
+    double un_bar_abs = ((fabs(unr) * rhor) + (fabs(unl) * rhol)) / (rhor + rhol);
+    double unp = ((1.0 - g) * un_bar_abs) + (g * fabs(unl));
+    double unm = ((1.0 - g) * un_bar_abs) + (g * fabs(unr));
+    double dm = 0.5 * (((rhol * (unl + unp)) + (rhor * (unr - unm))) - ((chi / a_bar) * dp));
+    double betar, betal;
+    if (fabs(mr) < 1.0) betar = (0.25 * (2.0 + mr)) * ((mr - 1.0) * (mr - 1.0));
+    else betar = 0.5 * (1.0 - fsign1(mr));
+    if (fabs(ml) < 1.0) betal = (0.25 * (2.0 - ml)) * ((ml + 1.0) * (ml + 1.0));
+    else betal = 0.5 * (1.0 + fsign1(ml));
+    double p_tilde = (p_bar + ((0.5 * (betal - betar)) * (pl - pr))) + (((1.0 - chi) * ((betar + betal) - 1.0)) * p_bar);
+    double cp = 0.5 * (dm + fabs(dm)), cm = 0.5 * (dm - fabs(dm));
+#pragma unroll
+    for (int k = 0; k < NY; k++) tg[k] = ((cp * wl[4 + k]) + (cm * wr[4 + k])) + (p_tilde * 0.0);
+    tg[NY] = ((cp * ul) + (cm * ur)) + (p_tilde * n1);
+    tg[NY + 1] = ((cp * vl) + (cm * vr)) + (p_tilde * n2);
+    tg[NY + 2] = ((cp * wl[dq + 2]) + (cm * wr[dq + 2])) + (p_tilde * 0.0);
+}
+
+// viscous flux of one face: viscosity/viscous/no-nV/2d/sch_viscous.f90:106-178
+// (DIR 0: i-face (i,j) between i and i+1; DIR 1: j-face between j and j+1)
+template <int NY, int DIR>
+__device__ inline void viscous_face(const DevCtx& d, const DevBlock& B, int i, int j, double* out) {
+    constexpr int dq = NY + 3, ig = dq + 1, ir = dq + 3, imu = dq + 4;
+    const long long P = B.pitch;
+    const long long c = cidx(B, i, j);
+    const long long c1 = DIR == 0 ? c + 1 : c + P;
+    auto Wf = [&](int k, long long cc) { return FLD(d, SHCFD_F_W, k, cc); };
+    auto Td = [&](long long cc) { return (Wf(IW_P, cc) / Wf(IW_RHO, cc)) / Wf(ir, cc); };
+    const double Sc = 1.0, Pr = 1.0;
+    double rho = (Wf(IW_RHO, c) + Wf(IW_RHO, c1)) * 0.5;
+    double u = (Wf(IW_U, c) + Wf(IW_U, c1)) * 0.5;
+    double v = (Wf(IW_V, c) + Wf(IW_V, c1)) * 0.5;
+    double gamm = (Wf(ig, c) + Wf(ig, c1)) * 0.5;
+    double Rg = (Wf(ir, c) + Wf(ir, c1)) * 0.5;
+    double mu = (Wf(imu, c) + Wf(imu, c1)) * 0.5;
+    double lhi[NY], dwdxi[NY], dwdeta[NY], dwdx[NY], dwdr[NY];
+#pragma unroll
+    for (int k = 0; k < NY; k++) lhi[k] = (FLD(d, SHCFD_F_VHI, k, c) + FLD(d, SHCFD_F_VHI, k, c1)) * 0.5;
+    double D = (mu / rho) / Sc;
+    double k_heat = (((mu * gamm) / (gamm - 1.0)) * Rg) / Pr;
+    double dudxi, dvdxi, dTdxi, dudeta, dvdeta, dTdeta;
+    const double* gj;
+    int fg;
+    if (DIR == 0) {
+        // neighbours (i+1,j+1),(i,j+1),(i+1,j-1),(i,j-1)
+        long long a = c + 1 + P, bq = c + P, e = c + 1 - P, f = c - P;
+        dudxi = Wf(IW_U, c1) - Wf(IW_U, c);
+        dvdxi = Wf(IW_V, c1) - Wf(IW_V, c);
+        dTdxi = Td(c1) - Td(c);
+#pragma unroll
+        for (int k = 0; k < NY; k++) dwdxi[k] = Wf(IW_Y(k), c1) - Wf(IW_Y(k), c);
+        dudeta = (((Wf(IW_U, a) + Wf(IW_U, bq)) - Wf(IW_U, e)) - Wf(IW_U, f)) * 0.25;
+        dvdeta = (((Wf(IW_V, a) + Wf(IW_V, bq)) - Wf(IW_V, e)) - Wf(IW_V, f)) * 0.25;
+        dTdeta = (((Td(a) + Td(bq)) - Td(e)) - Td(f)) * 0.25;
+#pragma unroll
+        for (int k = 0; k < NY; k++)
+            dwdeta[k] = (((Wf(IW_Y(k), a) + Wf(IW_Y(k), bq)) - Wf(IW_Y(k), e)) - Wf(IW_Y(k), f)) * 0.25;
+        fg = SHCFD_F_GEOJACI;
+    } else {
+        // neighbours (i+1,j+1),(i+1,j),(i-1,j+1),(i-1,j)
+        long long a = c + 1 + P, bq = c + 1, e = c - 1 + P, f = c - 1;
+        dudxi = (((Wf(IW_U, a) + Wf(IW_U, bq)) - Wf(IW_U, e)) - Wf(IW_U, f)) * 0.25;
+        dvdxi = (((Wf(IW_V, a) + Wf(IW_V, bq)) - Wf(IW_V, e)) - Wf(IW_V, f)) * 0.25;
+        dTdxi = (((Td(a) + Td(bq)) - Td(e)) - Td(f)) * 0.25;
+#pragma unroll
+        for (int k = 0; k < NY; k++)
+            dwdxi[k] = (((Wf(IW_Y(k), a) + Wf(IW_Y(k), bq)) - Wf(IW_Y(k), e)) - Wf(IW_Y(k), f)) * 0.25;
+        dudeta = Wf(IW_U, c1) - Wf(IW_U, c);
+        dvdeta = Wf(IW_V, c1) - Wf(IW_V, c);
+        dTdeta = Td(c1) - Td(c);
+#pragma unroll
+        for (int k = 0; k < NY; k++) dwdeta[k] = Wf(IW_Y(k), c1) - Wf(IW_Y(k), c);
+        fg = SHCFD_F_GEOJACJ;
+    }
+    (void)gj;
+    double g11 = FLD(d, fg, 0, c), g21 = FLD(d, fg, 1, c), g12 = FLD(d, fg, 2, c), g22 = FLD(d, fg, 3, c);
+    double dudx = (dudxi * g11) + (dudeta * g21);
+    double dvdx = (dvdxi * g11) + (dvdeta * g21);
+    double dTdx = (dTdxi * g11) + (dTdeta * g21);
+    double dudr = (dudxi * g12) + (dudeta * g22);
+    double dvdr = (dvdxi * g12) + (dvdeta * g22);
+    double dTdr = (dTdxi * g12) + (dTdeta * g22);
+    double hidwdx = 0.0, hidwdr = 0.0;
+#pragma unroll
+    for (int k = 0; k < NY; k++) {
+        dwdx[k] = (dwdxi[k] * g11) + (dwdeta[k] * g21);
+        hidwdx = hidwdx + (dwdx[k] * lhi[k]);
+        dwdr[k] = (dwdxi[k] * g12) + (dwdeta[k] * g22);
+        hidwdr = hidwdr + (dwdr[k] * lhi[k]);
+    }
+    double divu = dudx + dvdr;
+    double t_xx = -(((2.0 / 3.0) * mu) * divu) + ((2.0 * mu) * dudx);
+    double t_rr = -(((2.0 / 3.0) * mu) * divu) + ((2.0 * mu) * dvdr);
+    double t_xr = mu * (dvdx + dudr);
+    double qx = -(k_heat * dTdx) - ((rho * D) * hidwdx);
+    double qr = -(k_heat * dTdr) - ((rho * D) * hidwdr);
+    const int fvn = DIR == 0 ? SHCFD_F_VNI : SHCFD_F_VNJ;
+    double n1 = FLD(d, fvn, 0, c), n2 = FLD(d, fvn, 1, c);
+#pragma unroll
+    for (int k = 0; k < NY; k++) out[k] = (((rho * D) * dwdx[k]) * n1) + (((rho * D) * dwdr[k]) * n2);
+    out[NY] = (t_xx * n1) + (t_xr * n2);
+    out[NY + 1] = (t_xr * n1) + (t_rr * n2);
+    out[NY + 2] = ((((u * t_xx) + (v * t_xr)) - qx) * n1) + ((((u * t_xr) + (v * t_rr)) - qr) * n2);
+}
+
+// flux balance of one cell: LU-SGS/2d/time_lusgs_viscous.f90:12-22 (Area=0)
+template <int NY>
+__device__ inline void flux_balance(const DevCtx& d, const DevBlock& B, long long c, double* s) {
+    const long long P = B.pitch;
+    double di0 = FLD(d, SHCFD_F_DSI, 0, c - 1), di1 = FLD(d, SHCFD_F_DSI, 0, c);
+    double dj0 = FLD(d, SHCFD_F_DSJ, 0, c - P), dj1 = FLD(d, SHCFD_F_DSJ, 0, c);
+#pragma unroll
+    for (int k = 0; k < NY + 3; k++) {
+        double v = ((((di0 * (FLD(d, SHCFD_F_TGI, k, c - 1) - FLD(d, SHCFD_F_TGVI, k, c - 1))) -
+                      (di1 * (FLD(d, SHCFD_F_TGI, k, c) - FLD(d, SHCFD_F_TGVI, k, c)))) +
+                     (dj0 * (FLD(d, SHCFD_F_TGJ, k, c - P) - FLD(d, SHCFD_F_TGVJ, k, c - P)))) -
+                    (dj1 * (FLD(d, SHCFD_F_TGJ, k, c) - FLD(d, SHCFD_F_TGVJ, k, c))));
+        s[k] = v + 0.0;
+    }
+}
+
+// Cell-centre split flux Jacobians and LU-SGS scalars of one cell:
+// LU-SGS/2d/sch_lusgs.raw.f90:26-136, preconLU-SGS/2d/sch_precon.raw.f90:28-197,
+// dual/2d/sch_dual.raw.f90:28-204.
+template <int NY>
+struct CellJac {
+    static constexpr int dq = NY + 3;
+    double Ap[dq * dq], Am[dq * dq], Bp[dq * dq], Bm[dq * dq];
+    double alpha, phi, dtm;
+    double pre1[dq], pre2[dq], ps[4];
+    bool precond;
+};
+
+template <int NY>
+__device__ inline void cell_jacobian(const DevCtx& d, long long c, long long P, CellJac<NY>& J) {
+    constexpr int dq = NY + 3, ig = dq + 1, ih = dq + 2, imu = dq + 4;
+    const int scheme = d.p.scheme;
+    double wc[NY + 8], dhi[NY];
+#pragma unroll
+    for (int k = 0; k < NY + 8; k++) wc[k] = FLD(d, SHCFD_F_W, k, c);
+#pragma unroll
+    for (int k = 0; k < NY; k++) dhi[k] = FLD(d, SHCFD_F_DHI, k, c);
+    double rho = wc[0], u = wc[1], v = wc[2], p = wc[3], gamm = wc[ig], H = wc[ih];
+    double cs = sqrt((gamm * p) / rho);
+    double gmmad = gamm - 1.0;
+    double Vs = (u * u) + (v * v);
+    double n1a = FLD(d, SHCFD_F_VNCI, 0, c), n2a = FLD(d, SHCFD_F_VNCI, 1, c);
+    double n1b = FLD(d, SHCFD_F_VNCJ, 0, c), n2b = FLD(d, SHCFD_F_VNCJ, 1, c);
+    double unA = (n1a * u) + (n2a * v);
+    double unB = (n1b * u) + (n2b * v);
+    double dsci = FLD(d, SHCFD_F_DSCI, 0, c), dscj = FLD(d, SHCFD_F_DSCJ, 0, c);
+    double vol = FLD(d, SHCFD_F_VOL, 0, c);
+    double dtv = (d.p.time_step == 0) ? d.scal[SCAL_DT_GRBL] : FLD(d, SHCFD_F_DT, 0, c);
+    double nuA, nuB;
+    J.precond = (scheme == SHCFD_SCHEME_DUAL || scheme == SHCFD_SCHEME_PRECON_LUSGS);
+    double pscale = 1.0;
+    J.phi = 0.0;
+    J.dtm = 0.0;
+    if (!J.precond) {
+        nuA = fabs(unA) + cs;
+        nuB = fabs(unB) + cs;
+    } else {
+        double dmn = (dsci < dscj) ? dsci : dscj;
+        double Vv = (wc[imu] / rho) / dmn;
+        double sv = sqrt(Vs);
+        double Ur = (Vv > sv) ? Vv : sv;
+        Ur = (Ur < cs) ? Ur : cs;
+        double phi = (1.0 / (Ur * Ur)) - (1.0 / (cs * cs));
+        J.phi = phi;
+        double rr = Ur / cs;
+        double alp = 0.5 * (1.0 - rr * rr);
+        double ud = unA * (1.0 - alp);
+        double cd = sqrt(((alp * alp) * (unA * unA)) + (Ur * Ur));
+        nuA = fabs(ud) + cd;
+        ud = unB * (1.0 - alp);
+        cd = sqrt(((alp * alp) * (unB * unB)) + (Ur * Ur));
+        nuB = fabs(ud) + cd;
+        double uds = (nuA * (FLD(d, SHCFD_F_DSI, 0, c) + FLD(d, SHCFD_F_DSI, 0, c - 1))) +
+                     (nuB * (FLD(d, SHCFD_F_DSJ, 0, c) + FLD(d, SHCFD_F_DSJ, 0, c - P)));
+        double nr = wc[imu] / rho;
+        double mx = (uds > nr) ? uds : nr;
+        if (scheme == SHCFD_SCHEME_PRECON_LUSGS) {
+            double dtm = (d.p.cfl * vol) / mx;
+            J.dtm = dtm;
+            dtv = dtm;
+            double phh = -1.0 / ((cs * cs) + (1.0 / phi));
+#pragma unroll
+            for (int k = 0; k < NY; k++) J.pre1[k] = phh * wc[4 + k];
+            J.pre1[NY] = phh * u;
+            J.pre1[NY + 1] = phh * v;
+            J.pre1[NY + 2] = phh * H;
+        } else {
+            double dtau = (d.p.cfl_tau * vol) / mx;
+            double alpha = 1.0 / (((vol / dtau) + (dsci * nuA)) + (dscj * nuB));
+            J.alpha = alpha;
+            double beta = ((3.0 * vol) / (2.0 * dtv)) * alpha;
+            double ip = 1.0 / phi;
+            J.ps[0] = 1.0 + beta;
+            J.ps[1] = -1.0 / (ip + (cs * cs));
+            J.ps[2] = (-1.0 / (ip + (cs * cs))) / ((1.0 / beta) + 1.0);
+            J.ps[3] = beta / (((1.0 + beta) / phi) + (cs * cs));
+            pscale = J.ps[1];
+#pragma unroll
+            for (int k = 0; k < NY; k++) J.pre1[k] = wc[4 + k];
+            J.pre1[NY] = u;
+            J.pre1[NY + 1] = v;
+            J.pre1[NY + 2] = H;
+        }
+#pragma unroll
+        for (int k = 0; k < NY; k++) J.pre2[k] = (0.5 * gmmad) * Vs + dhi[k];
+        J.pre2[NY] = -(gmmad * u);
+        J.pre2[NY + 1] = -(gmmad * v);
+        J.pre2[NY + 2] = gmmad;
+    }
+    // Euler Jacobians A, B (column-major)
+    double A[dq * dq], Bm_[dq * dq], tA[dq], tB[dq];
+    const int a1 = NY, a2 = NY + 1, a3 = NY + 2;
+#define AM_(k, l) A[(k) + dq * (l)]
+#define BM_(k, l) Bm_[(k) + dq * (l)]
+    AM_(a1, a1) = u * (3.0 - gamm);
+    AM_(a2, a1) = v;
+    AM_(a3, a1) = H - (u * u) * gmmad;
+    AM_(a1, a2) = -(v * gmmad);
+    AM_(a2, a2) = u;
+    AM_(a3, a2) = -((u * v) * gmmad);
+    AM_(a1, a3) = gmmad;
+    AM_(a2, a3) = 0.0;
+    AM_(a3, a3) = u * gamm;
+#pragma unroll
+    for (int k = 0; k < NY; k++) {
+        tA[k] = -(wc[4 + k] * u);
+        AM_(k, a1) = wc[4 + k];
+        AM_(k, a2) = 0.0;
+        AM_(k, a3) = 0.0;
+    }
+    tA[a1] = -(u * u) + (0.5 * gmmad) * Vs;
+    tA[a2] = -(u * v);
+    tA[a3] = u * (-H + (0.5 * gmmad) * Vs);
+    BM_(a1, a1) = v;
+    BM_(a2, a1) = -(u * gmmad);
+    BM_(a3, a1) = -((u * v) * gmmad);
+    BM_(a1, a2) = u;
+    BM_(a2, a2) = v * (3.0 - gamm);
+    BM_(a3, a2) = H - (v * v) * gmmad;
+    BM_(a1, a3) = 0.0;
+    BM_(a2, a3) = gmmad;
+    BM_(a3, a3) = v * gamm;
+#pragma unroll
+    for (int k = 0; k < NY; k++) {
+        tB[k] = -(wc[4 + k] * v);
+        BM_(k, a1) = 0.0;
+        BM_(k, a2) = wc[4 + k];
+        BM_(k, a3) = 0.0;
+    }
+    tB[a1] = -(v * u);
+    tB[a2] = -(v * v) + (0.5 * gmmad) * Vs;
+    tB[a3] = v * (-H + (0.5 * gmmad) * Vs);
+#pragma unroll
+    for (int k = 0; k < NY; k++) {
+#pragma unroll
+        for (int l = 0; l < NY; l++) {
+            AM_(l, k) = tA[l];
+            BM_(l, k) = tB[l];
+        }
+        AM_(k, k) = tA[k] + u;
+        AM_(a1, k) = tA[a1] + dhi[k];
+        AM_(a2, k) = tA[a2];
+        AM_(a3, k) = tA[a3] + dhi[k] * u;
+        BM_(k, k) = tB[k] + v;
+        BM_(a1, k) = tB[a1];
+        BM_(a2, k) = tB[a2] + dhi[k];
+        BM_(a3, k) = tB[a3] + dhi[k] * v;
+    }
+    if (J.precond) {
+#pragma unroll
+        for (int pass = 0; pass < 2; pass++) {
+            double* M = pass == 0 ? A : Bm_;
+#pragma unroll
+            for (int k = 0; k < dq; k++) {
+                double tmp = 0.0;
+#pragma unroll
+                for (int l = 0; l < dq; l++) tmp = tmp + J.pre2[l] * M[l + dq * k];
+                if (scheme == SHCFD_SCHEME_DUAL) tmp = pscale * tmp;
+#pragma unroll
+                for (int l = 0; l < dq; l++) M[l + dq * k] = M[l + dq * k] + J.pre1[l] * tmp;
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < dq * dq; e++) {
+        double at = (n1a * A[e]) + (n2a * Bm_[e]);
+        double bt = (n1b * A[e]) + (n2b * Bm_[e]);
+        J.Ap[e] = at * 0.5;
+        J.Am[e] = at * 0.5;
+        J.Bp[e] = bt * 0.5;
+        J.Bm[e] = bt * 0.5;
+    }
+#pragma unroll
+    for (int k = 0; k < dq; k++) {
+        int e = k + dq * k;
+        J.Ap[e] = J.Ap[e] + nuA * 0.5;
+        J.Am[e] = J.Am[e] - nuA * 0.5;
+        J.Bp[e] = J.Bp[e] + nuB * 0.5;
+        J.Bm[e] = J.Bm[e] - nuB * 0.5;
+    }
+#undef AM_
+#undef BM_
+    if (scheme != SHCFD_SCHEME_DUAL) J.alpha = 1.0 / (((vol / dtv) + (dsci * nuA)) + (dscj * nuB));
+}
+
+// Cells touched by one copy of section_exchange (cond/no-nV/condition.tail.f90:19-68):
+// copy 0/1 fill the first/second ghost layer of plane p1 from p2, copy 2/3
+// those of p2 from p1.  Returns false for an unknown geo code.
+__host__ __device__ inline bool cut_map(const int* t, int inc, int copy, int& bd, int& di, int& dj, int& bs, int& si,
+                                        int& sj) {
+    int geo = t[0], order = t[2];
+    int p1 = t[3] - 1, c1 = t[4], s1 = t[5], pm1 = t[6];
+    int p2 = t[7] - 1, c2 = t[8], s2 = t[9], pm2 = t[10];
+    int a1 = s1 + inc, a2 = s2 + inc * order;
+    bool to1 = copy < 2;
+    int lay = copy & 1;
+    int g = to1 ? c1 + (lay ? 2 * pm1 : pm1) : c2 + (lay ? 2 * pm2 : pm2);
+    int src = to1 ? (lay ? c2 - pm2 : c2) : (lay ? c1 - pm1 : c1);
+    bd = to1 ? p1 : p2;
+    bs = to1 ? p2 : p1;
+    switch (geo) {
+    case 1: if (to1) { di = g; dj = a1; si = src; sj = a2; } else { di = g; dj = a2; si = src; sj = a1; } return true;
+    case 2: if (to1) { di = g; dj = a1; si = a2; sj = src; } else { di = a2; dj = g; si = src; sj = a1; } return true;
+    case 3: if (to1) { di = a1; dj = g; si = src; sj = a2; } else { di = g; dj = a2; si = a1; sj = src; } return true;
+    case 4: if (to1) { di = a1; dj = g; si = a2; sj = src; } else { di = a2; dj = g; si = a1; sj = src; } return true;
+    default: return false;
+    }
+}
+
+// ---- kernel declarations ----
+template <int NY> __global__ void k_set_w(DevCtx, const Tile*);
+template <int NY> __global__ void k_thermo_ideal(DevCtx, const Tile*);
+template <int NY> __global__ void k_bc_cut(DevCtx, const int*, int, int);
+template <int NY> __global__ void k_bc_side(DevCtx, const int*, int, const double*, int);
+template <int NY> __global__ void k_bc_corner(DevCtx);
+template <int NY> __global__ void k_set_hv(DevCtx, const Tile*);
+template <int NY> __global__ void k_set_tg(DevCtx, const Tile*);
+template <int NY> __global__ void k_set_tgv(DevCtx, const Tile*);
+template <int NY> __global__ void k_set_dt(DevCtx, const Tile*, double*);
+template <int NY> __global__ void k_rms_part(DevCtx, const Tile*, double*);
+template <int NY> __global__ void k_set_abpm(DevCtx, const Tile*);
+template <int NY> __global__ void k_set_dqdt(DevCtx, const Tile*);
+template <int NY> __global__ void k_explicit(DevCtx, const Tile*, int);
+template <int NY> __global__ void k_copy_q(DevCtx, const Tile*, int, int);
+__global__ void k_zero_field(double*, long long);
+__global__ void k_reduce_final(DevCtx, const double*, int, int, int);
+
+// implicit update (k_implicit.hip)
+template <int NY> __global__ void k_rhs(DevCtx, const Tile*, int*, int);
+template <int NY> __global__ void k_sweep_fwd_plain(DevCtx, const Strip*, int*, double*, long long, long long);
+template <int NY> __global__ void k_sweep_bwd_plain(DevCtx, const Strip*, int, int*, double*, long long, double*, long long);
+template <int NY> __global__ void k_update(DevCtx, const Tile*);
+__global__ void k_reduce_res_omega(DevCtx, const double*, int);
+__global__ void k_copy_scal(DevCtx, int, int);
+// geometry (k_geom.hip)
+__global__ void k_geom_a(DevCtx, const Tile*);
+__global__ void k_geom_ghost(DevCtx, int);
+__global__ void k_geojac(DevCtx, const Tile*);
+
+}  // namespace shcfd

@@ -1,0 +1,235 @@
+"""ctypes binding of libshcfd.so (the HIP library) and a host-side solver
+object that drives it the way the reference's generated main program drives
+its Fortran subroutines (store/core/main.body.raw.f90,
+store/time_scheme/*/main.main.raw.f90).
+
+There is no CPU fallback: if the library is missing, or no GPU is present,
+every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import cases as C
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libshcfd.so")
+_LIB = None
+
+# every symbol declared in include/shcfd.h
+EXPORTS = [
+    "shcfd_create", "shcfd_destroy", "shcfd_set_params", "shcfd_strerror", "shcfd_version",
+    "shcfd_upload_field", "shcfd_download_field", "shcfd_set_cut_table", "shcfd_set_bc_table",
+    "shcfd_init_geometry", "shcfd_set_time", "shcfd_get_time", "shcfd_synchronize", "shcfd_set_w",
+    "shcfd_set_thermo_prop", "shcfd_set_bc", "shcfd_set_hv", "shcfd_set_tg", "shcfd_set_tgv", "shcfd_set_dt",
+    "shcfd_set_abpm", "shcfd_set_dqdt", "shcfd_calc_next_step_implicit", "shcfd_calc_next_step",
+    "shcfd_calc_next_step_rk2", "shcfd_copy_state", "shcfd_energy_rms", "shcfd_prepare", "shcfd_step",
+    "shcfd_check_error", "shcfd_profile_enable", "shcfd_profile_read",
+]
+
+# kernel classes of shcfd_profile_read (include/shcfd.h)
+KCLASS = ["PRIMITIVE", "BC", "HV", "TG", "TGV", "DT", "ABPM", "DQDT", "RHS", "SWEEP_FWD", "SWEEP_BWD",
+          "REDUCE", "UPDATE", "EXPLICIT", "RESIDUAL", "PREP"]
+
+
+class ShcfdError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        msg = lib().shcfd_strerror(code).decode() if _LIB is not None else str(code)
+        super().__init__(f"{where}: {msg} (code {code})")
+
+
+def lib():
+    """Load libshcfd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built: run `python -m shared_cfd_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        L.shcfd_strerror.restype = ctypes.c_char_p
+        L.shcfd_strerror.argtypes = [ctypes.c_int]
+        L.shcfd_create.argtypes = [ctypes.c_int, ip, ip, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(vp)]
+        L.shcfd_destroy.argtypes = [vp]
+        L.shcfd_set_params.argtypes = [vp, ctypes.c_void_p]
+        L.shcfd_upload_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.shcfd_download_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.shcfd_set_cut_table.argtypes = [vp, ip, ctypes.c_int]
+        L.shcfd_set_bc_table.argtypes = [vp, ip, ctypes.c_int, dp, ctypes.c_int]
+        for n in ("shcfd_init_geometry", "shcfd_synchronize", "shcfd_set_w", "shcfd_set_hv", "shcfd_set_tg",
+                  "shcfd_set_tgv", "shcfd_set_abpm", "shcfd_set_dqdt", "shcfd_calc_next_step", "shcfd_prepare",
+                  "shcfd_check_error"):
+            getattr(L, n).argtypes = [vp]
+        L.shcfd_set_thermo_prop.argtypes = [vp, ctypes.c_int]
+        L.shcfd_set_bc.argtypes = [vp, ctypes.c_int]
+        L.shcfd_set_dt.argtypes = [vp, ctypes.c_int, dp]
+        L.shcfd_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+        L.shcfd_get_time.argtypes = [vp, dp, dp]
+        L.shcfd_calc_next_step_implicit.argtypes = [vp, ctypes.c_int, dp, dp]
+        L.shcfd_calc_next_step_rk2.argtypes = [vp, ctypes.c_int]
+        L.shcfd_copy_state.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.shcfd_energy_rms.argtypes = [vp, dp]
+        L.shcfd_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.shcfd_profile_enable.argtypes = [vp, ctypes.c_int]
+        L.shcfd_profile_read.argtypes = [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def check(rc, where):
+    if rc != 0:
+        raise ShcfdError(rc, where)
+
+
+class Solver:
+    """A device-resident case: all blocks' state in HBM, driven through the
+    C ABI.  Method names follow the reference's subroutine names."""
+
+    def __init__(self, case: C.Case, load_q: bool = True):
+        L = lib()
+        self.L = L
+        self.case = case
+        self.params = case.params
+        self.ni = np.array(case.ni, dtype=np.int32)
+        self.nj = np.array(case.nj, dtype=np.int32)
+        h = ctypes.c_void_p()
+        check(L.shcfd_create(case.nblocks, _ip(self.ni), _ip(self.nj), case.ny, ctypes.byref(self.params),
+                             ctypes.byref(h)), "shcfd_create")
+        self.ctx = h
+        for b in range(case.nblocks):
+            xh, rh = case.nodes_ref(b)
+            self.upload("XH", b, xh)
+            self.upload("RH", b, rh)
+            if load_q:
+                self.upload("Q", b, case.q0[b])
+        cut = np.ascontiguousarray(case.cut_rows, dtype=np.int32)
+        check(L.shcfd_set_cut_table(self.ctx, _ip(cut), int(cut.shape[0])), "shcfd_set_cut_table")
+        bc = np.ascontiguousarray(case.bc_rows, dtype=np.int32)
+        pool = np.ascontiguousarray(case.bc_pool, dtype=np.float64)
+        check(L.shcfd_set_bc_table(self.ctx, _ip(bc), int(bc.shape[0]), _dp(pool), int(pool.size)),
+              "shcfd_set_bc_table")
+        check(L.shcfd_init_geometry(self.ctx), "shcfd_init_geometry")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.shcfd_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        self.params = params
+        check(self.L.shcfd_set_params(self.ctx, ctypes.byref(params)), "shcfd_set_params")
+
+    def upload(self, name, b, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        k = C.field_ncomp(name, self.case.ny)
+        assert a.shape == (self.case.nj[b] + 4, self.case.ni[b] + 4, k), (name, a.shape)
+        check(self.L.shcfd_upload_field(self.ctx, C.F[name], b, _dp(a)), "shcfd_upload_field")
+
+    def download(self, name, b):
+        k = C.field_ncomp(name, self.case.ny)
+        a = np.zeros((self.case.nj[b] + 4, self.case.ni[b] + 4, k), dtype=np.float64)
+        check(self.L.shcfd_download_field(self.ctx, C.F[name], b, _dp(a)), "shcfd_download_field")
+        return a
+
+    # ---- per-routine drop-ins (reference names) ----
+    def set_w(self):
+        check(self.L.shcfd_set_w(self.ctx), "set_w")
+
+    def set_thermo_prop(self, step=0):
+        check(self.L.shcfd_set_thermo_prop(self.ctx, step), "set_thermo_prop")
+
+    def set_BC(self, step=0):
+        check(self.L.shcfd_set_bc(self.ctx, step), "set_BC")
+
+    def set_HV(self):
+        check(self.L.shcfd_set_hv(self.ctx), "set_HV")
+
+    def set_TG(self):
+        check(self.L.shcfd_set_tg(self.ctx), "set_TG")
+
+    def set_TGv(self):
+        check(self.L.shcfd_set_tgv(self.ctx), "set_TGv")
+
+    def set_dt(self, step=0):
+        v = ctypes.c_double(0.0)
+        check(self.L.shcfd_set_dt(self.ctx, step, ctypes.byref(v)), "set_dt")
+        return v.value
+
+    def set_ABpm(self):
+        check(self.L.shcfd_set_abpm(self.ctx), "set_ABpm")
+
+    def set_dqdt(self):
+        check(self.L.shcfd_set_dqdt(self.ctx), "set_dqdt")
+
+    def calc_next_step_implicit(self, step_internal=1):
+        r, o = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        check(self.L.shcfd_calc_next_step_implicit(self.ctx, step_internal, ctypes.byref(r), ctypes.byref(o)),
+              "calc_next_step_implicit")
+        return r.value, o.value
+
+    def calc_next_step(self):
+        check(self.L.shcfd_calc_next_step(self.ctx), "calc_next_step")
+
+    def calc_next_step_rk2(self, stage):
+        check(self.L.shcfd_calc_next_step_rk2(self.ctx, stage), "calc_next_step_rk2")
+
+    def copy_state(self, dst, src):
+        check(self.L.shcfd_copy_state(self.ctx, C.F[dst], C.F[src]), "copy_state")
+
+    def energy_rms(self):
+        v = ctypes.c_double(0.0)
+        check(self.L.shcfd_energy_rms(self.ctx, ctypes.byref(v)), "energy_rms")
+        return v.value
+
+    def set_time(self, tt, first_dual):
+        check(self.L.shcfd_set_time(self.ctx, tt, int(first_dual)), "set_time")
+
+    def get_time(self):
+        a, b = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        check(self.L.shcfd_get_time(self.ctx, ctypes.byref(a), ctypes.byref(b)), "get_time")
+        return a.value, b.value
+
+    def synchronize(self):
+        check(self.L.shcfd_synchronize(self.ctx), "synchronize")
+
+    # ---- whole steps ----
+    def prepare(self):
+        check(self.L.shcfd_prepare(self.ctx), "prepare")
+
+    def step(self, step=1, step_res=0):
+        r1, r2 = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        check(self.L.shcfd_step(self.ctx, step, step_res, ctypes.byref(r1), ctypes.byref(r2)), "step")
+        return r1.value, r2.value
+
+    # ---- per-kernel timing ----
+    def profile(self, classes=None):
+        """Time the given kernel classes (names of KCLASS; None = all, [] = off)."""
+        mask = -1 if classes is None else sum(1 << KCLASS.index(c) for c in classes)
+        check(self.L.shcfd_profile_enable(self.ctx, mask), "profile_enable")
+
+    def profile_read(self, reset=True):
+        """{class: (total_ms, launches)} for every kernel class with launches."""
+        out = {}
+        for k, name in enumerate(KCLASS):
+            ms, n = ctypes.c_double(0.0), ctypes.c_longlong(0)
+            check(self.L.shcfd_profile_read(self.ctx, k, ctypes.byref(ms), ctypes.byref(n), int(reset)), "profile_read")
+            if n.value:
+                out[name] = (ms.value, n.value)
+        return out
