@@ -50,6 +50,7 @@ def main(argv=None):
     ap.add_argument("--n-per-unit", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--plain-steps", type=int, default=1)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -97,6 +98,24 @@ def main(argv=None):
                 "alg_bytes_per_launch": bytes_launch,
                 "share_of_step": {k: round(v[0] / sum(x[0] for x in prof_all.values()), 4) for k, v in prof_all.items()}}
 
+    # the per-routine drop-in ("plain GPU": one kernel per reference routine,
+    # stored Jacobians) on the same workload, for comparison
+    plain = None
+    if a.plain_steps > 0:
+        s.close()
+        sp = Solver(case, fast=False)
+        sp.prepare()
+        sp.step(1, 0)
+        sp.synchronize()
+        tp0 = time.perf_counter()
+        for k in range(a.plain_steps):
+            sp.step(2 + k, 0)
+        sp.synchronize()
+        tp1 = time.perf_counter()
+        plain = {"value": round(cells * inner * a.plain_steps / (tp1 - tp0) / 1e6, 3), "unit": "Mcell-updates/s",
+                 "steps": a.plain_steps}
+        sp.close()
+
     cpu = None
     if not a.no_cpu:
         from oracle.oracle import Oracle
@@ -118,7 +137,7 @@ def main(argv=None):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": case.description + ", dual time 20 inner iterations/step, MUSCL3 + SLAU, FP64",
                    "cells": cells, "inner_iterations_per_step": inner, "blocks": case.nblocks},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "plain_gpu": plain,
     }
     print(json.dumps(out))
     return 0

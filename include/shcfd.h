@@ -38,6 +38,14 @@ int shcfd_destroy(shcfd_ctx *ctx);
  * store/high_order/muscl/muscl.f90:76-116, store/time_scheme/dual/2d/sch_dual.raw.f90:301-340 */
 int shcfd_set_params(shcfd_ctx *ctx, const shcfd_params *params);
 const char *shcfd_strerror(int code);
+/* options (no counterpart in the reference) */
+enum {
+    SHCFD_OPT_FAST_IMPLICIT = 1, /* 1 (default): optimised LU-SGS sweeps for nY = 1;
+                                    0: the per-routine path with stored Jacobians */
+    SHCFD_OPT_SPIN_LIMIT = 2     /* bound on polls of a sweep hand-off before
+                                    SHCFD_E_SPIN is raised                          */
+};
+int shcfd_set_option(shcfd_ctx *ctx, int option, int value);
 int shcfd_version(void);
 
 /* ---- data movement (reference layout, see shcfd_types.h) ----------------- */

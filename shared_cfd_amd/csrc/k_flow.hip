@@ -395,8 +395,8 @@ __global__ void __launch_bounds__(256) k_rms_part(DevCtx d, const Tile* tiles, d
 // ---------------------------------------------------------------------------
 // set_ABpm (LU-SGS, preconLU-SGS, dual) and set_dqdt
 // ---------------------------------------------------------------------------
-template <int NY>
-__global__ void __launch_bounds__(256) k_set_abpm(DevCtx d, const Tile* tiles) {
+template <int NY, bool MATS>
+__device__ inline void set_abpm_body(const DevCtx& d, const Tile* tiles) {
     using D = Dims<NY>;
     constexpr int dq = D::dq;
     TileCoord t = tile_coord(tiles);
@@ -406,6 +406,7 @@ __global__ void __launch_bounds__(256) k_set_abpm(DevCtx d, const Tile* tiles) {
     const long long c = cidx(B, i, j), P = B.pitch;
     CellJac<NY> J;
     cell_jacobian<NY>(d, c, P, J);
+    if (MATS)
     for (int e = 0; e < dq * dq; e++) {
         FLD(d, SHCFD_F_AP, e, c) = J.Ap[e];
         FLD(d, SHCFD_F_AM, e, c) = J.Am[e];
@@ -426,6 +427,11 @@ __global__ void __launch_bounds__(256) k_set_abpm(DevCtx d, const Tile* tiles) {
         }
     }
 }
+
+template <int NY>
+__global__ void __launch_bounds__(256) k_set_abpm(DevCtx d, const Tile* tiles) { set_abpm_body<NY, true>(d, tiles); }
+template <int NY>
+__global__ void __launch_bounds__(256) k_set_abpm_scal(DevCtx d, const Tile* tiles) { set_abpm_body<NY, false>(d, tiles); }
 
 template <int NY>
 __global__ void __launch_bounds__(256) k_set_dqdt(DevCtx d, const Tile* tiles) {
@@ -493,6 +499,7 @@ __global__ void __launch_bounds__(256) k_copy_q(DevCtx d, const Tile* tiles, int
     template __global__ void k_set_dt<NY>(DevCtx, const Tile*, double*);                                 \
     template __global__ void k_rms_part<NY>(DevCtx, const Tile*, double*);                               \
     template __global__ void k_set_abpm<NY>(DevCtx, const Tile*);                                        \
+    template __global__ void k_set_abpm_scal<NY>(DevCtx, const Tile*);                                   \
     template __global__ void k_set_dqdt<NY>(DevCtx, const Tile*);                                        \
     template __global__ void k_explicit<NY>(DevCtx, const Tile*, int);                                   \
     template __global__ void k_copy_q<NY>(DevCtx, const Tile*, int, int);

@@ -9,6 +9,7 @@
 
 #include "../../include/shcfd.h"
 #include "shcfd_kernels.h"
+#include "shcfd_fast.h"
 
 using namespace shcfd;
 
@@ -161,6 +162,10 @@ int shcfd_create(int nblocks, const int* ni, const int* nj, int ny, const shcfd_
 int shcfd_destroy(shcfd_ctx* ctx) {
     if (!ctx) return SHCFD_E_ARG;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->fast) { ctx->fast->release(); delete ctx->fast; ctx->fast = nullptr; }
+    for (auto& e : ctx->prof_free) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+    for (int k = 0; k < PROF_COUNT; k++)
+        for (auto& e : ctx->prof_pending[k]) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
     for (int fld = 0; fld < SHCFD_F_COUNT; fld++) if (ctx->f[fld]) hipFree(ctx->f[fld]);
     void* ptrs[] = {ctx->dblk, ctx->dtiles, ctx->dstrips, ctx->dprog, ctx->dbnd, ctx->dpart, ctx->scal,
                     ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool};
@@ -287,6 +292,7 @@ int shcfd_set_bc_table(shcfd_ctx* ctx, const int* rows, int nrows, const double*
 }
 
 static int ntiles(shcfd_ctx* ctx) { return (int)ctx->htiles.size(); }
+static bool fast_enabled(shcfd_ctx* ctx);
 
 int shcfd_init_geometry(shcfd_ctx* ctx) {
     if (!ctx) return SHCFD_E_ARG;
@@ -514,12 +520,55 @@ int shcfd_set_abpm(shcfd_ctx* ctx) {
     return SHCFD_OK;
 }
 
+// set_ABpm without the stored Jacobians (alpha, pre-conditioner vectors and
+// scalars, local dt): what the optimised sweeps need from it
+static int abpm_for_step(shcfd_ctx* ctx) {
+    if (!fast_enabled(ctx)) return shcfd_set_abpm(ctx);
+    ProfScope ps(ctx, PROF_ABPM);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_abpm_scal<NY>));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
 int shcfd_set_dqdt(shcfd_ctx* ctx) {
     if (!ctx) return SHCFD_E_ARG;
     ProfScope ps(ctx, PROF_DQDT);
     DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_dqdt<NY>));
     LAUNCHCHK();
     return SHCFD_OK;
+}
+
+static void prof_begin(Ctx* c, int id) {
+    if (!((c->prof >> id) & 1u)) return;
+    if (c->prof_free.empty()) {
+        EventPair e{};
+        hipEventCreate(&e.a);
+        hipEventCreate(&e.b);
+        c->prof_free.push_back(e);
+    }
+    c->prof_open[id] = c->prof_free.back();
+    c->prof_free.pop_back();
+    hipEventRecord(c->prof_open[id].a, c->stream);
+}
+static void prof_end(Ctx* c, int id) {
+    if (!((c->prof >> id) & 1u)) return;
+    hipEventRecord(c->prof_open[id].b, c->stream);
+    c->prof_pending[id].push_back(c->prof_open[id]);
+}
+
+static bool fast_enabled(shcfd_ctx* ctx) {
+    if (!ctx->use_fast || ctx->ny != 1) return false;
+    if (!ctx->fast) {
+        ctx->fast = new FastPath();
+        if (ctx->fast->init(*ctx) != SHCFD_OK) {
+            ctx->fast->release();
+            delete ctx->fast;
+            ctx->fast = nullptr;
+            ctx->use_fast = 0;
+            return false;
+        }
+    }
+    return true;
 }
 
 static int implicit_async(shcfd_ctx* ctx, int step_internal) {
@@ -529,6 +578,11 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
     int* prog_b = ctx->dprog + ns;
     { ProfScope ps(ctx, PROF_RHS);
     DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_rhs<NY>, ctx->dprog, 2 * ns)); }
+    if (fast_enabled(ctx)) {
+        int rc = ctx->fast->implicit(*ctx, step_internal, prof_begin, prof_end);
+        LAUNCHCHK();
+        return rc;
+    }
     { ProfScope ps(ctx, PROF_SWEEP_FWD);
     DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep_fwd_plain<NY>, dim3(ns), dim3(64), 0, ctx->stream, d,
                                             ctx->dstrips, prog_f, ctx->dbnd, ctx->bnd_stride, ctx->spin_limit)); }
@@ -546,6 +600,21 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
     }
     LAUNCHCHK();
     return SHCFD_OK;
+}
+
+int shcfd_set_option(shcfd_ctx* ctx, int option, int value) {
+    if (!ctx) return SHCFD_E_ARG;
+    switch (option) {
+    case SHCFD_OPT_FAST_IMPLICIT:
+        ctx->use_fast = value ? 1 : 0;
+        return SHCFD_OK;
+    case SHCFD_OPT_SPIN_LIMIT:
+        if (value < 1) return SHCFD_E_ARG;
+        ctx->spin_limit = value;
+        return SHCFD_OK;
+    default:
+        return SHCFD_E_ARG;
+    }
 }
 
 int shcfd_calc_next_step_implicit(shcfd_ctx* ctx, int step_internal, double* res_out, double* omega_out) {
@@ -638,14 +707,14 @@ int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double
         TRY(part_primitive(ctx));
         TRY(shcfd_set_dt(ctx, step, nullptr));
         TRY(part_main(ctx));
-        TRY(shcfd_set_abpm(ctx));
+        TRY(abpm_for_step(ctx));
         TRY(implicit_async(ctx, 0));
         break;
     case SHCFD_SCHEME_PRECON_LUSGS:
         TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
         TRY(part_primitive(ctx));
         TRY(part_main(ctx));
-        TRY(shcfd_set_abpm(ctx));
+        TRY(abpm_for_step(ctx));
         TRY(implicit_async(ctx, 0));
         break;
     case SHCFD_SCHEME_EULER:
@@ -673,7 +742,7 @@ int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double
         for (int j = 1; j < ctx->p.num_internal_loop; j++) {
             TRY(part_primitive(ctx));
             TRY(part_main(ctx));
-            TRY(shcfd_set_abpm(ctx));
+            TRY(abpm_for_step(ctx));
             TRY(shcfd_set_dqdt(ctx));
             TRY(implicit_async(ctx, j));
         }

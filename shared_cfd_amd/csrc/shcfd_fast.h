@@ -1,0 +1,24 @@
+// shcfd_fast.h -- host side of the optimised implicit update (k_fast.hip)
+#pragma once
+#include "shcfd_internal.h"
+
+namespace shcfd {
+
+struct QStrip;
+
+struct FastPath {
+    bool ready = false;
+    int nstrips = 0, nwork = 0;
+    long long nslots = 0, hstride = 0;
+    QStrip* strips = nullptr;
+    int2* work = nullptr;
+    double *R = nullptr, *V = nullptr, *X = nullptr, *Z = nullptr;
+    double *GA = nullptr, *GB = nullptr, *HA = nullptr, *HB = nullptr;
+    double* hand = nullptr;
+    double* part = nullptr;
+    int init(Ctx& c);
+    void release();
+    int implicit(Ctx& c, int step_internal, void (*prof_begin)(Ctx*, int), void (*prof_end)(Ctx*, int));
+};
+
+}  // namespace shcfd

@@ -95,6 +95,10 @@ struct Ctx {
     std::vector<EventPair> prof_free;
     double prof_ms[PROF_COUNT] = {};
     long long prof_n[PROF_COUNT] = {};
+    EventPair prof_open[PROF_COUNT] = {};
+    // optimised implicit update (k_fast.hip); owned, created on first use
+    struct FastPath* fast = nullptr;
+    int use_fast = 1;              // 1: optimised sweeps when available
     int nb = 0, ny = 0, dimq = 0, dimw = 0;
     shcfd_params p{};
     std::vector<DevBlock> hblk;

@@ -394,6 +394,7 @@ template <int NY> __global__ void k_set_tgv(DevCtx, const Tile*);
 template <int NY> __global__ void k_set_dt(DevCtx, const Tile*, double*);
 template <int NY> __global__ void k_rms_part(DevCtx, const Tile*, double*);
 template <int NY> __global__ void k_set_abpm(DevCtx, const Tile*);
+template <int NY> __global__ void k_set_abpm_scal(DevCtx, const Tile*);
 template <int NY> __global__ void k_set_dqdt(DevCtx, const Tile*);
 template <int NY> __global__ void k_explicit(DevCtx, const Tile*, int);
 template <int NY> __global__ void k_copy_q(DevCtx, const Tile*, int, int);

@@ -27,8 +27,9 @@ EXPORTS = [
     "shcfd_set_thermo_prop", "shcfd_set_bc", "shcfd_set_hv", "shcfd_set_tg", "shcfd_set_tgv", "shcfd_set_dt",
     "shcfd_set_abpm", "shcfd_set_dqdt", "shcfd_calc_next_step_implicit", "shcfd_calc_next_step",
     "shcfd_calc_next_step_rk2", "shcfd_copy_state", "shcfd_energy_rms", "shcfd_prepare", "shcfd_step",
-    "shcfd_check_error", "shcfd_profile_enable", "shcfd_profile_read",
+    "shcfd_check_error", "shcfd_profile_enable", "shcfd_profile_read", "shcfd_set_option",
 ]
+OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT = 1, 2
 
 # kernel classes of shcfd_profile_read (include/shcfd.h)
 KCLASS = ["PRIMITIVE", "BC", "HV", "TG", "TGV", "DT", "ABPM", "DQDT", "RHS", "SWEEP_FWD", "SWEEP_BWD",
@@ -74,6 +75,7 @@ def lib():
         L.shcfd_energy_rms.argtypes = [vp, dp]
         L.shcfd_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
         L.shcfd_profile_enable.argtypes = [vp, ctypes.c_int]
+        L.shcfd_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.shcfd_profile_read.argtypes = [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
         _LIB = L
     return _LIB
@@ -96,7 +98,7 @@ class Solver:
     """A device-resident case: all blocks' state in HBM, driven through the
     C ABI.  Method names follow the reference's subroutine names."""
 
-    def __init__(self, case: C.Case, load_q: bool = True):
+    def __init__(self, case: C.Case, load_q: bool = True, fast: bool = True):
         L = lib()
         self.L = L
         self.case = case
@@ -120,6 +122,10 @@ class Solver:
         check(L.shcfd_set_bc_table(self.ctx, _ip(bc), int(bc.shape[0]), _dp(pool), int(pool.size)),
               "shcfd_set_bc_table")
         check(L.shcfd_init_geometry(self.ctx), "shcfd_init_geometry")
+        self.set_option(OPT_FAST_IMPLICIT, int(fast))
+
+    def set_option(self, option, value):
+        check(self.L.shcfd_set_option(self.ctx, option, int(value)), "shcfd_set_option")
 
     def close(self):
         if getattr(self, "ctx", None):
