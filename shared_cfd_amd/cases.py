@@ -102,6 +102,7 @@ class Case:
     bc_pool: np.ndarray
     cut_rows: np.ndarray
     description: str = ""
+    meta: dict = dataclasses.field(default_factory=dict)
 
     @property
     def nblocks(self) -> int:
@@ -228,6 +229,77 @@ def forward_step(n_per_unit=320, scheme=SCHEME_DUAL, num_internal_loop=21, cfl=0
     return Case("forward_step", [n1i, n2i], [n1j, n2j], 1, params, [xh1, xh2], [rh1, rh2], q,
                 np.array(rows, dtype=np.int32), pool, np.array(cut, dtype=np.int32),
                 description=f"Mach-3 forward step, h=1/{n_per_unit}, {n1i}x{n1j}+{n2i}x{n2j} cells")
+
+
+# ---- the same grid cut into two blocks (block-interface tests) -----------------------
+def _rotate_block(xh, rh, q):
+    """Re-index a block by a quarter turn: new (i', j') = (j, n_i+1-i).
+    Physical coordinates and velocity components are unchanged, the index
+    space stays right-handed."""
+    xr = xh[::-1, :].T.copy()
+    rr = rh[::-1, :].T.copy()
+    # interior cells: new[j'-1, i'-1] = old[j-1, i-1] with j = i', i = n+1-j'
+    qi = q[2:-2, 2:-2, :]
+    qn = np.transpose(qi, (1, 0, 2))[::-1, :, :].copy()
+    qq = ref_zeros(q.shape[-1], qn.shape[1], qn.shape[0])
+    qq[2:-2, 2:-2, :] = qn
+    return xr, rr, qq
+
+
+def to_original(case: "Case", b: int, interior_arr: np.ndarray) -> np.ndarray:
+    """Interior array of a split block re-indexed back to the unsplit grid's
+    orientation (identity unless the block was rotated)."""
+    if case.meta.get("rotated", [False] * case.nblocks)[b]:
+        return interior_arr[::-1].transpose(1, 0, 2)
+    return interior_arr
+
+
+def split_case_i(case: "Case", k: int, orient: str = "ii") -> "Case":
+    """Cut block 1 of `case` after column k into two blocks joined by one cut.
+    orient: 'ii' (geo 1), 'ij' (block 2 rotated, geo 2), 'ji' (block 1 rotated,
+    geo 3) or 'jj' (both rotated, geo 4).  All physical sides become
+    zero-gradient except the inflow (i-min), kept as in `case`."""
+    assert case.nblocks == 1
+    ni, nj = case.ni[0], case.nj[0]
+    xh, rh, q = case.xh[0], case.rh[0], case.q0[0]
+    n1, n2 = k, ni - k
+    x1, r1 = xh[: k + 1].copy(), rh[: k + 1].copy()
+    x2, r2 = xh[k:].copy(), rh[k:].copy()
+    q1 = ref_zeros(q.shape[-1], n1, nj)
+    q1[2:-2, 2:-2] = q[2:-2, 2:2 + n1]
+    q2 = ref_zeros(q.shape[-1], n2, nj)
+    q2[2:-2, 2:-2] = q[2:-2, 2 + n1:2 + ni]
+    rot1, rot2 = orient[0] == "j", orient[1] == "j"
+    if rot1:
+        x1, r1, q1 = _rotate_block(x1, r1, q1)
+    if rot2:
+        x2, r2, q2 = _rotate_block(x2, r2, q2)
+    # sides of the original orientation -> sides of the (possibly rotated) block
+    rot_side = {SIDE_IMIN: SIDE_JMAX, SIDE_IMAX: SIDE_JMIN, SIDE_JMIN: SIDE_IMIN, SIDE_JMAX: SIDE_IMAX}
+    inflow = [r for r in case.bc_rows if r[1] == SIDE_IMIN]
+    rows = []
+    for r in inflow:
+        rows.append([1, rot_side[SIDE_IMIN] if rot1 else SIDE_IMIN, 1, nj, int(r[4]), int(r[5])])
+    for side in (SIDE_JMIN, SIDE_JMAX):
+        rows.append([1, rot_side[side] if rot1 else side, 1, n1, BC_EXTRAP, 0])
+        rows.append([2, rot_side[side] if rot2 else side, 1, n2, BC_EXTRAP, 0])
+    rows.append([2, rot_side[SIDE_IMAX] if rot2 else SIDE_IMAX, 1, nj, BC_EXTRAP, 0])
+    # cut: block 1 column n1 <-> block 2 column 1
+    if not rot1 and not rot2:
+        cut = [1, nj, 1, 1, n1, 1, 1, 2, 1, 1, -1]
+    elif not rot1 and rot2:   # block 2 side is j-max (j' = n2)
+        cut = [2, nj, 1, 1, n1, 1, 1, 2, n2, 1, 1]
+    elif rot1 and not rot2:   # block 1 side is j-min (j' = 1)
+        cut = [3, nj, 1, 1, 1, 1, -1, 2, 1, 1, -1]
+    else:
+        cut = [4, nj, 1, 1, 1, 1, -1, 2, n2, 1, 1]
+    nis = [nj if rot1 else n1, nj if rot2 else n2]
+    njs = [n1 if rot1 else nj, n2 if rot2 else nj]
+    out = Case(case.name + "_split" + orient, nis, njs, case.ny, case.params, [x1, x2], [r1, r2], [q1, q2],
+               np.array(rows, dtype=np.int32), case.bc_pool.copy(), np.array([cut], dtype=np.int32),
+               description=case.description + f" split at i={k} ({orient})")
+    out.meta["rotated"] = [rot1, rot2]
+    return out
 
 
 # ---- random smooth states on skewed grids (parity tests) -----------------------------
