@@ -69,11 +69,14 @@ def build_fortran() -> str | None:
     os.makedirs(out_dir, exist_ok=True)
     mod = os.path.join(FORT, "shcfd_f.f90")
     drivers = [f for f in sorted(os.listdir(FORT)) if f.startswith("main_") and f.endswith(".f90")]
-    _run([flang, "-O2", "-c", mod, "-module-dir", out_dir, "-o", os.path.join(out_dir, "shcfd_f.o")])
+    # unformatted files big-endian, as the reference builds them (-convert big_endian,
+    # store/architecture/PC/Makefile.head:4)
+    fflags = ["-O2", "-fconvert=big-endian"]
+    _run([flang, *fflags, "-c", mod, "-module-dir", out_dir, "-o", os.path.join(out_dir, "shcfd_f.o")])
     exe = None
     for drv in drivers:
         exe = os.path.join(HERE, drv[:-4])
-        _run([flang, "-O2", "-I", out_dir, os.path.join(FORT, drv), os.path.join(out_dir, "shcfd_f.o"),
+        _run([flang, *fflags, "-I", out_dir, os.path.join(FORT, drv), os.path.join(out_dir, "shcfd_f.o"),
               "-L" + HERE, "-lshcfd", "-Wl,-rpath,$ORIGIN", "-o", exe])
     return exe
 

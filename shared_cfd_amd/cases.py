@@ -231,6 +231,63 @@ def forward_step(n_per_unit=320, scheme=SCHEME_DUAL, num_internal_loop=21, cfl=0
                 description=f"Mach-3 forward step, h=1/{n_per_unit}, {n1i}x{n1j}+{n2i}x{n2j} cells")
 
 
+# ---- files of the Fortran host driver ---------------------------------------------
+def write_case(case: "Case", path: str, max_step: int, out_period: int = 1, min_rms: float = 1e-14,
+               fast: bool = True) -> None:
+    """case.inp for shared_cfd_amd/main_shcfd (the Fortran main program)."""
+    p = case.params
+    with open(path, "w") as f:
+        f.write("SHCFD-CASE\n")
+        f.write(f"{case.nblocks} {case.ny}\n")
+        for a, b in zip(case.ni, case.nj):
+            f.write(f"{a} {b}\n")
+        f.write(f"{p.scheme} {p.time_step} {p.high_order} {p.limiter} {p.viscous} {p.thermal} "
+                f"{p.num_internal_loop} {int(fast)}\n")
+        vals = [p.phi_mus, p.k_mus, p.cfl, p.cfl_tau, p.omega_max, p.omega_min, p.dqmax,
+                min(p.res_rate_warning, 1e300), min(p.res_rate_error, 1e300), p.kappa, p.rgas, p.nu]
+        f.write(" ".join(repr(float(v)) for v in vals) + "\n")
+        f.write(f"{float(max_step)!r} {min_rms!r} {out_period}\n")
+        f.write(f"{case.bc_rows.shape[0]} {case.bc_pool.size}\n")
+        for r in case.bc_rows:
+            f.write(" ".join(str(int(x)) for x in r) + "\n")
+        if case.bc_pool.size:
+            f.write(" ".join(repr(float(x)) for x in case.bc_pool) + "\n")
+        f.write(f"{case.cut_rows.shape[0]}\n")
+        for r in case.cut_rows:
+            f.write(" ".join(str(int(x)) for x in r) + "\n")
+        for b in range(case.nblocks):
+            ni, nj = case.ni[b], case.nj[b]
+            for arr in (case.xh[b], case.rh[b]):
+                f.write(" ".join(repr(float(x)) for x in arr.T.ravel()) + "\n")  # i fastest
+            qi = case.q0[b][2:nj + 2, 2:ni + 2, :]
+            f.write(" ".join(repr(float(x)) for x in qi.ravel()) + "\n")  # component fastest
+
+
+def read_result_bin(path: str, case: "Case"):
+    """Read result.bin / restart.bin in the out_bin layout of
+    store/core/inout.f90:44-56 (big-endian Fortran sequential records):
+    step, tt, then (q(1:dimq), R, gamma, p) for i=0..ni+1, j=0..nj+1 of
+    every plane.  Returns (step, tt, [array (nj+2, ni+2, dimq+3)])."""
+    import struct
+
+    with open(path, "rb") as f:
+        data = f.read()
+    recs, pos = [], 0
+    while pos < len(data):
+        (n,) = struct.unpack(">i", data[pos:pos + 4])
+        recs.append(data[pos + 4:pos + 4 + n])
+        pos += 8 + n
+    step = struct.unpack(">i", recs[0])[0]
+    tt = struct.unpack(">d", recs[1])[0]
+    vals = np.frombuffer(recs[2], dtype=">f8").astype(np.float64)
+    out, k = [], 0
+    for ni, nj in zip(case.ni, case.nj):
+        n = (nj + 2) * (ni + 2) * (case.dimq + 3)
+        out.append(vals[k:k + n].reshape(nj + 2, ni + 2, case.dimq + 3))
+        k += n
+    return step, tt, out
+
+
 # ---- the same grid cut into two blocks (block-interface tests) -----------------------
 def _rotate_block(xh, rh, q):
     """Re-index a block by a quarter turn: new (i', j') = (j, n_i+1-i).

@@ -171,6 +171,7 @@ __device__ inline bool handoff_wait(double& x, const double* slotp, bool mine, l
 constexpr int PF = 8;   // coefficient prefetch depth (steps)
 constexpr int PH = 16;  // hand-off prefetch depth (steps)
 constexpr int UNR = 16; // unroll (multiple of PF and PH)
+constexpr int HM = 16;  // extra hand-off lead (columns) kept after a miss
 
 // ---------------------------------------------------------------------------
 // forward sweep: one wavefront per quad strip
@@ -225,6 +226,21 @@ __global__ void __launch_bounds__(64) k_fast_fwd(DevCtx d, const QStrip* strips,
                 const int i0 = dd + 1;  // column of row 0
                 if (has_prev && i0 <= ni) {
                     double x = hb[u % PH];
+                    if (__ballot(cons && is_sentinel(x)) != 0ULL) {
+                        // the producer was not far enough ahead when the ring was
+                        // filled: let it get PH+HM columns ahead, then refill the
+                        // whole ring so the following steps do not miss again
+                        const int ahead = min(i0 + PH + HM, ni);
+                        double y = cons ? load_agent(hin + 4LL * ahead) : 0.0;
+                        if (!handoff_wait(y, hin + 4LL * ahead, cons, spin_limit)) {
+                            atomicCAS(d.iflag + IFLAG_ERR, 0, SHCFD_E_SPIN);
+                            return;
+                        }
+                        x = cons ? load_agent(hin + 4LL * i0) : 0.0;
+#pragma unroll
+                        for (int m = 1; m < PH; m++)
+                            hb[(u + m) % PH] = (cons && i0 + m <= ni) ? load_agent(hin + 4LL * (i0 + m)) : 0.0;
+                    }
                     if (!handoff_wait(x, hin + 4LL * i0, cons, spin_limit)) {
                         atomicCAS(d.iflag + IFLAG_ERR, 0, SHCFD_E_SPIN);
                         return;
@@ -342,6 +358,19 @@ __global__ void __launch_bounds__(64) k_fast_bwd(DevCtx d, const QStrip* strips,
                 const int i15 = dd - (QR - 2);  // column of row 15
                 if (has_next && i15 >= 1 && i15 <= ni) {
                     double x = hb[u % PH];
+                    if (__ballot(cons && is_sentinel(x)) != 0ULL) {
+                        // resynchronise as in the forward sweep (columns descend)
+                        const int ahead = max(i15 - PH - HM, 1);
+                        double y = cons ? load_agent(hin + 4LL * ahead) : 0.0;
+                        if (!handoff_wait(y, hin + 4LL * ahead, cons, spin_limit)) {
+                            atomicCAS(d.iflag + IFLAG_ERR, 0, SHCFD_E_SPIN);
+                            return;
+                        }
+                        x = cons ? load_agent(hin + 4LL * i15) : 0.0;
+#pragma unroll
+                        for (int m = 1; m < PH; m++)
+                            hb[(u + m) % PH] = (cons && i15 - m >= 1) ? load_agent(hin + 4LL * (i15 - m)) : 0.0;
+                    }
                     if (!handoff_wait(x, hin + 4LL * i15, cons, spin_limit)) {
                         atomicCAS(d.iflag + IFLAG_ERR, 0, SHCFD_E_SPIN);
                         return;

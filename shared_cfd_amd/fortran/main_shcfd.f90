@@ -1,0 +1,180 @@
+! main_shcfd.f90 -- the solver's main program on the GPU, written like the
+! reference's generated main.f90: initialisation (store/core/main.part_init.f90),
+! IC (store/core/main.body.raw.f90), then the per-scheme loop body of
+! store/time_scheme/<scheme>/main.main.raw.f90 calling the reference-named
+! routines of module shcfd_f.  Reads case.inp (grid, BC and cut tables,
+! initial state, control keys; written by shared_cfd_amd.cases.write_case)
+! and writes RMS.dat and result.bin (the out_bin record layout of
+! store/core/inout.f90:44-56: step, tt, then (q(1:dimq), R, gamma, p) over
+! i=0..ni+1, j=0..nj+1 for every plane).
+program main
+   use iso_c_binding
+   use shcfd_f
+   implicit none
+   integer :: i, step_res, j, b, k, nb, nbc, npool, ncut, dimq, dimw, ii, jj
+   integer, allocatable :: ni(:), nj(:), bc(:), cut(:)
+   double precision, allocatable :: pool(:), a(:), nodes(:,:), qin(:), w(:), pool_out(:)
+   character(len=32) :: magic
+   integer(c_int) :: fast
+
+   open(10, file="case.inp", status="old", action="read")
+   read(10, *) magic
+   if (trim(magic) /= "SHCFD-CASE") stop "case.inp: bad header"
+   read(10, *) nb, ny_
+   nplane = nb
+   allocate(ni(nb), nj(nb))
+   do b = 1, nb
+      read(10, *) ni(b), nj(b)
+   end do
+   read(10, *) prm%scheme, prm%time_step, prm%high_order, prm%limiter, prm%viscous, prm%thermal, &
+               prm%num_internal_loop, fast
+   read(10, *) prm%phi_mus, prm%k_mus, prm%cfl, prm%cfl_tau, prm%omega_max, prm%omega_min, prm%dqmax, &
+               prm%res_rate_warning, prm%res_rate_error, prm%kappa, prm%rgas, prm%nu
+   prm%pad0 = 0
+   read(10, *) max_step, min_RMS, OutPeriod
+   read(10, *) nbc, npool
+   allocate(bc(6*max(nbc,1)), pool(max(npool,1)))
+   do k = 1, nbc
+      read(10, *) bc(6*k-5:6*k)
+   end do
+   if (npool > 0) read(10, *) pool(1:npool)
+   read(10, *) ncut
+   allocate(cut(11*max(ncut,1)))
+   do k = 1, ncut
+      read(10, *) cut(11*k-10:11*k)
+   end do
+
+   ! --- init: set_MPI / init_geometry / read_control equivalents ---
+   call shcfd_check(shcfd_create(int(nb, c_int), ni, nj, int(ny_, c_int), prm, ctx), "shcfd_create")
+   call shcfd_check(shcfd_set_option(ctx, 1_c_int, fast), "set_option")
+   dimq = ny_ + 3
+   dimw = dimq + 5
+   do b = 1, nb
+      allocate(nodes(0:ni(b), 0:nj(b)), a((ni(b)+4)*(nj(b)+4)), qin(dimq*ni(b)*nj(b)))
+      do k = 1, 2
+         read(10, *) nodes
+         a = 0d0
+         do jj = 0, nj(b)
+            do ii = 0, ni(b)
+               a((ii+1) + (ni(b)+4)*(jj+1) + 1) = nodes(ii, jj)
+            end do
+         end do
+         if (k == 1) then
+            call shcfd_check(shcfd_upload_field(ctx, SHCFD_F_XH, int(b-1, c_int), a), "upload xh")
+         else
+            call shcfd_check(shcfd_upload_field(ctx, SHCFD_F_RH, int(b-1, c_int), a), "upload rh")
+         end if
+      end do
+      read(10, *) qin
+      deallocate(a)
+      allocate(a(dimq*(ni(b)+4)*(nj(b)+4)))
+      a = 0d0
+      do jj = 1, nj(b)
+         do ii = 1, ni(b)
+            do k = 1, dimq
+               a(k + dimq*((ii+1) + (ni(b)+4)*(jj+1))) = qin(k + dimq*((ii-1) + ni(b)*(jj-1)))
+            end do
+         end do
+      end do
+      call shcfd_check(shcfd_upload_field(ctx, SHCFD_F_Q, int(b-1, c_int), a), "upload q")
+      deallocate(nodes, a, qin)
+   end do
+   close(10)
+   call shcfd_check(shcfd_set_bc_table(ctx, bc, int(nbc, c_int), pool, int(npool, c_int)), "set_bc_table")
+   call shcfd_check(shcfd_set_cut_table(ctx, cut, int(ncut, c_int)), "set_cut_table")
+   call shcfd_check(shcfd_init_geometry(ctx), "init_geometry")
+
+   ! --- set initial condition (store/core/main.body.raw.f90) ---
+   step_res = 0
+   tt = 0d0
+   i = step_res + 1
+   call set_w()
+   call set_thermo_prop(i)
+   call set_BC(i)
+   open(82, file="RMS.dat", status="replace")
+   close(82)
+
+   ! --- main loop ---
+   do while (check_convergence(i, step_res))
+      select case (prm%scheme)
+      case (SCHEME_LUSGS)                  ! time_scheme/LU-SGS/2d/main.main.raw.f90
+         call copy_qp()
+         call set_w(); call set_thermo_prop(i)
+         call set_dt(i)
+         call set_BC(i); call set_HV(); call set_TG(); call set_TGv()
+         call set_ABpm()
+         call calc_next_step_implicit(0)
+      case (SCHEME_PRECON_LUSGS)           ! time_scheme/preconLU-SGS/2d/main.main.raw.f90
+         call copy_qp()
+         call set_w(); call set_thermo_prop(i)
+         call set_BC(i); call set_HV(); call set_TG(); call set_TGv()
+         call set_ABpm()
+         call calc_next_step_implicit(0)
+      case (SCHEME_EULER)                  ! time_scheme/euler/main.main.raw.f90
+         call copy_qp()
+         call set_w(); call set_thermo_prop(i)
+         call set_dt(i)
+         call set_BC(i); call set_HV(); call set_TG(); call set_TGv()
+         call calc_next_step()
+      case (SCHEME_RK2)                    ! time_scheme/RK2/main.main.raw.f90
+         call copy_qp()
+         call set_w(); call set_thermo_prop(i)
+         call set_dt(i)
+         call set_BC(i); call set_HV(); call set_TG(); call set_TGv()
+         call calc_next_step_RK2_first()
+         call set_w(); call set_thermo_prop(i)
+         call set_BC(i); call set_HV(); call set_TG(); call set_TGv()
+         call calc_next_step_RK2_second()
+      case (SCHEME_DUAL)                   ! time_scheme/dual/2d/main.main.raw.f90
+         call set_w(); call set_thermo_prop(i)
+         call set_dt(i)
+         call copy_qpp()
+         call copy_qp()
+         j = 1
+         do while (j < prm%num_internal_loop)
+            call set_w(); call set_thermo_prop(i)
+            call set_BC(i); call set_HV(); call set_TG(); call set_TGv()
+            call set_ABpm()
+            call set_dqdt()
+            call calc_next_step_implicit(j)
+            j = j + 1
+         end do
+         call check_internal_loop_convergence(i, step_res)
+      case default
+         stop "unsupported time scheme"
+      end select
+      i = i + 1
+   end do
+
+   ! --- out_bin (store/core/inout.f90:44-56) ---
+   call shcfd_check(shcfd_get_time(ctx, dt_grbl, tt), "get_time")
+   open(45, file="result.bin", form="unformatted", status="replace")
+   write(45) i - 1
+   write(45) tt
+   ! one record holding every plane, as out_bin writes it
+   k = 0
+   do b = 1, nb
+      k = k + (dimq + 3) * (ni(b) + 2) * (nj(b) + 2)
+   end do
+   allocate(pool_out(k))
+   k = 0
+   do b = 1, nb
+      allocate(a(dimq*(ni(b)+4)*(nj(b)+4)), w(dimw*(ni(b)+4)*(nj(b)+4)))
+      call shcfd_check(shcfd_download_field(ctx, SHCFD_F_Q, int(b-1, c_int), a), "download q")
+      call shcfd_check(shcfd_download_field(ctx, SHCFD_F_W, int(b-1, c_int), w), "download w")
+      do jj = 0, nj(b) + 1
+         do ii = 0, ni(b) + 1
+            pool_out(k+1:k+dimq) = a(1 + dimq*((ii+1) + (ni(b)+4)*(jj+1)):dimq*((ii+1) + (ni(b)+4)*(jj+1) + 1))
+            pool_out(k+dimq+1) = w(dimq+4 + dimw*((ii+1) + (ni(b)+4)*(jj+1)))
+            pool_out(k+dimq+2) = w(dimq+2 + dimw*((ii+1) + (ni(b)+4)*(jj+1)))
+            pool_out(k+dimq+3) = w(4 + dimw*((ii+1) + (ni(b)+4)*(jj+1)))
+            k = k + dimq + 3
+         end do
+      end do
+      deallocate(a, w)
+   end do
+   write(45) pool_out
+   close(45)
+   call shcfd_check(shcfd_destroy(ctx), "destroy")
+   print *, "normal end"
+end program main
