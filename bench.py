@@ -30,7 +30,9 @@ METRIC = "FP64 Mcell-updates/s vs plain GPU & CPU; adjoint/primal cost at 1e-10 
 HBM_PEAK_GBS = 8000.0
 
 # algorithmic HBM bytes per cell for one launch of each kernel class
-# (minimum-pass schedule, dimq = 4, 13 inviscid geometry doubles)
+# (minimum-pass schedule: residual reads q, qp, qpp and 13 inviscid geometry
+# doubles and writes R; each sweep/update pass reads two dimq vectors and
+# writes one; neighbour reads are counted once)
 def alg_bytes_per_cell(kclass: str, dimq: int = 4, geo: int = 13) -> float:
     d = 8.0
     return {
@@ -54,7 +56,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
-              "blocks would shard across GPUs as described in DESIGN.md, but no multi-GPU path is built.",
+              "whole blocks would shard across GPUs, but no multi-GPU path is built.",
               file=sys.stderr)
         return 2
 

@@ -4,8 +4,8 @@ oracle (a C restatement of the reference Fortran).
 Every device routine reads only device state produced by the previous device
 routine, so agreement here is end-to-end.  Kernels and oracle evaluate the
 same expressions in the same order with FMA contraction off, so most fields
-agree to the last bit; the tolerances are the north-star bounds (1e-12
-relative per iteration)."""
+agree to the last bit; the tolerances are the parity bounds of the path
+(1e-12 relative per iteration, 1e-10 on converged fields)."""
 import numpy as np
 import pytest
 
