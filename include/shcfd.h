@@ -114,13 +114,48 @@ int shcfd_step(shcfd_ctx *ctx, int step, int step_res, double *res_first, double
 /* pending asynchronous error (omega, NaN, sweep flag timeout); clears it */
 int shcfd_check_error(shcfd_ctx *ctx);
 
+/* ---- Chemkin chemistry ---------------------------------------------------- */
+/* read_cheminp (store/therm_lib/chemkin/core/sub_chem.f90:191-598) on
+ * chem_inp with the element weights of mw_inp, and set_trans_data
+ * (:788-889) on trans_inp (NULL: no transport data, mu = 0).  Host only.
+ * out->rtol/atol (control_chem.inp, read_fo_composition :934-935) are set
+ * to 0 and left to the caller.  On error a message goes to errbuf. */
+int shcfd_mech_parse(const char *chem_inp, const char *mw_inp, const char *trans_inp, shcfd_mech *out, char *errbuf,
+                     int errlen);
+/* make mech the context's mechanism (the reference's module chem state, set
+ * up by init_therm); mech->ns must equal nY */
+int shcfd_set_mechanism(shcfd_ctx *ctx, const shcfd_mech *mech);
+/* proceed_reaction: store/therm_lib/chemkin/flow/reactive/thermal_model.f90:45-73
+ * (reaction_chemeq2 per cell over dt_grbl; or the point-implicit update,
+ * params.reaction) */
+int shcfd_proceed_reaction(shcfd_ctx *ctx);
+/* calc_boundary: thermal_model.f90:326-356 -- rho_rel2abs (sub_chem.f90:1036-1085)
+ * and calc_T (:1507-1645) for a fixed state of pressure p [Pa], temperature
+ * T [K] and relative composition vrho_rel(ns).  Evaluated on the host.
+ * w(dimw) = rho, 0, 0, p, Y(ns), gamma, H, R, mu; vhi(ns); dhi(ns) (may be NULL). */
+int shcfd_chem_boundary_state(const shcfd_mech *mech, double p, double T, const double *vrho_rel, double *w,
+                              double *vhi, double *dhi);
+/* Batched 0-D evaluation on the GPU of independent states (the per-cell
+ * routines the reference's DVODE callbacks and chem_drive use).  Row
+ * layouts per state (N = ns+1):
+ *   SHCFD_CHEM_FEX      in n(ns),T          out dndt(N)          Fex,  FJ.f90:152-300
+ *   SHCFD_CHEM_FEXVAR   in n(ns),T          out p(N), m(N)       Fex_var, FJ.f90:301-453
+ *   SHCFD_CHEM_JEX      in n(ns),T          out dndn(N,N) col-major  Jex, FJ.f90:454-708
+ *   SHCFD_CHEM_CALCT    in vrho(ns),rho,E,T0  out T,kappa,MWave,mu,DHi(ns),vhi(ns)  calc_T
+ *   SHCFD_CHEM_CHEMEQ2  in vrho(ns),T       out vrho(ns)   reaction_chemeq2 over dt
+ *   SHCFD_CHEM_POINT_IMPLICIT  in vrho(ns),T  out vrho(ns),T1  point-implicit (backward-Euler) update over dt
+ * status[i] (may be NULL): Newton iterations / CHEMEQ2 attempts, or -code.
+ * Synchronous; allocates and frees its own device buffers. */
+int shcfd_chem_eval(const shcfd_mech *mech, int op, int nstates, const double *in, double *out, int *status,
+                    double dt);
+
 /* ---- per-kernel timing (no counterpart in the reference, whose timers
  * are commented-out stubs: store/debug/time_barrier.f90:1-37) ------------- */
 /* kernel classes */
 enum {
     SHCFD_K_PRIMITIVE = 0, SHCFD_K_BC, SHCFD_K_HV, SHCFD_K_TG, SHCFD_K_TGV, SHCFD_K_DT, SHCFD_K_ABPM,
     SHCFD_K_DQDT, SHCFD_K_RHS, SHCFD_K_SWEEP_FWD, SHCFD_K_SWEEP_BWD, SHCFD_K_REDUCE, SHCFD_K_UPDATE,
-    SHCFD_K_EXPLICIT, SHCFD_K_RESIDUAL, SHCFD_K_PREP, SHCFD_K_COUNT
+    SHCFD_K_EXPLICIT, SHCFD_K_RESIDUAL, SHCFD_K_PREP, SHCFD_K_THERMO_CHEM, SHCFD_K_REACTION, SHCFD_K_COUNT
 };
 /* mask: bit k set = record HIP events (on the context stream) around every
  * launch of kernel class k; -1 = all classes, 0 = off */

@@ -123,6 +123,71 @@ enum {
     SHCFD_F_COUNT
 };
 
+/* ---- thermal models ("thermal model" of checkout.inp) -------------------- */
+enum {
+    SHCFD_THERMAL_IDEAL = 0,    /* store/therm_lib/ideal                        */
+    SHCFD_THERMAL_CHEMKIN = 2   /* store/therm_lib/chemkin (NASA-7, reactive)   */
+};
+/* ---- chemistry source per physical step (the point-implicit slot of
+ * store/time_scheme/{LU-SGS/2d,RK2}/main.main.point_implicit.f90) --------- */
+enum {
+    SHCFD_REACTION_NONE = 0,
+    SHCFD_REACTION_CHEMEQ2 = 1, /* reaction_chemeq2 per cell (sub_chem.f90:1684-1819) */
+    SHCFD_REACTION_POINT_IMPLICIT = 2 /* one linearised backward-Euler step per
+                                  cell: (I - dt J) dn = dt F with the analytic
+                                  Jacobian Jex and LINPACK-style LU (dvode.f DGEFA/DGESL) */
+};
+
+/* ---- Chemkin mechanism tables --------------------------------------------
+ * What read_cheminp / set_trans_data produce
+ * (store/therm_lib/chemkin/core/sub_chem.f90:191-598, :788-889; module chem,
+ * mod_chem.raw.f90:30-70), with 0-based indices.  Units are the reference's:
+ * cgs (mol/cm^3, erg), abe = (ln A, beta, Ea/Rc [K]) with Rc = 1.987 cal/mol/K
+ * as a single-precision literal (mod_chem.raw.f90:9), troe = (a, 1/T***,
+ * 1/T*, T**), men = efficiency - 1. */
+#define SHCFD_CHEM_MAX_NE 8
+#define SHCFD_CHEM_MAX_NS 16
+#define SHCFD_CHEM_MAX_NR 48
+#define SHCFD_CHEM_MAX_TSEC 5
+enum {
+    SHCFD_RXN_NONE = 0,  /* reverse rate from K_c                          */
+    SHCFD_RXN_REV = 1,   /* explicit REV /A b E/                           */
+    SHCFD_RXN_LOW = 2,   /* Lindemann fall-off (+M) with LOW /A b E/       */
+    SHCFD_RXN_TROE = 3,  /* Troe fall-off                                   */
+    SHCFD_RXN_M = 4,     /* +M third body                                   */
+    SHCFD_RXN_M_REV = 5  /* +M with REV                                     */
+};
+/* 0-D chemistry operations of shcfd_chem_eval */
+enum {
+    SHCFD_CHEM_FEX = 0, SHCFD_CHEM_FEXVAR = 1, SHCFD_CHEM_JEX = 2, SHCFD_CHEM_CALCT = 3, SHCFD_CHEM_CHEMEQ2 = 4,
+    SHCFD_CHEM_POINT_IMPLICIT = 5
+};
+typedef struct shcfd_mech {
+    int ne, ns, nr, nt;                                  /* elements, species, reactions, transport species */
+    double mw[SHCFD_CHEM_MAX_NS];                        /* g/mol                                     */
+    double inv_mw[SHCFD_CHEM_MAX_NS];
+    double tthre[SHCFD_CHEM_MAX_NS][3];                  /* Tlow, Thigh, Tcommon                      */
+    double coeff[SHCFD_CHEM_MAX_NS][2][7];               /* NASA-7: [0] T <= Tcommon, [1] above       */
+    double es[SHCFD_CHEM_MAX_NS][SHCFD_CHEM_MAX_NE];     /* element counts                            */
+    int reverse_off[SHCFD_CHEM_MAX_NR];                  /* Rstate(:,1): 1 for "=>"                   */
+    int kind[SHCFD_CHEM_MAX_NR];                         /* Rstate(:,2): SHCFD_RXN_*                  */
+    int num_nu[SHCFD_CHEM_MAX_NR][2];                    /* reactant / product count (<= 3 each)      */
+    int ind_nu[SHCFD_CHEM_MAX_NR][2][3];                 /* species index per occurrence              */
+    int snu[SHCFD_CHEM_MAX_NR];                          /* num_nu[1] - num_nu[0]                     */
+    int exist_m[SHCFD_CHEM_MAX_NR];
+    int num_m[SHCFD_CHEM_MAX_NR];
+    int ind_m[SHCFD_CHEM_MAX_NR][SHCFD_CHEM_MAX_NS];
+    double men[SHCFD_CHEM_MAX_NR][SHCFD_CHEM_MAX_NS];
+    double abe[SHCFD_CHEM_MAX_NR][3];
+    double cabe[SHCFD_CHEM_MAX_NR][3];                   /* REV or LOW parameters                     */
+    double troe[SHCFD_CHEM_MAX_NR][4];
+    int tr2th[SHCFD_CHEM_MAX_NS];                        /* transport entry -> species index          */
+    int nsec_trans[SHCFD_CHEM_MAX_NS];
+    double trange[SHCFD_CHEM_MAX_NS][SHCFD_CHEM_MAX_TSEC][2];
+    double trans[SHCFD_CHEM_MAX_NS][SHCFD_CHEM_MAX_TSEC][4]; /* ln mu = A lnT + B/T + C/T^2 + D (micropoise) */
+    double rtol, atol;                                   /* control_chem.inp tolerances (CHEMEQ2)     */
+} shcfd_mech;
+
 /* ---- per-case parameters --------------------------------------------------
  * Build-time choices of the reference's checkout.inp (checkout_flow.py:66-318)
  * and the run-time keys of control.inp (read_control.f90:61-92,
@@ -135,9 +200,9 @@ typedef struct shcfd_params {
     int high_order;        /* 0 upwind, 1 MUSCL                               */
     int limiter;           /* SHCFD_LIMITER_*                                 */
     int viscous;           /* 0 inviscid, 1 viscous no-nV/2d                  */
-    int thermal;           /* 0 ideal gas                                     */
+    int thermal;           /* SHCFD_THERMAL_*                                 */
     int num_internal_loop; /* "InternalLoop Max Number" (N-1 inner iterations)*/
-    int pad0;
+    int reaction;          /* SHCFD_REACTION_*: chemistry source per step     */
     double phi_mus;        /* "MUSCL ON(1)/OFF(0)"                            */
     double k_mus;          /* -1 (order 2) or 1/3 (order 3)                   */
     double cfl;            /* "CFL Number"                                    */

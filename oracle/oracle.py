@@ -60,6 +60,29 @@ def lib():
         L.or_prepare.argtypes = [vp]
         L.or_step.argtypes = [vp, dp, dp]
         L.or_omp_threads.restype = ctypes.c_int
+        mp = ctypes.c_void_p
+        L.or_set_mechanism.argtypes = [vp, mp]
+        L.or_chem_error.argtypes = [vp]
+        L.or_proceed_reaction.argtypes = [vp]
+        L.or_chem_fex.argtypes = [mp, dp, dp]
+        L.or_chem_fex.restype = None
+        L.or_chem_fex_var.argtypes = [mp, dp, dp, dp]
+        L.or_chem_fex_var.restype = None
+        L.or_chem_jex.argtypes = [mp, dp, dp]
+        L.or_chem_jex.restype = None
+        L.or_chem_rho_rel2abs.argtypes = [mp, ctypes.c_double, ctypes.c_double, dp, dp, dp]
+        L.or_chem_rho_rel2abs.restype = None
+        L.or_chem_calc_t.argtypes = [mp, dp, ctypes.c_double, ctypes.c_double, dp, dp]
+        L.or_chem_chemeq2.argtypes = [mp, dp, ctypes.c_double, ctypes.c_double, ip]
+        L.or_chem_reaction_chemeq2.argtypes = [mp, ctypes.c_double, dp, ctypes.c_double, ip]
+        L.or_chem_point_implicit.argtypes = [mp, ctypes.c_double, dp, ctypes.c_double, ip, dp]
+        L.or_dgefa.argtypes = [dp, ctypes.c_int, ip]
+        L.or_dgesl.argtypes = [dp, ctypes.c_int, ip, dp]
+        L.or_dgesl.restype = None
+        L.or_exp.argtypes = [ctypes.c_double]
+        L.or_exp.restype = ctypes.c_double
+        L.or_log.argtypes = [ctypes.c_double]
+        L.or_log.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -101,6 +124,17 @@ class Oracle:
         L.or_set_bc_table(self.ctx, _ip(bc), int(bc.shape[0]), _dp(pool), int(pool.size))
         if init_geometry:
             L.or_init_geometry(self.ctx)
+        self.mech = None
+        if getattr(case, "mech", None) is not None:
+            from oracle.chem_inp import OrMech
+
+            # the oracle keeps its own copy of the tables (same C layout)
+            self.mech = OrMech.from_buffer_copy(bytes(case.mech))
+            if L.or_set_mechanism(self.ctx, ctypes.byref(self.mech)) != 0:
+                raise RuntimeError("or_set_mechanism failed")
+        if getattr(case, "w0", None) is not None:
+            for b in range(case.nblocks):
+                self.field("W", b)[...] = case.w0[b]
 
     def __del__(self):
         try:
@@ -139,3 +173,68 @@ class Oracle:
         res, om = ctypes.c_double(0.0), ctypes.c_double(0.0)
         err = self.L.or_calc_next_step_implicit(self.ctx, step_internal, ctypes.byref(res), ctypes.byref(om))
         return err, res.value, om.value
+
+
+# ---- 0-D chemistry (per state; n = ns concentrations then T) --------------
+def chem_fex(m, n):
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    out = np.zeros(m.ns + 1)
+    lib().or_chem_fex(ctypes.byref(m), _dp(n), _dp(out))
+    return out
+
+
+def chem_fex_var(m, n):
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    p, q = np.zeros(m.ns + 1), np.zeros(m.ns + 1)
+    lib().or_chem_fex_var(ctypes.byref(m), _dp(n), _dp(p), _dp(q))
+    return p, q
+
+
+def chem_jex(m, n):
+    """Jacobian d(dndt_a)/d(n_b) as a (ns+1, ns+1) array indexed [a, b]."""
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    N = m.ns + 1
+    J = np.zeros(N * N)
+    lib().or_chem_jex(ctypes.byref(m), _dp(n), _dp(J))
+    return J.reshape(N, N, order="F")
+
+
+def chem_rho_rel2abs(m, p, T, vrho_rel):
+    v = np.array(vrho_rel, dtype=np.float64)
+    rho, E = ctypes.c_double(), ctypes.c_double()
+    lib().or_chem_rho_rel2abs(ctypes.byref(m), p, T, _dp(v), ctypes.byref(rho), ctypes.byref(E))
+    return v, rho.value, E.value
+
+
+def chem_calc_t(m, vrho, rho, E, T0):
+    v = np.ascontiguousarray(vrho, dtype=np.float64)
+    T = ctypes.c_double(T0)
+    out = np.zeros(3 + 2 * m.ns)
+    it = lib().or_chem_calc_t(ctypes.byref(m), _dp(v), rho, E, ctypes.byref(T), _dp(out))
+    return T.value, out, it
+
+
+def chem_reaction_chemeq2(m, T, vrho, dt):
+    v = np.array(vrho, dtype=np.float64)
+    att = ctypes.c_int(0)
+    e = lib().or_chem_reaction_chemeq2(ctypes.byref(m), T, _dp(v), dt, ctypes.byref(att))
+    return e, v, att.value
+
+
+def chem_point_implicit(m, T, vrho, dt):
+    v = np.array(vrho, dtype=np.float64)
+    it = ctypes.c_int(0)
+    T1 = ctypes.c_double(0.0)
+    e = lib().or_chem_point_implicit(ctypes.byref(m), T, _dp(v), dt, ctypes.byref(it), ctypes.byref(T1))
+    return e, v, it.value, T1.value
+
+
+def dgefa_dgesl(a, b):
+    a = np.array(a, dtype=np.float64, order="F")
+    n = a.shape[0]
+    ipvt = np.zeros(n, dtype=np.int32)
+    x = np.array(b, dtype=np.float64)
+    info = lib().or_dgefa(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, _ip(ipvt))
+    if info == 0:
+        lib().or_dgesl(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, _ip(ipvt), _dp(x))
+    return info, x

@@ -37,6 +37,9 @@ struct or_ctx {
     int first_dual;
     double res1;
     double *scratch; /* per-cell res contributions, max block size */
+    shcfd_mech mech; /* Chemkin tables (thermal = SHCFD_THERMAL_CHEMKIN) */
+    int has_mech;
+    int chem_err;    /* first chemistry failure since the step began (or_step clears it) */
 };
 
 /* element (k,i,j) of field fld, block b; i,j are Fortran indices in
@@ -331,7 +334,78 @@ void or_set_w(or_ctx *c) {
     }
 }
 
+int or_set_mechanism(or_ctx *c, const shcfd_mech *m) {
+    if (!m || m->ns != c->ny) return SHCFD_E_ARG;
+    c->mech = *m;
+    c->has_mech = 1;
+    return 0;
+}
+int or_chem_error(or_ctx *c) { return c->chem_err; }
+
+/* therm_lib/chemkin/flow/reactive/thermal_model.f90:13-42: per cell, T from
+ * e by calc_T starting at the previous p/(rho R), then R, p, gamma, H, mu,
+ * DHi, vhi.  R_uni = 8.3144621e3 J/(kmol K) (chem_var). */
+static void set_thermo_prop_chemkin(or_ctx *c) {
+    int ny = c->ny, err = 0;
+    const shcfd_mech *m = &c->mech;
+    for (int b = 0; b < c->nb; b++) {
+        int ni = c->ni[b], nj = c->nj[b];
+#pragma omp parallel for reduction(max : err)
+        for (int j = 1; j <= nj; j++)
+            for (int i = 1; i <= ni; i++) {
+                double rho = W(IW_RHO, i, j), u = W(IW_U, i, j), v = W(IW_V, i, j);
+                double ei = Q(ny + 2, i, j) / rho - 0.5 * (u * u + v * v);
+                double T = W(IW_P, i, j) / (rho * W(IW_R(c), i, j));
+                double vrho[SHCFD_CHEM_MAX_NS], out[3 + 2 * SHCFD_CHEM_MAX_NS];
+                for (int k = 0; k < ny; k++) vrho[k] = Q(k, i, j);
+                if (or_chem_calc_t(m, vrho, rho, ei, &T, out) < 0) err = SHCFD_E_TNEWTON;
+                double rg = 8.3144621e3 / out[1];
+                W(IW_R(c), i, j) = rg;
+                W(IW_P, i, j) = rho * rg * T;
+                W(IW_G(c), i, j) = out[0];
+                W(IW_H(c), i, j) = (Q(ny + 2, i, j) + W(IW_P, i, j)) / rho;
+                W(IW_MU(c), i, j) = out[2];
+                for (int k = 0; k < ny; k++) {
+                    DHI(k, i, j) = out[3 + k];
+                    VHI(k, i, j) = out[3 + ny + k];
+                }
+            }
+    }
+    if (err && !c->chem_err) c->chem_err = err;
+}
+
+/* proceed_reaction: thermal_model.f90:45-73 -- per interior cell,
+ * T = p/(rho R) and reaction_chemeq2(T, q(1:nY), dt_grbl); or, for
+ * SHCFD_REACTION_POINT_IMPLICIT, the linearised backward-Euler update. */
+int or_proceed_reaction(or_ctx *c) {
+    int ny = c->ny, err = 0;
+    const shcfd_mech *m = &c->mech;
+    double dt = c->dt_grbl;
+    if (!c->has_mech) return SHCFD_E_STATE;
+    for (int b = 0; b < c->nb; b++) {
+        int ni = c->ni[b], nj = c->nj[b];
+#pragma omp parallel for reduction(max : err)
+        for (int j = 1; j <= nj; j++)
+            for (int i = 1; i <= ni; i++) {
+                double T = W(IW_P, i, j) / W(IW_RHO, i, j) / W(IW_R(c), i, j);
+                double vrho[SHCFD_CHEM_MAX_NS];
+                for (int k = 0; k < ny; k++) vrho[k] = Q(k, i, j);
+                int e = (c->p.reaction == SHCFD_REACTION_POINT_IMPLICIT)
+                            ? or_chem_point_implicit(m, T, vrho, dt, NULL, NULL)
+                            : or_chem_reaction_chemeq2(m, T, vrho, dt, NULL);
+                if (e) err = e > err ? e : err;
+                for (int k = 0; k < ny; k++) Q(k, i, j) = vrho[k];
+            }
+    }
+    if (err && !c->chem_err) c->chem_err = err;
+    return err;
+}
+
 void or_set_thermo_prop(or_ctx *c) {
+    if (c->p.thermal == SHCFD_THERMAL_CHEMKIN) {
+        set_thermo_prop_chemkin(c);
+        return;
+    }
     /* therm_lib/ideal/thermal_model.raw.f90:7-33 */
     int ny = c->ny;
     double kap = c->p.kappa, rg = c->p.rgas, nu = c->p.nu;
@@ -1225,8 +1299,21 @@ int or_prepare(or_ctx *c) {
 
 int or_step(or_ctx *c, double *res_first, double *res_last) {
     int err = 0;
+    c->chem_err = 0;
     switch (c->p.scheme) {
     case SHCFD_SCHEME_LUSGS: /* LU-SGS/2d/main.main.raw.f90:1-6 */
+        if (c->p.reaction) { /* LU-SGS/2d/main.main.point_implicit.f90:1-9 */
+            part_primitive(c);
+            or_set_dt(c);
+            err = or_proceed_reaction(c);
+            or_copy_field(c, SHCFD_F_QP, SHCFD_F_Q);
+            part_primitive(c);
+            part_main(c);
+            or_set_abpm(c);
+            int e = or_calc_next_step_implicit(c, 0, NULL, NULL);
+            if (!err) err = e;
+            break;
+        }
         or_copy_field(c, SHCFD_F_QP, SHCFD_F_Q);
         part_primitive(c);
         or_set_dt(c);
@@ -1249,6 +1336,19 @@ int or_step(or_ctx *c, double *res_first, double *res_last) {
         or_calc_next_step(c);
         break;
     case SHCFD_SCHEME_RK2: /* RK2/main.main.raw.f90 */
+        if (c->p.reaction) { /* RK2/main.main.point_implicit.f90:1-11 */
+            part_primitive(c);
+            or_set_dt(c);
+            err = or_proceed_reaction(c);
+            or_copy_field(c, SHCFD_F_QP, SHCFD_F_Q);
+            part_primitive(c);
+            part_main(c);
+            or_calc_next_step_rk2(c, 1);
+            part_primitive(c);
+            part_main(c);
+            or_calc_next_step_rk2(c, 2);
+            break;
+        }
         or_copy_field(c, SHCFD_F_QP, SHCFD_F_Q);
         part_primitive(c);
         or_set_dt(c);
@@ -1283,5 +1383,6 @@ int or_step(or_ctx *c, double *res_first, double *res_last) {
     default:
         return SHCFD_E_STATE;
     }
+    if (!err && c->chem_err) err = c->chem_err;
     return err;
 }

@@ -64,6 +64,24 @@ int or_prepare(or_ctx *c);            /* main.body.raw.f90: part_primitive + set
 int or_step(or_ctx *c, double *res_first, double *res_last);
 int or_omp_threads(void);
 
+/* ---- Chemkin chemistry (shcfd_chem_oracle.c) ----------------------------
+ * n/y vectors: ns concentrations (mol/cm^3) then T (K). */
+int or_set_mechanism(or_ctx *c, const shcfd_mech *m);
+int or_chem_error(or_ctx *c);
+int or_proceed_reaction(or_ctx *c);   /* chemkin/flow/reactive/thermal_model.f90:45-73 */
+void or_chem_fex(const shcfd_mech *m, const double *n, double *dndt);                    /* FJ.f90:152-300 */
+void or_chem_fex_var(const shcfd_mech *m, const double *n, double *dndtp, double *dndtm); /* FJ.f90:301-453 */
+void or_chem_jex(const shcfd_mech *m, const double *n, double *J);                       /* FJ.f90:454-708 */
+void or_chem_rho_rel2abs(const shcfd_mech *m, double p, double T, double *vrho, double *rho, double *E);
+int or_chem_calc_t(const shcfd_mech *m, const double *vrho, double rho, double E, double *T, double *out);
+int or_chem_chemeq2(const shcfd_mech *m, double *y, double tout, double rtol, int *attempts);
+int or_chem_reaction_chemeq2(const shcfd_mech *m, double T, double *vrho, double tout, int *attempts);
+int or_chem_point_implicit(const shcfd_mech *m, double T, double *vrho, double dt, int *iters, double *T_out);
+int or_dgefa(double *a, int n, int *ipvt);
+double or_exp(double x);
+double or_log(double x);
+void or_dgesl(const double *a, int n, const int *ipvt, double *b);
+
 #ifdef __cplusplus
 }
 #endif

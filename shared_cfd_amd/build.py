@@ -41,11 +41,11 @@ def _run(cmd, cwd=None):
 
 def build_lib(jobs: int = 8, verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip") or f.endswith(".cpp"))
     objs = []
 
     def comp(src):
-        obj = os.path.join(OBJ, src[:-4] + ".o")
+        obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         out = _run(cmd)
         if verbose and out.strip():

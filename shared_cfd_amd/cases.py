@@ -63,7 +63,7 @@ class Params(ctypes.Structure):
     _fields_ = [
         ("scheme", ctypes.c_int), ("time_step", ctypes.c_int), ("high_order", ctypes.c_int),
         ("limiter", ctypes.c_int), ("viscous", ctypes.c_int), ("thermal", ctypes.c_int),
-        ("num_internal_loop", ctypes.c_int), ("pad0", ctypes.c_int),
+        ("num_internal_loop", ctypes.c_int), ("reaction", ctypes.c_int),
         ("phi_mus", ctypes.c_double), ("k_mus", ctypes.c_double), ("cfl", ctypes.c_double),
         ("cfl_tau", ctypes.c_double), ("omega_max", ctypes.c_double), ("omega_min", ctypes.c_double),
         ("dqmax", ctypes.c_double), ("res_rate_warning", ctypes.c_double),
@@ -74,7 +74,7 @@ class Params(ctypes.Structure):
 
 def make_params(**kw) -> Params:
     d = dict(scheme=SCHEME_LUSGS, time_step=0, high_order=1, limiter=LIMITER_REFERENCE, viscous=0,
-             thermal=0, num_internal_loop=21, pad0=0, phi_mus=1.0, k_mus=-1.0, cfl=0.5, cfl_tau=1e-2,
+             thermal=0, num_internal_loop=21, reaction=0, phi_mus=1.0, k_mus=-1.0, cfl=0.5, cfl_tau=1e-2,
              omega_max=0.1, omega_min=1e-3, dqmax=1e-2, res_rate_warning=1e-5, res_rate_error=1e-3,
              kappa=1.4, rgas=287.0, nu=1.6e-5)
     d.update(kw)
@@ -103,6 +103,8 @@ class Case:
     cut_rows: np.ndarray
     description: str = ""
     meta: dict = dataclasses.field(default_factory=dict)
+    mech: Optional[object] = None           # chem.Mech for the Chemkin thermal model
+    w0: Optional[List[np.ndarray]] = None   # initial w per block (T start of calc_T)
 
     @property
     def nblocks(self) -> int:
@@ -420,3 +422,75 @@ def random_case(seed=0, ni=(13,), nj=(9,), ny=1, scheme=SCHEME_LUSGS, viscous=0,
     return Case(f"random{seed}", list(ni), list(nj), ny, params, xhs, rhs, qs,
                 np.array(rows, dtype=np.int32), pool,
                 np.array(cut, dtype=np.int32).reshape(-1, 11), description="random smooth test state")
+
+
+# ---- reacting flame-sheet mixing layer (Chemkin thermo, nY = ns) ---------------
+def mixing_layer(mech, ni=1024, nj=512, seed=0, scheme=SCHEME_LUSGS, reaction=2, viscous=1, cfl=0.5,
+                 length=0.05, height=0.025, T_fuel=300.0, T_ox=1500.0, u_fuel=60.0, u_ox=120.0,
+                 p0=1.01325e5, order=3, fuel=None, ox=None, species=None) -> Case:
+    """Two parallel streams meeting at a flame sheet: fuel (y < height/2,
+    T_fuel) and oxidiser (y > height/2, T_ox) at pressure p0, uniform
+    Cartesian grid.  Mass fractions are drawn from ``seed`` and normalised
+    unless given: fuel H2 in [0.05, 0.3] with N2, oxidiser O2 in [0.2, 0.3],
+    H2O in [0, 0.1] with N2.  Inflow (i-min) holds each stream fixed, as
+    calc_boundary does in sample/no27.chemkin_reaction/condition.f90
+    (static enthalpy plus the kinetic term); the outflow is zero-gradient
+    and the j sides are slip walls.  The initial state is the two streams
+    with the sheet at mid-height.  ``mech`` is a chem.Mech with ns = nY;
+    ``species`` names its species in order (needed for drawn compositions)."""
+    from . import chem
+
+    ns = mech.ns
+    rng = np.random.default_rng(seed)
+    species = list(species) if species is not None else None
+
+    def rel(ys):
+        r = np.zeros(ns)
+        for name, y in ys.items():
+            r[species.index(name)] = y
+        return r / r.sum()
+
+    if fuel is None or ox is None:
+        if species is None:
+            raise ValueError("mixing_layer: give the species names or explicit stream compositions")
+        yH2 = rng.uniform(0.05, 0.3)
+        fuel = rel({"H2": yH2, "N2": 1.0 - yH2})
+        yO2, yH2O = rng.uniform(0.2, 0.3), rng.uniform(0.0, 0.1)
+        ox = rel({"O2": yO2, "H2O": yH2O, "N2": 1.0 - yO2 - yH2O})
+    states = []
+    for T, y, u in ((T_fuel, fuel, u_fuel), (T_ox, ox, u_ox)):
+        w, vhi, dhi = chem.boundary_state(mech, p0, T, y)
+        w = w.copy()
+        w[1] = u
+        w[ns + 3 + 2] = w[ns + 3 + 2] + 0.5 * u * u   # H += kinetic energy
+        states.append((w, vhi))
+    dimq, dimw = ns + 3, ns + 8
+    jm = nj // 2
+    qcell = np.zeros((nj, ni, dimq))
+    wcell = np.zeros((nj, ni, dimw))
+    for s, (w, vhi) in enumerate(states):
+        rows = slice(0, jm) if s == 0 else slice(jm, nj)
+        rho, u, p = w[0], w[1], w[3]
+        e_int = w[ns + 5] - 0.5 * u * u - p / rho       # H - u^2/2 - p/rho
+        qcell[rows, :, :ns] = rho * w[4:4 + ns]
+        qcell[rows, :, ns] = rho * u
+        qcell[rows, :, ns + 1] = 0.0
+        qcell[rows, :, ns + 2] = rho * (e_int + 0.5 * u * u)
+        wcell[rows, :, :] = w
+    w0 = ref_zeros(dimw, ni, nj)
+    w0[2:nj + 2, 2:ni + 2, :] = wcell
+    pool = np.concatenate([np.concatenate(st) for st in states])
+    off = dimw + ns
+    rows = [[1, SIDE_IMIN, 1, jm, BC_FIXED, 0], [1, SIDE_IMIN, jm + 1, nj, BC_FIXED, off],
+            [1, SIDE_IMAX, 1, nj, BC_EXTRAP, 0], [1, SIDE_JMIN, 1, ni, BC_REFLECT, 0],
+            [1, SIDE_JMAX, 1, ni, BC_REFLECT, 0]]
+    xh, rh = rect_nodes(ni, nj, 0.0, length, 0.0, height)
+    k_mus = -1.0 if order == 2 else 1.0 / 3.0
+    params = make_params(scheme=scheme, time_step=0, high_order=1, limiter=LIMITER_REFERENCE, viscous=viscous,
+                         thermal=2, reaction=reaction, cfl=cfl, k_mus=k_mus, kappa=1.4, rgas=287.0, nu=0.0,
+                         num_internal_loop=2)
+    return Case("mixing_layer", [ni], [nj], ns, params, [xh], [rh], [_fill_q(ni, nj, qcell)],
+                np.array(rows, dtype=np.int32), pool, np.zeros((0, 11), dtype=np.int32),
+                description=f"reacting mixing layer {ni}x{nj}, {ns} species / {mech.nr} reactions, "
+                            f"fuel {T_fuel:g} K / oxidiser {T_ox:g} K, p {p0:g} Pa",
+                mech=mech, w0=[w0])

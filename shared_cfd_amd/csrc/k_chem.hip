@@ -1,0 +1,167 @@
+// k_chem.hip -- Chemkin thermochemistry kernels: the reactive
+// set_thermo_prop (calc_T per cell), proceed_reaction (CHEMEQ2 or the
+// point-implicit LU update per cell) and a batched 0-D harness (Fex, Jex,
+// calc_T, CHEMEQ2, point-implicit on arrays of independent states).
+//
+// One lane owns one cell; the per-cell species vectors and the 10x10
+// Jacobian stay in registers (shcfd_chem.h).  The mechanism tables are read
+// through uniform (scalar) loads.
+#include "shcfd_chem.h"
+#include "shcfd_device.h"
+#include "shcfd_kernels.h"
+
+namespace shcfd {
+
+// store/therm_lib/chemkin/flow/reactive/thermal_model.f90:13-42
+template <int NS>
+__global__ void __launch_bounds__(256) k_thermo_chem(DevCtx d, const Tile* tiles, const shcfd_mech* __restrict__ m) {
+    using D = Dims<NS>;
+    TileCoord t = tile_coord(tiles);
+    const DevBlock B = d.blk[t.b];
+    if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) return;
+    long long c = cidx(B, t.i, t.j);
+    double rho = FLD(d, SHCFD_F_W, IW_RHO, c), u = FLD(d, SHCFD_F_W, IW_U, c), v = FLD(d, SHCFD_F_W, IW_V, c);
+    double qe = FLD(d, SHCFD_F_Q, NS + 2, c);
+    double ei = qe / rho - 0.5 * (u * u + v * v);
+    double T = FLD(d, SHCFD_F_W, IW_P, c) / (rho * FLD(d, SHCFD_F_W, D::ir, c));
+    double vrho[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) vrho[k] = FLD(d, SHCFD_F_Q, k, c);
+    chem::ThermoOut<NS> o;
+    if (chem::calc_t<NS>(m, vrho, rho, ei, T, o) < 0) atomicCAS(d.iflag + IFLAG_ERR, 0, SHCFD_E_TNEWTON);
+    double rg = chem::R_UNI / o.mwave;
+    double p = rho * rg * T;
+    FLD(d, SHCFD_F_W, D::ir, c) = rg;
+    FLD(d, SHCFD_F_W, IW_P, c) = p;
+    FLD(d, SHCFD_F_W, D::ig, c) = o.kappa;
+    FLD(d, SHCFD_F_W, D::ih, c) = (qe + p) / rho;
+    FLD(d, SHCFD_F_W, D::imu, c) = o.mu;
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        FLD(d, SHCFD_F_DHI, k, c) = o.dhi[k];
+        FLD(d, SHCFD_F_VHI, k, c) = o.vhi[k];
+    }
+}
+
+// proceed_reaction, thermal_model.f90:45-73 (mode 1: reaction_chemeq2;
+// mode 2: point-implicit LU update), over dt_grbl
+template <int NS>
+__global__ void __launch_bounds__(256) k_reaction(DevCtx d, const Tile* tiles, const shcfd_mech* __restrict__ m,
+                                                  int mode) {
+    using D = Dims<NS>;
+    TileCoord t = tile_coord(tiles);
+    const DevBlock B = d.blk[t.b];
+    if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) return;
+    long long c = cidx(B, t.i, t.j);
+    double dt = d.scal[SCAL_DT_GRBL];
+    double T = FLD(d, SHCFD_F_W, IW_P, c) / FLD(d, SHCFD_F_W, IW_RHO, c) / FLD(d, SHCFD_F_W, D::ir, c);
+    double vrho[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) vrho[k] = FLD(d, SHCFD_F_Q, k, c);
+    int e, att = 0;
+    double T1;
+    if (mode == SHCFD_REACTION_POINT_IMPLICIT)
+        e = chem::point_implicit<NS>(m, T, vrho, dt, att, T1);
+    else
+        e = chem::reaction_chemeq2<NS>(m, T, vrho, dt, att);
+    if (e) {
+        atomicCAS(d.iflag + IFLAG_ERR, 0, e);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < NS; k++) FLD(d, SHCFD_F_Q, k, c) = vrho[k];
+}
+
+// batched 0-D harness: cell q's input row starts at in + q*ldi, output row at
+// out + q*ldo; status[q] receives an iteration/attempt count or -code.
+template <int NS, int op>
+__global__ void __launch_bounds__(256) k_chem0d(const shcfd_mech* __restrict__ m, int n, const double* in, int ldi,
+                                                double* out, int ldo, int* status, double dt) {
+    constexpr int N = NS + 1;
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double* x = in + (long long)q * ldi;
+    double* y = out + (long long)q * ldo;
+    int st = 0;
+    if constexpr (op == CHEM_OP_FEX || op == CHEM_OP_FEXVAR || op == CHEM_OP_JEX) {
+        double v[N];
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = x[k];
+        if constexpr (op == CHEM_OP_FEX) {
+            double f[N];
+            chem::fex<NS>(m, v, f);
+#pragma unroll
+            for (int k = 0; k < N; k++) y[k] = f[k];
+        } else if constexpr (op == CHEM_OP_FEXVAR) {
+            double p[N], mm[N];
+            chem::fex_var<NS>(m, v, p, mm);
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                y[k] = p[k];
+                y[N + k] = mm[k];
+            }
+        } else {
+            double J[N][N], f[N];
+            chem::jex<NS>(m, v, J, f);
+            // column-major, as the reference's dndn(ns+1, ns+1)
+#pragma unroll
+            for (int b = 0; b < N; b++)
+#pragma unroll
+                for (int a = 0; a < N; a++) y[a + N * b] = J[a][b];
+        }
+    } else if constexpr (op == CHEM_OP_CALCT) {
+        // in: vrho(NS), rho, E, T start; out: T, kappa, MWave, mu, DHi(NS), vhi(NS)
+        double vrho[NS];
+#pragma unroll
+        for (int k = 0; k < NS; k++) vrho[k] = x[k];
+        double T = x[NS + 2];
+        chem::ThermoOut<NS> o;
+        st = chem::calc_t<NS>(m, vrho, x[NS], x[NS + 1], T, o);
+        y[0] = T;
+        y[1] = o.kappa;
+        y[2] = o.mwave;
+        y[3] = o.mu;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            y[4 + k] = o.dhi[k];
+            y[4 + NS + k] = o.vhi[k];
+        }
+    } else if constexpr (op == CHEM_OP_POINT_IMPLICIT) {
+        // in: vrho(NS), T; out: vrho(NS), backward-Euler T
+        double vrho[NS];
+#pragma unroll
+        for (int k = 0; k < NS; k++) vrho[k] = x[k];
+        int it = 0;
+        double T1 = x[NS];
+        int e = chem::point_implicit<NS>(m, x[NS], vrho, dt, it, T1);
+        st = e ? -e : it;
+#pragma unroll
+        for (int k = 0; k < NS; k++) y[k] = vrho[k];
+        y[NS] = T1;
+    } else {
+        // in: vrho(NS), T; out: vrho(NS)
+        double vrho[NS];
+#pragma unroll
+        for (int k = 0; k < NS; k++) vrho[k] = x[k];
+        int att = 0;
+        int e = chem::reaction_chemeq2<NS>(m, x[NS], vrho, dt, att);
+        st = e ? -e : att;
+#pragma unroll
+        for (int k = 0; k < NS; k++) y[k] = vrho[k];
+    }
+    status[q] = st;
+}
+
+#define SHCFD_CHEM_FLOW_INST(NS)                                                        \
+    template __global__ void k_thermo_chem<NS>(DevCtx, const Tile*, const shcfd_mech*); \
+    template __global__ void k_reaction<NS>(DevCtx, const Tile*, const shcfd_mech*, int);
+#define SHCFD_CHEM0D_OP(NS, OP) \
+    template __global__ void k_chem0d<NS, OP>(const shcfd_mech*, int, const double*, int, double*, int, int*, double);
+#define SHCFD_CHEM0D_INST(NS)                                                                     \
+    SHCFD_CHEM0D_OP(NS, CHEM_OP_FEX) SHCFD_CHEM0D_OP(NS, CHEM_OP_FEXVAR) SHCFD_CHEM0D_OP(NS, CHEM_OP_JEX) \
+    SHCFD_CHEM0D_OP(NS, CHEM_OP_CALCT) SHCFD_CHEM0D_OP(NS, CHEM_OP_CHEMEQ2)                          \
+    SHCFD_CHEM0D_OP(NS, CHEM_OP_POINT_IMPLICIT)
+SHCFD_FOR_EACH_FLOW_NS(SHCFD_CHEM_FLOW_INST)
+SHCFD_FOR_EACH_NS(SHCFD_CHEM0D_INST)
+
+}  // namespace shcfd

@@ -63,6 +63,11 @@ int field_ncomp(int fld, int ny) {
     case 9: { constexpr int NY = 9; CALL; } break; \
     default: return SHCFD_E_ARG;         \
     }
+#define DISPATCH_FLOW_NS(NSV, CALL)       \
+    switch (NSV) {                       \
+    case 9: { constexpr int NS = 9; CALL; } break; \
+    default: return SHCFD_E_ARG;         \
+    }
 
 static bool ny_supported(int ny) { return ny == 1 || ny == 2 || ny == 9; }
 
@@ -168,7 +173,7 @@ int shcfd_destroy(shcfd_ctx* ctx) {
         for (auto& e : ctx->prof_pending[k]) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
     for (int fld = 0; fld < SHCFD_F_COUNT; fld++) if (ctx->f[fld]) hipFree(ctx->f[fld]);
     void* ptrs[] = {ctx->dblk, ctx->dtiles, ctx->dstrips, ctx->dprog, ctx->dbnd, ctx->dpart, ctx->scal,
-                    ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool};
+                    ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool, ctx->dmech};
     for (void* p : ptrs) if (p) hipFree(p);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -431,9 +436,37 @@ int shcfd_set_w(shcfd_ctx* ctx) {
 int shcfd_set_thermo_prop(shcfd_ctx* ctx, int step) {
     (void)step;
     if (!ctx) return SHCFD_E_ARG;
-    if (ctx->p.thermal != 0) return SHCFD_E_STATE;
+    if (ctx->p.thermal == SHCFD_THERMAL_CHEMKIN) {
+        if (!ctx->has_mech) return SHCFD_E_STATE;
+        ProfScope ps(ctx, PROF_THERMO_CHEM);
+        DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH(k_thermo_chem<NS>, ctx->dmech));
+        LAUNCHCHK();
+        return SHCFD_OK;
+    }
+    if (ctx->p.thermal != SHCFD_THERMAL_IDEAL) return SHCFD_E_STATE;
     ProfScope ps(ctx, PROF_PRIMITIVE);
     DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_thermo_ideal<NY>));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+int shcfd_set_mechanism(shcfd_ctx* ctx, const shcfd_mech* mech) {
+    if (!ctx || !mech || mech->ns != ctx->ny || mech->nr < 1 || mech->nr > SHCFD_CHEM_MAX_NR ||
+        mech->nt < 0 || mech->nt > mech->ns)
+        return SHCFD_E_ARG;
+    if (!ctx->dmech) HIPCHK(hipMalloc((void**)&ctx->dmech, sizeof(shcfd_mech)));
+    HIPCHK(hipMemcpy(ctx->dmech, mech, sizeof(shcfd_mech), hipMemcpyHostToDevice));
+    ctx->has_mech = 1;
+    return SHCFD_OK;
+}
+
+int shcfd_proceed_reaction(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    if (!ctx->has_mech || ctx->p.thermal != SHCFD_THERMAL_CHEMKIN ||
+        (ctx->p.reaction != SHCFD_REACTION_CHEMEQ2 && ctx->p.reaction != SHCFD_REACTION_POINT_IMPLICIT))
+        return SHCFD_E_STATE;
+    ProfScope ps(ctx, PROF_REACTION);
+    DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH(k_reaction<NS>, ctx->dmech, ctx->p.reaction));
     LAUNCHCHK();
     return SHCFD_OK;
 }
@@ -703,6 +736,18 @@ int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double
 #define TRY(x) do { rc = (x); if (rc) return rc; } while (0)
     switch (ctx->p.scheme) {
     case SHCFD_SCHEME_LUSGS:
+        if (ctx->p.reaction) {  // LU-SGS/2d/main.main.point_implicit.f90:1-9
+            TRY(shcfd_set_w(ctx));
+            TRY(shcfd_set_thermo_prop(ctx, step));
+            TRY(shcfd_set_dt(ctx, step, nullptr));
+            TRY(shcfd_proceed_reaction(ctx));
+            TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
+            TRY(part_primitive(ctx));
+            TRY(part_main(ctx));
+            TRY(abpm_for_step(ctx));
+            TRY(implicit_async(ctx, 0));
+            break;
+        }
         TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
         TRY(part_primitive(ctx));
         TRY(shcfd_set_dt(ctx, step, nullptr));
@@ -725,9 +770,18 @@ int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double
         TRY(shcfd_calc_next_step(ctx));
         break;
     case SHCFD_SCHEME_RK2:
-        TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
-        TRY(part_primitive(ctx));
-        TRY(shcfd_set_dt(ctx, step, nullptr));
+        if (ctx->p.reaction) {  // RK2/main.main.point_implicit.f90:1-11
+            TRY(shcfd_set_w(ctx));
+            TRY(shcfd_set_thermo_prop(ctx, step));
+            TRY(shcfd_set_dt(ctx, step, nullptr));
+            TRY(shcfd_proceed_reaction(ctx));
+            TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
+            TRY(part_primitive(ctx));
+        } else {
+            TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
+            TRY(part_primitive(ctx));
+            TRY(shcfd_set_dt(ctx, step, nullptr));
+        }
         TRY(part_main(ctx));
         TRY(shcfd_calc_next_step_rk2(ctx, 1));
         TRY(part_primitive(ctx));

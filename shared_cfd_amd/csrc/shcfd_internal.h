@@ -78,7 +78,7 @@ struct Strip {
 enum {
     PROF_PRIMITIVE = 0, PROF_BC, PROF_HV, PROF_TG, PROF_TGV, PROF_DT, PROF_ABPM, PROF_DQDT, PROF_RHS,
     PROF_SWEEP_FWD, PROF_SWEEP_BWD, PROF_REDUCE, PROF_UPDATE, PROF_EXPLICIT, PROF_RESIDUAL, PROF_PREP,
-    PROF_COUNT
+    PROF_THERMO_CHEM, PROF_REACTION, PROF_COUNT
 };
 
 struct EventPair {
@@ -134,6 +134,9 @@ struct Ctx {
     std::string last_error;
     int sweep_impl = 0;            // 0 = plain (stored Jacobians)
     long long spin_limit = 1LL << 23;
+    // Chemkin mechanism (module chem of the reference), device copy
+    shcfd_mech* dmech = nullptr;
+    int has_mech = 0;
 
     DevCtx dev() const {
         DevCtx d{};

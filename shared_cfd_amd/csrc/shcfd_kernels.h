@@ -5,6 +5,10 @@
 
 // species counts with compiled kernels (1 = ideal gas)
 #define SHCFD_FOR_EACH_NY(X) X(1) X(2) X(9)
+// species counts with compiled chemistry (0-D harness); flows with
+// chemistry need NY = NS in both lists
+#define SHCFD_FOR_EACH_NS(X) X(8) X(9)
+#define SHCFD_FOR_EACH_FLOW_NS(X) X(9)
 
 namespace shcfd {
 
@@ -408,6 +412,12 @@ template <int NY> __global__ void k_sweep_bwd_plain(DevCtx, const Strip*, int, i
 template <int NY> __global__ void k_update(DevCtx, const Tile*);
 __global__ void k_reduce_res_omega(DevCtx, const double*, int);
 __global__ void k_copy_scal(DevCtx, int, int);
+// chemistry (k_chem.hip)
+enum { CHEM_OP_FEX = 0, CHEM_OP_FEXVAR = 1, CHEM_OP_JEX = 2, CHEM_OP_CALCT = 3, CHEM_OP_CHEMEQ2 = 4,
+       CHEM_OP_POINT_IMPLICIT = 5 };
+template <int NS> __global__ void k_thermo_chem(DevCtx, const Tile*, const shcfd_mech*);
+template <int NS> __global__ void k_reaction(DevCtx, const Tile*, const shcfd_mech*, int);
+template <int NS, int OP> __global__ void k_chem0d(const shcfd_mech*, int, const double*, int, double*, int, int*, double);
 // geometry (k_geom.hip)
 __global__ void k_geom_a(DevCtx, const Tile*);
 __global__ void k_geom_ghost(DevCtx, int);

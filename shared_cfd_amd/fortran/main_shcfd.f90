@@ -30,7 +30,7 @@ program main
                prm%num_internal_loop, fast
    read(10, *) prm%phi_mus, prm%k_mus, prm%cfl, prm%cfl_tau, prm%omega_max, prm%omega_min, prm%dqmax, &
                prm%res_rate_warning, prm%res_rate_error, prm%kappa, prm%rgas, prm%nu
-   prm%pad0 = 0
+   prm%reaction = 0
    read(10, *) max_step, min_RMS, OutPeriod
    read(10, *) nbc, npool
    allocate(bc(6*max(nbc,1)), pool(max(npool,1)))

@@ -15,7 +15,7 @@ module shcfd_f
 
    type, bind(C) :: shcfd_params_t
       integer(c_int) :: scheme, time_step, high_order, limiter, viscous, thermal
-      integer(c_int) :: num_internal_loop, pad0
+      integer(c_int) :: num_internal_loop, reaction
       real(c_double) :: phi_mus, k_mus, cfl, cfl_tau, omega_max, omega_min, dqmax
       real(c_double) :: res_rate_warning, res_rate_error, kappa, rgas, nu
    end type shcfd_params_t

@@ -28,12 +28,14 @@ EXPORTS = [
     "shcfd_set_abpm", "shcfd_set_dqdt", "shcfd_calc_next_step_implicit", "shcfd_calc_next_step",
     "shcfd_calc_next_step_rk2", "shcfd_copy_state", "shcfd_energy_rms", "shcfd_prepare", "shcfd_step",
     "shcfd_check_error", "shcfd_profile_enable", "shcfd_profile_read", "shcfd_set_option",
+    "shcfd_mech_parse", "shcfd_set_mechanism", "shcfd_proceed_reaction", "shcfd_chem_boundary_state",
+    "shcfd_chem_eval",
 ]
 OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT = 1, 2
 
 # kernel classes of shcfd_profile_read (include/shcfd.h)
 KCLASS = ["PRIMITIVE", "BC", "HV", "TG", "TGV", "DT", "ABPM", "DQDT", "RHS", "SWEEP_FWD", "SWEEP_BWD",
-          "REDUCE", "UPDATE", "EXPLICIT", "RESIDUAL", "PREP"]
+          "REDUCE", "UPDATE", "EXPLICIT", "RESIDUAL", "PREP", "THERMO_CHEM", "REACTION"]
 
 
 class ShcfdError(RuntimeError):
@@ -121,6 +123,14 @@ class Solver:
         pool = np.ascontiguousarray(case.bc_pool, dtype=np.float64)
         check(L.shcfd_set_bc_table(self.ctx, _ip(bc), int(bc.shape[0]), _dp(pool), int(pool.size)),
               "shcfd_set_bc_table")
+        if case.mech is not None:
+            from . import chem
+
+            chem._lib()
+            check(L.shcfd_set_mechanism(self.ctx, ctypes.byref(case.mech)), "shcfd_set_mechanism")
+        if case.w0 is not None and load_q:
+            for b in range(case.nblocks):
+                self.upload("W", b, case.w0[b])
         check(L.shcfd_init_geometry(self.ctx), "shcfd_init_geometry")
         self.set_option(OPT_FAST_IMPLICIT, int(fast))
 
@@ -163,6 +173,10 @@ class Solver:
 
     def set_BC(self, step=0):
         check(self.L.shcfd_set_bc(self.ctx, step), "set_BC")
+
+    def proceed_reaction(self):
+        check(self.L.shcfd_proceed_reaction(self.ctx), "proceed_reaction")
+        check(self.L.shcfd_check_error(self.ctx), "proceed_reaction")
 
     def set_HV(self):
         check(self.L.shcfd_set_hv(self.ctx), "set_HV")

@@ -1,0 +1,47 @@
+"""Host-side chemistry of the library (no GPU): boundary states and the
+parser of the 9-species test mechanism, against the CPU oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle.chem_inp import read_cheminp
+from shared_cfd_amd import chem
+
+from chem_cases import no27_plus_h2o2
+
+
+@pytest.fixture(scope="module")
+def mechs(tmp_path_factory):
+    d = tmp_path_factory.mktemp("m")
+    c, t = chem.write_synthetic_mechanism(3, str(d / "syn"))
+    p9 = no27_plus_h2o2(str(d / "no27"))
+    return {"syn": (chem.parse(c, t), read_cheminp(c, chem.MW_INP, t)),
+            "no27_9": (chem.parse(p9), read_cheminp(p9, chem.MW_INP))}
+
+
+@pytest.mark.parametrize("which", ["syn", "no27_9"])
+def test_boundary_state_matches_oracle_bitwise(mechs, which):
+    lm, om = mechs[which]
+    assert lm.ns == 9
+    rng = np.random.default_rng(0)
+    for T in (300.0, 1000.0, 1500.0, 2400.0):
+        rel = rng.random(9) + 0.01
+        w, vhi, dhi = chem.boundary_state(lm, 1.01325e5, T, rel)
+        vrho, rho, E = O.chem_rho_rel2abs(om, 1.01325e5, T, rel)
+        T1, out, it = O.chem_calc_t(om, vrho, rho, E, T)
+        ns = 9
+        assert w[0] == rho and w[3] == 1.01325e5
+        assert np.array_equal(w[4:4 + ns], vrho / rho)
+        assert w[ns + 4] == out[0]                        # gamma
+        assert w[ns + 5] == E + 1.01325e5 / rho           # H (u = v = 0)
+        assert w[ns + 6] == 8.3144621e3 / out[1]          # R
+        assert w[ns + 7] == out[2]                        # mu
+        assert np.array_equal(vhi, out[3 + ns:3 + 2 * ns]) and np.array_equal(dhi, out[3:3 + ns])
+
+
+def test_no27_with_h2o2_parses(mechs):
+    lm, om = mechs["no27_9"]
+    assert (lm.ns, lm.nr) == (9, 9)
+    a, b = chem.Mech.from_buffer_copy(bytes(lm)), om
+    a.rtol = a.atol = b.rtol = b.atol = 0.0
+    assert bytes(a) == bytes(b)

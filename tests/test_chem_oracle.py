@@ -1,0 +1,399 @@
+"""CPU pins of the chemistry oracle (oracle/shcfd_chem_oracle.c) and of the
+mechanism parsers.  Each check is independent of the restated formula:
+finite differences, conservation laws, equilibrium constants rebuilt from
+the NASA-7 polynomials in numpy, LAPACK, a stiff ODE solver, closed forms.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle.chem_inp import OrMech, read_cheminp
+from shared_cfd_amd import chem
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NO27 = os.path.join(HERE, "golden", "no27_chem.inp")
+RU = 8.3144621e7
+PST = 1.01325e6
+
+
+@pytest.fixture(scope="module")
+def synth(tmp_path_factory):
+    d = tmp_path_factory.mktemp("mech")
+    c, t = chem.write_synthetic_mechanism(7, str(d))
+    return c, t
+
+
+@pytest.fixture(scope="module")
+def no27():
+    return read_cheminp(NO27, chem.MW_INP)
+
+
+@pytest.fixture(scope="module")
+def syn(synth):
+    return read_cheminp(synth[0], chem.MW_INP, synth[1])
+
+
+# ---- exp / log ----------------------------------------------------------------
+def test_exp_log_within_2ulp():
+    L = O.lib()
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(-700, 700, 4000), rng.uniform(-1, 1, 4000), [0.0, 1e-300, -1e-18, 0.5]])
+    for x in xs:
+        e = L.or_exp(float(x))
+        ref = math.exp(x)
+        assert abs(e - ref) <= 2 * math.ulp(ref), x
+    ys = np.concatenate([10.0 ** rng.uniform(-300, 300, 4000), rng.uniform(0.5, 2.0, 4000), [1.0, 2.0, 1e-310]])
+    for y in ys:
+        v = L.or_log(float(y))
+        ref = math.log(y)
+        assert abs(v - ref) <= 2 * math.ulp(ref) + 1e-300, y
+    assert L.or_log(1.0) == 0.0 and L.or_exp(0.0) == 1.0
+
+
+# ---- parsers ------------------------------------------------------------------
+def test_no27_known_values(no27):
+    m = no27
+    assert (m.ne, m.ns, m.nr) == (4, 8, 9)
+    assert abs(m.mw[0] - 2 * 14.0067) < 1e-12  # N2
+    assert abs(m.mw[7] - (2 * 1.00797 + 15.9994)) < 1e-12  # H2O
+    # H+HO2=H2+O2  4.280E+013 0.000 1410.1
+    assert m.abe[0][0] == math.log(4.28e13)
+    assert m.abe[0][2] == 1410.1 * (1.0 / float(np.float32(1.987)))
+    assert [m.ind_nu[0][0][k] for k in range(2)] == [5, 6] and m.num_nu[0][1] == 2
+    # O2+H+M=HO2+M with N2/0.223/ O2/0.133/ H2O/0/
+    assert m.kind[1] == 4 and m.exist_m[1] == 1 and m.num_m[1] == 3
+    assert [m.ind_m[1][k] for k in range(3)] == [0, 1, 7]
+    assert abs(m.men[1][0] - (0.223 - 1)) < 1e-15 and m.men[1][2] == -1.0
+    assert m.snu[1] == -1 and m.snu[0] == 0
+    assert m.coeff[0][1][0] == 0.29525407e01 and m.coeff[0][0][6] == 0.29674391e01
+    assert list(m.tthre[0]) == [200.0, 6000.0, 1000.0]
+
+
+@pytest.mark.parametrize("which", ["no27", "synthetic"])
+def test_library_parser_matches_restatement(which, synth):
+    if which == "no27":
+        a = chem.parse(NO27)
+        b = read_cheminp(NO27, chem.MW_INP)
+    else:
+        a = chem.parse(synth[0], synth[1])
+        b = read_cheminp(synth[0], chem.MW_INP, synth[1])
+    a.rtol = a.atol = b.rtol = b.atol = 1e-6
+    assert bytes(a) == bytes(b)
+
+
+def test_trans_reader_skips_binary_and_unknown(tmp_path, synth):
+    t = tmp_path / "trans.inp"
+    t.write_text(
+        "title\n"
+        "N2              O2                V1C0  binary\n"
+        " V  300.0   1000.0   0.1E+01 0.2E+02 0.3E+03 0.4E+01\n"
+        "XX                                V1C1  unknown species\n"
+        " V  300.0   1000.0   0.1E+01 0.2E+02 0.3E+03 0.4E+01\n"
+        " C  300.0   1000.0   0.1E+01 0.2E+02 0.3E+03 0.4E+01\n"
+        "H2                                V2C1  kept\n"
+        " V  200.0   1000.0   0.74553182E 00 0.43555109E 02-0.32579340E 04 0.13556243E 00\n"
+        " V 1000.0   5000.0   0.96730605E+00 0.67931897E+03-0.21025179E+06-0.18251697E+01\n"
+        " C  200.0   1000.0   0.1E+01 0.2E+02 0.3E+03 0.4E+01\n"
+        "END\n")
+    a = chem.parse(synth[0], str(t))
+    b = read_cheminp(synth[0], chem.MW_INP, str(t))
+    assert a.nt == b.nt == 1 and a.tr2th[0] == 2 and a.nsec_trans[0] == 2
+    assert a.trans[0][0][1] == 43.555109 and a.trans[0][1][2] == -0.21025179e6
+    a.rtol = a.atol = b.rtol = b.atol = 0.0
+    assert bytes(a) == bytes(b)
+
+
+def test_parser_rejects_unknown_species(tmp_path):
+    bad = open(NO27).read().replace("H+HO2=H2+O2", "H+HO3=H2+O2")
+    p = tmp_path / "bad.inp"
+    p.write_text(bad)
+    with pytest.raises(ValueError):
+        chem.parse(str(p))
+
+
+# ---- NASA-7 thermo rebuilt in numpy ----------------------------------------------
+def _nasa(m, k, T):
+    c = np.array(m.coeff[k][0 if T <= m.tthre[k][2] else 1])
+    cp = c[0] + c[1] * T + c[2] * T ** 2 + c[3] * T ** 3 + c[4] * T ** 4
+    h = c[0] + c[1] * T / 2 + c[2] * T ** 2 / 3 + c[3] * T ** 3 / 4 + c[4] * T ** 4 / 5 + c[5] / T
+    s = c[0] * math.log(T) + c[1] * T + c[2] * T ** 2 / 2 + c[3] * T ** 3 / 3 + c[4] * T ** 4 / 4 + c[6]
+    return cp, h, s  # cp/R, h/RT, s/R
+
+
+def _state(m, rng, T=None):
+    n = 10.0 ** rng.uniform(-9, -5, m.ns)
+    return np.append(n, T if T is not None else rng.uniform(800.0, 2600.0))
+
+
+@pytest.mark.parametrize("which", ["no27", "syn"])
+def test_jex_matches_central_differences(which, no27, syn):
+    m = no27 if which == "no27" else syn
+    rng = np.random.default_rng(1)
+    N = m.ns + 1
+    for _ in range(6):
+        y = _state(m, rng)
+        J = O.chem_jex(m, y)
+        f0 = O.chem_fex(m, y)
+        Jfd = np.zeros((N, N))
+        for b in range(N):
+            h = 1e-6 * y[b]
+            yp, ym = y.copy(), y.copy()
+            yp[b] += h
+            ym[b] -= h
+            Jfd[:, b] = (O.chem_fex(m, yp) - O.chem_fex(m, ym)) / (2 * h)
+        # scale per row by the size of the rates it is built from
+        for a in range(N):
+            scale = np.max(np.abs(Jfd[a] * y)) / y.max() + np.max(np.abs(J[a])) * 1e-12 + 1e-300
+            err = np.max(np.abs((J[a] - Jfd[a]) * y)) / (np.max(np.abs(Jfd[a] * y)) + 1e-300)
+            assert err < 2e-6, (which, a, err)
+        assert np.all(np.isfinite(f0))
+
+
+@pytest.mark.parametrize("which", ["no27", "syn"])
+def test_fex_conserves_mass_and_elements(which, no27, syn):
+    m = no27 if which == "no27" else syn
+    rng = np.random.default_rng(2)
+    mw = np.array(m.mw[:m.ns])
+    es = np.array([[m.es[k][e] for e in range(m.ne)] for k in range(m.ns)])
+    for _ in range(10):
+        y = _state(m, rng)
+        f = O.chem_fex(m, y)[:m.ns]
+        scale = np.sum(np.abs(f) * mw)
+        assert abs(np.sum(f * mw)) <= 1e-13 * scale
+        for e in range(m.ne):
+            s = np.sum(np.abs(f) * es[:, e]) + 1e-300
+            assert abs(np.sum(f * es[:, e])) <= 1e-13 * s
+
+
+def _single_reaction(m, i):
+    r = OrMech.from_buffer_copy(bytes(m))
+    src = OrMech.from_buffer_copy(bytes(m))
+    for name in ("reverse_off", "kind", "num_nu", "ind_nu", "snu", "exist_m", "num_m", "ind_m", "men", "abe",
+                 "cabe", "troe"):
+        getattr(r, name)[0] = getattr(src, name)[i]
+    r.nr = 1
+    return r
+
+
+@pytest.mark.parametrize("i", range(9))
+def test_reverse_rate_is_forward_over_kc(no27, i):
+    """k_r = k_f / K_c with K_c = exp(-sum nu g/RT) (pst/(Ru T))^dnu rebuilt
+    from the NASA-7 polynomials here (no27 reactions use K_c)."""
+    m = _single_reaction(no27, i)
+    rng = np.random.default_rng(10 + i)
+    y = _state(m, rng, T=1500.0)
+    T = y[-1]
+    reac = [m.ind_nu[0][0][k] for k in range(m.num_nu[0][0])]
+    prod = [m.ind_nu[0][1][k] for k in range(m.num_nu[0][1])]
+    # third-body concentration
+    M = np.sum(y[:m.ns]) + sum(m.men[0][k] * y[m.ind_m[0][k]] for k in range(m.num_m[0]))
+    kf = math.exp(m.abe[0][0]) * T ** m.abe[0][1] * math.exp(-m.abe[0][2] / T) * (M if m.kind[0] == 4 else 1.0)
+    g = {k: _nasa(m, k, T)[1] - _nasa(m, k, T)[2] for k in set(reac + prod)}
+    dnu = len(prod) - len(reac)
+    lnKc = -(sum(g[k] for k in prod) - sum(g[k] for k in reac)) + dnu * math.log(PST / (RU * T))
+    kr = kf / math.exp(lnKc)
+    rate = kf * np.prod(y[reac]) - kr * np.prod(y[prod])
+    f = O.chem_fex(m, y)
+    k0 = prod[0]
+    expect = rate * prod.count(k0) - (rate * reac.count(k0))
+    assert abs(f[k0] - expect) <= 1e-9 * (kf * np.prod(y[reac]) + kr * np.prod(y[prod])) * 2
+
+
+def test_equilibrium_has_zero_net_rate(no27):
+    """Products set to K_c-balanced values make Fex vanish for that reaction."""
+    m = _single_reaction(no27, 6)  # H2+O=OH+H
+    T = 1800.0
+    y = np.full(m.ns + 1, 1e-7)
+    y[-1] = T
+    reac = [m.ind_nu[0][0][k] for k in range(m.num_nu[0][0])]
+    prod = [m.ind_nu[0][1][k] for k in range(m.num_nu[0][1])]
+    g = {k: _nasa(m, k, T)[1] - _nasa(m, k, T)[2] for k in set(reac + prod)}
+    Kc = math.exp(-(sum(g[k] for k in prod) - sum(g[k] for k in reac)))
+    # choose [OH] so that [OH][H]/([H2][O]) = Kc
+    y[prod[0]] = Kc * y[reac[0]] * y[reac[1]] / y[prod[1]]
+    f = O.chem_fex(m, y)
+    kf = math.exp(m.abe[0][0]) * T ** m.abe[0][1] * math.exp(-m.abe[0][2] / T)
+    assert abs(f[reac[0]]) <= 1e-12 * kf * y[reac[0]] * y[reac[1]]
+
+
+@pytest.mark.parametrize("which", ["no27", "syn"])
+def test_fex_var_splits_fex(which, no27, syn):
+    m = no27 if which == "no27" else syn
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        y = _state(m, rng)
+        p, q = O.chem_fex_var(m, y)
+        f = O.chem_fex(m, y)
+        assert np.all(p >= 0) and np.all(q <= 0)
+        scale = np.abs(p) + np.abs(q) + 1e-300
+        assert np.all(np.abs(p[:-1] + q[:-1] - f[:-1]) <= 1e-12 * scale[:-1])
+        assert abs((p[-1] + q[-1]) - f[-1]) <= 1e-9 * (abs(f[-1]) + 1e-300)
+
+
+def test_dT_dt_conserves_energy(syn):
+    """de/dt = 0 at constant volume: sum (e_k/R) dn_k/dt + cv/R dT/dt = 0."""
+    m = syn
+    rng = np.random.default_rng(4)
+    for _ in range(5):
+        y = _state(m, rng)
+        f = O.chem_fex(m, y)
+        T = y[-1]
+        ek = np.array([(_nasa(m, k, T)[1] - 1.0) * T for k in range(m.ns)])
+        cv = np.array([_nasa(m, k, T)[0] - 1.0 for k in range(m.ns)])
+        de = np.sum(ek * f[:m.ns]) + np.sum(cv * y[:m.ns]) * f[-1]
+        assert abs(de) <= 1e-11 * np.sum(np.abs(ek * f[:m.ns]))
+
+
+# ---- calc_T / rho_rel2abs / Wilke ----------------------------------------------------
+@pytest.mark.parametrize("T", [300.0, 999.0, 1000.0, 1001.5, 1800.0, 2900.0])
+def test_calc_t_round_trip(syn, T):
+    m = syn
+    rng = np.random.default_rng(int(T))
+    rel = rng.random(m.ns) + 0.05
+    vrho, rho, E = O.chem_rho_rel2abs(m, 1.01325e5, T, rel)
+    T1, out, it = O.chem_calc_t(m, vrho, rho, E, 0.7 * T + 150.0)
+    assert it > 0 and abs(T1 - T) < 1e-7 * T
+    # p = rho R T with R = R_uni / MW
+    n = vrho / np.array(m.mw[:m.ns]) * 1e-3
+    assert abs(rho / n.sum() * 1e-3 - out[1]) <= 1e-14 * out[1]
+    # gamma = cp/cv of the mixture from the polynomials
+    cv = sum(n[k] * (_nasa(m, k, T1)[0] - 1.0) for k in range(m.ns))
+    assert abs(out[0] - (1.0 + n.sum() / cv)) < 1e-10
+    # vhi = h_k (J/kg) and DHi = R_k T (gamma - (gamma-1) h_k/(R_k T))
+    for k in range(m.ns):
+        Rk = 8.3144621e3 / m.mw[k]
+        hk = _nasa(m, k, T1)[1]
+        assert abs(out[3 + m.ns + k] - Rk * T1 * hk) <= 1e-10 * abs(Rk * T1 * hk) + 1e-8
+        assert abs(out[3 + k] - Rk * T1 * (out[0] - (out[0] - 1) * hk)) <= 1e-9 * abs(Rk * T1 * hk) + 1e-6
+    # E from rho_rel2abs: sum n_k e_k / rho
+    e = sum(n[k] * (_nasa(m, k, T)[1] - 1.0) for k in range(m.ns)) * RU * T / rho / 10.0
+    assert abs(e - E) <= 1e-12 * abs(E) + 1e-9
+
+
+def test_wilke_single_species_is_pure_fit(syn):
+    m = syn
+    T = 1234.0
+    for k in range(m.ns):
+        rel = np.zeros(m.ns)
+        rel[k] = 1.0
+        vrho, rho, E = O.chem_rho_rel2abs(m, 1e5, T, rel)
+        T1, out, _ = O.chem_calc_t(m, vrho, rho, E, T)
+        t = [i for i in range(m.nt) if m.tr2th[i] == k][0]
+        sec = 0 if T1 < m.trange[t][0][1] else 1
+        a = m.trans[t][sec]
+        mu = math.exp(a[0] * math.log(T1) + a[1] / T1 + a[2] / T1 ** 2 + a[3]) * 1e-7
+        assert abs(out[2] - mu) <= 1e-13 * mu
+
+
+def test_wilke_binary_closed_form(syn):
+    """Two species: Wilke's formula written out by hand."""
+    m = syn
+    T = 900.0
+    rel = np.zeros(m.ns)
+    rel[0], rel[2] = 0.7, 0.3  # N2, H2 (mass-based relative amounts)
+    vrho, rho, E = O.chem_rho_rel2abs(m, 1e5, T, rel)
+    T1, out, _ = O.chem_calc_t(m, vrho, rho, E, T)
+    n = vrho / np.array(m.mw[:m.ns]) * 1e-3
+    mus, mws, ns_ = [], [], []
+    for k in (0, 2):
+        t = [i for i in range(m.nt) if m.tr2th[i] == k][0]
+        a = m.trans[t][0 if T1 < m.trange[t][0][1] else 1]
+        mus.append(math.exp(a[0] * math.log(T1) + a[1] / T1 + a[2] / T1 ** 2 + a[3]))
+        mws.append(m.mw[k])
+        ns_.append(n[k])
+
+    def phi(i, j):
+        return (1 + math.sqrt(mus[i] / mus[j]) * (mws[j] / mws[i]) ** 0.25) ** 2 / math.sqrt(8 * (1 + mws[i] / mws[j]))
+
+    mu = sum(ns_[i] * mus[i] / sum(ns_[j] * phi(i, j) for j in range(2)) for i in range(2)) * 1e-7
+    assert abs(out[2] - mu) <= 1e-12 * mu
+
+
+# ---- LINPACK LU and the point-implicit update ---------------------------------------
+def test_dgefa_dgesl_match_lapack():
+    rng = np.random.default_rng(5)
+    for n in (3, 10, 12):
+        A = rng.standard_normal((n, n))
+        b = rng.standard_normal(n)
+        info, x = O.dgefa_dgesl(A, b)
+        assert info == 0
+        assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-12, atol=1e-12)
+    A = np.array([[0.0, 2.0, 1.0], [1.0, 1.0, 0.0], [3.0, 0.0, 1.0]])  # needs pivoting
+    info, x = O.dgefa_dgesl(A, [1.0, 2.0, 3.0])
+    assert info == 0 and np.allclose(A @ x, [1.0, 2.0, 3.0], atol=1e-14)
+    info, _ = O.dgefa_dgesl(np.array([[1.0, 2.0], [2.0, 4.0]]), [1.0, 1.0])
+    assert info == 2
+
+
+@pytest.mark.parametrize("which", ["no27", "syn"])
+def test_point_implicit_conserves_mass_and_tends_to_euler(which, no27, syn):
+    m = no27 if which == "no27" else syn
+    rng = np.random.default_rng(6)
+    mw = np.array(m.mw[:m.ns])
+    for _ in range(4):
+        y = _state(m, rng, T=1600.0)
+        vrho = y[:m.ns] * mw * 1e3
+        f = O.chem_fex(m, y)
+        dt_exp = 1e-3 / (np.max(np.abs(f[:m.ns]) / y[:m.ns]) + 1e-300)
+        for dt in (1e3 * dt_exp, dt_exp, 1e-3 * dt_exp):
+            e, v, it, T1 = O.chem_point_implicit(m, 1600.0, vrho, dt)
+            assert e == 0 and it >= 1
+            assert abs(v.sum() - vrho.sum()) <= 1e-13 * vrho.sum()
+            # the result solves backward Euler: y1 = y0 + dt F(y1)
+            y1 = np.append(v / mw * 1e-3, T1)
+            g = y1 - y - dt * O.chem_fex(m, y1)
+            scale = np.abs(y1 - y) + 1e-9 * np.abs(y) + 1e-300
+            assert np.all(np.abs(g[:m.ns]) <= 1e-6 * scale[:m.ns] + 1e-14 * y[:m.ns].sum())
+            assert abs(g[-1]) <= 1e-8 * T1
+            if dt < dt_exp:  # small steps: first order in dt, i.e. explicit Euler
+                euler = (y[:m.ns] + dt * f[:m.ns]) * mw * 1e3
+                assert np.max(np.abs(v - euler)) <= 1e-2 * np.max(np.abs(euler - vrho))
+        # a huge step either converges to a finite positive state or reports
+        # SHCFD_E_CHEM (iteration cap, singular matrix or T <= 0), with the
+        # input left unchanged
+        e, v, it, T1 = O.chem_point_implicit(m, 1600.0, vrho, 1.0)
+        assert e in (0, 8) and 1 <= it <= 32
+        if e == 0:
+            assert np.all(v > 0) and np.all(np.isfinite(v))
+        else:
+            assert np.array_equal(v, vrho)
+
+
+# ---- CHEMEQ2 against a stiff reference integrator ---------------------------------------
+def test_chemeq2_tracks_bdf_solution(no27):
+    scipy_integrate = pytest.importorskip("scipy.integrate")
+    m = OrMech.from_buffer_copy(bytes(no27))
+    m.rtol = 1e-6
+    mw = np.array(m.mw[:m.ns])
+    # H2/air at 1200 K, 1 atm, slightly radical-seeded
+    rel = np.array([0.70, 0.20, 0.028, 1e-4, 1e-4, 1e-4, 1e-5, 1e-3])
+    vrho, rho, E = O.chem_rho_rel2abs(m, 1.01325e5, 1200.0, rel)
+    y0 = np.append(vrho / mw * 1e-3, 1200.0)
+    dt = 2e-7
+    e, v, att = O.chem_reaction_chemeq2(m, 1200.0, vrho, dt)
+    assert e == 0 and 1 <= att <= 10
+    sol = scipy_integrate.solve_ivp(lambda t, y: O.chem_fex(m, y), (0.0, dt), y0, method="BDF", rtol=1e-11,
+                                    atol=1e-22, jac=lambda t, y: O.chem_jex(m, y))
+    ref = sol.y[:m.ns, -1] * mw * 1e3
+    change = np.abs(ref - vrho)
+    moved = change > 1e-3 * vrho
+    # the step is a real reaction interval (radicals change by O(1)) ...
+    assert moved.sum() >= 5 and np.max(change / vrho) > 1.0
+    # ... and CHEMEQ2 (second order, error control on rtol) lands within a
+    # few percent of each species' change
+    assert np.max(np.abs(v - ref)[moved] / change[moved]) < 0.05
+
+
+def test_chemeq2_reports_fallback_when_too_stiff(no27):
+    """More than 10 attempts is the reference's switch to DVODE
+    (sub_chem.f90:1751); this build reports SHCFD_E_CHEM instead.  H2/air
+    ignition at 1200 K over 1 us needs more than 10 CHEMEQ2 attempts."""
+    m = OrMech.from_buffer_copy(bytes(no27))
+    m.rtol = 1e-6
+    rel = np.array([0.70, 0.20, 0.028, 1e-4, 1e-4, 1e-4, 1e-5, 1e-3])
+    vrho, rho, E = O.chem_rho_rel2abs(m, 1.01325e5, 1200.0, rel)
+    e, v, att = O.chem_reaction_chemeq2(m, 1200.0, vrho, 1e-6)
+    assert e == 8 and att == 11
