@@ -125,6 +125,13 @@ int shcfd_mech_parse(const char *chem_inp, const char *mw_inp, const char *trans
 /* make mech the context's mechanism (the reference's module chem state, set
  * up by init_therm); mech->ns must equal nY */
 int shcfd_set_mechanism(shcfd_ctx *ctx, const shcfd_mech *mech);
+/* init_therm of the reactive model (store/therm_lib/chemkin/flow/reactive/
+ * main.part_init.f90 -> read_cheminp, set_trans_data, control_chem.inp
+ * tolerances): shcfd_mech_parse of the three files, then
+ * shcfd_set_mechanism.  For hosts that do not hold the table themselves
+ * (the Fortran driver). */
+int shcfd_load_mechanism(shcfd_ctx *ctx, const char *chem_inp, const char *mw_inp, const char *trans_inp, double rtol,
+                         double atol);
 /* proceed_reaction: store/therm_lib/chemkin/flow/reactive/thermal_model.f90:45-73
  * (reaction_chemeq2 per cell over dt_grbl; or the point-implicit update,
  * params.reaction) */

@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import ctypes
 import dataclasses
+import os
 from typing import List, Optional
 
 import numpy as np
@@ -235,8 +236,10 @@ def forward_step(n_per_unit=320, scheme=SCHEME_DUAL, num_internal_loop=21, cfl=0
 
 # ---- files of the Fortran host driver ---------------------------------------------
 def write_case(case: "Case", path: str, max_step: int, out_period: int = 1, min_rms: float = 1e-14,
-               fast: bool = True) -> None:
-    """case.inp for shared_cfd_amd/main_shcfd (the Fortran main program)."""
+               fast: bool = True, mech_files=None) -> None:
+    """case.inp for shared_cfd_amd/main_shcfd (the Fortran main program).
+    ``mech_files`` = (chem.inp, MW.inp, trans.inp or None) for the Chemkin
+    thermal model."""
     p = case.params
     with open(path, "w") as f:
         f.write("SHCFD-CASE\n")
@@ -248,6 +251,15 @@ def write_case(case: "Case", path: str, max_step: int, out_period: int = 1, min_
         vals = [p.phi_mus, p.k_mus, p.cfl, p.cfl_tau, p.omega_max, p.omega_min, p.dqmax,
                 min(p.res_rate_warning, 1e300), min(p.res_rate_error, 1e300), p.kappa, p.rgas, p.nu]
         f.write(" ".join(repr(float(v)) for v in vals) + "\n")
+        f.write(f"{p.reaction} {int(case.w0 is not None)}\n")
+        if p.thermal == 2:
+            if mech_files is None:
+                raise ValueError("write_case: the Chemkin thermal model needs mech_files")
+            chem_inp, mw_inp, trans_inp = mech_files
+            for fn in (chem_inp, mw_inp, trans_inp or "-"):
+                f.write(os.path.abspath(fn) if fn != "-" else "-")
+                f.write("\n")
+            f.write(f"{case.mech.rtol!r} {case.mech.atol!r}\n")
         f.write(f"{float(max_step)!r} {min_rms!r} {out_period}\n")
         f.write(f"{case.bc_rows.shape[0]} {case.bc_pool.size}\n")
         for r in case.bc_rows:
@@ -263,6 +275,9 @@ def write_case(case: "Case", path: str, max_step: int, out_period: int = 1, min_
                 f.write(" ".join(repr(float(x)) for x in arr.T.ravel()) + "\n")  # i fastest
             qi = case.q0[b][2:nj + 2, 2:ni + 2, :]
             f.write(" ".join(repr(float(x)) for x in qi.ravel()) + "\n")  # component fastest
+            if case.w0 is not None:
+                wi = case.w0[b][2:nj + 2, 2:ni + 2, :]
+                f.write(" ".join(repr(float(x)) for x in wi.ravel()) + "\n")
 
 
 def read_result_bin(path: str, case: "Case"):

@@ -1,6 +1,8 @@
 // shcfd_chem_api.hip -- C ABI of the chemistry: host-evaluated boundary
 // states and the batched 0-D harness on the GPU.
+#include <cstdio>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "../../include/shcfd.h"
@@ -85,6 +87,24 @@ int shcfd_chem_boundary_state(const shcfd_mech* m, double p, double T, const dou
     case 9: return boundary_state<9>(m, p, T, vrho_rel, w, vhi, dhi);
     default: return SHCFD_E_ARG;
     }
+}
+
+int shcfd_load_mechanism(shcfd_ctx* ctx, const char* chem_inp, const char* mw_inp, const char* trans_inp, double rtol,
+                         double atol) {
+    if (!ctx || !chem_inp || !mw_inp) return SHCFD_E_ARG;
+    shcfd_mech* m = new (std::nothrow) shcfd_mech;
+    if (!m) return SHCFD_E_NOMEM;
+    char err[256];
+    int rc = shcfd_mech_parse(chem_inp, mw_inp, (trans_inp && trans_inp[0]) ? trans_inp : nullptr, m, err, 256);
+    if (rc == SHCFD_OK) {
+        m->rtol = rtol;
+        m->atol = atol;
+        rc = shcfd_set_mechanism(ctx, m);
+    } else {
+        std::fprintf(stderr, "shcfd_load_mechanism: %s\n", err);
+    }
+    delete m;
+    return rc;
 }
 
 int shcfd_chem_eval(const shcfd_mech* m, int op, int n, const double* in, double* out, int* status, double dt) {

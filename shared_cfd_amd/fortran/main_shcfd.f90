@@ -15,6 +15,9 @@ program main
    integer, allocatable :: ni(:), nj(:), bc(:), cut(:)
    double precision, allocatable :: pool(:), a(:), nodes(:,:), qin(:), w(:), pool_out(:)
    character(len=32) :: magic
+   character(len=1024) :: chem_inp, mw_inp, trans_inp
+   double precision :: rtol, atol
+   integer :: has_w
    integer(c_int) :: fast
 
    open(10, file="case.inp", status="old", action="read")
@@ -30,7 +33,14 @@ program main
                prm%num_internal_loop, fast
    read(10, *) prm%phi_mus, prm%k_mus, prm%cfl, prm%cfl_tau, prm%omega_max, prm%omega_min, prm%dqmax, &
                prm%res_rate_warning, prm%res_rate_error, prm%kappa, prm%rgas, prm%nu
-   prm%reaction = 0
+   read(10, *) prm%reaction, has_w
+   if (prm%thermal == 2) then
+      read(10, '(a)') chem_inp
+      read(10, '(a)') mw_inp
+      read(10, '(a)') trans_inp
+      if (trim(trans_inp) == "-") trans_inp = ""
+      read(10, *) rtol, atol
+   end if
    read(10, *) max_step, min_RMS, OutPeriod
    read(10, *) nbc, npool
    allocate(bc(6*max(nbc,1)), pool(max(npool,1)))
@@ -77,8 +87,23 @@ program main
          end do
       end do
       call shcfd_check(shcfd_upload_field(ctx, SHCFD_F_Q, int(b-1, c_int), a), "upload q")
+      if (has_w /= 0) then   ! primitive start state (p and R give calc_T its first guess)
+         deallocate(qin, a)
+         allocate(qin(dimw*ni(b)*nj(b)), a(dimw*(ni(b)+4)*(nj(b)+4)))
+         read(10, *) qin
+         a = 0d0
+         do jj = 1, nj(b)
+            do ii = 1, ni(b)
+               do k = 1, dimw
+                  a(k + dimw*((ii+1) + (ni(b)+4)*(jj+1))) = qin(k + dimw*((ii-1) + ni(b)*(jj-1)))
+               end do
+            end do
+         end do
+         call shcfd_check(shcfd_upload_field(ctx, SHCFD_F_W, int(b-1, c_int), a), "upload w")
+      end if
       deallocate(nodes, a, qin)
    end do
+   if (prm%thermal == 2) call init_therm(chem_inp, mw_inp, trans_inp, rtol, atol)
    close(10)
    call shcfd_check(shcfd_set_bc_table(ctx, bc, int(nbc, c_int), pool, int(npool, c_int)), "set_bc_table")
    call shcfd_check(shcfd_set_cut_table(ctx, cut, int(ncut, c_int)), "set_cut_table")
@@ -98,6 +123,18 @@ program main
    do while (check_convergence(i, step_res))
       select case (prm%scheme)
       case (SCHEME_LUSGS)                  ! time_scheme/LU-SGS/2d/main.main.raw.f90
+         if (prm%reaction /= 0) then       ! time_scheme/LU-SGS/2d/main.main.point_implicit.f90
+            call set_w(); call set_thermo_prop(i)
+            call set_dt(i)
+            call proceed_reaction()
+            call copy_qp()
+            call set_w(); call set_thermo_prop(i)
+            call set_BC(i); call set_HV(); call set_TG(); call set_TGv()
+            call set_ABpm()
+            call calc_next_step_implicit(0)
+            i = i + 1
+            cycle
+         end if
          call copy_qp()
          call set_w(); call set_thermo_prop(i)
          call set_dt(i)
@@ -117,9 +154,17 @@ program main
          call set_BC(i); call set_HV(); call set_TG(); call set_TGv()
          call calc_next_step()
       case (SCHEME_RK2)                    ! time_scheme/RK2/main.main.raw.f90
-         call copy_qp()
-         call set_w(); call set_thermo_prop(i)
-         call set_dt(i)
+         if (prm%reaction /= 0) then       ! time_scheme/RK2/main.main.point_implicit.f90
+            call set_w(); call set_thermo_prop(i)
+            call set_dt(i)
+            call proceed_reaction()
+            call copy_qp()
+            call set_w(); call set_thermo_prop(i)
+         else
+            call copy_qp()
+            call set_w(); call set_thermo_prop(i)
+            call set_dt(i)
+         end if
          call set_BC(i); call set_HV(); call set_TG(); call set_TGv()
          call calc_next_step_RK2_first()
          call set_w(); call set_thermo_prop(i)

@@ -144,6 +144,21 @@ module shcfd_f
          type(c_ptr), value :: ctx
          real(c_double), intent(out) :: dt, tt
       end function
+      integer(c_int) function shcfd_load_mechanism(ctx, chem_inp, mw_inp, trans_inp, rtol, atol) &
+            bind(C, name="shcfd_load_mechanism")
+         import :: c_int, c_ptr, c_char, c_double
+         type(c_ptr), value :: ctx
+         character(kind=c_char), intent(in) :: chem_inp(*), mw_inp(*), trans_inp(*)
+         real(c_double), value :: rtol, atol
+      end function
+      integer(c_int) function shcfd_proceed_reaction_c(ctx) bind(C, name="shcfd_proceed_reaction")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function shcfd_check_error(ctx) bind(C, name="shcfd_check_error")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
       integer(c_int) function shcfd_set_option(ctx, option, value) bind(C, name="shcfd_set_option")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
@@ -174,6 +189,10 @@ contains
             print *, "res is NaN (", where, ")"
          case (6)
             print *, "ERROR. EXCEED MAXIMUM RESIDUAL."
+         case (7)
+            print '(a)', "Error: Exceed max_T_loop at temperature determination iteration"
+         case (8)
+            print *, "not converted at CHEMEQ2 (", where, ")"
          case default
             print *, "shcfd error ", rc, " in ", where
          end select
@@ -185,9 +204,19 @@ contains
    subroutine set_w()                       ! store/core/scheme.f90:1-29
       call shcfd_check(shcfd_set_w_c(ctx), "set_w")
    end subroutine
-   subroutine set_thermo_prop(step)         ! store/therm_lib/ideal/thermal_model.raw.f90:7-33
-      integer, intent(in) :: step
+   subroutine set_thermo_prop(step)         ! therm_lib/ideal/thermal_model.raw.f90:7-33,
+      integer, intent(in) :: step           ! therm_lib/chemkin/flow/reactive/thermal_model.f90:13-42
       call shcfd_check(shcfd_set_thermo_prop_c(ctx, int(step, c_int)), "set_thermo_prop")
+   end subroutine
+   subroutine proceed_reaction()            ! therm_lib/chemkin/flow/reactive/thermal_model.f90:45-73
+      call shcfd_check(shcfd_proceed_reaction_c(ctx), "proceed_reaction")
+      call shcfd_check(shcfd_check_error(ctx), "proceed_reaction")
+   end subroutine
+   subroutine init_therm(chem_inp, mw_inp, trans_inp, rtol, atol)   ! reactive main.part_init.f90
+      character(*), intent(in) :: chem_inp, mw_inp, trans_inp
+      double precision, intent(in) :: rtol, atol
+      call shcfd_check(shcfd_load_mechanism(ctx, trim(chem_inp)//c_null_char, trim(mw_inp)//c_null_char, &
+                                            trim(trans_inp)//c_null_char, rtol, atol), "init_therm")
    end subroutine
    subroutine set_BC(step)                  ! store/cond/no-nV/condition.tail.f90
       integer, intent(in) :: step

@@ -28,8 +28,8 @@ EXPORTS = [
     "shcfd_set_abpm", "shcfd_set_dqdt", "shcfd_calc_next_step_implicit", "shcfd_calc_next_step",
     "shcfd_calc_next_step_rk2", "shcfd_copy_state", "shcfd_energy_rms", "shcfd_prepare", "shcfd_step",
     "shcfd_check_error", "shcfd_profile_enable", "shcfd_profile_read", "shcfd_set_option",
-    "shcfd_mech_parse", "shcfd_set_mechanism", "shcfd_proceed_reaction", "shcfd_chem_boundary_state",
-    "shcfd_chem_eval",
+    "shcfd_mech_parse", "shcfd_set_mechanism", "shcfd_load_mechanism", "shcfd_proceed_reaction",
+    "shcfd_chem_boundary_state", "shcfd_chem_eval",
 ]
 OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT = 1, 2
 

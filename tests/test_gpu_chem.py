@@ -91,18 +91,27 @@ def test_chemeq2(mechs):
     lm, om = mechs["no27"]
     om.rtol = lm.rtol = 1e-6
     ns = om.ns
+    # H2/air with radicals at 1e-4 of the mixture, 900-1500 K, over 0.1 us:
+    # CHEMEQ2 needs 4-10 attempts, and a few states exceed 10 (the
+    # reference's DVODE hand-over, reported as SHCFD_E_CHEM)
+    rng = np.random.default_rng(4)
     st_in, ref = [], []
-    for vrho, rho, E, T in mixture_states(om, 200, seed=4, tmin=800.0, tmax=1600.0):
+    for _ in range(200):
+        rel = np.array([0.7, 0.2, 0.03, 1e-4, 1e-4, 1e-4, 1e-4, 0.01]) * rng.uniform(0.5, 1.5, 8)
+        T = rng.uniform(900.0, 1500.0)
+        vrho, rho, E = O.chem_rho_rel2abs(om, 1.01325e5, T, rel)
         st_in.append(np.append(vrho, T))
-        ref.append(O.chem_reaction_chemeq2(om, T, vrho, 5e-8))
-    out, st = chem.chem_eval(lm, chem.CHEMEQ2, np.array(st_in), dt=5e-8)
-    ok = 0
+        ref.append(O.chem_reaction_chemeq2(om, T, vrho, 1e-7))
+    out, st = chem.chem_eval(lm, chem.CHEMEQ2, np.array(st_in), dt=1e-7)
+    ok = fail = 0
     for s, (e, v, att) in enumerate(ref):
         assert st[s] == (-e if e else att)
         if e == 0:
             ok += 1
             assert _rel(out[s], v) <= 1e-13
-    assert ok > 100
+        else:
+            fail += 1
+    assert ok > 150 and fail > 0
 
 
 def _pair(case):

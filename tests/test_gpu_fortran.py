@@ -57,3 +57,25 @@ def test_fortran_driver_matches_oracle(tmp_path, name, fast):
         q_o = o.field("Q", b)[2:nj + 2, 2:ni + 2]
         q_d = blocks[b][1:nj + 1, 1:ni + 1, :case.dimq]
         np.testing.assert_allclose(q_d, q_o, rtol=1e-11, atol=1e-11 * np.max(np.abs(q_o)))
+
+
+def test_fortran_driver_reacting(tmp_path):
+    """Point-implicit LU-SGS with Chemkin thermo (init_therm, proceed_reaction)
+    on a small mixing layer of the no27 mechanism plus H2O2."""
+    from shared_cfd_amd import chem
+
+    from chem_cases import NO27_9, no27_plus_h2o2
+
+    p9 = no27_plus_h2o2(str(tmp_path / "mech"))
+    m = chem.parse(p9)
+    case = C.mixing_layer(m, ni=24, nj=12, seed=2, species=NO27_9, reaction=2, T_ox=1200.0)
+    n = 4
+    C.write_case(case, str(tmp_path / "case.inp"), max_step=n, out_period=1, fast=True,
+                 mech_files=(p9, chem.MW_INP, None))
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "normal end" in r.stdout, r.stdout + r.stderr
+    step, tt, blocks = C.read_result_bin(str(tmp_path / "result.bin"), case)
+    o, _ = _run_oracle(case, n, 1)
+    q_o = o.field("Q", 0)[2:14, 2:26]
+    q_d = blocks[0][1:13, 1:25, :case.dimq]
+    np.testing.assert_allclose(q_d, q_o, rtol=1e-10, atol=1e-10 * np.max(np.abs(q_o)))
