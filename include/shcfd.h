@@ -42,8 +42,12 @@ const char *shcfd_strerror(int code);
 enum {
     SHCFD_OPT_FAST_IMPLICIT = 1, /* 1 (default): optimised LU-SGS sweeps for nY = 1;
                                     0: the per-routine path with stored Jacobians */
-    SHCFD_OPT_SPIN_LIMIT = 2     /* bound on polls of a sweep hand-off before
+    SHCFD_OPT_SPIN_LIMIT = 2,    /* bound on polls of a sweep hand-off before
                                     SHCFD_E_SPIN is raised                          */
+    SHCFD_OPT_SKEW_SWEEP = 3     /* 1 (default): whole steps with nY > 1 keep the
+                                    split Jacobians in the strip-skewed layout that
+                                    the row-per-lane sweeps read coalesced; 0: the
+                                    Ap/Am/Bp/Bm fields                              */
 };
 int shcfd_set_option(shcfd_ctx *ctx, int option, int value);
 int shcfd_version(void);
@@ -136,6 +140,10 @@ int shcfd_load_mechanism(shcfd_ctx *ctx, const char *chem_inp, const char *mw_in
  * (reaction_chemeq2 per cell over dt_grbl; or the point-implicit update,
  * params.reaction) */
 int shcfd_proceed_reaction(shcfd_ctx *ctx);
+/* instrumentation, no counterpart in the reference: Newton iterations
+ * (point-implicit) or CHEMEQ2 attempts summed over all cells and calls since
+ * the last reset (synchronises) */
+int shcfd_chem_iterations(shcfd_ctx *ctx, double *total, int reset);
 /* calc_boundary: thermal_model.f90:326-356 -- rho_rel2abs (sub_chem.f90:1036-1085)
  * and calc_T (:1507-1645) for a fixed state of pressure p [Pa], temperature
  * T [K] and relative composition vrho_rel(ns).  Evaluated on the host.

@@ -43,15 +43,12 @@ __global__ void __launch_bounds__(256) k_thermo_chem(DevCtx d, const Tile* tiles
     }
 }
 
-// proceed_reaction, thermal_model.f90:45-73 (mode 1: reaction_chemeq2;
-// mode 2: point-implicit LU update), over dt_grbl
+// one cell of proceed_reaction; returns the Newton iterations / CHEMEQ2
+// attempts it took
 template <int NS>
-__global__ void __launch_bounds__(256) k_reaction(DevCtx d, const Tile* tiles, const shcfd_mech* __restrict__ m,
-                                                  int mode) {
+__device__ inline int reaction_cell(const DevCtx& d, const DevBlock& B, const TileCoord& t, const shcfd_mech* m,
+                                    int mode) {
     using D = Dims<NS>;
-    TileCoord t = tile_coord(tiles);
-    const DevBlock B = d.blk[t.b];
-    if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) return;
     long long c = cidx(B, t.i, t.j);
     double dt = d.scal[SCAL_DT_GRBL];
     double T = FLD(d, SHCFD_F_W, IW_P, c) / FLD(d, SHCFD_F_W, IW_RHO, c) / FLD(d, SHCFD_F_W, D::ir, c);
@@ -66,10 +63,28 @@ __global__ void __launch_bounds__(256) k_reaction(DevCtx d, const Tile* tiles, c
         e = chem::reaction_chemeq2<NS>(m, T, vrho, dt, att);
     if (e) {
         atomicCAS(d.iflag + IFLAG_ERR, 0, e);
-        return;
+        return att;
     }
 #pragma unroll
     for (int k = 0; k < NS; k++) FLD(d, SHCFD_F_Q, k, c) = vrho[k];
+    return att;
+}
+
+// proceed_reaction, thermal_model.f90:45-73 (mode 1: reaction_chemeq2;
+// mode 2: point-implicit backward Euler), over dt_grbl.  The per-cell
+// iteration counts are summed into SCAL_CHEM_ITERS (one atomic per wave).
+template <int NS>
+__global__ void __launch_bounds__(256) k_reaction(DevCtx d, const Tile* tiles, const shcfd_mech* __restrict__ m,
+                                                  int mode) {
+    TileCoord t = tile_coord(tiles);
+    const DevBlock B = d.blk[t.b];
+    bool inside = !(t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj);
+    int att = 0;
+    if (inside) att = reaction_cell<NS>(d, B, t, m, mode);
+    double w = (double)att;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
+    if ((threadIdx.x & 63) == 0 && w > 0.0) atomicAdd(d.scal + SCAL_CHEM_ITERS, w);
 }
 
 // batched 0-D harness: cell q's input row starts at in + q*ldi, output row at

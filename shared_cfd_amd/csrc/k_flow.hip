@@ -395,24 +395,30 @@ __global__ void __launch_bounds__(256) k_rms_part(DevCtx d, const Tile* tiles, d
 // ---------------------------------------------------------------------------
 // set_ABpm (LU-SGS, preconLU-SGS, dual) and set_dqdt
 // ---------------------------------------------------------------------------
-template <int NY, bool MATS>
-__device__ inline void set_abpm_body(const DevCtx& d, const Tile* tiles) {
+// MATS: 0 = no stored Jacobians, 1 = Ap/Am/Bp/Bm fields (cell SoA),
+// 2 = the strip-skewed copy read by the skewed sweeps (slot given)
+template <int NY, int MATS>
+__device__ inline void set_abpm_cell(const DevCtx& d, const DevBlock& B, int i, int j, double* skew, long long nsl,
+                                     long long slot) {
     using D = Dims<NY>;
     constexpr int dq = D::dq;
-    TileCoord t = tile_coord(tiles);
-    const DevBlock B = d.blk[t.b];
-    const int i = t.i, j = t.j;
-    if (i < 1 || i > B.ni || j < 1 || j > B.nj) return;
     const long long c = cidx(B, i, j), P = B.pitch;
     CellJac<NY> J;
     cell_jacobian<NY>(d, c, P, J);
-    if (MATS)
-    for (int e = 0; e < dq * dq; e++) {
-        FLD(d, SHCFD_F_AP, e, c) = J.Ap[e];
-        FLD(d, SHCFD_F_AM, e, c) = J.Am[e];
-        FLD(d, SHCFD_F_BP, e, c) = J.Bp[e];
-        FLD(d, SHCFD_F_BM, e, c) = J.Bm[e];
-    }
+    if (MATS == 1)
+        for (int e = 0; e < dq * dq; e++) {
+            FLD(d, SHCFD_F_AP, e, c) = J.Ap[e];
+            FLD(d, SHCFD_F_AM, e, c) = J.Am[e];
+            FLD(d, SHCFD_F_BP, e, c) = J.Bp[e];
+            FLD(d, SHCFD_F_BM, e, c) = J.Bm[e];
+        }
+    if (MATS == 2)
+        for (int e = 0; e < dq * dq; e++) {
+            skew[(long long)(SKEW_AP * dq * dq + e) * nsl + slot] = J.Ap[e];
+            skew[(long long)(SKEW_BP * dq * dq + e) * nsl + slot] = J.Bp[e];
+            skew[(long long)(SKEW_AM * dq * dq + e) * nsl + slot] = J.Am[e];
+            skew[(long long)(SKEW_BM * dq * dq + e) * nsl + slot] = J.Bm[e];
+        }
     FLD(d, SHCFD_F_ALPHA, 0, c) = J.alpha;
     if (J.precond) {
         FLD(d, SHCFD_F_PHI, 0, c) = J.phi;
@@ -428,10 +434,37 @@ __device__ inline void set_abpm_body(const DevCtx& d, const Tile* tiles) {
     }
 }
 
+template <int NY, int MATS>
+__device__ inline void set_abpm_body(const DevCtx& d, const Tile* tiles) {
+    TileCoord t = tile_coord(tiles);
+    const DevBlock B = d.blk[t.b];
+    if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) return;
+    set_abpm_cell<NY, MATS>(d, B, t.i, t.j, nullptr, 0, 0);
+}
+
 template <int NY>
-__global__ void __launch_bounds__(256) k_set_abpm(DevCtx d, const Tile* tiles) { set_abpm_body<NY, true>(d, tiles); }
+__global__ void __launch_bounds__(256) k_set_abpm(DevCtx d, const Tile* tiles) { set_abpm_body<NY, 1>(d, tiles); }
 template <int NY>
-__global__ void __launch_bounds__(256) k_set_abpm_scal(DevCtx d, const Tile* tiles) { set_abpm_body<NY, false>(d, tiles); }
+__global__ void __launch_bounds__(256) k_set_abpm_scal(DevCtx d, const Tile* tiles) { set_abpm_body<NY, 0>(d, tiles); }
+
+// set_ABpm writing the split Jacobians in the strip-skewed layout of the
+// row-per-lane sweeps: strip s, sweep step t, lane L (row j0+L) holds cell
+// (i = t-L+1, j0+L) at slot sbase[s] + 64 t + L, so every step of both sweeps
+// reads 64 consecutive slots per matrix element.  One wavefront per step t
+// (4 per workgroup); grid = (strips, ceil((max ni + 63)/4)).
+template <int NY>
+__global__ void __launch_bounds__(256) k_set_abpm_skew(DevCtx d, const Strip* strips, const long long* sbase,
+                                                       double* skew, long long nsl) {
+    const int s = blockIdx.x;
+    const int t = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    const int L = (int)(threadIdx.x & 63);
+    const Strip S = strips[s];
+    const DevBlock B = d.blk[S.b];
+    if (t >= B.ni + 63) return;
+    const int i = t - L + 1, j = S.j0 + L;
+    if (i < 1 || i > B.ni || j > B.nj) return;
+    set_abpm_cell<NY, 2>(d, B, i, j, skew, nsl, sbase[s] + 64LL * t + L);
+}
 
 template <int NY>
 __global__ void __launch_bounds__(256) k_set_dqdt(DevCtx d, const Tile* tiles) {
@@ -500,6 +533,7 @@ __global__ void __launch_bounds__(256) k_copy_q(DevCtx d, const Tile* tiles, int
     template __global__ void k_rms_part<NY>(DevCtx, const Tile*, double*);                               \
     template __global__ void k_set_abpm<NY>(DevCtx, const Tile*);                                        \
     template __global__ void k_set_abpm_scal<NY>(DevCtx, const Tile*);                                   \
+    template __global__ void k_set_abpm_skew<NY>(DevCtx, const Strip*, const long long*, double*, long long); \
     template __global__ void k_set_dqdt<NY>(DevCtx, const Tile*);                                        \
     template __global__ void k_explicit<NY>(DevCtx, const Tile*, int);                                   \
     template __global__ void k_copy_q<NY>(DevCtx, const Tile*, int, int);

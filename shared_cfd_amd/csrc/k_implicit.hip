@@ -72,6 +72,21 @@ __device__ inline void matvec_scaled(const DevCtx& d, int F, long long c, double
     }
 }
 
+// the same product from the strip-skewed Jacobian copy (k_set_abpm_skew):
+// element e of matrix m at skew[(m dq^2 + e) nsl + slot]
+template <int DQN>
+__device__ inline void matvec_skew(const double* __restrict__ skew, long long nsl, int m, long long slot, double scale,
+                                   const double* x, double* y) {
+    const double* base = skew + (long long)m * DQN * DQN * nsl + slot;
+#pragma unroll
+    for (int k = 0; k < DQN; k++) {
+        double s = 0.0;
+#pragma unroll
+        for (int l = 0; l < DQN; l++) s = s + base[(long long)(k + DQN * l) * nsl] * x[l];
+        y[k] = scale * s;
+    }
+}
+
 __device__ inline void set_error(const DevCtx& d, int code) {
     atomicCAS(d.iflag + IFLAG_ERR, 0, code);
 }
@@ -94,9 +109,10 @@ __device__ inline bool wait_progress(const int* prog, int other, int need, int& 
 // ---------------------------------------------------------------------------
 // forward sweep (plain: Jacobians read from the stored Ap/Bp arrays)
 // ---------------------------------------------------------------------------
-template <int NY>
+template <int NY, bool SKEW>
 __global__ void __launch_bounds__(64) k_sweep_fwd_plain(DevCtx d, const Strip* strips, int* prog, double* bnd,
-                                                        long long bstride, long long spin_limit) {
+                                                        long long bstride, long long spin_limit, const double* skew,
+                                                        long long nsl, const long long* sbase) {
     constexpr int dq = NY + 3;
     const int s = blockIdx.x;
     const int lane = threadIdx.x;
@@ -148,8 +164,14 @@ __global__ void __launch_bounds__(64) k_sweep_fwd_plain(DevCtx d, const Strip* s
             }
 #pragma unroll
             for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = x[k];
-            matvec_scaled<dq>(d, SHCFD_F_AP, c, FLD(d, SHCFD_F_DSCI, 0, c), x, a_in);
-            matvec_scaled<dq>(d, SHCFD_F_BP, c, FLD(d, SHCFD_F_DSCJ, 0, c), x, bo);
+            if (SKEW) {
+                const long long slot = sbase[s] + 64LL * t + lane;
+                matvec_skew<dq>(skew, nsl, SKEW_AP, slot, FLD(d, SHCFD_F_DSCI, 0, c), x, a_in);
+                matvec_skew<dq>(skew, nsl, SKEW_BP, slot, FLD(d, SHCFD_F_DSCJ, 0, c), x, bo);
+            } else {
+                matvec_scaled<dq>(d, SHCFD_F_AP, c, FLD(d, SHCFD_F_DSCI, 0, c), x, a_in);
+                matvec_scaled<dq>(d, SHCFD_F_BP, c, FLD(d, SHCFD_F_DSCJ, 0, c), x, bo);
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < dq; k++) bo[k] = 0.0;
@@ -175,10 +197,11 @@ __global__ void __launch_bounds__(64) k_sweep_fwd_plain(DevCtx d, const Strip* s
 // preconditioned LU-SGS the update q += Δq is applied here; for dual time the
 // per-strip (res, omega) partials are written and k_update applies q += ωΔq.
 // ---------------------------------------------------------------------------
-template <int NY>
+template <int NY, bool SKEW>
 __global__ void __launch_bounds__(64) k_sweep_bwd_plain(DevCtx d, const Strip* strips, int nstrips, int* prog,
                                                         double* bnd, long long bstride, double* part,
-                                                        long long spin_limit) {
+                                                        long long spin_limit, const double* skew, long long nsl,
+                                                        const long long* sbase) {
     constexpr int dq = NY + 3;
     const int s = nstrips - 1 - (int)blockIdx.x;
     const int lane = threadIdx.x;
@@ -252,8 +275,15 @@ __global__ void __launch_bounds__(64) k_sweep_bwd_plain(DevCtx d, const Strip* s
             }
 #pragma unroll
             for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = x[k];
-            matvec_scaled<dq>(d, SHCFD_F_AM, c, FLD(d, SHCFD_F_DSCI, 0, c), x, a_in);
-            matvec_scaled<dq>(d, SHCFD_F_BM, c, FLD(d, SHCFD_F_DSCJ, 0, c), x, bo);
+            if (SKEW) {
+                // cell (i, j0+lane) sits at forward step i + lane - 1 = ni + 62 - t
+                const long long slot = sbase[s] + 64LL * (ni + 62 - t) + lane;
+                matvec_skew<dq>(skew, nsl, SKEW_AM, slot, FLD(d, SHCFD_F_DSCI, 0, c), x, a_in);
+                matvec_skew<dq>(skew, nsl, SKEW_BM, slot, FLD(d, SHCFD_F_DSCJ, 0, c), x, bo);
+            } else {
+                matvec_scaled<dq>(d, SHCFD_F_AM, c, FLD(d, SHCFD_F_DSCI, 0, c), x, a_in);
+                matvec_scaled<dq>(d, SHCFD_F_BM, c, FLD(d, SHCFD_F_DSCJ, 0, c), x, bo);
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < dq; k++) bo[k] = 0.0;
@@ -325,8 +355,14 @@ __global__ void __launch_bounds__(256) k_update(DevCtx d, const Tile* tiles) {
 
 #define INST(NY)                                                                                               \
     template __global__ void k_rhs<NY>(DevCtx, const Tile*, int*, int);                                        \
-    template __global__ void k_sweep_fwd_plain<NY>(DevCtx, const Strip*, int*, double*, long long, long long); \
-    template __global__ void k_sweep_bwd_plain<NY>(DevCtx, const Strip*, int, int*, double*, long long, double*, long long); \
+    template __global__ void k_sweep_fwd_plain<NY, false>(DevCtx, const Strip*, int*, double*, long long, long long, \
+                                                          const double*, long long, const long long*);          \
+    template __global__ void k_sweep_fwd_plain<NY, true>(DevCtx, const Strip*, int*, double*, long long, long long,  \
+                                                         const double*, long long, const long long*);           \
+    template __global__ void k_sweep_bwd_plain<NY, false>(DevCtx, const Strip*, int, int*, double*, long long, double*, \
+                                                          long long, const double*, long long, const long long*); \
+    template __global__ void k_sweep_bwd_plain<NY, true>(DevCtx, const Strip*, int, int*, double*, long long, double*, \
+                                                         long long, const double*, long long, const long long*);  \
     template __global__ void k_update<NY>(DevCtx, const Tile*);
 SHCFD_FOR_EACH_NY(INST)
 #undef INST

@@ -173,7 +173,7 @@ int shcfd_destroy(shcfd_ctx* ctx) {
         for (auto& e : ctx->prof_pending[k]) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
     for (int fld = 0; fld < SHCFD_F_COUNT; fld++) if (ctx->f[fld]) hipFree(ctx->f[fld]);
     void* ptrs[] = {ctx->dblk, ctx->dtiles, ctx->dstrips, ctx->dprog, ctx->dbnd, ctx->dpart, ctx->scal,
-                    ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool, ctx->dmech};
+                    ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool, ctx->dmech, ctx->dskew, ctx->dsbase};
     for (void* p : ptrs) if (p) hipFree(p);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -471,6 +471,18 @@ int shcfd_proceed_reaction(shcfd_ctx* ctx) {
     return SHCFD_OK;
 }
 
+int shcfd_chem_iterations(shcfd_ctx* ctx, double* total, int reset) {
+    if (!ctx || !total) return SHCFD_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(total, ctx->scal + SCAL_CHEM_ITERS, sizeof(double), hipMemcpyDeviceToHost));
+    if (reset) {
+        hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(1), 0, ctx->stream, ctx->dev(), (int)SCAL_CHEM_ITERS, 0.0);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return SHCFD_OK;
+}
+
 int shcfd_set_bc(shcfd_ctx* ctx, int step) {
     (void)step;
     if (!ctx) return SHCFD_E_ARG;
@@ -550,13 +562,54 @@ int shcfd_set_abpm(shcfd_ctx* ctx) {
     ProfScope ps(ctx, PROF_ABPM);
     DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_abpm<NY>));
     LAUNCHCHK();
+    ctx->skew_active = 0;
     return SHCFD_OK;
 }
 
-// set_ABpm without the stored Jacobians (alpha, pre-conditioner vectors and
-// scalars, local dt): what the optimised sweeps need from it
+// the strip-skewed Jacobian copy is used inside whole steps for nY > 1
+// (where the optimised dimq = 4 sweeps do not apply); allocated on first use
+static bool skew_enabled(shcfd_ctx* ctx) {
+    if (!ctx->use_skew || ctx->ny == 1) return false;
+    if (ctx->dskew) return true;
+    const int ns = (int)ctx->hstrips.size();
+    std::vector<long long> sb(ns);
+    long long nsl = 0;
+    for (int s = 0; s < ns; s++) {
+        sb[s] = nsl;
+        nsl += 64LL * (ctx->hblk[ctx->hstrips[s].b].ni + 63);
+    }
+    const long long dq = ctx->dimq;
+    if (hipMalloc((void**)&ctx->dskew, sizeof(double) * 4 * dq * dq * nsl) != hipSuccess ||
+        hipMalloc((void**)&ctx->dsbase, sizeof(long long) * ns) != hipSuccess ||
+        hipMemcpy(ctx->dsbase, sb.data(), sizeof(long long) * ns, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(ctx->dskew);
+        hipFree(ctx->dsbase);
+        ctx->dskew = nullptr;
+        ctx->dsbase = nullptr;
+        ctx->use_skew = 0;
+        (void)hipGetLastError();
+        return false;
+    }
+    ctx->skew_nsl = nsl;
+    return true;
+}
+
+// set_ABpm inside a whole step: without stored Jacobians when the optimised
+// sweeps run (they need alpha, the pre-conditioner vectors/scalars and the
+// local dt), with the skewed copy for the row-per-lane sweeps at nY > 1
 static int abpm_for_step(shcfd_ctx* ctx) {
-    if (!fast_enabled(ctx)) return shcfd_set_abpm(ctx);
+    if (!fast_enabled(ctx)) {
+        if (!skew_enabled(ctx)) return shcfd_set_abpm(ctx);
+        ProfScope ps(ctx, PROF_ABPM);
+        int maxni = 0;
+        for (const auto& B : ctx->hblk) maxni = std::max(maxni, B.ni);
+        dim3 grid((unsigned)ctx->hstrips.size(), (unsigned)((maxni + 63 + 3) / 4));
+        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_set_abpm_skew<NY>, grid, dim3(256), 0, ctx->stream, ctx->dev(),
+                                                ctx->dstrips, ctx->dsbase, ctx->dskew, ctx->skew_nsl));
+        LAUNCHCHK();
+        ctx->skew_active = 1;
+        return SHCFD_OK;
+    }
     ProfScope ps(ctx, PROF_ABPM);
     DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_abpm_scal<NY>));
     LAUNCHCHK();
@@ -616,13 +669,26 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
         LAUNCHCHK();
         return rc;
     }
+    const bool sk = ctx->skew_active && ctx->dskew;
+    ctx->skew_active = 0;
     { ProfScope ps(ctx, PROF_SWEEP_FWD);
-    DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep_fwd_plain<NY>, dim3(ns), dim3(64), 0, ctx->stream, d,
-                                            ctx->dstrips, prog_f, ctx->dbnd, ctx->bnd_stride, ctx->spin_limit)); }
+    if (sk)
+        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL((k_sweep_fwd_plain<NY, true>), dim3(ns), dim3(64), 0, ctx->stream, d,
+                                                ctx->dstrips, prog_f, ctx->dbnd, ctx->bnd_stride, ctx->spin_limit,
+                                                ctx->dskew, ctx->skew_nsl, ctx->dsbase))
+    else
+        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL((k_sweep_fwd_plain<NY, false>), dim3(ns), dim3(64), 0, ctx->stream, d,
+                                                ctx->dstrips, prog_f, ctx->dbnd, ctx->bnd_stride, ctx->spin_limit,
+                                                nullptr, 0LL, nullptr)) }
     { ProfScope ps(ctx, PROF_SWEEP_BWD);
-    DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep_bwd_plain<NY>, dim3(ns), dim3(64), 0, ctx->stream, d,
-                                            ctx->dstrips, ns, prog_b, ctx->dbnd, ctx->bnd_stride, ctx->dpart,
-                                            ctx->spin_limit)); }
+    if (sk)
+        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL((k_sweep_bwd_plain<NY, true>), dim3(ns), dim3(64), 0, ctx->stream, d,
+                                                ctx->dstrips, ns, prog_b, ctx->dbnd, ctx->bnd_stride, ctx->dpart,
+                                                ctx->spin_limit, ctx->dskew, ctx->skew_nsl, ctx->dsbase))
+    else
+        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL((k_sweep_bwd_plain<NY, false>), dim3(ns), dim3(64), 0, ctx->stream, d,
+                                                ctx->dstrips, ns, prog_b, ctx->dbnd, ctx->bnd_stride, ctx->dpart,
+                                                ctx->spin_limit, nullptr, 0LL, nullptr)) }
     if (ctx->p.scheme == SHCFD_SCHEME_DUAL) {
         { ProfScope ps(ctx, PROF_REDUCE);
         hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, ctx->stream, d, ctx->dpart, ns); }
@@ -644,6 +710,9 @@ int shcfd_set_option(shcfd_ctx* ctx, int option, int value) {
     case SHCFD_OPT_SPIN_LIMIT:
         if (value < 1) return SHCFD_E_ARG;
         ctx->spin_limit = value;
+        return SHCFD_OK;
+    case SHCFD_OPT_SKEW_SWEEP:
+        ctx->use_skew = value ? 1 : 0;
         return SHCFD_OK;
     default:
         return SHCFD_E_ARG;

@@ -42,6 +42,7 @@ enum {
     SCAL_RES1 = 4,
     SCAL_RMS = 5,
     SCAL_FIRST_DUAL = 6,
+    SCAL_CHEM_ITERS = 7,  // Newton iterations / CHEMEQ2 attempts summed over cells
     SCAL_COUNT = 16
 };
 enum { IFLAG_ERR = 0, IFLAG_COUNT = 8 };
@@ -137,6 +138,12 @@ struct Ctx {
     // Chemkin mechanism (module chem of the reference), device copy
     shcfd_mech* dmech = nullptr;
     int has_mech = 0;
+    // strip-skewed Jacobian copy for the row-per-lane sweeps (nY > 1 steps)
+    int use_skew = 1;
+    int skew_active = 0;           // the last set_ABpm of a step wrote the skewed copy
+    double* dskew = nullptr;
+    long long skew_nsl = 0;        // slots: sum over strips of 64 (ni + 63)
+    long long* dsbase = nullptr;   // first slot of each strip
 
     DevCtx dev() const {
         DevCtx d{};

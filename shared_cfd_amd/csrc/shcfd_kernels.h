@@ -399,6 +399,9 @@ template <int NY> __global__ void k_set_dt(DevCtx, const Tile*, double*);
 template <int NY> __global__ void k_rms_part(DevCtx, const Tile*, double*);
 template <int NY> __global__ void k_set_abpm(DevCtx, const Tile*);
 template <int NY> __global__ void k_set_abpm_scal(DevCtx, const Tile*);
+// matrices of the strip-skewed Jacobian copy, in this order
+enum { SKEW_AP = 0, SKEW_BP = 1, SKEW_AM = 2, SKEW_BM = 3 };
+template <int NY> __global__ void k_set_abpm_skew(DevCtx, const Strip*, const long long*, double*, long long);
 template <int NY> __global__ void k_set_dqdt(DevCtx, const Tile*);
 template <int NY> __global__ void k_explicit(DevCtx, const Tile*, int);
 template <int NY> __global__ void k_copy_q(DevCtx, const Tile*, int, int);
@@ -407,8 +410,12 @@ __global__ void k_reduce_final(DevCtx, const double*, int, int, int);
 
 // implicit update (k_implicit.hip)
 template <int NY> __global__ void k_rhs(DevCtx, const Tile*, int*, int);
-template <int NY> __global__ void k_sweep_fwd_plain(DevCtx, const Strip*, int*, double*, long long, long long);
-template <int NY> __global__ void k_sweep_bwd_plain(DevCtx, const Strip*, int, int*, double*, long long, double*, long long);
+template <int NY, bool SKEW>
+__global__ void k_sweep_fwd_plain(DevCtx, const Strip*, int*, double*, long long, long long, const double*, long long,
+                                  const long long*);
+template <int NY, bool SKEW>
+__global__ void k_sweep_bwd_plain(DevCtx, const Strip*, int, int*, double*, long long, double*, long long,
+                                  const double*, long long, const long long*);
 template <int NY> __global__ void k_update(DevCtx, const Tile*);
 __global__ void k_reduce_res_omega(DevCtx, const double*, int);
 __global__ void k_copy_scal(DevCtx, int, int);

@@ -29,9 +29,9 @@ EXPORTS = [
     "shcfd_calc_next_step_rk2", "shcfd_copy_state", "shcfd_energy_rms", "shcfd_prepare", "shcfd_step",
     "shcfd_check_error", "shcfd_profile_enable", "shcfd_profile_read", "shcfd_set_option",
     "shcfd_mech_parse", "shcfd_set_mechanism", "shcfd_load_mechanism", "shcfd_proceed_reaction",
-    "shcfd_chem_boundary_state", "shcfd_chem_eval",
+    "shcfd_chem_boundary_state", "shcfd_chem_eval", "shcfd_chem_iterations",
 ]
-OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT = 1, 2
+OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT, OPT_SKEW_SWEEP = 1, 2, 3
 
 # kernel classes of shcfd_profile_read (include/shcfd.h)
 KCLASS = ["PRIMITIVE", "BC", "HV", "TG", "TGV", "DT", "ABPM", "DQDT", "RHS", "SWEEP_FWD", "SWEEP_BWD",
@@ -177,6 +177,13 @@ class Solver:
     def proceed_reaction(self):
         check(self.L.shcfd_proceed_reaction(self.ctx), "proceed_reaction")
         check(self.L.shcfd_check_error(self.ctx), "proceed_reaction")
+
+    def chem_iterations(self, reset=True):
+        """Newton iterations / CHEMEQ2 attempts summed over cells since the last reset."""
+        v = ctypes.c_double(0.0)
+        self.L.shcfd_chem_iterations.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        check(self.L.shcfd_chem_iterations(self.ctx, ctypes.byref(v), int(reset)), "chem_iterations")
+        return v.value
 
     def set_HV(self):
         check(self.L.shcfd_set_hv(self.ctx), "set_HV")
