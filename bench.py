@@ -187,8 +187,15 @@ def main(argv=None):
     avg_s = ms_dom / n_dom / 1e3
     bytes_launch = alg_bytes_per_cell(dominant) * cells
     achieved = bytes_launch / avg_s / 1e9
+    # measured HBM bytes per launch of the dominant kernel (rocprofv3 --pmc
+    # FETCH_SIZE / WRITE_SIZE runs on this workload, copy-calibrated)
+    traffic = None
+    tr_path = os.path.join(ROOT, "profiles", "forward_step_258k_traffic.json")
+    if a.n_per_unit == 320 and os.path.exists(tr_path):
+        with open(tr_path) as f:
+            traffic = json.load(f).get(dominant)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dominant,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dominant,
                 "avg_launch_us": round(avg_s * 1e6, 3),
                 "alg_bytes_per_launch": bytes_launch,
                 "share_of_step": {k: round(v[0] / sum(x[0] for x in prof_all.values()), 4) for k, v in prof_all.items()}}
