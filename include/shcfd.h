@@ -44,10 +44,12 @@ enum {
                                     0: the per-routine path with stored Jacobians */
     SHCFD_OPT_SPIN_LIMIT = 2,    /* bound on polls of a sweep hand-off before
                                     SHCFD_E_SPIN is raised                          */
-    SHCFD_OPT_SKEW_SWEEP = 3     /* 1 (default): whole steps with nY > 1 keep the
+    SHCFD_OPT_SKEW_SWEEP = 3,    /* 1 (default): whole steps with nY > 1 keep the
                                     split Jacobians in the strip-skewed layout that
                                     the row-per-lane sweeps read coalesced; 0: the
                                     Ap/Am/Bp/Bm fields                              */
+    SHCFD_OPT_TILE_ROWS = 4      /* rows of the 64-wide cell tiles of the per-cell
+                                    kernels: 1, 2 or 4 (default)                    */
 };
 int shcfd_set_option(shcfd_ctx *ctx, int option, int value);
 int shcfd_version(void);
@@ -163,6 +165,29 @@ int shcfd_chem_boundary_state(const shcfd_mech *mech, double p, double T, const 
  * Synchronous; allocates and frees its own device buffers. */
 int shcfd_chem_eval(const shcfd_mech *mech, int op, int nstates, const double *in, double *out, int *status,
                     double dt);
+
+/* ---- batches of independent blocks (no counterpart in the reference, whose
+ * kinematic viscosity is one compile-time constant, thermal_model.raw.f90:4,
+ * and whose run stops for all blocks at once) ----------------------------- */
+/* per-block kinematic viscosity of the ideal gas (mu = rho nu); shcfd_create
+ * sets every block to params->nu */
+int shcfd_set_block_nu(shcfd_ctx *ctx, const double *nu);
+/* per-block flag: 0 freezes a block (its implicit update leaves q unchanged) */
+int shcfd_set_block_active(shcfd_ctx *ctx, const int *active);
+/* the energy-change RMS of check_convergence (check_convergence.f90:17-28)
+ * for each block separately */
+int shcfd_energy_rms_blocks(shcfd_ctx *ctx, double *rms);
+/* nsteps physical steps (numbered from first_step) of a scheme other than
+ * dual time with no host synchronisation between steps; graph_steps > 0
+ * captures that many steps into one HIP graph and replays it, 0 launches
+ * eagerly, < 0 uses the schedule chosen by shcfd_autotune.  Dual time runs
+ * step by step through shcfd_step. */
+int shcfd_iterate(shcfd_ctx *ctx, int first_step, int nsteps, int graph_steps);
+/* times candidate schedules (optimised or per-routine sweeps, tile rows of
+ * the per-cell kernels, eager or graph launches) over trial_steps steps each
+ * and keeps the fastest; the flow state is restored afterwards.
+ * choice[4] (may be NULL) = fast sweeps, tile rows, graph steps, ms/step. */
+int shcfd_autotune(shcfd_ctx *ctx, int trial_steps, double *choice);
 
 /* ---- per-kernel timing (no counterpart in the reference, whose timers
  * are commented-out stubs: store/debug/time_barrier.f90:1-37) ------------- */

@@ -62,6 +62,7 @@ def lib():
         L.or_omp_threads.restype = ctypes.c_int
         mp = ctypes.c_void_p
         L.or_set_mechanism.argtypes = [vp, mp]
+        L.or_set_block_nu.argtypes = [vp, dp]
         L.or_chem_error.argtypes = [vp]
         L.or_proceed_reaction.argtypes = [vp]
         L.or_chem_fex.argtypes = [mp, dp, dp]
@@ -135,6 +136,9 @@ class Oracle:
         if getattr(case, "w0", None) is not None:
             for b in range(case.nblocks):
                 self.field("W", b)[...] = case.w0[b]
+        if getattr(case, "block_nu", None) is not None:
+            self._bnu = np.ascontiguousarray(case.block_nu, dtype=np.float64)
+            L.or_set_block_nu(self.ctx, _dp(self._bnu))
 
     def __del__(self):
         try:

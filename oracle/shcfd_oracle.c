@@ -37,6 +37,7 @@ struct or_ctx {
     int first_dual;
     double res1;
     double *scratch; /* per-cell res contributions, max block size */
+    double *bnu;     /* kinematic viscosity per block (default: params nu) */
     shcfd_mech mech; /* Chemkin tables (thermal = SHCFD_THERMAL_CHEMKIN) */
     int has_mech;
     int chem_err;    /* first chemistry failure since the step began (or_step clears it) */
@@ -113,6 +114,8 @@ or_ctx *or_create(int nb, const int *ni, const int *nj, int ny, const shcfd_para
     }
     c->scratch = (double *)calloc(maxcells, sizeof(double));
     if (p) c->p = *p;
+    c->bnu = (double *)malloc(sizeof(double) * nb);
+    for (int b = 0; b < nb; b++) c->bnu[b] = c->p.nu;
     c->first_dual = 1;
     return c;
 }
@@ -123,11 +126,17 @@ void or_destroy(or_ctx *c) {
         for (int b = 0; b < c->nb; b++) free(c->f[fld][b]);
         free(c->f[fld]);
     }
-    free(c->ni); free(c->nj); free(c->cut); free(c->bc); free(c->pool); free(c->scratch);
+    free(c->ni); free(c->nj); free(c->cut); free(c->bc); free(c->pool); free(c->scratch); free(c->bnu);
     free(c);
 }
 
 int or_set_params(or_ctx *c, const shcfd_params *p) { c->p = *p; return 0; }
+/* per-block kinematic viscosity: an extension for batches of independent
+ * cases (the reference has one compile-time NU, thermal_model.raw.f90:4) */
+int or_set_block_nu(or_ctx *c, const double *nu) {
+    for (int b = 0; b < c->nb; b++) c->bnu[b] = nu[b];
+    return 0;
+}
 double *or_field(or_ctx *c, int fld, int b) {
     if (fld < 0 || fld >= SHCFD_F_COUNT || b < 0 || b >= c->nb) return NULL;
     return c->f[fld][b];
@@ -408,9 +417,10 @@ void or_set_thermo_prop(or_ctx *c) {
     }
     /* therm_lib/ideal/thermal_model.raw.f90:7-33 */
     int ny = c->ny;
-    double kap = c->p.kappa, rg = c->p.rgas, nu = c->p.nu;
+    double kap = c->p.kappa, rg = c->p.rgas;
     for (int b = 0; b < c->nb; b++) {
         int ni = c->ni[b], nj = c->nj[b];
+        double nu = c->bnu[b];
 #pragma omp parallel for
         for (int j = 1; j <= nj; j++)
             for (int i = 1; i <= ni; i++) {
@@ -549,7 +559,7 @@ static void bc_segment(or_ctx *c, const int *row) {
             W(IW_G(c), ig1, jg1) = kap;
             W(IW_H(c), ig1, jg1) = ((kap / (kap - 1.0)) * rg) * T + 0.5 * ((u * u) + (v * v));
             W(IW_R(c), ig1, jg1) = rg;
-            W(IW_MU(c), ig1, jg1) = c->p.nu * rho;
+            W(IW_MU(c), ig1, jg1) = c->bnu[b] * rho;
             for (int k = 0; k < c->ny; k++) VHI(k, ig1, jg1) = ((kap / (kap - 1.0)) * rg) * T;
             break;
         }

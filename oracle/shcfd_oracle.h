@@ -27,6 +27,7 @@ typedef struct or_ctx or_ctx;
 or_ctx *or_create(int nb, const int *ni, const int *nj, int ny, const shcfd_params *p);
 void or_destroy(or_ctx *c);
 int or_set_params(or_ctx *c, const shcfd_params *p);
+int or_set_block_nu(or_ctx *c, const double *nu);
 /* pointer to the AoS array of one field/block (size or_field_size doubles) */
 double *or_field(or_ctx *c, int field, int block);
 int or_field_ncomp(or_ctx *c, int field);

@@ -106,6 +106,7 @@ class Case:
     meta: dict = dataclasses.field(default_factory=dict)
     mech: Optional[object] = None           # chem.Mech for the Chemkin thermal model
     w0: Optional[List[np.ndarray]] = None   # initial w per block (T start of calc_T)
+    block_nu: Optional[List[float]] = None  # per-block kinematic viscosity (batches)
 
     @property
     def nblocks(self) -> int:
@@ -509,3 +510,71 @@ def mixing_layer(mech, ni=1024, nj=512, seed=0, scheme=SCHEME_LUSGS, reaction=2,
                 description=f"reacting mixing layer {ni}x{nj}, {ns} species / {mech.nr} reactions, "
                             f"fuel {T_fuel:g} K / oxidiser {T_ox:g} K, p {p0:g} Pa",
                 mech=mech, w0=[w0])
+
+
+# ---- batch of independent laminar plate / channel blocks -------------------------
+def batched_plates(n=256, seed=0, ni_range=(64, 512), nj_range=(32, 256), mach_range=(0.05, 5.0),
+                   re_range=(1e3, 1e6), cfl=1.0, order=2, length=1.0, height=0.25) -> Case:
+    """``n`` independent laminar flat-plate or channel blocks (one block each,
+    no cuts), sizes ni x nj drawn uniformly from the ranges, Mach uniform and
+    Reynolds number log-uniform in the ranges (Re = U L / nu, so the
+    kinematic viscosity is set per block).  Air at rho = 1.2, p = 1e5, ideal
+    gas (gamma 1.4, R 287).  i-min: fixed free stream; i-max: zero gradient;
+    j-min: no-slip wall; j-max: no-slip wall (channel) or zero gradient
+    (plate), chosen by seed.  Uniform free stream as the initial state.
+    Preconditioned LU-SGS with local time steps, viscous."""
+    rng = np.random.default_rng(seed)
+    kappa, rgas = 1.4, 287.0
+    rho0, p0 = 1.2, 1e5
+    c0 = float(np.sqrt(kappa * p0 / rho0))
+    nis, njs, xhs, rhs, qs, nus = [], [], [], [], [], []
+    rows, pool = [], []
+    for b in range(n):
+        ni = int(rng.integers(ni_range[0], ni_range[1] + 1))
+        nj = int(rng.integers(nj_range[0], nj_range[1] + 1))
+        mach = float(rng.uniform(*mach_range))
+        re = float(10.0 ** rng.uniform(np.log10(re_range[0]), np.log10(re_range[1])))
+        channel = bool(rng.integers(0, 2))
+        u0 = mach * c0
+        nu = u0 * length / re
+        xh, rh = rect_nodes(ni, nj, 0.0, length, 0.0, height)
+        qcell = np.broadcast_to(cons_from_prim(np.array(rho0), np.array(u0), np.array(0.0), np.array(p0), kappa),
+                                (nj, ni, 4)).copy()
+        off = len(pool)
+        pool.extend(fixed_state_pool(rho0, u0, 0.0, p0, kappa, rgas, mu=rho0 * nu))
+        rows += [[b + 1, SIDE_IMIN, 1, nj, BC_FIXED, off], [b + 1, SIDE_IMAX, 1, nj, BC_EXTRAP, 0],
+                 [b + 1, SIDE_JMIN, 1, ni, BC_NOSLIP, 0],
+                 [b + 1, SIDE_JMAX, 1, ni, BC_NOSLIP if channel else BC_EXTRAP, 0]]
+        nis.append(ni)
+        njs.append(nj)
+        xhs.append(xh)
+        rhs.append(rh)
+        qs.append(_fill_q(ni, nj, qcell))
+        nus.append(nu)
+    k_mus = -1.0 if order == 2 else 1.0 / 3.0
+    params = make_params(scheme=SCHEME_PRECON_LUSGS, time_step=1, high_order=1, limiter=LIMITER_REFERENCE,
+                         viscous=1, cfl=cfl, k_mus=k_mus, kappa=kappa, rgas=rgas, nu=nus[0])
+    cells = int(sum(a * b for a, b in zip(nis, njs)))
+    return Case("batched_plates", nis, njs, 1, params, xhs, rhs, qs, np.array(rows, dtype=np.int32),
+                np.array(pool, dtype=np.float64), np.zeros((0, 11), dtype=np.int32),
+                description=f"{n} independent laminar plate/channel blocks, {cells} cells, "
+                            f"Mach {mach_range[0]:g}-{mach_range[1]:g}, Re {re_range[0]:g}-{re_range[1]:g}, "
+                            f"preconditioned LU-SGS, local dt, viscous",
+                block_nu=nus, meta={"seed": seed})
+
+
+def sub_case(case: Case, blocks) -> Case:
+    """The blocks ``blocks`` of a batch of independent blocks as a case of
+    their own (BC rows renumbered, pool shared)."""
+    blocks = list(blocks)
+    rows = []
+    for r in case.bc_rows:
+        if int(r[0]) - 1 in blocks:
+            rr = [int(x) for x in r]
+            rr[0] = blocks.index(int(r[0]) - 1) + 1
+            rows.append(rr)
+    return Case(case.name + "_sub", [case.ni[b] for b in blocks], [case.nj[b] for b in blocks], case.ny,
+                case.params, [case.xh[b] for b in blocks], [case.rh[b] for b in blocks],
+                [case.q0[b] for b in blocks], np.array(rows, dtype=np.int32), case.bc_pool,
+                np.zeros((0, 11), dtype=np.int32), description=case.description,
+                block_nu=None if case.block_nu is None else [case.block_nu[b] for b in blocks])

@@ -434,7 +434,7 @@ __global__ void __launch_bounds__(256) k_fast_post(DevCtx d, const QStrip* strip
             for (int k = 0; k < 4; k++) {
                 x[k] = al * z[k];
                 FLD(d, SHCFD_F_DQ, k, c) = x[k];
-                FLD(d, SHCFD_F_Q, k, c) = FLD(d, SHCFD_F_Q, k, c) + x[k];
+                if (B.active) FLD(d, SHCFD_F_Q, k, c) = FLD(d, SHCFD_F_Q, k, c) + x[k];
             }
         }
     }
@@ -543,7 +543,7 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
     if (dual) {
         hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, st, d, part, nwork);
         if (step_internal == 1) hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, st, d, (int)SCAL_RES1, (int)SCAL_RES);
-        hipLaunchKernelGGL(k_update<1>, dim3((unsigned)c.htiles.size()), dim3(256), 0, st, d, c.dtiles);
+        hipLaunchKernelGGL(k_update<1>, dim3((unsigned)c.htiles.size()), dim3(64 * c.tile_rows), 0, st, d, c.dtiles);
     }
     pe(&c, PROF_UPDATE);
     return hipGetLastError() == hipSuccess ? SHCFD_OK : SHCFD_E_HIP;

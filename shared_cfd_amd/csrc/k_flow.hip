@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(256) k_thermo_ideal(DevCtx d, const Tile* tile
     const DevBlock B = d.blk[t.b];
     if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) return;
     long long c = cidx(B, t.i, t.j);
-    const double kap = d.p.kappa, rg = d.p.rgas, nu = d.p.nu;
+    const double kap = d.p.kappa, rg = d.p.rgas, nu = B.nu;
     double rho = FLD(d, SHCFD_F_W, IW_RHO, c), u = FLD(d, SHCFD_F_W, IW_U, c), v = FLD(d, SHCFD_F_W, IW_V, c);
     double qe = FLD(d, SHCFD_F_Q, NY + 2, c);
     double ei = qe / rho - 0.5 * ((u * u) + (v * v));
@@ -131,7 +131,7 @@ __global__ void k_bc_side(DevCtx d, const int* bc, int nbc, const double* pool, 
         wv[D::ig] = kap;
         wv[D::ih] = ((kap / (kap - 1.0)) * rg) * T + 0.5 * ((u * u) + (v * v));
         wv[D::ir] = rg;
-        wv[D::imu] = d.p.nu * rho;
+        wv[D::imu] = B.nu * rho;
 #pragma unroll
         for (int k = 0; k < NY; k++) vh[k] = ((kap / (kap - 1.0)) * rg) * T;
         break;

@@ -270,8 +270,9 @@ __global__ void __launch_bounds__(64) k_sweep_bwd_plain(DevCtx d, const Strip* s
                 for (int k = 0; k < NY; k++)
                     if (x[k] < 0.0) omega = dmin(omega, (-FLD(d, SHCFD_F_Q, k, c) / x[k]) * dqmax);
             } else {
+                if (B.active)
 #pragma unroll
-                for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_Q, k, c) = FLD(d, SHCFD_F_Q, k, c) + x[k];
+                    for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_Q, k, c) = FLD(d, SHCFD_F_Q, k, c) + x[k];
             }
 #pragma unroll
             for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = x[k];
@@ -346,7 +347,7 @@ template <int NY>
 __global__ void __launch_bounds__(256) k_update(DevCtx d, const Tile* tiles) {
     TileCoord t = tile_coord(tiles);
     const DevBlock B = d.blk[t.b];
-    if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) return;
+    if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj || !B.active) return;
     const long long c = cidx(B, t.i, t.j);
     const double om = dmin(d.p.omega_max, d.scal[SCAL_OMEGA]);
 #pragma unroll

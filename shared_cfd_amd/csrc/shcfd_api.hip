@@ -71,6 +71,24 @@ int field_ncomp(int fld, int ny) {
 
 static bool ny_supported(int ny) { return ny == 1 || ny == 2 || ny == 9; }
 
+static void drop_graph(shcfd::Ctx* ctx) {
+    if (ctx->graph) hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+    ctx->graph_steps = 0;
+}
+
+// tiles of 64 x rows cells covering i in [0, ni+1], j in [0, nj+1] of every
+// block, block by block, rows then columns; one workgroup of 64*rows threads each
+static void build_tiles(shcfd::Ctx* ctx, int rows) {
+    ctx->htiles.clear();
+    ctx->tile_rows = rows;
+    for (int b = 0; b < ctx->nb; b++) {
+        const DevBlock& B = ctx->hblk[b];
+        for (int j0 = 0; j0 <= B.nj + 1; j0 += rows)
+            for (int i0 = 0; i0 <= B.ni + 1; i0 += TI) ctx->htiles.push_back(Tile{b, i0, j0, 0});
+    }
+}
+
 extern "C" {
 
 const char* shcfd_strerror(int code) {
@@ -112,15 +130,15 @@ int shcfd_create(int nblocks, const int* ni, const int* nj, int ny, const shcfd_
         B.nj = nj[b];
         B.pitch = ((ni[b] + 4) + 7) / 8 * 8;
         B.off = off;
+        B.active = 1;
+        B.nu = params->nu;
         off += (long long)B.pitch * (nj[b] + 4);
         off = (off + 63) / 64 * 64;
         ctx->hblk.push_back(B);
         maxni = std::max(maxni, ni[b]);
-        // tiles covering i in [0, ni+1], j in [0, nj+1]
-        for (int j0 = 0; j0 <= nj[b] + 1; j0 += TJ)
-            for (int i0 = 0; i0 <= ni[b] + 1; i0 += TI) ctx->htiles.push_back(Tile{b, i0, j0, 0});
     }
     ctx->NC = off;
+    build_tiles(ctx, TJ);
     // sweep strips, ordered by block then ascending j0
     for (int b = 0; b < nblocks; b++) {
         int first = (int)ctx->hstrips.size();
@@ -167,6 +185,7 @@ int shcfd_create(int nblocks, const int* ni, const int* nj, int ny, const shcfd_
 int shcfd_destroy(shcfd_ctx* ctx) {
     if (!ctx) return SHCFD_E_ARG;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    drop_graph(ctx);
     if (ctx->fast) { ctx->fast->release(); delete ctx->fast; ctx->fast = nullptr; }
     for (auto& e : ctx->prof_free) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
     for (int k = 0; k < PROF_COUNT; k++)
@@ -422,8 +441,9 @@ int shcfd_profile_read(shcfd_ctx* ctx, int kernel_class, double* total_ms, long 
 // ---------------------------------------------------------------------------
 // per-routine drop-ins
 // ---------------------------------------------------------------------------
-#define TILE_LAUNCH(KERNEL, ...)                                                                              \
-    hipLaunchKernelGGL(KERNEL, dim3(ntiles(ctx)), dim3(256), 0, ctx->stream, ctx->dev(), ctx->dtiles, ##__VA_ARGS__)
+#define TILE_LAUNCH(KERNEL, ...)                                                                   \
+    hipLaunchKernelGGL(KERNEL, dim3(ntiles(ctx)), dim3(64 * ctx->tile_rows), 0, ctx->stream, ctx->dev(), \
+                       ctx->dtiles, ##__VA_ARGS__)
 
 int shcfd_set_w(shcfd_ctx* ctx) {
     if (!ctx) return SHCFD_E_ARG;
@@ -706,6 +726,7 @@ int shcfd_set_option(shcfd_ctx* ctx, int option, int value) {
     switch (option) {
     case SHCFD_OPT_FAST_IMPLICIT:
         ctx->use_fast = value ? 1 : 0;
+        drop_graph(ctx);
         return SHCFD_OK;
     case SHCFD_OPT_SPIN_LIMIT:
         if (value < 1) return SHCFD_E_ARG;
@@ -713,7 +734,30 @@ int shcfd_set_option(shcfd_ctx* ctx, int option, int value) {
         return SHCFD_OK;
     case SHCFD_OPT_SKEW_SWEEP:
         ctx->use_skew = value ? 1 : 0;
+        drop_graph(ctx);
         return SHCFD_OK;
+    case SHCFD_OPT_TILE_ROWS: {
+        if (value != 1 && value != 2 && value != 4) return SHCFD_E_ARG;
+        if (value == ctx->tile_rows) return SHCFD_OK;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        build_tiles(ctx, value);
+        Tile* nt = nullptr;
+        double* np = nullptr;
+        int npart = (int)std::max(ctx->htiles.size(), ctx->hstrips.size());
+        if (hipMalloc((void**)&nt, sizeof(Tile) * ctx->htiles.size()) != hipSuccess ||
+            hipMalloc((void**)&np, sizeof(double) * 2 * npart) != hipSuccess) {
+            hipFree(nt);
+            return SHCFD_E_NOMEM;
+        }
+        HIPCHK(hipMemcpy(nt, ctx->htiles.data(), sizeof(Tile) * ctx->htiles.size(), hipMemcpyHostToDevice));
+        hipFree(ctx->dtiles);
+        hipFree(ctx->dpart);
+        ctx->dtiles = nt;
+        ctx->dpart = np;
+        ctx->npart = npart;
+        drop_graph(ctx);
+        return SHCFD_OK;
+    }
     default:
         return SHCFD_E_ARG;
     }
@@ -799,8 +843,10 @@ int shcfd_prepare(shcfd_ctx* ctx) {
     return rc;
 }
 
-int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double* res_last) {
-    if (!ctx) return SHCFD_E_ARG;
+// one physical step of a scheme without host-side checks inside the step
+// (all but dual time): only kernel launches on the context stream, so it can
+// be captured into a graph
+static int enqueue_step(shcfd_ctx* ctx, int step) {
     int rc = SHCFD_OK;
 #define TRY(x) do { rc = (x); if (rc) return rc; } while (0)
     switch (ctx->p.scheme) {
@@ -857,6 +903,22 @@ int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double
         TRY(part_main(ctx));
         TRY(shcfd_calc_next_step_rk2(ctx, 2));
         break;
+    default:
+        return SHCFD_E_STATE;
+    }
+#undef TRY
+    return rc;
+}
+
+int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double* res_last) {
+    if (!ctx) return SHCFD_E_ARG;
+    if (ctx->p.scheme != SHCFD_SCHEME_DUAL) {
+        int rc = enqueue_step(ctx, step);
+        return rc ? rc : shcfd_check_error(ctx);
+    }
+    int rc = SHCFD_OK;
+#define TRY(x) do { rc = (x); if (rc) return rc; } while (0)
+    switch (ctx->p.scheme) {
     case SHCFD_SCHEME_DUAL: {
         TRY(part_primitive(ctx));
         TRY(shcfd_set_dt(ctx, step, nullptr));
@@ -890,6 +952,169 @@ int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double
     }
 #undef TRY
     return shcfd_check_error(ctx);
+}
+
+// ---------------------------------------------------------------------------
+// batches of independent blocks
+// ---------------------------------------------------------------------------
+static int upload_blocks(shcfd_ctx* ctx) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(ctx->dblk, ctx->hblk.data(), sizeof(DevBlock) * ctx->nb, hipMemcpyHostToDevice));
+    return SHCFD_OK;
+}
+
+int shcfd_set_block_nu(shcfd_ctx* ctx, const double* nu) {
+    if (!ctx || !nu) return SHCFD_E_ARG;
+    for (int b = 0; b < ctx->nb; b++) {
+        if (!(nu[b] >= 0.0)) return SHCFD_E_ARG;
+        ctx->hblk[b].nu = nu[b];
+    }
+    return upload_blocks(ctx);
+}
+
+int shcfd_set_block_active(shcfd_ctx* ctx, const int* active) {
+    if (!ctx || !active) return SHCFD_E_ARG;
+    for (int b = 0; b < ctx->nb; b++) ctx->hblk[b].active = active[b] ? 1 : 0;
+    return upload_blocks(ctx);
+}
+
+int shcfd_energy_rms_blocks(shcfd_ctx* ctx, double* rms) {
+    if (!ctx || !rms) return SHCFD_E_ARG;
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_rms_part<NY>, ctx->dpart));
+    LAUNCHCHK();
+    std::vector<double> part(ctx->htiles.size());
+    HIPCHK(hipMemcpyAsync(part.data(), ctx->dpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<double> acc(ctx->nb, 0.0);
+    for (size_t k = 0; k < part.size(); k++) acc[ctx->htiles[k].b] = acc[ctx->htiles[k].b] + part[k];
+    for (int b = 0; b < ctx->nb; b++)
+        rms[b] = sqrt(acc[b]) / (double)(ctx->hblk[b].ni * ctx->hblk[b].nj);
+    return SHCFD_OK;
+}
+
+// allocate everything a step uses, outside any graph capture
+static void warm_allocations(shcfd_ctx* ctx) {
+    if (!fast_enabled(ctx)) skew_enabled(ctx);
+}
+
+int shcfd_iterate(shcfd_ctx* ctx, int first_step, int nsteps, int graph_steps) {
+    if (!ctx || nsteps < 0) return SHCFD_E_ARG;
+    if (graph_steps < 0) graph_steps = ctx->tuned_graph_steps;
+    if (ctx->p.scheme == SHCFD_SCHEME_DUAL) {
+        for (int k = 0; k < nsteps; k++) {
+            int rc = shcfd_step(ctx, first_step + k, first_step - 1, nullptr, nullptr);
+            if (rc) return rc;
+        }
+        return SHCFD_OK;
+    }
+    warm_allocations(ctx);
+    int done = 0;
+    if (graph_steps > 0 && nsteps >= graph_steps) {
+        if (ctx->graph_steps != graph_steps) {
+            drop_graph(ctx);
+            const unsigned prof = ctx->prof;
+            ctx->prof = 0;  // no event records inside the graph
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int rc = SHCFD_OK;
+            for (int k = 0; k < graph_steps && !rc; k++) rc = enqueue_step(ctx, first_step + k);
+            hipGraph_t g = nullptr;
+            hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+            ctx->prof = prof;
+            if (rc || ec != hipSuccess) {
+                if (g) hipGraphDestroy(g);
+                return rc ? rc : SHCFD_E_HIP;
+            }
+            hipError_t ei = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                ctx->graph = nullptr;
+                return SHCFD_E_HIP;
+            }
+            ctx->graph_steps = graph_steps;
+        }
+        for (; done + graph_steps <= nsteps; done += graph_steps) HIPCHK(hipGraphLaunch(ctx->graph, ctx->stream));
+    }
+    for (; done < nsteps; done++) {
+        int rc = enqueue_step(ctx, first_step + done);
+        if (rc) return rc;
+    }
+    return shcfd_check_error(ctx);
+}
+
+// Time each candidate schedule on the context's own blocks and keep the
+// fastest: optimised or per-routine sweeps (nY = 1), tile rows 1/2/4 of the
+// per-cell kernels, and eager launches or a graph of 8 steps.  The flow state
+// (q, qp, qpp and the time scalars) is saved before and restored after, so
+// tuning does not advance the solution.  choice[4] = fast, tile rows, graph
+// steps, best ms per step (may be NULL).
+int shcfd_autotune(shcfd_ctx* ctx, int trial_steps, double* choice) {
+    if (!ctx || trial_steps < 1 || ctx->p.scheme == SHCFD_SCHEME_DUAL) return SHCFD_E_ARG;
+    const size_t nq = (size_t)ctx->dimq * (size_t)ctx->NC;
+    double* save = nullptr;
+    double* ssave = nullptr;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (hipMalloc((void**)&save, sizeof(double) * 3 * nq) != hipSuccess ||
+        hipMalloc((void**)&ssave, sizeof(double) * SCAL_COUNT) != hipSuccess) {
+        hipFree(save);
+        return SHCFD_E_NOMEM;
+    }
+    const int fields[3] = {SHCFD_F_Q, SHCFD_F_QP, SHCFD_F_QPP};
+    for (int k = 0; k < 3; k++)
+        HIPCHK(hipMemcpy(save + k * nq, ctx->f[fields[k]], sizeof(double) * nq, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(ssave, ctx->scal, sizeof(double) * SCAL_COUNT, hipMemcpyDeviceToDevice));
+    auto restore = [&]() -> int {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (int k = 0; k < 3; k++)
+            HIPCHK(hipMemcpy(ctx->f[fields[k]], save + k * nq, sizeof(double) * nq, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(ctx->scal, ssave, sizeof(double) * SCAL_COUNT, hipMemcpyDeviceToDevice));
+        return SHCFD_OK;
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 1e300;
+    int bf = ctx->use_fast, br = ctx->tile_rows, bg = 0, rc = SHCFD_OK;
+    const int fasts[2] = {1, 0};
+    const int nfast = ctx->ny == 1 ? 2 : 1;
+    for (int fi = 0; fi < nfast && !rc; fi++)
+        for (int rows = 1; rows <= 4 && !rc; rows *= 2)
+            for (int gs = 0; gs <= 8 && !rc; gs += 8) {
+                rc = shcfd_set_option(ctx, SHCFD_OPT_FAST_IMPLICIT, fasts[fi]);
+                if (!rc) rc = shcfd_set_option(ctx, SHCFD_OPT_TILE_ROWS, rows);
+                if (!rc) rc = shcfd_iterate(ctx, 1, gs > 0 ? gs : 1, gs);  // warm-up (and capture)
+                if (rc) break;
+                const int n = gs > 0 ? ((trial_steps + gs - 1) / gs) * gs : trial_steps;
+                hipEventRecord(e0, ctx->stream);
+                rc = shcfd_iterate(ctx, 1, n, gs);
+                hipEventRecord(e1, ctx->stream);
+                hipEventSynchronize(e1);
+                float ms = 0.0f;
+                hipEventElapsedTime(&ms, e0, e1);
+                const double per = (double)ms / n;
+                if (!rc && per < best) {
+                    best = per;
+                    bf = fasts[fi];
+                    br = rows;
+                    bg = gs;
+                }
+                int r2 = restore();
+                if (!rc) rc = r2;
+            }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(save);
+    hipFree(ssave);
+    if (rc) return rc;
+    rc = shcfd_set_option(ctx, SHCFD_OPT_FAST_IMPLICIT, bf);
+    if (!rc) rc = shcfd_set_option(ctx, SHCFD_OPT_TILE_ROWS, br);
+    ctx->tuned_graph_steps = bg;
+    if (choice) {
+        choice[0] = bf;
+        choice[1] = br;
+        choice[2] = bg;
+        choice[3] = best;
+    }
+    return rc;
 }
 
 }  // extern "C"

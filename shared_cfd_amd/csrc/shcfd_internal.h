@@ -19,8 +19,9 @@
 namespace shcfd {
 
 struct DevBlock {
-    int ni, nj, pitch, pad;
-    long long off;  // index of cell (-1,-1)
+    int ni, nj, pitch, active;  // active = 0: the implicit update leaves q unchanged
+    long long off;              // index of cell (-1,-1)
+    double nu;                  // kinematic viscosity of the ideal gas (mu = rho nu)
 };
 
 // Device-side view passed by value to every kernel.
@@ -138,6 +139,11 @@ struct Ctx {
     // Chemkin mechanism (module chem of the reference), device copy
     shcfd_mech* dmech = nullptr;
     int has_mech = 0;
+    int tile_rows = TJ;            // rows per tile of the per-cell kernels (1, 2 or 4)
+    // captured whole-step graphs (shcfd_iterate)
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    int tuned_graph_steps = 0;     // schedule chosen by shcfd_autotune
     // strip-skewed Jacobian copy for the row-per-lane sweeps (nY > 1 steps)
     int use_skew = 1;
     int skew_active = 0;           // the last set_ABpm of a step wrote the skewed copy

@@ -30,8 +30,9 @@ EXPORTS = [
     "shcfd_check_error", "shcfd_profile_enable", "shcfd_profile_read", "shcfd_set_option",
     "shcfd_mech_parse", "shcfd_set_mechanism", "shcfd_load_mechanism", "shcfd_proceed_reaction",
     "shcfd_chem_boundary_state", "shcfd_chem_eval", "shcfd_chem_iterations",
+    "shcfd_set_block_nu", "shcfd_set_block_active", "shcfd_energy_rms_blocks", "shcfd_iterate", "shcfd_autotune",
 ]
-OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT, OPT_SKEW_SWEEP = 1, 2, 3
+OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT, OPT_SKEW_SWEEP, OPT_TILE_ROWS = 1, 2, 3, 4
 
 # kernel classes of shcfd_profile_read (include/shcfd.h)
 KCLASS = ["PRIMITIVE", "BC", "HV", "TG", "TGV", "DT", "ABPM", "DQDT", "RHS", "SWEEP_FWD", "SWEEP_BWD",
@@ -79,6 +80,11 @@ def lib():
         L.shcfd_profile_enable.argtypes = [vp, ctypes.c_int]
         L.shcfd_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.shcfd_profile_read.argtypes = [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+        L.shcfd_set_block_nu.argtypes = [vp, dp]
+        L.shcfd_set_block_active.argtypes = [vp, ip]
+        L.shcfd_energy_rms_blocks.argtypes = [vp, dp]
+        L.shcfd_iterate.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.shcfd_autotune.argtypes = [vp, ctypes.c_int, dp]
         _LIB = L
     return _LIB
 
@@ -131,6 +137,9 @@ class Solver:
         if case.w0 is not None and load_q:
             for b in range(case.nblocks):
                 self.upload("W", b, case.w0[b])
+        if case.block_nu is not None:
+            bnu = np.ascontiguousarray(case.block_nu, dtype=np.float64)
+            check(L.shcfd_set_block_nu(self.ctx, _dp(bnu)), "shcfd_set_block_nu")
         check(L.shcfd_init_geometry(self.ctx), "shcfd_init_geometry")
         self.set_option(OPT_FAST_IMPLICIT, int(fast))
 
@@ -177,6 +186,24 @@ class Solver:
     def proceed_reaction(self):
         check(self.L.shcfd_proceed_reaction(self.ctx), "proceed_reaction")
         check(self.L.shcfd_check_error(self.ctx), "proceed_reaction")
+
+    # ---- batches of independent blocks ----
+    def set_block_active(self, active):
+        a = np.ascontiguousarray(active, dtype=np.int32)
+        check(self.L.shcfd_set_block_active(self.ctx, _ip(a)), "set_block_active")
+
+    def energy_rms_blocks(self):
+        r = np.zeros(self.case.nblocks)
+        check(self.L.shcfd_energy_rms_blocks(self.ctx, _dp(r)), "energy_rms_blocks")
+        return r
+
+    def iterate(self, first_step, nsteps, graph_steps=-1):
+        check(self.L.shcfd_iterate(self.ctx, int(first_step), int(nsteps), int(graph_steps)), "iterate")
+
+    def autotune(self, trial_steps=4):
+        c = np.zeros(4)
+        check(self.L.shcfd_autotune(self.ctx, int(trial_steps), _dp(c)), "autotune")
+        return {"fast": int(c[0]), "tile_rows": int(c[1]), "graph_steps": int(c[2]), "ms_per_step": float(c[3])}
 
     def chem_iterations(self, reset=True):
         """Newton iterations / CHEMEQ2 attempts summed over cells since the last reset."""
