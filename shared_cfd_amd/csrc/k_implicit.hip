@@ -110,7 +110,7 @@ __device__ inline bool wait_progress(const int* prog, int other, int need, int& 
 // forward sweep (plain: Jacobians read from the stored Ap/Bp arrays)
 // ---------------------------------------------------------------------------
 template <int NY, bool SKEW>
-__global__ void __launch_bounds__(64) k_sweep_fwd_plain(DevCtx d, const Strip* strips, int* prog, double* bnd,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) k_sweep_fwd_plain(DevCtx d, const Strip* strips, int* prog, double* bnd,
                                                         long long bstride, long long spin_limit, const double* skew,
                                                         long long nsl, const long long* sbase) {
     constexpr int dq = NY + 3;
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(64) k_sweep_fwd_plain(DevCtx d, const Strip* s
 // per-strip (res, omega) partials are written and k_update applies q += ωΔq.
 // ---------------------------------------------------------------------------
 template <int NY, bool SKEW>
-__global__ void __launch_bounds__(64) k_sweep_bwd_plain(DevCtx d, const Strip* strips, int nstrips, int* prog,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) k_sweep_bwd_plain(DevCtx d, const Strip* strips, int nstrips, int* prog,
                                                         double* bnd, long long bstride, double* part,
                                                         long long spin_limit, const double* skew, long long nsl,
                                                         const long long* sbase) {
