@@ -122,9 +122,16 @@ constexpr unsigned long long SENTINEL = 0x7FF4DEADBEEFCAFEULL;
 // column i, for i from -HOFF on.
 constexpr int HOFF = 48;
 
+// Operator arrays G = (GA, GB) and H = (HA, HB): per step a 4 KB block of
+// four 1 KB parts, part p holding doubles (2p', 2p'+1) of every lane's row,
+// p = 2*(0 for A, 1 for B) + p'; lane L's pair sits at 16 L in its part.
+// Each 16-byte load of a sweep step then reads 1 KB of consecutive bytes.
+__device__ inline long long op_index(const QStrip& S, int dd, int L, int p, int e) {
+    return (S.base / 64 + dd) * 512 + p * 128 + 2 * L + e;
+}
+
 __global__ void __launch_bounds__(256) k_fast_prep(DevCtx d, const QStrip* strips, const int2* work, double* R_sl,
-                                                   double* GA, double* GB, double* HA, double* HB, double* hand,
-                                                   int nstrips, long long hstride) {
+                                                   double* G, double* H, double* hand, int nstrips, long long hstride) {
     // refill the hand-off rows of both sweeps: the sentinel for the columns
     // 1..ni of the strip's block, zero elsewhere
     {
@@ -155,22 +162,21 @@ __global__ void __launch_bounds__(256) k_fast_prep(DevCtx d, const QStrip* strip
     post_matrix(J, dual, Sm);
     const double dsci = FLD(d, SHCFD_F_DSCI, 0, c), dscj = FLD(d, SHCFD_F_DSCJ, 0, c);
     const double fa = dsci * J.alpha, fb = dscj * J.alpha;
-    const long long slot = S.base + (long long)dd * 64 + 4 * r;  // lane of component 0
     // R (component c in lane 4r+c)
 #pragma unroll
     for (int k = 0; k < 4; k++) R_sl[slot1(S.base, dd, 4 * r + k)] = FLD(d, SHCFD_F_DQ, k, c);
-    // rows of the four operators: lane 4r+c holds row c (4 doubles)
-    double* outs[4] = {GA, GB, HA, HB};
+    // rows of the four operators: lane 4r+k holds row k (4 doubles)
     const double* mats[4] = {J.Ap, J.Bp, J.Am, J.Bm};
     const double fs[4] = {fa, fb, fa, fb};
 #pragma unroll
     for (int m = 0; m < 4; m++) {
         mm4(mats[m], Sm, T);
-        double* o = outs[m] + 4 * slot;
+        double* o = m < 2 ? G : H;
 #pragma unroll
         for (int k = 0; k < 4; k++)
 #pragma unroll
-            for (int l = 0; l < 4; l++) o[4 * k + l] = fs[m] * T[k + 4 * l];
+            for (int l = 0; l < 4; l++)
+                o[op_index(S, dd, 4 * r + k, 2 * (m & 1) + (l >> 1), l & 1)] = fs[m] * T[k + 4 * l];
     }
 }
 
@@ -204,92 +210,231 @@ __device__ inline void handoff_wait(double& x, const double* slotp, bool mine, l
 // other counters left alone)
 __device__ inline void wait_all_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-constexpr int PF = 8;   // coefficient prefetch depth (steps); divides CH
-constexpr int PH = 8;   // hand-off prefetch depth (steps); divides CH
-constexpr int HM = 8;   // extra hand-off lead (columns) taken after a miss
 
-// Sweep schedule.  Per step a wave issues 2+2 16-byte coefficient loads,
-// half a 16-byte load of R/X, one hand-off load, half a 16-byte store of
-// V/Z and one hand-off store: 7 vector memory instructions, so PF = 8 steps
-// of prefetch stay within the 63 accesses a wave can keep in flight.  The
-// chunk body is straight-line code (no bounds tests, no masks), so the
-// compiler waits only for the data a step consumes.  A hand-off miss takes a
-// rare uniform branch that polls, refills the ring and then drains all
-// outstanding accesses, so it leaves no pending state behind for the common
-// path.
+// Buffer views.  Each sweep addresses its strip through buffer resources
+// (base in scalar registers, 32-bit lane offset, per-step offsets in scalar
+// registers or the instruction's immediate), so a step spends one
+// instruction per access instead of 64-bit address arithmetic per lane.
+// Lanes that must not store give an offset beyond the view's size: the
+// hardware's range check drops the access.
+typedef unsigned int shcfd_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int shcfd_u32x2 __attribute__((ext_vector_type(2)));
+constexpr int BUF_DROP = 0x7FFFFF00;     // lane offset beyond every view
+constexpr int AUX_AGENT = 16;            // sc1: device-coherent access
+__device__ inline __amdgpu_buffer_rsrc_t view(const double* base, long long ndoubles) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), 0, (int)(ndoubles * 8), 0x00020000);
+}
+__device__ inline double u2d(unsigned lo, unsigned hi) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline double2 ld_pair(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    const shcfd_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    return make_double2(u2d(v.x, v.y), u2d(v.z, v.w));
+}
+__device__ inline void st_pair(__amdgpu_buffer_rsrc_t r, double2 x, int voff, int soff) {
+    const unsigned long long a = (unsigned long long)__double_as_longlong(x.x);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x.y);
+    const shcfd_u32x4 v = {(unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
+}
+__device__ inline double ld_agent(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    const shcfd_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX_AGENT);
+    return u2d(v.x, v.y);
+}
+__device__ inline void st_agent(__amdgpu_buffer_rsrc_t r, double x, int voff, int soff) {
+    const unsigned long long a = (unsigned long long)__double_as_longlong(x);
+    const shcfd_u32x2 v = {(unsigned)a, (unsigned)(a >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, AUX_AGENT);
+}
+// byte offsets of step dd (dd >= -CH) inside a strip's views: operator
+// blocks of 4 KB per step, step pairs of 1 KB (dd even), hand-off columns of
+// 32 bytes
+__device__ inline int op_off(int dd) { return (dd + CH) * 4096; }
+__device__ inline int pair_off(int dd) { return (dd + CH) * 512; }
+__device__ inline int col_off(int col) { return (col + HOFF) * 32; }
+
+// Sweep schedule.  A sweep workgroup has two wavefronts on one CU: wave 0
+// computes, wave 1 streams the strip's operator blocks and R/X pairs into LDS
+// with LDS-DMA loads (buffer_load ... lds), CS steps per chunk, NSLOT chunk
+// slots.  The vector-memory path of a CU moves about 64 bytes per cycle, and
+// a step needs 4.5 KB of operators: issued by the computing wave, those loads
+// stall its dependency chain; issued by a second wave they overlap it.  The
+// waves meet at one s_barrier per chunk: before it the loader has waited for
+// the chunk the compute wave reads next, and the compute wave has finished
+// reading the slot the loader refills next.  The compute wave keeps the
+// hand-off ring (PH steps of device-coherent loads), its V/Z stores and the
+// hand-off stores.  Its step body is straight-line code; a hand-off miss takes
+// a rare uniform branch that polls the one column and then drains all
+// outstanding accesses, so it leaves no pending state for the common path.
+constexpr int CS = 4;                                   // steps per LDS chunk
+constexpr int NSLOT = 3;                                // LDS chunk slots
+constexpr int SLOT_G = CS * 4096;                       // operator bytes per slot
+constexpr int SLOT_BYTES = SLOT_G + (CS / 2) * 1024;    // + R/X step pairs
+constexpr int LD_PER_CHUNK = 4 * CS + CS / 2;           // LDS-DMA loads per chunk
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ inline void lds_dma(__amdgpu_buffer_rsrc_t r, char* lds, int voff, int soff) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void*)lds, 16, voff, soff, 0, 0);
+}
+// s_barrier that the compiler may not move memory accesses across
+__device__ inline void wg_barrier() { asm volatile("s_barrier" ::: "memory"); }
+// wait until at most N vector memory accesses of this wave are outstanding
+template <int N>
+__device__ inline void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
+}
+__device__ inline void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xC07F); }
+
+// loader wave: chunk j = aligned block m of CS steps (steps CS*m .. CS*m+CS-1)
+// into slot j % NSLOT; the caller orders blocks (ascending or descending)
+__device__ inline void load_block(__amdgpu_buffer_rsrc_t rop, __amdgpu_buffer_rsrc_t rpair, char* lds, int j,
+                                  int m, int vl) {
+    char* slot = lds + (j % NSLOT) * SLOT_BYTES;
+#pragma unroll
+    for (int t = 0; t < CS; t++)
+#pragma unroll
+        for (int p = 0; p < 4; p++) lds_dma(rop, slot + t * 4096 + p * 1024, vl, op_off(CS * m + t) + 1024 * p);
+#pragma unroll
+    for (int q = 0; q < CS / 2; q++) lds_dma(rpair, slot + SLOT_G + q * 1024, vl, pair_off(CS * m + 2 * q));
+}
+
+// loader side of the chunk protocol: nck chunks, block of chunk j = blk(j)
+template <typename BLK>
+__device__ inline void loader(__amdgpu_buffer_rsrc_t rop, __amdgpu_buffer_rsrc_t rpair, char* lds, int nck, int vl,
+                              BLK blk) {
+    load_block(rop, rpair, lds, 0, blk(0), vl);
+    load_block(rop, rpair, lds, 1, blk(1), vl);
+    load_block(rop, rpair, lds, 2, blk(2), vl);
+    wait_vmcnt<2 * LD_PER_CHUNK>();  // chunk 0 in LDS
+    wg_barrier();
+    wait_vmcnt<LD_PER_CHUNK>();      // chunk 1
+    wg_barrier();
+    for (int j = 0; j < nck; j++) {
+        // chunk j+3 takes the slot of chunk j, read before the last barrier;
+        // the blocks past the strip's end fall in its guard steps
+        load_block(rop, rpair, lds, j + 3, blk(j + 3), vl);
+        wait_vmcnt<LD_PER_CHUNK>();  // chunk j+2
+        wg_barrier();
+    }
+}
+
+// compute side: one chunk's operators and R/X pair(s) from LDS
+struct Chunk {
+    double2 g[CS][4];
+    double2 r[CS / 2];
+};
+__device__ inline void read_chunk(Chunk& c, const char* lds, int j, int vl) {
+    const char* slot = lds + (j % NSLOT) * SLOT_BYTES + vl;
+#pragma unroll
+    for (int t = 0; t < CS; t++)
+#pragma unroll
+        for (int p = 0; p < 4; p++) c.g[t][p] = *reinterpret_cast<const double2*>(slot + t * 4096 + p * 1024);
+#pragma unroll
+    for (int q = 0; q < CS / 2; q++) c.r[q] = *reinterpret_cast<const double2*>(slot + SLOT_G + q * 1024);
+}
+
+constexpr int PH = 4;   // hand-off prefetch depth (steps); divides 2*CS
+constexpr int HM = 4;   // extra hand-off lead (columns) awaited before the start
+
+__shared__ __attribute__((aligned(16))) char sweep_lds[NSLOT * SLOT_BYTES];
+
+// one forward step (lane L: component L&3 of row L>>2): step dd, slot step t
+// of chunk c, ring position u (compile-time after unrolling)
+__device__ inline void fwd_step(const Chunk& c, int t, int u, int dd, double& a, double& bo, double& vlo,
+                                double (&hb)[PH],
+                                bool cons, const double* hin, __amdgpu_buffer_rsrc_t rin,
+                                __amdgpu_buffer_rsrc_t rv, __amdgpu_buffer_rsrc_t rout, int vl, int vc, int vo,
+                                long long spin_limit, int* iflag) {
+    const double2 ga01 = c.g[t][0], ga23 = c.g[t][1], gb01 = c.g[t][2], gb23 = c.g[t][3];
+    const double Rc = (t & 1) ? c.r[t >> 1].y : c.r[t >> 1].x;
+    const int i0 = dd + 1;  // column of row 0
+    double x = hb[u % PH];
+    if (__builtin_expect((__builtin_amdgcn_ballot_w64(is_sentinel(x)) & 0xFULL) != 0ULL, 0)) {
+        // the producer fell behind after the start: poll this column only
+        // (a ring refill here would make the compiler copy the ring on the
+        // common path), then drain so the common path inherits no pending
+        // loads
+        handoff_wait(x, hin + 4LL * i0, cons, spin_limit, iflag);
+        wait_all_vmem();
+    }
+    double bin = shift_up4(bo);
+    if (cons) bin = x;
+    const double v = (Rc + a) + bin;
+    // refill the ring slot once x is dead, so the load lands in place
+    hb[u % PH] = ld_agent(rin, vc, col_off(i0 + PH));
+    if (t & 1)
+        st_pair(rv, make_double2(vlo, v), vl, pair_off(dd - 1));
+    else
+        vlo = v;
+    const double v0 = quad_bcast<0>(v), v1 = quad_bcast<1>(v), v2 = quad_bcast<2>(v), v3 = quad_bcast<3>(v);
+    a = fma(ga23.y, v3, fma(ga23.x, v2, fma(ga01.y, v1, ga01.x * v0)));
+    bo = fma(gb23.y, v3, fma(gb23.x, v2, fma(gb01.y, v1, gb01.x * v0)));
+    // row 15 hands column dd-14 to the strip below
+    st_agent(rout, bo, vo, col_off(dd - (QR - 2)));
+}
 
 // ---------------------------------------------------------------------------
-// forward sweep: one wavefront per quad strip
+// forward sweep: one workgroup (compute wave + loader wave) per quad strip
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-k_fast_fwd(DevCtx d, const QStrip* strips, int nstrips, const double* R_sl, const double* GA, const double* GB,
-           double* V_sl, double* hand, long long hstride, long long spin_limit) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_fast_fwd(DevCtx d, const QStrip* strips, int nstrips, const double* R_sl, const double* G, double* V_sl,
+           double* hand, long long hstride, long long spin_limit) {
     const int s = blockIdx.x;
-    const int L = threadIdx.x;
+    const int L = threadIdx.x & 63;
     const QStrip S = strips[s];
     const int ni = d.blk[S.b].ni;
-    const int nch = S.nch;
-    const bool has_next = S.next >= 0;
+    const int nck = S.nch * (CH / CS);  // chunks, processed in ascending order
+    const long long nstep = (long long)(S.nch + 2) * CH;
+    const int vl = 16 * L;
+    if (threadIdx.x >= 64) {
+        const __amdgpu_buffer_rsrc_t rg = view(G + (S.base / 64 - CH) * 512, nstep * 512);
+        const __amdgpu_buffer_rsrc_t rr = view(R_sl + S.base - 64 * CH, nstep * 64);
+        loader(rg, rr, sweep_lds, nck, vl, [](int j) { return j; });
+        return;
+    }
     const bool cons = L < 4;  // lanes of row 0: they take the hand-off
-    const double4* ga_p = reinterpret_cast<const double4*>(GA) + S.base + L;
-    const double4* gb_p = reinterpret_cast<const double4*>(GB) + S.base + L;
-    const double2* r_p = reinterpret_cast<const double2*>(R_sl) + S.base / 2 + L;
-    double2* v_p = reinterpret_cast<double2*>(V_sl) + S.base / 2 + L;
+    const __amdgpu_buffer_rsrc_t rv = view(V_sl + S.base - 64 * CH, nstep * 64);
     // a strip with no neighbour above reads the row of zeros
-    const double* hin = hand + (long long)(S.prev >= 0 ? S.prev : nstrips) * hstride + 4 * HOFF + (L & 3);
-    double* hout = hand + (long long)s * hstride + 4 * HOFF + (L & 3);
-    double4 gab[PF], gbb[PF];
-    double2 rb[PF / 2];
+    const long long prow = S.prev >= 0 ? S.prev : nstrips;
+    const __amdgpu_buffer_rsrc_t rin = view(hand + prow * hstride, hstride);
+    const __amdgpu_buffer_rsrc_t rout = view(hand + (long long)s * hstride, hstride);
+    const double* hin = hand + prow * hstride + 4 * HOFF + (L & 3);  // for polling
+    const int vc = 8 * (L & 3);
+    const int vo = (S.next >= 0 && L >= 60) ? vc : BUF_DROP;  // row 15 hands off
+    // start once the strip above is PH+HM columns ahead, so that the ring
+    // loads find their columns already written
+    {
+        const int ahead = min(PH + HM, ni);
+        double y = load_agent(hin + 4LL * ahead);
+        handoff_wait(y, hin + 4LL * ahead, cons, spin_limit, d.iflag);
+        wait_all_vmem();
+    }
     double hb[PH];
 #pragma unroll
-    for (int q = 0; q < PF; q++) {
-        gab[q] = ga_p[64LL * q];
-        gbb[q] = gb_p[64LL * q];
-    }
-#pragma unroll
-    for (int q = 0; q < PF / 2; q++) rb[q] = r_p[64LL * q];
-#pragma unroll
-    for (int q = 0; q < PH; q++) hb[q] = load_agent(hin + 4LL * (q + 1));
+    for (int q = 0; q < PH; q++) hb[q] = ld_agent(rin, vc, col_off(q + 1));
     double a = 0.0, bo = 0.0, vlo = 0.0;
-    for (int ch = 0; ch < nch; ch++) {
+    Chunk A, B;
+    wg_barrier();  // chunk 0 in LDS
+    read_chunk(A, sweep_lds, 0, vl);
+    wait_lgkm0();
+    wg_barrier();  // chunk 1 in LDS
+    for (int j = 0; j < nck; j += 2) {
+        read_chunk(B, sweep_lds, j + 1, vl);
 #pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int dd = ch * CH + u;
-            const double4 ga = gab[u % PF], gb = gbb[u % PF];
-            const double Rc = (u & 1) ? rb[(u / 2) % (PF / 2)].y : rb[(u / 2) % (PF / 2)].x;
-            gab[u % PF] = ga_p[64LL * (dd + PF)];
-            gbb[u % PF] = gb_p[64LL * (dd + PF)];
-            if (u & 1) rb[(u / 2) % (PF / 2)] = r_p[64LL * ((dd + PF) >> 1)];
-            const int i0 = dd + 1;  // column of row 0
-            double x = hb[u % PH];
-            if ((__builtin_amdgcn_ballot_w64(is_sentinel(x)) & 0xFULL) != 0ULL) {
-                // the producer was not far enough ahead when the ring was
-                // filled: let it get PH+HM columns ahead, refill the ring,
-                // then drain so the common path inherits no pending loads
-                const int ahead = min(i0 + PH + HM, ni);
-                double y = load_agent(hin + 4LL * ahead);
-                handoff_wait(y, hin + 4LL * ahead, cons, spin_limit, d.iflag);
-                x = load_agent(hin + 4LL * i0);
+        for (int t = 0; t < CS; t++)
+            fwd_step(A, t, t, CS * j + t, a, bo, vlo, hb, cons, hin, rin, rv, rout, vl, vc, vo, spin_limit,
+                     d.iflag);
+        wait_lgkm0();
+        wg_barrier();
+        read_chunk(A, sweep_lds, j + 2, vl);
 #pragma unroll
-                for (int m = 1; m < PH; m++) hb[(u + m) % PH] = load_agent(hin + 4LL * (i0 + m));
-                handoff_wait(x, hin + 4LL * i0, cons, spin_limit, d.iflag);
-                wait_all_vmem();
-            }
-            double bin = shift_up4(bo);
-            if (cons) bin = x;
-            const double v = (Rc + a) + bin;
-            // refill the ring slot once x is dead, so the load lands in place
-            hb[u % PH] = load_agent(hin + 4LL * (i0 + PH));
-            if (u & 1)
-                v_p[64LL * (dd >> 1)] = make_double2(vlo, v);
-            else
-                vlo = v;
-            const double v0 = quad_bcast<0>(v), v1 = quad_bcast<1>(v), v2 = quad_bcast<2>(v), v3 = quad_bcast<3>(v);
-            a = fma(ga.w, v3, fma(ga.z, v2, fma(ga.y, v1, ga.x * v0)));
-            bo = fma(gb.w, v3, fma(gb.z, v2, fma(gb.y, v1, gb.x * v0)));
-            // row 15 hands column dd-14 to the strip below
-            if (has_next && L >= 60) store_agent(hout + 4LL * (dd - (QR - 2)), bo);
-        }
+        for (int t = 0; t < CS; t++)
+            fwd_step(B, t, CS + t, CS * (j + 1) + t, a, bo, vlo, hb, cons, hin, rin, rv, rout, vl, vc, vo,
+                     spin_limit, d.iflag);
+        wait_lgkm0();
+        wg_barrier();
     }
 }
 
@@ -332,73 +477,94 @@ __global__ void __launch_bounds__(256) k_fast_mid(DevCtx d, const QStrip* strips
 // backward sweep: workgroup g handles strip (n-1-g), so the strip it waits
 // for (the one above) has a lower workgroup id
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-k_fast_bwd(DevCtx d, const QStrip* strips, int nstrips, const double* X_sl, const double* HA, const double* HB,
-           double* Z_sl, double* hand, long long hstride, long long spin_limit) {
+// one backward step: step dd, slot step t of chunk c, ring position u
+__device__ inline void bwd_step(const Chunk& c, int t, int u, int dd, double& a, double& bo, double& zhi,
+                                double (&hb)[PH], bool cons, const double* hin, __amdgpu_buffer_rsrc_t rin,
+                                __amdgpu_buffer_rsrc_t rz, __amdgpu_buffer_rsrc_t rout, int vl, int vc, int vo,
+                                long long spin_limit, int* iflag) {
+    const double2 ha01 = c.g[t][0], ha23 = c.g[t][1], hb01 = c.g[t][2], hb23 = c.g[t][3];
+    const double Xc = (t & 1) ? c.r[t >> 1].y : c.r[t >> 1].x;
+    const int i15 = dd - (QR - 2);  // column of row 15
+    double x = hb[u % PH];
+    if (__builtin_expect((__builtin_amdgcn_ballot_w64(is_sentinel(x)) >> 60) != 0ULL, 0)) {
+        // the producer fell behind: poll this column only
+        handoff_wait(x, hin + 4LL * i15, cons, spin_limit, iflag);
+        wait_all_vmem();
+    }
+    double bin = shift_down4(bo);
+    if (cons) bin = x;
+    const double z = Xc - (a + bin);
+    hb[u % PH] = ld_agent(rin, vc, col_off(i15 - PH));
+    if (t & 1)
+        zhi = z;
+    else
+        st_pair(rz, make_double2(z, zhi), vl, pair_off(dd));
+    const double z0 = quad_bcast<0>(z), z1 = quad_bcast<1>(z), z2 = quad_bcast<2>(z), z3 = quad_bcast<3>(z);
+    a = fma(ha23.y, z3, fma(ha23.x, z2, fma(ha01.y, z1, ha01.x * z0)));
+    bo = fma(hb23.y, z3, fma(hb23.x, z2, fma(hb01.y, z1, hb01.x * z0)));
+    // row 0 hands column dd+1 to the strip above
+    st_agent(rout, bo, vo, col_off(dd + 1));
+}
+
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_fast_bwd(DevCtx d, const QStrip* strips, int nstrips, const double* X_sl, const double* H, double* Z_sl,
+           double* hand, long long hstride, long long spin_limit) {
     const int s = nstrips - 1 - (int)blockIdx.x;
-    const int L = threadIdx.x;
+    const int L = threadIdx.x & 63;
     const QStrip S = strips[s];
-    const int nch = S.nch;
-    const bool has_prev = S.prev >= 0;
+    const int nck = S.nch * (CH / CS);  // chunk j holds block nck-1-j (steps descend)
+    const long long nstep = (long long)(S.nch + 2) * CH;
+    const int vl = 16 * L;
+    if (threadIdx.x >= 64) {
+        const __amdgpu_buffer_rsrc_t rh = view(H + (S.base / 64 - CH) * 512, nstep * 512);
+        const __amdgpu_buffer_rsrc_t rx = view(X_sl + S.base - 64 * CH, nstep * 64);
+        loader(rh, rx, sweep_lds, nck, vl, [nck](int j) { return nck - 1 - j; });
+        return;
+    }
     const bool cons = L >= 60;  // lanes of row 15: they take the hand-off
-    const double4* ha_p = reinterpret_cast<const double4*>(HA) + S.base + L;
-    const double4* hb_p = reinterpret_cast<const double4*>(HB) + S.base + L;
-    const double2* x_p = reinterpret_cast<const double2*>(X_sl) + S.base / 2 + L;
-    double2* z_p = reinterpret_cast<double2*>(Z_sl) + S.base / 2 + L;
+    const __amdgpu_buffer_rsrc_t rz = view(Z_sl + S.base - 64 * CH, nstep * 64);
     // a strip with no neighbour below reads the row of zeros
-    const double* hin = hand + (long long)(S.next >= 0 ? S.next : nstrips) * hstride + 4 * HOFF + (L & 3);
-    double* hout = hand + (long long)s * hstride + 4 * HOFF + (L & 3);
-    const int top = nch * CH - 1;  // first step of the sweep (steps descend)
-    double4 hab[PF], hbb[PF];
-    double2 xb[PF / 2];
+    const long long nrow = S.next >= 0 ? S.next : nstrips;
+    const __amdgpu_buffer_rsrc_t rin = view(hand + nrow * hstride, hstride);
+    const __amdgpu_buffer_rsrc_t rout = view(hand + (long long)s * hstride, hstride);
+    const double* hin = hand + nrow * hstride + 4 * HOFF + (L & 3);  // for polling
+    const int vc = 8 * (L & 3);
+    const int vo = (S.prev >= 0 && L < 4) ? vc : BUF_DROP;  // row 0 hands off
+    const int top = nck * CS - 1;  // first step of the sweep
+    // start once the strip below is PH+HM columns ahead (columns descend
+    // from ni; the padded columns above ni are zero)
+    {
+        const int ahead = max(d.blk[S.b].ni - PH - HM, 1);
+        double y = load_agent(hin + 4LL * ahead);
+        handoff_wait(y, hin + 4LL * ahead, cons, spin_limit, d.iflag);
+        wait_all_vmem();
+    }
+    // row 15's column at step dd is dd-14
     double hb[PH];
 #pragma unroll
-    for (int q = 0; q < PF; q++) {
-        hab[q] = ha_p[64LL * (top - q)];
-        hbb[q] = hb_p[64LL * (top - q)];
-    }
-#pragma unroll
-    for (int q = 0; q < PF / 2; q++) xb[q] = x_p[64LL * ((top >> 1) - q)];
-    // row 15's column at step dd is dd-14
-#pragma unroll
-    for (int q = 0; q < PH; q++) hb[q] = load_agent(hin + 4LL * (top - q - (QR - 2)));
+    for (int q = 0; q < PH; q++) hb[q] = ld_agent(rin, vc, col_off(top - q - (QR - 2)));
     double a = 0.0, bo = 0.0, zhi = 0.0;
-    for (int ch = 0; ch < nch; ch++) {
+    Chunk A, B;
+    wg_barrier();
+    read_chunk(A, sweep_lds, 0, vl);
+    wait_lgkm0();
+    wg_barrier();
+    for (int j = 0; j < nck; j += 2) {
+        read_chunk(B, sweep_lds, j + 1, vl);
+        const int m0 = nck - 1 - j;  // block of chunk j
 #pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int dd = top - (ch * CH + u);  // odd for even u
-            const double4 ha = hab[u % PF], hbm = hbb[u % PF];
-            const double Xc = (u & 1) ? xb[(u / 2) % (PF / 2)].x : xb[(u / 2) % (PF / 2)].y;
-            hab[u % PF] = ha_p[64LL * (dd - PF)];
-            hbb[u % PF] = hb_p[64LL * (dd - PF)];
-            if (u & 1) xb[(u / 2) % (PF / 2)] = x_p[64LL * ((dd - PF) >> 1)];
-            const int i15 = dd - (QR - 2);  // column of row 15
-            double x = hb[u % PH];
-            if ((__builtin_amdgcn_ballot_w64(is_sentinel(x)) >> 60) != 0ULL) {
-                // resynchronise as in the forward sweep (columns descend)
-                const int ahead = max(i15 - PH - HM, 1);
-                double y = load_agent(hin + 4LL * ahead);
-                handoff_wait(y, hin + 4LL * ahead, cons, spin_limit, d.iflag);
-                x = load_agent(hin + 4LL * i15);
+        for (int u = 0; u < CS; u++)
+            bwd_step(A, CS - 1 - u, u, CS * m0 + CS - 1 - u, a, bo, zhi, hb, cons, hin, rin, rz, rout, vl, vc, vo,
+                     spin_limit, d.iflag);
+        wait_lgkm0();
+        wg_barrier();
+        read_chunk(A, sweep_lds, j + 2, vl);
 #pragma unroll
-                for (int m = 1; m < PH; m++) hb[(u + m) % PH] = load_agent(hin + 4LL * (i15 - m));
-                handoff_wait(x, hin + 4LL * i15, cons, spin_limit, d.iflag);
-                wait_all_vmem();
-            }
-            double bin = shift_down4(bo);
-            if (cons) bin = x;
-            const double z = Xc - (a + bin);
-            hb[u % PH] = load_agent(hin + 4LL * (i15 - PH));
-            if (u & 1)
-                z_p[64LL * (dd >> 1)] = make_double2(z, zhi);
-            else
-                zhi = z;
-            const double z0 = quad_bcast<0>(z), z1 = quad_bcast<1>(z), z2 = quad_bcast<2>(z), z3 = quad_bcast<3>(z);
-            a = fma(ha.w, z3, fma(ha.z, z2, fma(ha.y, z1, ha.x * z0)));
-            bo = fma(hbm.w, z3, fma(hbm.z, z2, fma(hbm.y, z1, hbm.x * z0)));
-            // row 0 hands column dd+1 to the strip above
-            if (has_prev && L < 4) store_agent(hout + 4LL * (dd + 1), bo);
-        }
+        for (int u = 0; u < CS; u++)
+            bwd_step(B, CS - 1 - u, CS + u, CS * (m0 - 1) + CS - 1 - u, a, bo, zhi, hb, cons, hin, rin, rz, rout, vl,
+                     vc, vo, spin_limit, d.iflag);
+        wait_lgkm0();
+        wg_barrier();
     }
 }
 
@@ -507,8 +673,7 @@ int FastPath::init(Ctx& c) {
     bool ok = al((void**)&strips, sizeof(QStrip) * hs.size()) && al((void**)&work, sizeof(int2) * hw.size()) &&
               al((void**)&R, sizeof(double) * nslots) && al((void**)&V, sizeof(double) * nslots) &&
               al((void**)&X, sizeof(double) * nslots) && al((void**)&Z, sizeof(double) * nslots) &&
-              al((void**)&GA, sizeof(double) * 4 * nslots) && al((void**)&GB, sizeof(double) * 4 * nslots) &&
-              al((void**)&HA, sizeof(double) * 4 * nslots) && al((void**)&HB, sizeof(double) * 4 * nslots) &&
+              al((void**)&G, sizeof(double) * 8 * nslots) && al((void**)&H, sizeof(double) * 8 * nslots) &&
               al((void**)&hand, sizeof(double) * 2 * hstride * (nstrips + 1)) &&
               al((void**)&part, sizeof(double) * 2 * nwork);
     if (!ok) return SHCFD_E_NOMEM;
@@ -519,7 +684,7 @@ int FastPath::init(Ctx& c) {
 }
 
 void FastPath::release() {
-    void* ptrs[] = {strips, work, R, V, X, Z, GA, GB, HA, HB, hand, part};
+    void* ptrs[] = {strips, work, R, V, X, Z, G, H, hand, part};
     for (void* p : ptrs)
         if (p) hipFree(p);
     *this = FastPath();
@@ -532,11 +697,11 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
     double* hand_f = hand;
     double* hand_b = hand + hstride * (nstrips + 1);
     pb(&c, PROF_PREP);
-    hipLaunchKernelGGL(k_fast_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, R, GA, GB, HA, HB, hand, nstrips,
+    hipLaunchKernelGGL(k_fast_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, R, G, H, hand, nstrips,
                        hstride);
     pe(&c, PROF_PREP);
     pb(&c, PROF_SWEEP_FWD);
-    hipLaunchKernelGGL(k_fast_fwd, dim3(nstrips), dim3(64), 0, st, d, strips, nstrips, R, GA, GB, V, hand_f, hstride,
+    hipLaunchKernelGGL(k_fast_fwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, R, G, V, hand_f, hstride,
                        c.spin_limit);
     pe(&c, PROF_SWEEP_FWD);
     const double* Xin = V;
@@ -547,7 +712,7 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
         Xin = X;
     }
     pb(&c, PROF_SWEEP_BWD);
-    hipLaunchKernelGGL(k_fast_bwd, dim3(nstrips), dim3(64), 0, st, d, strips, nstrips, Xin, HA, HB, Z, hand_b, hstride,
+    hipLaunchKernelGGL(k_fast_bwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, Xin, H, Z, hand_b, hstride,
                        c.spin_limit);
     pe(&c, PROF_SWEEP_BWD);
     pb(&c, PROF_UPDATE);

@@ -13,7 +13,7 @@ struct FastPath {
     QStrip* strips = nullptr;
     int2* work = nullptr;
     double *R = nullptr, *V = nullptr, *X = nullptr, *Z = nullptr;
-    double *GA = nullptr, *GB = nullptr, *HA = nullptr, *HB = nullptr;
+    double *G = nullptr, *H = nullptr;  // (GA, GB) and (HA, HB), split-half layout
     double* hand = nullptr;
     double* part = nullptr;
     int init(Ctx& c);
