@@ -189,6 +189,39 @@ int shcfd_iterate(shcfd_ctx *ctx, int first_step, int nsteps, int graph_steps);
  * choice[4] (may be NULL) = fast sweeps, tile rows, graph steps, ms/step. */
 int shcfd_autotune(shcfd_ctx *ctx, int trial_steps, double *choice);
 
+/* ---- discrete adjoint (no counterpart in the reference, which has no
+ * adjoint; for shape and inflow-angle gradients of steady inviscid
+ * ideal-gas solutions: nY = 1, LU-SGS, optimised implicit path) ---------- */
+/* what of shcfd_adjoint_upload / shcfd_adjoint_download */
+enum {
+    SHCFD_ADJ_LAMBDA = 0, /* adjoint state, dimq per cell (reference cell layout) */
+    SHCFD_ADJ_QBAR = 1,   /* lambda^T dR/dq + dJ/dq per cell                      */
+    SHCFD_ADJ_XBAR = 2    /* lambda^T dR/dX + dJ/dX per node: (x, y) in the cell
+                             layout at the node's index, as node uploads use     */
+};
+/* objective J = sum over the j-min faces i in [i0, i1] of block (0-based
+ * block, 1-based i) of dsj * (TGj_u * ax + TGj_v * ay): the pressure
+ * (momentum) flux through a wall, so ax, ay = -e/(q_inf c) give the force
+ * coefficient along e */
+int shcfd_adjoint_objective(shcfd_ctx *ctx, int block, int i0, int i1, double ax, double ay);
+/* evaluates the residual pipeline at the current state and J */
+int shcfd_objective(shcfd_ctx *ctx, double *J);
+int shcfd_adjoint_upload(shcfd_ctx *ctx, int what, int block, const double *host);
+int shcfd_adjoint_download(shcfd_ctx *ctx, int what, int block, double *host);
+/* bars of the fixed boundary states (npool doubles, the bc pool layout) */
+int shcfd_adjoint_poolbar(shcfd_ctx *ctx, double *host);
+/* one reverse pass at the current state with the current lambda: fills
+ * QBAR, XBAR and the pool bars */
+int shcfd_residual_vjp(shcfd_ctx *ctx);
+/* solves (dR/dq)^T lambda = -(dJ/dq)^T by the transposed LU-SGS iteration
+ * lambda += M^-T (dR/dq^T lambda + dJ/dq^T), starting from the current
+ * lambda, until the RMS of the bracket falls below rtol times its first
+ * value or max_iter iterations; the RMS is checked every check_every
+ * iterations.  On return QBAR, XBAR and the pool bars belong to the final
+ * lambda, so XBAR is dJ/dX and the pool bars dJ/d(fixed states). */
+int shcfd_adjoint_solve(shcfd_ctx *ctx, int max_iter, double rtol, int check_every, double *res0, double *res,
+                        int *iters);
+
 /* ---- per-kernel timing (no counterpart in the reference, whose timers
  * are commented-out stubs: store/debug/time_barrier.f90:1-37) ------------- */
 /* kernel classes */

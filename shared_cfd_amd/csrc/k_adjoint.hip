@@ -1,0 +1,413 @@
+// k_adjoint.hip -- discrete-adjoint (reverse) kernels of the steady inviscid
+// ideal-gas residual
+//     R(c) = dsi(i-1) TGi(i-1) - dsi(i) TGi(i) + dsj(j-1) TGj(j-1) - dsj(j) TGj(j)
+// (LU-SGS/2d/time_lusgs_viscous.f90:12-22 with Area = 0) through the SLAU
+// flux (core/scheme.f90:71-153), MUSCL face states with the limiter
+// (high_order/muscl/muscl.f90:7-73), the boundary ghost fills
+// (cond/no-nV/condition.tail.f90:2-70 and the physical side rows), the
+// primitive/thermo map (core/scheme.f90:1-29,
+// therm_lib/ideal/thermal_model.raw.f90:7-33) and the face metrics
+// (dim/2d/geometry.f90:79-169).
+//
+// Given lambda (one vector per interior cell) and an objective
+//     J = sum over wall faces of dsj (TG_u a_x + TG_v a_y),
+// one pass computes lambda^T dR/dq + dJ/dq per cell, lambda^T dR/dX + dJ/dX
+// per node and the derivative with respect to the fixed boundary states.
+// Local Jacobians come from forward-mode dual evaluations of the same
+// scalar-generic functions the primal kernels use (shcfd_ad.h); their
+// transposes are applied here.  Every output is a gather (one writer per
+// element), except the fixed-state sums, which are reduced per boundary row
+// in a fixed order, so the results are deterministic.
+#include "shcfd_device.h"
+#include "shcfd_kernels.h"
+#include "shcfd_adj.h"
+
+namespace shcfd {
+
+using D1 = Dims<1>;
+
+__device__ inline bool interior(const DevBlock& B, int i, int j) { return i >= 1 && i <= B.ni && j >= 1 && j <= B.nj; }
+
+// ---------------------------------------------------------------------------
+// faces: flux bars -> face-state, normal and length bars.
+// DIR 0: i-face (i,j) between (i,j) and (i+1,j), i in [0,ni], j in [1,nj];
+// DIR 1: j-face (i,j) between (i,j) and (i,j+1), i in [1,ni], j in [0,nj].
+// ---------------------------------------------------------------------------
+constexpr int NFV = 7;  // face-state components the flux depends on: rho,u,v,p,Y,gamma,H
+__device__ inline int fv_comp(int k) { return k < 5 ? k : (k == 5 ? D1::ig : D1::ih); }
+
+template <int DIR>
+__global__ void __launch_bounds__(128) k_adj_face(DevCtx d, AdjView a, int b) {
+    const DevBlock B = d.blk[b];
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + (DIR == 0 ? 0 : 1);
+    const int j = (int)blockIdx.y + (DIR == 0 ? 1 : 0);
+    if (DIR == 0 ? (i > B.ni) : (i > B.ni || j > B.nj)) return;
+    const long long c0 = cidx(B, i, j), c1 = DIR == 0 ? c0 + 1 : c0 + B.pitch;
+    const int i1 = DIR == 0 ? i + 1 : i, j1 = DIR == 0 ? j : j + 1;
+    const bool in0 = interior(B, i, j), in1 = interior(B, i1, j1);
+    const int fds = DIR == 0 ? SHCFD_F_DSI : SHCFD_F_DSJ, fvn = DIR == 0 ? SHCFD_F_VNI : SHCFD_F_VNJ;
+    const int ftg = DIR == 0 ? SHCFD_F_TGI : SHCFD_F_TGJ;
+    const int fl = DIR == 0 ? SHCFD_F_WHLI : SHCFD_F_WHLJ, fr = DIR == 0 ? SHCFD_F_WHRI : SHCFD_F_WHRJ;
+    const double ds = FLD(d, fds, 0, c0);
+    double tgb[4], dsb = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double dl = (in1 ? a.lam[k * d.NC + c1] : 0.0) - (in0 ? a.lam[k * d.NC + c0] : 0.0);
+        tgb[k] = ds * dl;
+        dsb += FLD(d, ftg, k, c0) * dl;
+    }
+    // objective: momentum flux through the wall faces of the objective segment
+    if (DIR == 1 && j == 0 && b == a.obj_block && i >= a.obj_i0 && i <= a.obj_i1) {
+        tgb[1] += ds * a.obj_ax;
+        tgb[2] += ds * a.obj_ay;
+        dsb += FLD(d, ftg, 1, c0) * a.obj_ax + FLD(d, ftg, 2, c0) * a.obj_ay;
+    }
+    using T = Dual<2 * NFV + 2>;
+    T wl[D1::dw], wr[D1::dw], tg[4];
+#pragma unroll
+    for (int k = 0; k < D1::dw; k++) {
+        wl[k] = T(FLD(d, fl, k, c0));
+        wr[k] = T(FLD(d, fr, k, c0));
+    }
+#pragma unroll
+    for (int m = 0; m < NFV; m++) {
+        wl[fv_comp(m)].d[m] = 1.0;
+        wr[fv_comp(m)].d[NFV + m] = 1.0;
+    }
+    const T n1 = T::var(FLD(d, fvn, 0, c0), 2 * NFV), n2 = T::var(FLD(d, fvn, 1, c0), 2 * NFV + 1);
+    slau_face_t<1, T>(wl, wr, n1, n2, tg);
+    const int gl = DIR == 0 ? AdjView::WLI : AdjView::WLJ, gr = DIR == 0 ? AdjView::WRI : AdjView::WRJ;
+#pragma unroll
+    for (int k = 0; k < D1::dw; k++) {
+        a.face[gl][k * d.NC + c0] = 0.0;
+        a.face[gr][k * d.NC + c0] = 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < NFV; m++) {
+        double sl = 0.0, sr = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            sl += tgb[k] * tg[k].d[m];
+            sr += tgb[k] * tg[k].d[NFV + m];
+        }
+        a.face[gl][fv_comp(m) * d.NC + c0] = sl;
+        a.face[gr][fv_comp(m) * d.NC + c0] = sr;
+    }
+    double nb1 = 0.0, nb2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        nb1 += tgb[k] * tg[k].d[2 * NFV];
+        nb2 += tgb[k] * tg[k].d[2 * NFV + 1];
+    }
+    double* nbar = DIR == 0 ? a.nbar_i : a.nbar_j;
+    double* dsbar = DIR == 0 ? a.dsbar_i : a.dsbar_j;
+    nbar[c0] = nb1;
+    nbar[d.NC + c0] = nb2;
+    dsbar[c0] = dsb;
+}
+template __global__ void k_adj_face<0>(DevCtx, AdjView, int);
+template __global__ void k_adj_face<1>(DevCtx, AdjView, int);
+
+// ---------------------------------------------------------------------------
+// MUSCL: face-state bars -> w bars of the stencil cells (gather per cell,
+// ghosts included).  For face f of a line: wL(f) = w_f + hp du(w_{f+1}-w_f,
+// w_f-w_{f-1}), wR(f) = w_{f+1} - hp dl(w_{f+2}-w_{f+1}, w_{f+1}-w_f).
+// ---------------------------------------------------------------------------
+struct MusclD {
+    double l[3];  // dwL(f)/dw_{f-1}, dw_f, dw_{f+1}
+    double r[3];  // dwR(f)/dw_f, dw_{f+1}, dw_{f+2}
+};
+__device__ inline MusclD muscl_derivs(double wm, double w0, double w1, double w2, const DevCtx& d) {
+    MusclD m;
+    if (!d.p.high_order) {
+        m.l[0] = 0.0; m.l[1] = 1.0; m.l[2] = 0.0;
+        m.r[0] = 0.0; m.r[1] = 1.0; m.r[2] = 0.0;
+        return m;
+    }
+    const double hp = d.p.phi_mus * 0.5, kk = d.p.k_mus;
+    const int lim = d.p.limiter;
+    using T = Dual<2>;
+    T du, dl;
+    muscl_delta_t<T>(T::var(w1 - w0, 0), T::var(w0 - wm, 1), kk, lim, du, dl);
+    // du over (Du, Dl) of cell f
+    m.l[0] = -hp * du.d[1];
+    m.l[1] = 1.0 + hp * (du.d[1] - du.d[0]);
+    m.l[2] = hp * du.d[0];
+    T du1, dl1;
+    muscl_delta_t<T>(T::var(w2 - w1, 0), T::var(w1 - w0, 1), kk, lim, du1, dl1);
+    // dl over (Du', Dl') of cell f+1
+    m.r[0] = hp * dl1.d[1];
+    m.r[1] = 1.0 - hp * (dl1.d[1] - dl1.d[0]);
+    m.r[2] = -hp * dl1.d[0];
+    return m;
+}
+
+__global__ void __launch_bounds__(128) k_adj_muscl(DevCtx d, AdjView a, int b) {
+    const DevBlock B = d.blk[b];
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 1;
+    const int j = (int)blockIdx.y - 1;
+    if (i > B.ni + 2 || j > B.nj + 2) return;
+    const long long c = cidx(B, i, j), P = B.pitch;
+    const bool row = j >= 1 && j <= B.nj, col = i >= 1 && i <= B.ni;
+    for (int k = 0; k < D1::dw; k++) {
+        const double* W = d.f[SHCFD_F_W] + (long long)k * d.NC;
+        double s = 0.0;
+        if (row) {
+            // faces f = i-2 .. i+1 of row j (f in [0, ni])
+            for (int f = i - 2; f <= i + 1; f++) {
+                if (f < 0 || f > B.ni) continue;
+                const long long cf = cidx(B, f, j);
+                const MusclD m = muscl_derivs(W[cf - 1], W[cf], W[cf + 1], W[cf + 2], d);
+                const double bl = a.face[AdjView::WLI][k * d.NC + cf], br = a.face[AdjView::WRI][k * d.NC + cf];
+                const int o = i - f;  // position of this cell relative to f
+                if (o >= -1 && o <= 1) s += bl * m.l[o + 1];
+                if (o >= 0 && o <= 2) s += br * m.r[o];
+            }
+        }
+        if (col) {
+            for (int f = j - 2; f <= j + 1; f++) {
+                if (f < 0 || f > B.nj) continue;
+                const long long cf = cidx(B, i, f);
+                const MusclD m = muscl_derivs(W[cf - P], W[cf], W[cf + P], W[cf + 2 * P], d);
+                const double bl = a.face[AdjView::WLJ][k * d.NC + cf], br = a.face[AdjView::WRJ][k * d.NC + cf];
+                const int o = j - f;
+                if (o >= -1 && o <= 1) s += bl * m.l[o + 1];
+                if (o >= 0 && o <= 2) s += br * m.r[o];
+            }
+        }
+        a.wbar[k * d.NC + c] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// boundary rows: ghost bars -> source cells (one launch per row, one thread
+// per position along the side, one workgroup so the fixed-state sums reduce
+// in a fixed order)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_adj_bc_side(DevCtx d, AdjView a, const int* row, double* poolbar) {
+    __shared__ double red[256];
+    const int b = row[0] - 1, side = row[1], kind = row[4], off = row[5];
+    const DevBlock B = d.blk[b];
+    const int n = (side <= SHCFD_SIDE_IMAX) ? B.nj : B.ni;
+    const int s = row[2] < 1 ? 1 : row[2], e = row[3] > n ? n : row[3];
+    double pacc[D1::dw];
+#pragma unroll
+    for (int k = 0; k < D1::dw; k++) pacc[k] = 0.0;
+    for (int tpos = s + (int)threadIdx.x; tpos <= e; tpos += blockDim.x) {
+        int ii, jj, ig1, jg1, ig2, jg2;
+        switch (side) {
+        case SHCFD_SIDE_IMIN: ii = 1; jj = tpos; ig1 = 0; jg1 = tpos; ig2 = -1; jg2 = tpos; break;
+        case SHCFD_SIDE_IMAX: ii = B.ni; jj = tpos; ig1 = B.ni + 1; jg1 = tpos; ig2 = B.ni + 2; jg2 = tpos; break;
+        case SHCFD_SIDE_JMIN: ii = tpos; jj = 1; ig1 = tpos; jg1 = 0; ig2 = tpos; jg2 = -1; break;
+        default: ii = tpos; jj = B.nj; ig1 = tpos; jg1 = B.nj + 1; ig2 = tpos; jg2 = B.nj + 2; break;
+        }
+        const long long ci = cidx(B, ii, jj), c1 = cidx(B, ig1, jg1), c2 = cidx(B, ig2, jg2);
+        double gb[D1::dw];
+#pragma unroll
+        for (int k = 0; k < D1::dw; k++) {
+            gb[k] = a.wbar[k * d.NC + c1] + a.wbar[k * d.NC + c2];
+            a.wbar[k * d.NC + c1] = 0.0;
+            a.wbar[k * d.NC + c2] = 0.0;
+        }
+        if (kind == SHCFD_BC_FIXED) {
+#pragma unroll
+            for (int k = 0; k < D1::dw; k++) pacc[k] += gb[k];
+            continue;
+        }
+        if (kind == SHCFD_BC_REFLECT) {
+            const int kv = (side <= SHCFD_SIDE_IMAX) ? IW_U : IW_V;
+            gb[kv] = -gb[kv];
+        } else if (kind == SHCFD_BC_NOSLIP) {
+            gb[IW_U] = -gb[IW_U];
+            gb[IW_V] = -gb[IW_V];
+        } else if (kind == SHCFD_BC_OUTFLOW_P) {
+            gb[IW_P] = 0.0;
+        } else if (kind == SHCFD_BC_SLIP_NORMAL) {
+            long long cf;
+            double* nbar;
+            if (side == SHCFD_SIDE_IMIN) { cf = cidx(B, 0, jj); nbar = a.nbar_i; }
+            else if (side == SHCFD_SIDE_IMAX) { cf = cidx(B, B.ni, jj); nbar = a.nbar_i; }
+            else if (side == SHCFD_SIDE_JMIN) { cf = cidx(B, ii, 0); nbar = a.nbar_j; }
+            else { cf = cidx(B, ii, B.nj); nbar = a.nbar_j; }
+            const int fv = side <= SHCFD_SIDE_IMAX ? SHCFD_F_VNI : SHCFD_F_VNJ;
+            const double nx = FLD(d, fv, 0, cf), ny = FLD(d, fv, 1, cf);
+            const double u = FLD(d, SHCFD_F_W, IW_U, ci), v = FLD(d, SHCFD_F_W, IW_V, ci);
+            // ghost velocity g = u - 2 (n.u) n
+            const double ub = gb[IW_U], vb = gb[IW_V];
+            const double un = nx * u + ny * v, bn = nx * ub + ny * vb;
+            gb[IW_U] = ub - 2.0 * bn * nx;
+            gb[IW_V] = vb - 2.0 * bn * ny;
+            nbar[cf] += -2.0 * (bn * u + un * ub);
+            nbar[d.NC + cf] += -2.0 * (bn * v + un * vb);
+        }
+#pragma unroll
+        for (int k = 0; k < D1::dw; k++) a.wbar[k * d.NC + ci] += gb[k];
+    }
+    if (kind != SHCFD_BC_FIXED) return;
+    for (int k = 0; k < D1::dw; k++) {
+        red[threadIdx.x] = pacc[k];
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) poolbar[off + k] += red[0];
+        __syncthreads();
+    }
+}
+
+// one cut row: thread per position, copies in the order of the forward fill
+__global__ void k_adj_bc_cut(DevCtx d, AdjView a, const int* t) {
+    const int inc = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (inc >= t[1]) return;
+    for (int copy = 3; copy >= 0; copy--) {
+        int bd, di, dj, bs, si, sj;
+        if (!cut_map(t, inc, copy, bd, di, dj, bs, si, sj)) return;
+        const long long g = cidx(d.blk[bd], di, dj), src = cidx(d.blk[bs], si, sj);
+#pragma unroll
+        for (int k = 0; k < D1::dw; k++) {
+            a.wbar[k * d.NC + src] += a.wbar[k * d.NC + g];
+            a.wbar[k * d.NC + g] = 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// cells: w bars -> q bars through set_w and the ideal-gas thermo map
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline void ideal_w(const T* q, double kap, double rg, double nu, T* w) {
+    T rho = 0.0 + q[0];
+    T inv = 1.0 / rho;
+    T u = q[1] * inv, v = q[2] * inv;
+    w[IW_RHO] = rho;
+    w[IW_U] = u;
+    w[IW_V] = v;
+    w[IW_Y(0)] = q[0] * inv;
+    T ei = q[3] / rho - 0.5 * ((u * u) + (v * v));
+    T Tt = (ei * (kap - 1.0)) / rg;
+    T p = (rho * rg) * Tt;
+    w[IW_P] = p;
+    w[D1::ig] = T(kap);
+    w[D1::ih] = (q[3] + p) / rho;
+    w[D1::imu] = rho * nu;
+    w[D1::ir] = T(rg);
+}
+
+__global__ void __launch_bounds__(128) k_adj_cell(DevCtx d, AdjView a, int b) {
+    const DevBlock B = d.blk[b];
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1, j = (int)blockIdx.y + 1;
+    if (i > B.ni || j > B.nj) return;
+    const long long c = cidx(B, i, j);
+    using T = Dual<4>;
+    T q[4], w[D1::dw];
+#pragma unroll
+    for (int k = 0; k < 4; k++) q[k] = T::var(FLD(d, SHCFD_F_Q, k, c), k);
+    ideal_w<T>(q, d.p.kappa, d.p.rgas, B.nu, w);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < D1::dw; m++) s += a.wbar[m * d.NC + c] * w[m].d[k];
+        a.qbar[k * d.NC + c] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// nodes: face length and normal bars -> node coordinate bars.
+// j-face (i,j): A = node (i-1,j), B = node (i,j), n = (-dy, dx)/|d|;
+// i-face (i,j): A = node (i,j-1), B = node (i,j), n = (dy, -dx)/|d|.
+// ---------------------------------------------------------------------------
+__device__ inline void face_geom_bar(double ax, double ay, double bx, double by, double nb1, double nb2, double dsb,
+                                     bool jface, double& gx, double& gy) {
+    const double dx = bx - ax, dy = by - ay;
+    const double L = sqrt(dx * dx + dy * dy);
+    const double n1 = jface ? -dy / L : dy / L, n2 = jface ? dx / L : -dx / L;
+    const double dot = nb1 * n1 + nb2 * n2;
+    const double m1 = (nb1 - dot * n1) / L, m2 = (nb2 - dot * n2) / L;
+    gx = dsb * dx / L + (jface ? m2 : -m2);
+    gy = dsb * dy / L + (jface ? -m1 : m1);
+}
+
+__global__ void __launch_bounds__(128) k_adj_nodes(DevCtx d, AdjView a, int b) {
+    const DevBlock B = d.blk[b];
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)blockIdx.y;
+    if (i > B.ni || j > B.nj) return;
+    auto X = [&](int ii, int jj) { return FLD(d, SHCFD_F_XH, 0, cidx(B, ii, jj)); };
+    auto Y = [&](int ii, int jj) { return FLD(d, SHCFD_F_RH, 0, cidx(B, ii, jj)); };
+    double sx = 0.0, sy = 0.0, gx, gy;
+    // j-face (i,j): this node is B
+    if (i >= 1) {
+        const long long f = cidx(B, i, j);
+        face_geom_bar(X(i - 1, j), Y(i - 1, j), X(i, j), Y(i, j), a.nbar_j[f], a.nbar_j[d.NC + f], a.dsbar_j[f], true,
+                      gx, gy);
+        sx += gx; sy += gy;
+    }
+    // j-face (i+1,j): this node is A
+    if (i + 1 <= B.ni) {
+        const long long f = cidx(B, i + 1, j);
+        face_geom_bar(X(i, j), Y(i, j), X(i + 1, j), Y(i + 1, j), a.nbar_j[f], a.nbar_j[d.NC + f], a.dsbar_j[f], true,
+                      gx, gy);
+        sx -= gx; sy -= gy;
+    }
+    // i-face (i,j): this node is B
+    if (j >= 1) {
+        const long long f = cidx(B, i, j);
+        face_geom_bar(X(i, j - 1), Y(i, j - 1), X(i, j), Y(i, j), a.nbar_i[f], a.nbar_i[d.NC + f], a.dsbar_i[f], false,
+                      gx, gy);
+        sx += gx; sy += gy;
+    }
+    // i-face (i,j+1): this node is A
+    if (j + 1 <= B.nj) {
+        const long long f = cidx(B, i, j + 1);
+        face_geom_bar(X(i, j), Y(i, j), X(i, j + 1), Y(i, j + 1), a.nbar_i[f], a.nbar_i[d.NC + f], a.dsbar_i[f], false,
+                      gx, gy);
+        sx -= gx; sy -= gy;
+    }
+    const long long c = cidx(B, i, j);
+    a.xbar[c] = sx;
+    a.ybar[c] = sy;
+}
+
+// objective value (one workgroup, fixed-order reduction)
+__global__ void __launch_bounds__(256) k_adj_objective(DevCtx d, AdjView a, double* out) {
+    __shared__ double red[256];
+    const DevBlock B = d.blk[a.obj_block];
+    double s = 0.0;
+    for (int i = a.obj_i0 + (int)threadIdx.x; i <= a.obj_i1; i += blockDim.x) {
+        const long long c = cidx(B, i, 0);
+        s += FLD(d, SHCFD_F_DSJ, 0, c) * (FLD(d, SHCFD_F_TGJ, 1, c) * a.obj_ax + FLD(d, SHCFD_F_TGJ, 2, c) * a.obj_ay);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// sum of squares of the q bars over interior cells: per-workgroup partials
+__global__ void __launch_bounds__(256) k_adj_norm_part(DevCtx d, AdjView a, int b, double* part) {
+    __shared__ double red[256];
+    const DevBlock B = d.blk[b];
+    const long long n = (long long)B.ni * B.nj;
+    double s = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const long long c = cidx(B, (int)(t % B.ni) + 1, (int)(t / B.ni) + 1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double x = a.qbar[k * d.NC + c];
+            s += x * x;
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+}  // namespace shcfd

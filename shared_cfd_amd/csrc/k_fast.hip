@@ -625,6 +625,120 @@ __global__ void __launch_bounds__(256) k_fast_post(DevCtx d, const QStrip* strip
     }
 }
 
+// ---------------------------------------------------------------------------
+// Discrete adjoint: transposed LU-SGS.  With the folded operators above,
+// one LU-SGS update is Dq = diag(alpha) (I + U)^-1 (I - L)^-1 R, where L holds
+// GA, GB at the slots of the upwind neighbours and U holds HA, HB.  Its
+// transpose applied to r is
+//     y(c) = alpha(c) r(c) - HA(c)^T y(c - e_i) - HB(c)^T y(c - e_j)
+//     x(c) = y(c) + GA(c)^T x(c + e_i) + GB(c)^T x(c + e_j),
+// the same two sweeps with operators stored at the neighbour's slot:
+//     GA'(c) = -HA(c + e_i)^T,  GB'(c) = -HB(c + e_j)^T,
+//     HA'(c) = -GA(c - e_i)^T,  HB'(c) = -GB(c - e_j)^T,
+// zero where the neighbour is outside the block.  LU-SGS (S = I) only.
+// ---------------------------------------------------------------------------
+// one transposed operator of neighbour cn: -(fs * M)^T in lane rows
+__device__ inline void put_transposed(double* o, const QStrip& S, int dd, int r, int pbase, const double* M, double fs,
+                                      bool present) {
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+            o[op_index(S, dd, 4 * r + k, pbase + (l >> 1), l & 1)] = present ? -(fs * M[l + 4 * k]) : 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_fast_adj_prep(DevCtx d, const QStrip* strips, const int2* work, double* G,
+                                                       double* H) {
+    const int2 wk = work[blockIdx.x];
+    const QStrip S = strips[wk.x];
+    const DevBlock B = d.blk[S.b];
+    const int r = threadIdx.x & 15;
+    const int dd = wk.y + (threadIdx.x >> 4);
+    int i, j;
+    if (!slot_cell(S, B, dd, r, i, j)) return;
+    const long long c = cidx(B, i, j), P = B.pitch;
+    {
+        CellJac<1> J;
+        cell_jacobian<1>(d, c, P, J);
+        FLD(d, SHCFD_F_ALPHA, 0, c) = J.alpha;
+    }
+    // GA' from HA(c+e_i): dsci alpha Am of the right neighbour
+    {
+        CellJac<1> J;
+        const bool ok = i < B.ni;
+        const long long cn = ok ? c + 1 : c;
+        cell_jacobian<1>(d, cn, P, J);
+        put_transposed(G, S, dd, r, 0, J.Am, FLD(d, SHCFD_F_DSCI, 0, cn) * J.alpha, ok);
+        // HA' from GA(c-e_i)
+    }
+    {
+        CellJac<1> J;
+        const bool ok = j < B.nj;
+        const long long cn = ok ? c + P : c;
+        cell_jacobian<1>(d, cn, P, J);
+        put_transposed(G, S, dd, r, 2, J.Bm, FLD(d, SHCFD_F_DSCJ, 0, cn) * J.alpha, ok);
+    }
+    {
+        CellJac<1> J;
+        const bool ok = i > 1;
+        const long long cn = ok ? c - 1 : c;
+        cell_jacobian<1>(d, cn, P, J);
+        put_transposed(H, S, dd, r, 0, J.Ap, FLD(d, SHCFD_F_DSCI, 0, cn) * J.alpha, ok);
+    }
+    {
+        CellJac<1> J;
+        const bool ok = j > 1;
+        const long long cn = ok ? c - P : c;
+        cell_jacobian<1>(d, cn, P, J);
+        put_transposed(H, S, dd, r, 2, J.Bp, FLD(d, SHCFD_F_DSCJ, 0, cn) * J.alpha, ok);
+    }
+}
+
+// right-hand side of the transposed sweep: alpha r into the skewed layout
+__global__ void __launch_bounds__(256) k_fast_adj_rhs(DevCtx d, const QStrip* strips, const int2* work,
+                                                      const double* r_cell, double* R_sl) {
+    const int2 wk = work[blockIdx.x];
+    const QStrip S = strips[wk.x];
+    const DevBlock B = d.blk[S.b];
+    const int r = threadIdx.x & 15;
+    const int dd = wk.y + (threadIdx.x >> 4);
+    int i, j;
+    if (!slot_cell(S, B, dd, r, i, j)) return;
+    const long long c = cidx(B, i, j);
+    const double al = FLD(d, SHCFD_F_ALPHA, 0, c);
+#pragma unroll
+    for (int k = 0; k < 4; k++) R_sl[slot1(S.base, dd, 4 * r + k)] = al * r_cell[k * d.NC + c];
+}
+
+// lambda += x (no alpha: the transposed product ends with the forward sweep's L^T)
+__global__ void __launch_bounds__(256) k_fast_adj_post(DevCtx d, const QStrip* strips, const int2* work,
+                                                       const double* Z_sl, double* lam) {
+    const int2 wk = work[blockIdx.x];
+    const QStrip S = strips[wk.x];
+    const DevBlock B = d.blk[S.b];
+    const int r = threadIdx.x & 15;
+    const int dd = wk.y + (threadIdx.x >> 4);
+    int i, j;
+    if (!slot_cell(S, B, dd, r, i, j)) return;
+    const long long c = cidx(B, i, j);
+#pragma unroll
+    for (int k = 0; k < 4; k++) lam[k * d.NC + c] += Z_sl[slot1(S.base, dd, 4 * r + k)];
+}
+
+// refill of the hand-off rows alone (the adjoint sweeps do not run k_fast_prep)
+__global__ void __launch_bounds__(256) k_fast_hand_fill(DevCtx d, const QStrip* strips, double* hand, int nstrips,
+                                                        long long hstride) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long nhand = 2LL * (nstrips + 1) * hstride;
+    const double sv = __longlong_as_double((long long)SENTINEL);
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nhand; k += stride) {
+        const int row = (int)((k / hstride) % (nstrips + 1));
+        const long long col = (k % hstride) / 4 - HOFF;
+        const bool live = row < nstrips && col >= 1 && col <= d.blk[strips[row].b].ni;
+        hand[k] = live ? sv : 0.0;
+    }
+}
+
 }  // namespace shcfd
 
 // ---------------------------------------------------------------------------
@@ -684,7 +798,7 @@ int FastPath::init(Ctx& c) {
 }
 
 void FastPath::release() {
-    void* ptrs[] = {strips, work, R, V, X, Z, G, H, hand, part};
+    void* ptrs[] = {strips, work, R, V, X, Z, G, H, hand, part, aR, aV, aZ, aG, aH, ahand};
     for (void* p : ptrs)
         if (p) hipFree(p);
     *this = FastPath();
@@ -723,6 +837,36 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
         hipLaunchKernelGGL(k_update<1>, dim3((unsigned)c.htiles.size()), dim3(64 * c.tile_rows), 0, st, d, c.dtiles);
     }
     pe(&c, PROF_UPDATE);
+    return hipGetLastError() == hipSuccess ? SHCFD_OK : SHCFD_E_HIP;
+}
+
+int FastPath::adjoint_prep(Ctx& c) {
+    if (c.p.scheme != SHCFD_SCHEME_LUSGS) return SHCFD_E_STATE;
+    if (!aG) {
+        auto al = [&](void** p, size_t bytes) {
+            if (hipMalloc(p, bytes) != hipSuccess) return false;
+            return hipMemset(*p, 0, bytes) == hipSuccess;
+        };
+        bool ok = al((void**)&aR, sizeof(double) * nslots) && al((void**)&aV, sizeof(double) * nslots) &&
+                  al((void**)&aZ, sizeof(double) * nslots) && al((void**)&aG, sizeof(double) * 8 * nslots) &&
+                  al((void**)&aH, sizeof(double) * 8 * nslots) &&
+                  al((void**)&ahand, sizeof(double) * 2 * hstride * (nstrips + 1));
+        if (!ok) return SHCFD_E_NOMEM;
+    }
+    hipLaunchKernelGGL(k_fast_adj_prep, dim3(nwork), dim3(256), 0, c.stream, c.dev(), strips, work, aG, aH);
+    return hipGetLastError() == hipSuccess ? SHCFD_OK : SHCFD_E_HIP;
+}
+
+int FastPath::adjoint_apply(Ctx& c, const double* r_cell, double* lam) {
+    DevCtx d = c.dev();
+    hipStream_t st = c.stream;
+    hipLaunchKernelGGL(k_fast_adj_rhs, dim3(nwork), dim3(256), 0, st, d, strips, work, r_cell, aR);
+    hipLaunchKernelGGL(k_fast_hand_fill, dim3(256), dim3(256), 0, st, d, strips, ahand, nstrips, hstride);
+    hipLaunchKernelGGL(k_fast_fwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, aR, aG, aV, ahand, hstride,
+                       c.spin_limit);
+    hipLaunchKernelGGL(k_fast_bwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, aV, aH, aZ,
+                       ahand + hstride * (nstrips + 1), hstride, c.spin_limit);
+    hipLaunchKernelGGL(k_fast_adj_post, dim3(nwork), dim3(256), 0, st, d, strips, work, aZ, lam);
     return hipGetLastError() == hipSuccess ? SHCFD_OK : SHCFD_E_HIP;
 }
 

@@ -1,0 +1,39 @@
+// shcfd_adj.h -- device view and host state of the discrete adjoint
+// (k_adjoint.hip, the transposed sweeps in k_fast.hip, shcfd_adjoint.hip).
+#pragma once
+#include "shcfd_internal.h"
+
+namespace shcfd {
+
+// Arrays indexed like the cell fields (component k of cell c at k*NC + c);
+// face arrays are stored at the index of the face's left/lower cell, node
+// arrays at the index of the node.
+struct AdjView {
+    enum { WLI = 0, WRI = 1, WLJ = 2, WRJ = 3 };
+    double* lam;       // adjoint state (4 per interior cell)
+    double* qbar;      // lambda^T dR/dq + dJ/dq (4 per interior cell)
+    double* wbar;      // bars of w (dimw per cell, ghosts included)
+    double* face[4];   // bars of the MUSCL face states (dimw per face)
+    double* nbar_i;    // bars of the i-face unit normals (2 per face)
+    double* nbar_j;
+    double* dsbar_i;   // bars of the face lengths
+    double* dsbar_j;
+    double* xbar;      // bars of the node coordinates
+    double* ybar;
+    int obj_block, obj_i0, obj_i1;  // wall faces j = 0, i in [i0, i1] of a block
+    double obj_ax, obj_ay;          // objective weights of the momentum fluxes
+};
+
+struct Adjoint {
+    bool ready = false;
+    double* mem = nullptr;     // one allocation for all arrays of AdjView
+    double* poolbar = nullptr; // bars of the fixed boundary states (npool)
+    double* part = nullptr;    // reduction partials
+    double* objval = nullptr;  // objective value
+    int npart = 0;
+    AdjView v{};
+    int init(Ctx& c);
+    void release();
+};
+
+}  // namespace shcfd

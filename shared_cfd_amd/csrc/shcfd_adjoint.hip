@@ -1,0 +1,302 @@
+// shcfd_adjoint.hip -- C-ABI of the discrete adjoint (steady, inviscid, ideal
+// gas, LU-SGS on the optimised implicit path).  The reference has no adjoint;
+// these entry points drive the reverse kernels of k_adjoint.hip and the
+// transposed sweeps of k_fast.hip.
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "../../include/shcfd.h"
+#include "shcfd_kernels.h"
+#include "shcfd_adj.h"
+#include "shcfd_fast.h"
+
+// the opaque handle of the C ABI is the context itself (as in shcfd_api.hip)
+struct shcfd_ctx : public shcfd::Ctx {};
+
+namespace shcfd {
+
+// k_adjoint.hip
+template <int DIR> __global__ void k_adj_face(DevCtx, AdjView, int);
+__global__ void k_adj_muscl(DevCtx, AdjView, int);
+__global__ void k_adj_bc_side(DevCtx, AdjView, const int*, double*);
+__global__ void k_adj_bc_cut(DevCtx, AdjView, const int*);
+__global__ void k_adj_cell(DevCtx, AdjView, int);
+__global__ void k_adj_nodes(DevCtx, AdjView, int);
+__global__ void k_adj_objective(DevCtx, AdjView, double*);
+__global__ void k_adj_norm_part(DevCtx, AdjView, int, double*);
+// shcfd_api.hip
+bool fast_ready(Ctx* ctx);
+
+int Adjoint::init(Ctx& c) {
+    const long long NC = c.NC;
+    const long long n = NC * (4 + 4 + 9 + 4 * 9 + 2 + 2 + 1 + 1 + 1 + 1);
+    npart = 256;
+    if (hipMalloc((void**)&mem, sizeof(double) * n) != hipSuccess) return SHCFD_E_NOMEM;
+    if (hipMalloc((void**)&poolbar, sizeof(double) * (c.npool > 0 ? c.npool : 1)) != hipSuccess) return SHCFD_E_NOMEM;
+    if (hipMalloc((void**)&part, sizeof(double) * (npart + 1)) != hipSuccess) return SHCFD_E_NOMEM;
+    objval = part + npart;
+    if (hipMemset(mem, 0, sizeof(double) * n) != hipSuccess) return SHCFD_E_HIP;
+    if (hipMemset(poolbar, 0, sizeof(double) * (c.npool > 0 ? c.npool : 1)) != hipSuccess) return SHCFD_E_HIP;
+    double* p = mem;
+    auto take = [&](long long k) { double* r = p; p += k * NC; return r; };
+    v.lam = take(4);
+    v.qbar = take(4);
+    v.wbar = take(9);
+    for (int f = 0; f < 4; f++) v.face[f] = take(9);
+    v.nbar_i = take(2);
+    v.nbar_j = take(2);
+    v.dsbar_i = take(1);
+    v.dsbar_j = take(1);
+    v.xbar = take(1);
+    v.ybar = take(1);
+    v.obj_block = -1;
+    ready = true;
+    return SHCFD_OK;
+}
+
+void Adjoint::release() {
+    void* ptrs[] = {mem, poolbar, part};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    *this = Adjoint();
+}
+
+void adjoint_release(Ctx* c) {
+    if (c->adj) {
+        c->adj->release();
+        delete c->adj;
+        c->adj = nullptr;
+    }
+}
+
+}  // namespace shcfd
+
+using namespace shcfd;
+
+namespace {
+
+int supported(shcfd_ctx* ctx) {
+    if (ctx->ny != 1 || ctx->p.viscous || ctx->p.thermal != SHCFD_THERMAL_IDEAL || ctx->p.scheme != SHCFD_SCHEME_LUSGS)
+        return SHCFD_E_STATE;
+    return SHCFD_OK;
+}
+
+int ensure(shcfd_ctx* ctx) {
+    int rc = supported(ctx);
+    if (rc) return rc;
+    if (!ctx->adj) {
+        ctx->adj = new (std::nothrow) Adjoint();
+        if (!ctx->adj) return SHCFD_E_NOMEM;
+        rc = ctx->adj->init(*ctx);
+        if (rc) { adjoint_release(ctx); return rc; }
+    }
+    return SHCFD_OK;
+}
+
+// residual pipeline at the current state (face states, fluxes, time steps)
+int residual_state(shcfd_ctx* ctx) {
+    int rc = shcfd_set_w(ctx);
+    if (!rc) rc = shcfd_set_thermo_prop(ctx, 0);
+    if (!rc) rc = shcfd_set_bc(ctx, 0);
+    if (!rc) rc = shcfd_set_hv(ctx);
+    if (!rc) rc = shcfd_set_tg(ctx);
+    return rc;
+}
+
+// one reverse pass with the current lambda (launches only)
+void vjp_async(shcfd_ctx* ctx) {
+    DevCtx d = ctx->dev();
+    AdjView& a = ctx->adj->v;
+    hipStream_t st = ctx->stream;
+    for (int b = 0; b < ctx->nb; b++) {
+        const DevBlock& B = ctx->hblk[b];
+        hipLaunchKernelGGL(k_adj_face<0>, dim3((unsigned)((B.ni + 1 + 127) / 128), (unsigned)B.nj), dim3(128), 0, st,
+                           d, a, b);
+        hipLaunchKernelGGL(k_adj_face<1>, dim3((unsigned)((B.ni + 127) / 128), (unsigned)(B.nj + 1)), dim3(128), 0, st,
+                           d, a, b);
+    }
+    for (int b = 0; b < ctx->nb; b++) {
+        const DevBlock& B = ctx->hblk[b];
+        hipLaunchKernelGGL(k_adj_muscl, dim3((unsigned)((B.ni + 4 + 127) / 128), (unsigned)(B.nj + 4)), dim3(128), 0, st,
+                           d, a, b);
+    }
+    hipMemsetAsync(ctx->adj->poolbar, 0, sizeof(double) * (ctx->npool > 0 ? ctx->npool : 1), st);
+    // reverse of the forward fill order: corners (unused by inviscid fluxes),
+    // side rows, cut rows
+    for (int r = ctx->nbc - 1; r >= 0; r--)
+        hipLaunchKernelGGL(k_adj_bc_side, dim3(1), dim3(256), 0, st, d, a, ctx->dbc + SHCFD_BC_ROW * r,
+                           ctx->adj->poolbar);
+    for (int r = ctx->ncut - 1; r >= 0; r--) {
+        const int w = ctx->hcut[SHCFD_CUT_ROW * r + 1];
+        if (w > 0)
+            hipLaunchKernelGGL(k_adj_bc_cut, dim3((unsigned)((w + 127) / 128)), dim3(128), 0, st, d, a,
+                               ctx->dcut + SHCFD_CUT_ROW * r);
+    }
+    for (int b = 0; b < ctx->nb; b++) {
+        const DevBlock& B = ctx->hblk[b];
+        hipLaunchKernelGGL(k_adj_cell, dim3((unsigned)((B.ni + 127) / 128), (unsigned)B.nj), dim3(128), 0, st, d, a, b);
+        hipLaunchKernelGGL(k_adj_nodes, dim3((unsigned)((B.ni + 1 + 127) / 128), (unsigned)(B.nj + 1)), dim3(128), 0,
+                           st, d, a, b);
+    }
+}
+
+// RMS of the q bars over all interior cells (synchronises)
+int qbar_rms(shcfd_ctx* ctx, double* out) {
+    DevCtx d = ctx->dev();
+    Adjoint& A = *ctx->adj;
+    double total = 0.0;
+    long long cells = 0;
+    std::vector<double> h(A.npart);
+    for (int b = 0; b < ctx->nb; b++) {
+        hipLaunchKernelGGL(k_adj_norm_part, dim3(A.npart), dim3(256), 0, ctx->stream, d, A.v, b, A.part);
+        if (hipMemcpyAsync(h.data(), A.part, sizeof(double) * A.npart, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            return SHCFD_E_HIP;
+        for (int k = 0; k < A.npart; k++) total += h[k];
+        cells += (long long)ctx->hblk[b].ni * ctx->hblk[b].nj;
+    }
+    *out = std::sqrt(total / (double)cells);
+    return std::isfinite(*out) ? SHCFD_OK : SHCFD_E_NAN;
+}
+
+// host <-> device copy of K cell-indexed components of one block in the
+// reference layout (k fastest, then i = -1..ni+2, then j = -1..nj+2)
+int copy_block(shcfd_ctx* ctx, double* const* dev, int K, int block, double* host, bool to_device) {
+    const DevBlock& B = ctx->hblk[block];
+    const size_t rows = (size_t)B.nj + 4, w = (size_t)B.ni + 4;
+    std::vector<double> plane(rows * B.pitch, 0.0);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return SHCFD_E_HIP;
+    for (int k = 0; k < K; k++) {
+        double* base = dev[k] + B.off;
+        if (to_device) {
+            for (size_t j = 0; j < rows; j++)
+                for (size_t i = 0; i < w; i++) plane[j * B.pitch + i] = host[k + (size_t)K * (i + w * j)];
+            if (hipMemcpy(base, plane.data(), sizeof(double) * plane.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return SHCFD_E_HIP;
+        } else {
+            if (hipMemcpy(plane.data(), base, sizeof(double) * plane.size(), hipMemcpyDeviceToHost) != hipSuccess)
+                return SHCFD_E_HIP;
+            for (size_t j = 0; j < rows; j++)
+                for (size_t i = 0; i < w; i++) host[k + (size_t)K * (i + w * j)] = plane[j * B.pitch + i];
+        }
+    }
+    return SHCFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int shcfd_adjoint_objective(shcfd_ctx* ctx, int block, int i0, int i1, double ax, double ay) {
+    if (!ctx || block < 0 || block >= ctx->nb || i0 < 1 || i1 > ctx->hblk[block].ni || i0 > i1) return SHCFD_E_ARG;
+    int rc = ensure(ctx);
+    if (rc) return rc;
+    AdjView& a = ctx->adj->v;
+    a.obj_block = block;
+    a.obj_i0 = i0;
+    a.obj_i1 = i1;
+    a.obj_ax = ax;
+    a.obj_ay = ay;
+    return SHCFD_OK;
+}
+
+int shcfd_objective(shcfd_ctx* ctx, double* J) {
+    if (!ctx || !J) return SHCFD_E_ARG;
+    int rc = ensure(ctx);
+    if (!rc && ctx->adj->v.obj_block < 0) rc = SHCFD_E_STATE;
+    if (!rc) rc = residual_state(ctx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_adj_objective, dim3(1), dim3(256), 0, ctx->stream, ctx->dev(), ctx->adj->v, ctx->adj->objval);
+    if (hipMemcpyAsync(J, ctx->adj->objval, sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return SHCFD_E_HIP;
+    return shcfd_check_error(ctx);
+}
+
+int shcfd_adjoint_upload(shcfd_ctx* ctx, int what, int block, const double* host) {
+    if (!ctx || !host || block < 0 || block >= ctx->nb || what != SHCFD_ADJ_LAMBDA) return SHCFD_E_ARG;
+    int rc = ensure(ctx);
+    if (rc) return rc;
+    double* dev[4];
+    for (int k = 0; k < 4; k++) dev[k] = ctx->adj->v.lam + (long long)k * ctx->NC;
+    return copy_block(ctx, dev, 4, block, const_cast<double*>(host), true);
+}
+
+int shcfd_adjoint_download(shcfd_ctx* ctx, int what, int block, double* host) {
+    if (!ctx || !host || block < 0 || block >= ctx->nb) return SHCFD_E_ARG;
+    int rc = ensure(ctx);
+    if (rc) return rc;
+    AdjView& a = ctx->adj->v;
+    double* dev[4];
+    switch (what) {
+    case SHCFD_ADJ_LAMBDA:
+        for (int k = 0; k < 4; k++) dev[k] = a.lam + (long long)k * ctx->NC;
+        return copy_block(ctx, dev, 4, block, host, false);
+    case SHCFD_ADJ_QBAR:
+        for (int k = 0; k < 4; k++) dev[k] = a.qbar + (long long)k * ctx->NC;
+        return copy_block(ctx, dev, 4, block, host, false);
+    case SHCFD_ADJ_XBAR:
+        dev[0] = a.xbar;
+        dev[1] = a.ybar;
+        return copy_block(ctx, dev, 2, block, host, false);
+    default:
+        return SHCFD_E_ARG;
+    }
+}
+
+int shcfd_adjoint_poolbar(shcfd_ctx* ctx, double* host) {
+    if (!ctx || !host) return SHCFD_E_ARG;
+    int rc = ensure(ctx);
+    if (rc) return rc;
+    if (ctx->npool <= 0) return SHCFD_OK;
+    if (hipMemcpyAsync(host, ctx->adj->poolbar, sizeof(double) * ctx->npool, hipMemcpyDeviceToHost, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return SHCFD_E_HIP;
+    return SHCFD_OK;
+}
+
+int shcfd_residual_vjp(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    int rc = ensure(ctx);
+    if (!rc && ctx->adj->v.obj_block < 0) rc = SHCFD_E_STATE;
+    if (!rc) rc = residual_state(ctx);
+    if (rc) return rc;
+    vjp_async(ctx);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) return SHCFD_E_HIP;
+    return shcfd_check_error(ctx);
+}
+
+int shcfd_adjoint_solve(shcfd_ctx* ctx, int max_iter, double rtol, int check_every, double* res0, double* res,
+                        int* iters) {
+    if (!ctx || max_iter < 0 || check_every < 1) return SHCFD_E_ARG;
+    int rc = ensure(ctx);
+    if (!rc && ctx->adj->v.obj_block < 0) rc = SHCFD_E_STATE;
+    if (!rc && !fast_ready(ctx)) rc = SHCFD_E_STATE;
+    if (!rc) rc = residual_state(ctx);
+    if (!rc) rc = shcfd_set_dt(ctx, 0, nullptr);
+    if (!rc) rc = ctx->fast->adjoint_prep(*ctx);
+    if (rc) return rc;
+    AdjView& a = ctx->adj->v;
+    double r0 = 0.0, r = 0.0;
+    int it = 0;
+    vjp_async(ctx);
+    rc = qbar_rms(ctx, &r0);
+    r = r0;
+    while (!rc && it < max_iter && r > rtol * r0) {
+        rc = ctx->fast->adjoint_apply(*ctx, a.qbar, a.lam);
+        if (rc) break;
+        vjp_async(ctx);
+        it++;
+        if (it % check_every == 0 || it == max_iter) rc = qbar_rms(ctx, &r);
+    }
+    if (!rc) rc = qbar_rms(ctx, &r);
+    if (res0) *res0 = r0;
+    if (res) *res = r;
+    if (iters) *iters = it;
+    if (!rc) rc = shcfd_check_error(ctx);
+    return rc;
+}
+
+}  // extern "C"

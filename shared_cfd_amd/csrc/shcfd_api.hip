@@ -187,6 +187,7 @@ int shcfd_destroy(shcfd_ctx* ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     drop_graph(ctx);
     if (ctx->fast) { ctx->fast->release(); delete ctx->fast; ctx->fast = nullptr; }
+    adjoint_release(ctx);
     for (auto& e : ctx->prof_free) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
     for (int k = 0; k < PROF_COUNT; k++)
         for (auto& e : ctx->prof_pending[k]) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
@@ -675,6 +676,13 @@ static bool fast_enabled(shcfd_ctx* ctx) {
         }
     }
     return true;
+}
+
+extern "C++" {
+namespace shcfd {
+// the optimised implicit path of this context, created on first use
+bool fast_ready(Ctx* ctx) { return fast_enabled(static_cast<shcfd_ctx*>(ctx)); }
+}  // namespace shcfd
 }
 
 static int implicit_async(shcfd_ctx* ctx, int step_internal) {

@@ -16,9 +16,13 @@ struct FastPath {
     double *G = nullptr, *H = nullptr;  // (GA, GB) and (HA, HB), split-half layout
     double* hand = nullptr;
     double* part = nullptr;
+    // transposed sweeps of the discrete adjoint (allocated on first use)
+    double *aR = nullptr, *aV = nullptr, *aZ = nullptr, *aG = nullptr, *aH = nullptr, *ahand = nullptr;
     int init(Ctx& c);
     void release();
     int implicit(Ctx& c, int step_internal, void (*prof_begin)(Ctx*, int), void (*prof_end)(Ctx*, int));
+    int adjoint_prep(Ctx& c);                                      // transposed operators at the current state
+    int adjoint_apply(Ctx& c, const double* r_cell, double* lam);  // lam += M^-T r
 };
 
 }  // namespace shcfd

@@ -100,6 +100,8 @@ struct Ctx {
     EventPair prof_open[PROF_COUNT] = {};
     // optimised implicit update (k_fast.hip); owned, created on first use
     struct FastPath* fast = nullptr;
+    // discrete adjoint (shcfd_adjoint.hip); owned, created on first use
+    struct Adjoint* adj = nullptr;
     int use_fast = 1;              // 1: optimised sweeps when available
     int nb = 0, ny = 0, dimq = 0, dimw = 0;
     shcfd_params p{};
@@ -162,5 +164,8 @@ struct Ctx {
 };
 
 int field_ncomp(int fld, int ny);
+
+// frees the adjoint state of a context (shcfd_adjoint.hip)
+void adjoint_release(Ctx* c);
 
 }  // namespace shcfd

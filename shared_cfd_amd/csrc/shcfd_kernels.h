@@ -2,6 +2,7 @@
 // per-routine kernels and the fused kernels, plus kernel declarations.
 #pragma once
 #include "shcfd_device.h"
+#include "shcfd_ad.h"
 
 // species counts with compiled kernels (1 = ideal gas)
 #define SHCFD_FOR_EACH_NY(X) X(1) X(2) X(9)
@@ -13,61 +14,74 @@
 namespace shcfd {
 
 // MUSCL slope pair of one cell: high_order/muscl/muscl.f90:28-35
-// (limiter 1 = minmod, an addition)
-__device__ inline void muscl_delta(double Du, double Dl, double k_mus, int limiter, double& du, double& dl) {
+// (limiter 1 = minmod, an addition).  Generic in the scalar type (see
+// shcfd_ad.h); the double instantiation is the primal.
+template <typename T>
+__device__ inline void muscl_delta_t(const T& Du, const T& Dl, double k_mus, int limiter, T& du, T& dl) {
     if (limiter == SHCFD_LIMITER_MINMOD) {
-        double m;
-        if (Du * Dl <= 0.0) m = 0.0;
-        else m = (fabs(Du) < fabs(Dl)) ? Du : Dl;
+        if (val(Du) * val(Dl) <= 0.0) {
+            du = T(0.0);
+            dl = T(0.0);
+            return;
+        }
+        const T m = (tfabs(Du) < tfabs(Dl)) ? Du : Dl;
         du = m;
         dl = m;
         return;
     }
-    double s = ((2.0 * Du) * Dl) / (((Du * Du) + (Dl * Dl)) + 1e-300);
+    T s = ((2.0 * Du) * Dl) / (((Du * Du) + (Dl * Dl)) + 1e-300);
     du = (0.5 * s) * (((1.0 + k_mus * s) * Du) + ((1.0 - k_mus * s) * Dl));
     dl = (0.5 * s) * (((1.0 + k_mus * s) * Dl) + ((1.0 - k_mus * s) * Du));
 }
+__device__ inline void muscl_delta(double Du, double Dl, double k_mus, int limiter, double& du, double& dl) {
+    muscl_delta_t<double>(Du, Dl, k_mus, limiter, du, dl);
+}
 
 // SLAU flux of one face: core/scheme.f90:71-153.  wl/wr are full w
-// vectors (rho,u,v,p,Y(NY),gamma,H,...), n the unit face normal.
-template <int NY>
-__device__ inline void slau_face(const double* wl, const double* wr, double n1, double n2, double* tg) {
+// vectors (rho,u,v,p,Y(NY),gamma,H,...), n the unit face normal.  Generic in
+// the scalar type; slau_face below is the primal (double) form.
+template <int NY, typename T>
+__device__ inline void slau_face_t(const T* wl, const T* wr, const T& n1, const T& n2, T* tg) {
     constexpr int dq = NY + 3;
-    double rhol = wl[0], ul = wl[1], vl = wl[2], pl = wl[3], gl = wl[dq + 1];
-    double rhor = wr[0], ur = wr[1], vr = wr[2], pr = wr[3], gr = wr[dq + 1];
-    double al = sqrt((gl * pl) / rhol);
-    double unl = (n1 * ul) + (n2 * vl);
-    double ar = sqrt((gr * pr) / rhor);
-    double unr = (n1 * ur) + (n2 * vr);
-    double dp = pr - pl;
-    double a_bar = 0.5 * (ar + al);
-    double p_bar = 0.5 * (pr + pl);
-    double ml = unl / a_bar;
-    double mr = unr / a_bar;
-    double mt = (1.0 / a_bar) * sqrt(0.5 * ((((ul * ul) + (vl * vl)) + (ur * ur)) + (vr * vr)));
-    double m_tilde = (1.0 < mt) ? 1.0 : mt;
-    double chi = (1.0 - m_tilde) * (1.0 - m_tilde);
-    double tt = (ml < 0.0) ? ml : 0.0;
-    double gp = -((tt > -1.0) ? tt : -1.0);
-    tt = (mr > 0.0) ? mr : 0.0;
-    double gm = (tt < 1.0) ? tt : 1.0;
-    double g = gp * gm;
-    double un_bar_abs = ((fabs(unr) * rhor) + (fabs(unl) * rhol)) / (rhor + rhol);
-    double unp = ((1.0 - g) * un_bar_abs) + (g * fabs(unl));
-    double unm = ((1.0 - g) * un_bar_abs) + (g * fabs(unr));
-    double dm = 0.5 * (((rhol * (unl + unp)) + (rhor * (unr - unm))) - ((chi / a_bar) * dp));
-    double betar, betal;
-    if (fabs(mr) < 1.0) betar = (0.25 * (2.0 + mr)) * ((mr - 1.0) * (mr - 1.0));
-    else betar = 0.5 * (1.0 - fsign1(mr));
-    if (fabs(ml) < 1.0) betal = (0.25 * (2.0 - ml)) * ((ml + 1.0) * (ml + 1.0));
-    else betal = 0.5 * (1.0 + fsign1(ml));
-    double p_tilde = (p_bar + ((0.5 * (betal - betar)) * (pl - pr))) + (((1.0 - chi) * ((betar + betal) - 1.0)) * p_bar);
-    double cp = 0.5 * (dm + fabs(dm)), cm = 0.5 * (dm - fabs(dm));
+    T rhol = wl[0], ul = wl[1], vl = wl[2], pl = wl[3], gl = wl[dq + 1];
+    T rhor = wr[0], ur = wr[1], vr = wr[2], pr = wr[3], gr = wr[dq + 1];
+    T al = sqrt((gl * pl) / rhol);
+    T unl = (n1 * ul) + (n2 * vl);
+    T ar = sqrt((gr * pr) / rhor);
+    T unr = (n1 * ur) + (n2 * vr);
+    T dp = pr - pl;
+    T a_bar = 0.5 * (ar + al);
+    T p_bar = 0.5 * (pr + pl);
+    T ml = unl / a_bar;
+    T mr = unr / a_bar;
+    T mt = (1.0 / a_bar) * sqrt(0.5 * ((((ul * ul) + (vl * vl)) + (ur * ur)) + (vr * vr)));
+    T m_tilde = (1.0 < val(mt)) ? T(1.0) : mt;
+    T chi = (1.0 - m_tilde) * (1.0 - m_tilde);
+    T tt = (val(ml) < 0.0) ? ml : T(0.0);
+    T gp = -((val(tt) > -1.0) ? tt : T(-1.0));
+    tt = (val(mr) > 0.0) ? mr : T(0.0);
+    T gm = (val(tt) < 1.0) ? tt : T(1.0);
+    T g = gp * gm;
+    T un_bar_abs = ((tfabs(unr) * rhor) + (tfabs(unl) * rhol)) / (rhor + rhol);
+    T unp = ((1.0 - g) * un_bar_abs) + (g * tfabs(unl));
+    T unm = ((1.0 - g) * un_bar_abs) + (g * tfabs(unr));
+    T dm = 0.5 * (((rhol * (unl + unp)) + (rhor * (unr - unm))) - ((chi / a_bar) * dp));
+    T betar, betal;
+    if (::fabs(val(mr)) < 1.0) betar = (0.25 * (2.0 + mr)) * ((mr - 1.0) * (mr - 1.0));
+    else betar = T(0.5 * (1.0 - sgn1(mr)));
+    if (::fabs(val(ml)) < 1.0) betal = (0.25 * (2.0 - ml)) * ((ml + 1.0) * (ml + 1.0));
+    else betal = T(0.5 * (1.0 + sgn1(ml)));
+    T p_tilde = (p_bar + ((0.5 * (betal - betar)) * (pl - pr))) + (((1.0 - chi) * ((betar + betal) - 1.0)) * p_bar);
+    T cp = 0.5 * (dm + tfabs(dm)), cm = 0.5 * (dm - tfabs(dm));
 #pragma unroll
     for (int k = 0; k < NY; k++) tg[k] = ((cp * wl[4 + k]) + (cm * wr[4 + k])) + (p_tilde * 0.0);
     tg[NY] = ((cp * ul) + (cm * ur)) + (p_tilde * n1);
     tg[NY + 1] = ((cp * vl) + (cm * vr)) + (p_tilde * n2);
     tg[NY + 2] = ((cp * wl[dq + 2]) + (cm * wr[dq + 2])) + (p_tilde * 0.0);
+}
+template <int NY>
+__device__ inline void slau_face(const double* wl, const double* wr, double n1, double n2, double* tg) {
+    slau_face_t<NY, double>(wl, wr, n1, n2, tg);
 }
 
 // viscous flux of one face: viscosity/viscous/no-nV/2d/sch_viscous.f90:106-178

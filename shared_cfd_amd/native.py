@@ -31,7 +31,10 @@ EXPORTS = [
     "shcfd_mech_parse", "shcfd_set_mechanism", "shcfd_load_mechanism", "shcfd_proceed_reaction",
     "shcfd_chem_boundary_state", "shcfd_chem_eval", "shcfd_chem_iterations",
     "shcfd_set_block_nu", "shcfd_set_block_active", "shcfd_energy_rms_blocks", "shcfd_iterate", "shcfd_autotune",
+    "shcfd_adjoint_objective", "shcfd_objective", "shcfd_adjoint_upload", "shcfd_adjoint_download",
+    "shcfd_adjoint_poolbar", "shcfd_residual_vjp", "shcfd_adjoint_solve",
 ]
+ADJ_LAMBDA, ADJ_QBAR, ADJ_XBAR = 0, 1, 2
 OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT, OPT_SKEW_SWEEP, OPT_TILE_ROWS = 1, 2, 3, 4
 
 # kernel classes of shcfd_profile_read (include/shcfd.h)
@@ -80,6 +83,15 @@ def lib():
         L.shcfd_profile_enable.argtypes = [vp, ctypes.c_int]
         L.shcfd_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.shcfd_profile_read.argtypes = [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+        L.shcfd_adjoint_objective.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_double]
+        L.shcfd_objective.argtypes = [vp, dp]
+        L.shcfd_adjoint_upload.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.shcfd_adjoint_download.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.shcfd_adjoint_poolbar.argtypes = [vp, dp]
+        L.shcfd_residual_vjp.argtypes = [vp]
+        L.shcfd_adjoint_solve.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp,
+                                          ctypes.POINTER(ctypes.c_int)]
         L.shcfd_set_block_nu.argtypes = [vp, dp]
         L.shcfd_set_block_active.argtypes = [vp, ip]
         L.shcfd_energy_rms_blocks.argtypes = [vp, dp]
@@ -271,6 +283,41 @@ class Solver:
         r1, r2 = ctypes.c_double(0.0), ctypes.c_double(0.0)
         check(self.L.shcfd_step(self.ctx, step, step_res, ctypes.byref(r1), ctypes.byref(r2)), "step")
         return r1.value, r2.value
+
+    # ---- discrete adjoint (include/shcfd.h) ----
+    def adjoint_objective(self, block, i0, i1, ax, ay):
+        """J = sum over j-min faces i0..i1 of `block` of dsj (TG_u ax + TG_v ay)."""
+        check(self.L.shcfd_adjoint_objective(self.ctx, block, i0, i1, ax, ay), "adjoint_objective")
+
+    def objective(self):
+        v = ctypes.c_double(0.0)
+        check(self.L.shcfd_objective(self.ctx, ctypes.byref(v)), "objective")
+        return v.value
+
+    def adjoint_upload_lambda(self, b, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == (self.case.nj[b] + 4, self.case.ni[b] + 4, self.case.dimq)
+        check(self.L.shcfd_adjoint_upload(self.ctx, ADJ_LAMBDA, b, _dp(a)), "adjoint_upload")
+
+    def adjoint_download(self, what, b):
+        k = 2 if what == ADJ_XBAR else self.case.dimq
+        out = np.zeros((self.case.nj[b] + 4, self.case.ni[b] + 4, k), dtype=np.float64)
+        check(self.L.shcfd_adjoint_download(self.ctx, what, b, _dp(out)), "adjoint_download")
+        return out
+
+    def adjoint_poolbar(self):
+        out = np.zeros(max(1, self.case.bc_pool.size), dtype=np.float64)
+        check(self.L.shcfd_adjoint_poolbar(self.ctx, _dp(out)), "adjoint_poolbar")
+        return out[: self.case.bc_pool.size]
+
+    def residual_vjp(self):
+        check(self.L.shcfd_residual_vjp(self.ctx), "residual_vjp")
+
+    def adjoint_solve(self, max_iter=20000, rtol=1e-10, check_every=100):
+        r0, r, it = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_int(0)
+        check(self.L.shcfd_adjoint_solve(self.ctx, max_iter, rtol, check_every, ctypes.byref(r0), ctypes.byref(r),
+                                         ctypes.byref(it)), "adjoint_solve")
+        return r0.value, r.value, it.value
 
     # ---- per-kernel timing ----
     def profile(self, classes=None):
