@@ -177,15 +177,67 @@ def bench_batched(a):
     }
 
 
+def bench_adjoint(a, fd=True):
+    """Discrete adjoint on the NACA 4-digit (12 %) C-mesh, 512x128 cells,
+    Mach 0.8, alpha 1.25 deg, 64 Hicks-Henne amplitudes drawn from seed 0 in
+    [-1e-3, 1e-3] chord: the steady primal and the adjoint of C_L (then C_D)
+    each iterated to 1e-10 relative; cost = wall time of the adjoint solve
+    over that of the primal solve.  First-order face states: with the
+    reference limiter the second-order scheme stalls at Mach 0.8, and its
+    exact linearisation does not converge under the LU-SGS iteration."""
+    import numpy as np
+    from shared_cfd_amd.adjoint import AirfoilFlow
+
+    mesh = dict(ni=a.adj_ni, nj=a.adj_nj, nwake=a.adj_ni // 8, mach=0.8, cfl=2.0, order=1)
+    rng = np.random.default_rng(0)
+    p0 = rng.uniform(-1e-3, 1e-3, 64)
+    f = AirfoilFlow(amplitudes=p0, alpha_deg=1.25, **mesh)
+    pr = f.converge(rtol=1e-10, max_steps=400000)
+    cl, cd = f.coefficients()
+    gl = f.gradient(1.0, 0.0, rtol=1e-10, max_iter=400000)
+    gd = f.gradient(0.0, 1.0, rtol=1e-10, max_iter=400000)
+    f.close()
+    out = {
+        "workload": f"NACA 4-digit t=0.12 C-mesh {mesh['ni']}x{mesh['nj']}, M=0.8, alpha=1.25 deg, inviscid, "
+                    "LU-SGS local dt (first-order face states), 64 Hicks-Henne amplitudes (seed 0)",
+        "cells": mesh["ni"] * mesh["nj"], "tol": 1e-10,
+        "primal": {"steps": pr.steps, "seconds": round(pr.seconds, 3), "rms_ratio": pr.rms / pr.rms0},
+        "adjoint_cl": {"iterations": gl.iterations, "seconds": round(gl.seconds, 3), "res_ratio": gl.res / gl.res0},
+        "adjoint_cd": {"iterations": gd.iterations, "seconds": round(gd.seconds, 3), "res_ratio": gd.res / gd.res0},
+        "cost_ratio": round(gl.seconds / pr.seconds, 3),
+        "cost_ratio_per_iteration": round((gl.seconds / gl.iterations) / (pr.seconds / pr.steps), 3),
+        "CL": cl, "CD": cd, "dCL_dalpha": gl.dJ_dalpha, "dCD_dalpha": gd.dJ_dalpha,
+        "dCL_dp_norm": float(np.linalg.norm(gl.dJ_dp)),
+    }
+    if fd:
+        h, k = 1e-7, 20
+        vals = []
+        for sgn in (1.0, -1.0):
+            p = p0.copy()
+            p[k] += sgn * h
+            ff = AirfoilFlow(amplitudes=p, alpha_deg=1.25, **mesh)
+            ff.converge(rtol=1e-12, max_steps=600000)
+            vals.append(ff.coefficients()[0])
+            ff.close()
+        fdv = (vals[0] - vals[1]) / (2 * h)
+        out["fd_check"] = {"param": k, "h": h, "adjoint": float(gl.dJ_dp[k]), "central_fd": fdv,
+                           "rel_err": abs(fdv - gl.dJ_dp[k]) / abs(fdv)}
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seq-instances", type=int, default=256)
     ap.add_argument("--seq-steps", type=int, default=5)
-    ap.add_argument("--workload", choices=["step", "reacting", "batched"], default="step",
+    ap.add_argument("--workload", choices=["step", "reacting", "batched", "adjoint"], default="step",
                     help="step: Mach-3 forward step (the headline metric); reacting: mixing layer with "
-                         "chemistry; batched: 256 independent plate/channel blocks")
+                         "chemistry; batched: 256 independent plate/channel blocks; adjoint: airfoil shape "
+                         "gradients with a finite-difference spot check")
+    ap.add_argument("--adj-ni", type=int, default=512)
+    ap.add_argument("--adj-nj", type=int, default=128)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--ni", type=int, default=1024)
     ap.add_argument("--nj", type=int, default=512)
     ap.add_argument("--steps", type=int, default=5)
@@ -207,6 +259,12 @@ def main(argv=None):
         return 0
     if a.workload == "batched":
         print(json.dumps(bench_batched(a)))
+        return 0
+    if a.workload == "adjoint":
+        r = bench_adjoint(a, fd=True)
+        print(json.dumps({"metric": "adjoint/primal cost at 1e-10 tol", "value": r["cost_ratio"], "unit": "ratio",
+                          "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+                          "vs_baseline": None, "config": {"workload": r["workload"]}, "adjoint": r}))
         return 0
 
     from shared_cfd_amd import cases as C
@@ -297,6 +355,12 @@ def main(argv=None):
                    "cells": cells, "inner_iterations_per_step": inner, "blocks": case.nblocks},
         "roofline": roofline, "cpu_baseline": cpu, "plain_gpu": plain,
     }
+    # the metric's second figure: adjoint/primal cost at 1e-10 (airfoil
+    # shape-gradient workload; the finite-difference check runs only with
+    # --workload adjoint)
+    if not a.no_adjoint:
+        s.close()
+        out["adjoint"] = bench_adjoint(a, fd=False)
     print(json.dumps(out))
     return 0
 

@@ -60,3 +60,56 @@ def test_vjp_matches_torch_autograd(order):
     assert np.max(np.abs(xb[..., 1] - gy)) <= 1e-10 * scale
     gp = pool.grad.numpy()[:9]
     assert np.max(np.abs(pbar[:9] - gp)) <= 1e-10 * max(np.max(np.abs(gp)), 1e-300)
+
+
+def test_gradients_match_central_differences():
+    """dC_L/dp and dC_L/dalpha from one adjoint solve against central
+    differences (h = 1e-7) of converged first-order solutions; the
+    adjoint iteration also has to converge as the primal does."""
+    from shared_cfd_amd.adjoint import AirfoilFlow
+    mesh = dict(ni=96, nj=24, nwake=12, mach=0.8, cfl=5.0, order=1)
+    p0 = np.zeros(64)
+    f = AirfoilFlow(amplitudes=p0, **mesh)
+    pr = f.converge(rtol=1e-12, max_steps=300000)
+    assert pr.rms <= 1e-12 * pr.rms0
+    g = f.gradient(1.0, 0.0, rtol=1e-12, max_iter=300000)
+    assert g.res <= 1e-12 * g.res0
+    # the adjoint iteration costs about what the primal does
+    assert g.iterations <= 2 * pr.steps
+    f.close()
+    h = 1e-7
+
+    def cl(p, dalpha=0.0):
+        ff = AirfoilFlow(amplitudes=p, alpha_deg=1.25 + np.rad2deg(dalpha), **mesh)
+        ff.converge(rtol=1e-12, max_steps=300000)
+        v = ff.coefficients()[0]
+        ff.close()
+        return v
+
+    for k in (20, 45):
+        pp, pm = p0.copy(), p0.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (cl(pp) - cl(pm)) / (2 * h)
+        assert abs(fd - g.dJ_dp[k]) <= 1e-6 * abs(fd), (k, fd, g.dJ_dp[k])
+    fd = (cl(p0, h) - cl(p0, -h)) / (2 * h)
+    assert abs(fd - g.dJ_dalpha) <= 1e-6 * abs(fd), (fd, g.dJ_dalpha)
+
+
+def test_autograd_function_backward_is_the_adjoint_gradient():
+    from shared_cfd_amd.adjoint import AirfoilCoefficients
+    opts = {"mesh": dict(ni=64, nj=16, nwake=8, mach=0.8, cfl=5.0, order=1), "rtol": 1e-11}
+    p = torch.zeros(64, dtype=torch.float64, requires_grad=True)
+    a = torch.tensor(np.deg2rad(1.25), dtype=torch.float64, requires_grad=True)
+    c = AirfoilCoefficients.apply(p, a, opts)
+    (2.0 * c[0] - 3.0 * c[1]).backward()
+    assert p.grad is not None and torch.isfinite(p.grad).all() and float(p.grad.abs().max()) > 0
+    # directional check along one amplitude against central differences
+    h = 1e-7
+    e = torch.zeros(64, dtype=torch.float64)
+    e[20] = 1.0
+    with torch.no_grad():
+        cp = AirfoilCoefficients.apply(p + h * e, a, opts)
+        cm = AirfoilCoefficients.apply(p - h * e, a, opts)
+    fd = float((2.0 * (cp[0] - cm[0]) - 3.0 * (cp[1] - cm[1])) / (2 * h))
+    assert abs(fd - float(p.grad[20])) <= 1e-5 * abs(fd)
