@@ -244,7 +244,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n-per-unit", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="CPU sample length (default: 15 physical steps; reacting: 30 steps)")
     ap.add_argument("--plain-steps", type=int, default=1)
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -253,8 +254,8 @@ def main(argv=None):
               file=sys.stderr)
         return 2
     if a.workload == "reacting":
-        if a.cpu_steps == 5:
-            a.cpu_steps = 1
+        if a.cpu_steps is None:
+            a.cpu_steps = 30
         print(json.dumps(bench_reacting(a)))
         return 0
     if a.workload == "batched":
@@ -270,6 +271,8 @@ def main(argv=None):
     from shared_cfd_amd import cases as C
     from shared_cfd_amd.native import Solver
 
+    if a.cpu_steps is None:
+        a.cpu_steps = 15
     case = C.forward_step(n_per_unit=a.n_per_unit, num_internal_loop=21)
     inner = case.params.num_internal_loop - 1
     cells = case.ncells
