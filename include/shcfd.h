@@ -92,13 +92,16 @@ int shcfd_set_tgv(shcfd_ctx *ctx);
  * dt_grbl_out (may be NULL) receives the global step (synchronises) */
 int shcfd_set_dt(shcfd_ctx *ctx, int step, double *dt_grbl_out);
 /* set_ABpm: store/time_scheme/LU-SGS/2d/sch_lusgs.raw.f90:1-141,
+ * NR/2d/sch_NR.raw.f90:1-141, precon/2d/sch_precon.raw.f90:1-201,
  * preconLU-SGS/2d/sch_precon.raw.f90:1-202, dual/2d/sch_dual.raw.f90:1-209 */
 int shcfd_set_abpm(shcfd_ctx *ctx);
-/* set_dqdt: store/time_scheme/dual/2d/sch_dual.raw.f90:211-240 */
+/* set_dqdt: store/time_scheme/dual/2d/sch_dual.raw.f90:211-240 (the same in
+ * NR/2d/sch_NR.raw.f90:143-172 and precon/2d/sch_precon.raw.f90:203-232) */
 int shcfd_set_dqdt(shcfd_ctx *ctx);
 /* calc_next_step_implicit: LU-SGS/2d/time_lusgs_viscous.f90:1-65,
- * preconLU-SGS/2d/time_precon_viscous.f90:1-66, dual/2d/time_dual_viscous.f90:1-125.
- * res_out/omega_out (dual only, may be NULL) synchronise. */
+ * preconLU-SGS/2d/time_precon_viscous.f90:1-66, NR/2d/time_NR_viscous.f90:1-102,
+ * precon/2d/time_precon_viscous.f90:1-103, dual/2d/time_dual_viscous.f90:1-125.
+ * res_out/omega_out (inner-loop schemes only, may be NULL) synchronise. */
 int shcfd_calc_next_step_implicit(shcfd_ctx *ctx, int step_internal, double *res_out, double *omega_out);
 /* calc_next_step (explicit Euler): store/time_scheme/euler/time_euler_viscous.raw.f90:1-23 */
 int shcfd_calc_next_step(shcfd_ctx *ctx);
@@ -113,9 +116,9 @@ int shcfd_energy_rms(shcfd_ctx *ctx, double *rms_out);
 /* main.body.raw.f90:14-15: part_primitive + set_BC before the main loop */
 int shcfd_prepare(shcfd_ctx *ctx);
 /* one physical step of store/time_scheme/<scheme>/main.main.raw.f90 with
- * check_internal_loop_convergence (sch_dual.raw.f90:284-299) for dual
- * time.  res_first/res_last: dual-time inner residuals of the first and
- * last inner iteration (may be NULL). */
+ * check_internal_loop_convergence (sch_dual.raw.f90:284-299) for the
+ * inner-loop schemes (NR, precon, dual).  res_first/res_last: inner
+ * residuals of the first and last inner iteration (may be NULL). */
 int shcfd_step(shcfd_ctx *ctx, int step, int step_res, double *res_first, double *res_last);
 /* pending asynchronous error (omega, NaN, sweep flag timeout); clears it */
 int shcfd_check_error(shcfd_ctx *ctx);
@@ -177,11 +180,12 @@ int shcfd_set_block_active(shcfd_ctx *ctx, const int *active);
 /* the energy-change RMS of check_convergence (check_convergence.f90:17-28)
  * for each block separately */
 int shcfd_energy_rms_blocks(shcfd_ctx *ctx, double *rms);
-/* nsteps physical steps (numbered from first_step) of a scheme other than
- * dual time with no host synchronisation between steps; graph_steps > 0
- * captures that many steps into one HIP graph and replays it, 0 launches
- * eagerly, < 0 uses the schedule chosen by shcfd_autotune.  Dual time runs
- * step by step through shcfd_step. */
+/* nsteps physical steps (numbered from first_step) with no host
+ * synchronisation between steps; graph_steps > 0 captures that many steps
+ * into one HIP graph and replays it, 0 launches eagerly, < 0 uses the
+ * schedule chosen by shcfd_autotune.  The schemes with a dual-time inner
+ * loop (NR, precon, dual), which check the inner residual on the host after
+ * every step, run step by step through shcfd_step. */
 int shcfd_iterate(shcfd_ctx *ctx, int first_step, int nsteps, int graph_steps);
 /* times candidate schedules (optimised or per-routine sweeps, tile rows of
  * the per-cell kernels, eager or graph launches) over trial_steps steps each

@@ -42,6 +42,8 @@ enum {
     SHCFD_SCHEME_EULER = 0,
     SHCFD_SCHEME_RK2 = 1,
     SHCFD_SCHEME_LUSGS = 2,
+    SHCFD_SCHEME_NR = 3,         /* dual-time inner loop, plain LU-SGS operator: NR/2d        */
+    SHCFD_SCHEME_PRECON = 4,     /* dual-time inner loop, preconditioned operator: precon/2d  */
     SHCFD_SCHEME_DUAL = 5,
     SHCFD_SCHEME_PRECON_LUSGS = 6
 };

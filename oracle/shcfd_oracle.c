@@ -965,9 +965,10 @@ void or_set_abpm(or_ctx *c) {
                 double nuA, nuB;
                 double dtv = (c->p.time_step == 0) ? c->dt_grbl : SC(SHCFD_F_DT, i, j);
                 double pre1[32], pre2[32], pscale = 1.0;
-                int precond = (scheme == SHCFD_SCHEME_DUAL || scheme == SHCFD_SCHEME_PRECON_LUSGS);
+                int precond = (scheme == SHCFD_SCHEME_DUAL || scheme == SHCFD_SCHEME_PRECON_LUSGS ||
+                               scheme == SHCFD_SCHEME_PRECON);
                 if (!precond) {
-                    /* sch_lusgs.raw.f90:38-41 */
+                    /* sch_lusgs.raw.f90:38-41, sch_NR.raw.f90:40-41 */
                     nuA = fabs(unA) + cs;
                     nuB = fabs(unB) + cs;
                 } else {
@@ -991,7 +992,8 @@ void or_set_abpm(or_ctx *c) {
                                  (nuB * (SC(SHCFD_F_DSJ, i, j) + SC(SHCFD_F_DSJ, i, j - 1)));
                     double nr = wc[dq + 4] / rho;
                     double mx = (uds > nr) ? uds : nr;
-                    if (scheme == SHCFD_SCHEME_PRECON_LUSGS) {
+                    if (scheme == SHCFD_SCHEME_PRECON_LUSGS || scheme == SHCFD_SCHEME_PRECON) {
+                        /* precon/2d/sch_precon.raw.f90:58-68 (dt_mat and the phh-scaled pre1) */
                         double dtm = (c->p.cfl * vol) / mx;
                         SC(SHCFD_F_DT, i, j) = dtm;
                         dtv = dtm;
@@ -1058,7 +1060,10 @@ void or_set_abpm(or_ctx *c) {
                     Bp[e] = Bp[e] + nuB * 0.5;
                     Bm[e] = Bm[e] - nuB * 0.5;
                 }
-                if (scheme != SHCFD_SCHEME_DUAL)
+                if (scheme == SHCFD_SCHEME_NR || scheme == SHCFD_SCHEME_PRECON)
+                    /* BDF2 diagonal: NR/2d/sch_NR.raw.f90:134-136, precon/2d/sch_precon.raw.f90:194-196 */
+                    SC(SHCFD_F_ALPHA, i, j) = 1.0 / ((((3.0 * vol) / (2.0 * dtv)) + (dsci * nuA)) + (dscj * nuB));
+                else if (scheme != SHCFD_SCHEME_DUAL)
                     SC(SHCFD_F_ALPHA, i, j) = 1.0 / (((vol / dtv) + (dsci * nuA)) + (dscj * nuB));
             }
     }
@@ -1085,6 +1090,7 @@ void or_set_dqdt(or_ctx *c) {
 /* ======================================================================== */
 /* implicit update: LU-SGS/2d/time_lusgs_viscous.f90:1-65,                  */
 /* preconLU-SGS/2d/time_precon_viscous.f90:1-66,                            */
+/* NR/2d/time_NR_viscous.f90:1-102, precon/2d/time_precon_viscous.f90:1-103, */
 /* dual/2d/time_dual_viscous.f90:1-125                                      */
 /* ======================================================================== */
 /* flux balance of one cell: time_lusgs_viscous.f90:12-22 (Area = 0 in 2-D) */
@@ -1116,8 +1122,10 @@ static inline double dotp(int dq, const double *a, const double *x) {
 int or_calc_next_step_implicit(or_ctx *c, int step_internal, double *res_out, double *omega_out) {
     int dq = c->dimq, ny = c->ny;
     int scheme = c->p.scheme;
-    int dual = scheme == SHCFD_SCHEME_DUAL;
-    int prec = scheme == SHCFD_SCHEME_PRECON_LUSGS;
+    int dual = scheme == SHCFD_SCHEME_DUAL; /* S-scaled three-stage sweeps */
+    /* inner loop of dual time: dq/dt term, res and the omega-limited update */
+    int inner = dual || scheme == SHCFD_SCHEME_NR || scheme == SHCFD_SCHEME_PRECON;
+    int prec = scheme == SHCFD_SCHEME_PRECON_LUSGS || scheme == SHCFD_SCHEME_PRECON;
     double omega = 1e300, res = 0.0;
     int err = 0;
     for (int b = 0; b < c->nb; b++) {
@@ -1137,10 +1145,16 @@ int or_calc_next_step_implicit(or_ctx *c, int step_internal, double *res_out, do
                     double ps2 = VEC(SHCFD_F_PRESCA, 1, i, j);
                     for (int k = 0; k < dq; k++) DQ(k, i, j) = d[k] + (ps2 * dt) * VEC(SHCFD_F_PRE1, k, i, j);
                 } else if (prec) {
+                    /* time_precon_viscous.f90:24-26 (precon/2d), :24 (preconLU-SGS/2d) */
                     double dt = dotp(dq, &VEC(SHCFD_F_PRE2, 0, i, j), s);
-                    for (int k = 0; k < dq; k++) DQ(k, i, j) = s[k] + VEC(SHCFD_F_PRE1, k, i, j) * dt;
+                    for (int k = 0; k < dq; k++) {
+                        double x = s[k] + VEC(SHCFD_F_PRE1, k, i, j) * dt;
+                        DQ(k, i, j) = inner ? x - SC(SHCFD_F_VOL, i, j) * VEC(SHCFD_F_DQDT, k, i, j) : x;
+                    }
                 } else {
-                    for (int k = 0; k < dq; k++) DQ(k, i, j) = s[k];
+                    /* time_lusgs_viscous.f90:21, time_NR_viscous.f90:24 */
+                    for (int k = 0; k < dq; k++)
+                        DQ(k, i, j) = inner ? s[k] - SC(SHCFD_F_VOL, i, j) * VEC(SHCFD_F_DQDT, k, i, j) : s[k];
                 }
             }
         /* forward sweep on hyperplanes sm=i+j ascending (ghost Dq = 0) */
@@ -1197,14 +1211,16 @@ int or_calc_next_step_implicit(or_ctx *c, int step_internal, double *res_out, do
                     double ps4 = VEC(SHCFD_F_PRESCA, 3, i, j), ips1 = 1.0 / VEC(SHCFD_F_PRESCA, 0, i, j);
                     for (int k = 0; k < dq; k++) DQ(k, i, j) = DQ(k, i, j) + (ps4 * d) * VEC(SHCFD_F_PRE1, k, i, j);
                     for (int k = 0; k < dq; k++) DQ(k, i, j) = ips1 * DQ(k, i, j);
+                }
+                if (inner) {
                     double e = DQ(ny + 2, i, j);
                     cres[(size_t)(i - 1) + (size_t)ni * (size_t)(j - 1)] = e * e;
                 }
             }
         }
-        if (dual) {
+        if (inner) {
             /* res and omega in the reference's single-thread visiting order:
-             * time_dual_viscous.f90:68-93 */
+             * time_dual_viscous.f90:68-93, time_NR_viscous.f90:55-69 */
             for (int sm = ni + nj; sm >= 2; sm--) {
                 int ilo = (1 > sm - nj) ? 1 : sm - nj, ihi = (sm - 1 < ni) ? sm - 1 : ni;
                 for (int i = ilo; i <= ihi; i++) {
@@ -1226,9 +1242,9 @@ int or_calc_next_step_implicit(or_ctx *c, int step_internal, double *res_out, do
                     for (int k = 0; k < dq; k++) Q(k, i, j) = Q(k, i, j) + DQ(k, i, j);
         }
     }
-    if (dual) {
-        /* time_dual_viscous.f90:98-124 */
-        if (res != res) err = err ? err : SHCFD_E_NAN;
+    if (inner) {
+        /* time_dual_viscous.f90:98-124, time_NR_viscous.f90:74-101 (no NaN stop) */
+        if (dual && res != res) err = err ? err : SHCFD_E_NAN;
         if (step_internal == 1) c->res1 = res;
         if (omega_out) *omega_out = omega;
         double om = (c->p.omega_max < omega) ? c->p.omega_max : omega;
@@ -1368,9 +1384,11 @@ int or_step(or_ctx *c, double *res_first, double *res_last) {
         part_main(c);
         or_calc_next_step_rk2(c, 2);
         break;
+    case SHCFD_SCHEME_NR:     /* NR/2d/main.main.raw.f90:1-20 (the same loop as dual) */
+    case SHCFD_SCHEME_PRECON: /* precon/2d/main.main.raw.f90:1-19 (no set_dt: set_ABpm sets dt_mat) */
     case SHCFD_SCHEME_DUAL: { /* dual/2d/main.main.raw.f90:1-20 */
         part_primitive(c);
-        or_set_dt(c);
+        if (c->p.scheme != SHCFD_SCHEME_PRECON) or_set_dt(c);
         or_copy_field(c, SHCFD_F_QPP, SHCFD_F_QP);
         or_copy_field(c, SHCFD_F_QP, SHCFD_F_Q);
         double res = 0.0;

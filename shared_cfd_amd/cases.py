@@ -26,7 +26,7 @@ from typing import List, Optional
 import numpy as np
 
 # ---- mirrors of include/shcfd_types.h ----------------------------------------
-SCHEME_EULER, SCHEME_RK2, SCHEME_LUSGS, SCHEME_DUAL, SCHEME_PRECON_LUSGS = 0, 1, 2, 5, 6
+SCHEME_EULER, SCHEME_RK2, SCHEME_LUSGS, SCHEME_NR, SCHEME_PRECON, SCHEME_DUAL, SCHEME_PRECON_LUSGS = 0, 1, 2, 3, 4, 5, 6
 LIMITER_REFERENCE, LIMITER_MINMOD = 0, 1
 BC_EXTRAP, BC_REFLECT, BC_NOSLIP, BC_FIXED, BC_INFLOW_TUV, BC_OUTFLOW_P, BC_SLIP_NORMAL = range(7)
 SIDE_IMIN, SIDE_IMAX, SIDE_JMIN, SIDE_JMAX = range(4)

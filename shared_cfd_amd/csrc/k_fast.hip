@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(256) k_fast_prep(DevCtx d, const QStrip* strip
     CellJac<1> J;
     cell_jacobian<1>(d, c, B.pitch, J);
     FLD(d, SHCFD_F_ALPHA, 0, c) = J.alpha;
-    if (d.p.scheme == SHCFD_SCHEME_PRECON_LUSGS) FLD(d, SHCFD_F_DT, 0, c) = J.dtm;
+    if (J.precond && !dual) FLD(d, SHCFD_F_DT, 0, c) = J.dtm;
     double Sm[16], T[16];
     post_matrix(J, dual, Sm);
     const double dsci = FLD(d, SHCFD_F_DSCI, 0, c), dscj = FLD(d, SHCFD_F_DSCJ, 0, c);
@@ -569,8 +569,8 @@ k_fast_bwd(DevCtx d, const QStrip* strips, int nstrips, const double* X_sl, cons
 }
 
 // ---------------------------------------------------------------------------
-// post: Dq = alpha*S*z into the natural DQ field; dual: (res, omega) partials
-// per workgroup; otherwise q += Dq
+// post: Dq = alpha*S*z into the natural DQ field; inner-loop schemes: (res,
+// omega) partials per workgroup; otherwise q += Dq
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_fast_post(DevCtx d, const QStrip* strips, const int2* work, const double* Z_sl,
                                                    double* part) {
@@ -581,6 +581,7 @@ __global__ void __launch_bounds__(256) k_fast_post(DevCtx d, const QStrip* strip
     const int r = threadIdx.x & 15;
     const int dd = wk.y + (threadIdx.x >> 4);
     const bool dual = d.p.scheme == SHCFD_SCHEME_DUAL;
+    const bool inner = inner_loop_scheme(d.p.scheme);
     double res = 0.0, om = 1e300;
     int i, j;
     if (slot_cell(S, B, dd, r, i, j)) {
@@ -607,11 +608,15 @@ __global__ void __launch_bounds__(256) k_fast_post(DevCtx d, const QStrip* strip
             for (int k = 0; k < 4; k++) {
                 x[k] = al * z[k];
                 FLD(d, SHCFD_F_DQ, k, c) = x[k];
-                if (B.active) FLD(d, SHCFD_F_Q, k, c) = FLD(d, SHCFD_F_Q, k, c) + x[k];
+                if (B.active && !inner) FLD(d, SHCFD_F_Q, k, c) = FLD(d, SHCFD_F_Q, k, c) + x[k];
+            }
+            if (inner) {
+                res = x[3] * x[3];
+                if (x[0] < 0.0) om = (-FLD(d, SHCFD_F_Q, 0, c) / x[0]) * d.p.dqmax;
             }
         }
     }
-    if (!dual) return;
+    if (!inner) return;
     for (int off = 32; off >= 1; off >>= 1) {
         res = res + __shfl_xor(res, off);
         om = dmin(om, __shfl_xor(om, off));
@@ -831,7 +836,7 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
     pe(&c, PROF_SWEEP_BWD);
     pb(&c, PROF_UPDATE);
     hipLaunchKernelGGL(k_fast_post, dim3(nwork), dim3(256), 0, st, d, strips, work, Z, part);
-    if (dual) {
+    if (inner_loop_scheme(c.p.scheme)) {
         hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, st, d, part, nwork);
         if (step_internal == 1) hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, st, d, (int)SCAL_RES1, (int)SCAL_RES);
         hipLaunchKernelGGL(k_update<1>, dim3((unsigned)c.htiles.size()), dim3(64 * c.tile_rows), 0, st, d, c.dtiles);

@@ -1,6 +1,8 @@
 // k_implicit.hip -- right-hand side, LU-SGS hyperplane sweeps and update of
 // calc_next_step_implicit for LU-SGS (LU-SGS/2d/time_lusgs_viscous.f90),
-// preconditioned LU-SGS (preconLU-SGS/2d/time_precon_viscous.f90) and the
+// preconditioned LU-SGS (preconLU-SGS/2d/time_precon_viscous.f90), the
+// dual-time inner loops with the plain and the preconditioned operator
+// (NR/2d/time_NR_viscous.f90, precon/2d/time_precon_viscous.f90) and the
 // dual-time three-stage form (dual/2d/time_dual_viscous.f90).
 //
 // Sweep schedule ("strip wavefront").  The reference walks hyperplanes
@@ -21,8 +23,10 @@
 namespace shcfd {
 
 // ---------------------------------------------------------------------------
-// RHS: time_lusgs_viscous.f90:12-22, time_precon_viscous.f90:12-21,
-// time_dual_viscous.f90:16-28.  Tile 0 also clears the sweep progress flags.
+// RHS: time_lusgs_viscous.f90:12-22, time_precon_viscous.f90:12-21
+// (preconLU-SGS), time_NR_viscous.f90:16-25, time_precon_viscous.f90:16-27
+// (precon), time_dual_viscous.f90:16-28.  Tile 0 also clears the sweep
+// progress flags.
 // ---------------------------------------------------------------------------
 template <int NY>
 __global__ void __launch_bounds__(256) k_rhs(DevCtx d, const Tile* tiles, int* prog, int nprog) {
@@ -48,12 +52,21 @@ __global__ void __launch_bounds__(256) k_rhs(DevCtx d, const Tile* tiles, int* p
         double ps2 = FLD(d, SHCFD_F_PRESCA, 1, c);
 #pragma unroll
         for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = x[k] + (ps2 * dt) * FLD(d, SHCFD_F_PRE1, k, c);
-    } else if (scheme == SHCFD_SCHEME_PRECON_LUSGS) {
+    } else if (scheme == SHCFD_SCHEME_PRECON_LUSGS || scheme == SHCFD_SCHEME_PRECON) {
+        const bool inner = scheme == SHCFD_SCHEME_PRECON;
+        const double vol = inner ? FLD(d, SHCFD_F_VOL, 0, c) : 0.0;
         double dt = 0.0;
 #pragma unroll
         for (int k = 0; k < dq; k++) dt = dt + FLD(d, SHCFD_F_PRE2, k, c) * s[k];
 #pragma unroll
-        for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = s[k] + FLD(d, SHCFD_F_PRE1, k, c) * dt;
+        for (int k = 0; k < dq; k++) {
+            const double x = s[k] + FLD(d, SHCFD_F_PRE1, k, c) * dt;
+            FLD(d, SHCFD_F_DQ, k, c) = inner ? x - vol * FLD(d, SHCFD_F_DQDT, k, c) : x;
+        }
+    } else if (scheme == SHCFD_SCHEME_NR) {
+        const double vol = FLD(d, SHCFD_F_VOL, 0, c);
+#pragma unroll
+        for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = s[k] - vol * FLD(d, SHCFD_F_DQDT, k, c);
     } else {
 #pragma unroll
         for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = s[k];
@@ -195,7 +208,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 // backward sweep (plain).  Workgroup g handles strip nstrips-1-g so that the
 // strip it waits on (the one above) has a lower workgroup id.  For LU-SGS and
 // preconditioned LU-SGS the update q += Δq is applied here; for dual time the
-// per-strip (res, omega) partials are written and k_update applies q += ωΔq.
+// per-strip (res, omega) partials are written and k_update applies q += ωΔq
+// (also for the NR and precon inner loops, without the S-scaling stages).
 // ---------------------------------------------------------------------------
 template <int NY, bool SKEW>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) k_sweep_bwd_plain(DevCtx d, const Strip* strips, int nstrips, int* prog,
@@ -211,6 +225,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
     const int j = S.j0 + lane;
     const bool rowok = j <= nj;
     const bool dual = d.p.scheme == SHCFD_SCHEME_DUAL;
+    const bool inner = inner_loop_scheme(d.p.scheme);
     const bool has_prev = S.prev >= 0, has_next = S.next >= 0;
     const double dqmax = d.p.dqmax;
     double a_in[dq], bo[dq];
@@ -265,6 +280,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
                 for (int k = 0; k < dq; k++) x[k] = x[k] + (ps4 * dd) * FLD(d, SHCFD_F_PRE1, k, c);
 #pragma unroll
                 for (int k = 0; k < dq; k++) x[k] = ips1 * x[k];
+            }
+            if (inner) {
                 res = res + x[NY + 2] * x[NY + 2];
 #pragma unroll
                 for (int k = 0; k < NY; k++)
@@ -303,7 +320,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
             }
         }
     }
-    if (dual) {
+    if (inner) {
         // fixed-order tree over the 64 rows of the strip
         for (int off = 32; off >= 1; off >>= 1) {
             res = res + __shfl_xor(res, off);
@@ -335,14 +352,14 @@ __global__ void __launch_bounds__(256) k_reduce_res_omega(DevCtx d, const double
         for (int k = 1; k < (int)(blockDim.x >> 6); k++) { R = R + shr[k]; O = dmin(O, sho[k]); }
         d.scal[SCAL_RES] = R;
         d.scal[SCAL_OMEGA] = O;
-        if (R != R) set_error(d, SHCFD_E_NAN);
+        if (R != R && d.p.scheme == SHCFD_SCHEME_DUAL) set_error(d, SHCFD_E_NAN);  // time_dual_viscous.f90:104-107
         if (O < d.p.omega_min) set_error(d, SHCFD_E_OMEGA);
     }
 }
 
 __global__ void k_copy_scal(DevCtx d, int dst, int src) { d.scal[dst] = d.scal[src]; }
 
-// q += min(omega_max, omega) * Δq : time_dual_viscous.f90:113-124
+// q += min(omega_max, omega) * Δq : time_dual_viscous.f90:113-124, time_NR_viscous.f90:90-101
 template <int NY>
 __global__ void __launch_bounds__(256) k_update(DevCtx d, const Tile* tiles) {
     TileCoord t = tile_coord(tiles);

@@ -110,8 +110,29 @@ const char* shcfd_strerror(int code) {
 
 int shcfd_version(void) { return 100; }
 
+// the scheme / time-step pairs the reference's checkout accepts
+// (store/checkout/checkout_flow.py:120-185): precon and preconLU-SGS need
+// local time steps, dual time a global one
+static bool params_valid(const shcfd_params* p) {
+    if (p->time_step != 0 && p->time_step != 1) return false;
+    switch (p->scheme) {
+    case SHCFD_SCHEME_EULER:
+    case SHCFD_SCHEME_RK2:
+    case SHCFD_SCHEME_LUSGS:
+    case SHCFD_SCHEME_NR:
+        return true;
+    case SHCFD_SCHEME_PRECON:
+    case SHCFD_SCHEME_PRECON_LUSGS:
+        return p->time_step == 1;
+    case SHCFD_SCHEME_DUAL:
+        return p->time_step == 0;
+    default:
+        return false;
+    }
+}
+
 int shcfd_create(int nblocks, const int* ni, const int* nj, int ny, const shcfd_params* params, shcfd_ctx** out) {
-    if (!out || nblocks <= 0 || !ni || !nj || !params || !ny_supported(ny)) return SHCFD_E_ARG;
+    if (!out || nblocks <= 0 || !ni || !nj || !params || !ny_supported(ny) || !params_valid(params)) return SHCFD_E_ARG;
     *out = nullptr;
     for (int b = 0; b < nblocks; b++)
         if (ni[b] < 2 || nj[b] < 2) return SHCFD_E_ARG;
@@ -201,7 +222,7 @@ int shcfd_destroy(shcfd_ctx* ctx) {
 }
 
 int shcfd_set_params(shcfd_ctx* ctx, const shcfd_params* params) {
-    if (!ctx || !params) return SHCFD_E_ARG;
+    if (!ctx || !params || !params_valid(params)) return SHCFD_E_ARG;
     ctx->p = *params;
     return SHCFD_OK;
 }
@@ -717,7 +738,7 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
         DISPATCH_NY(ctx->ny, hipLaunchKernelGGL((k_sweep_bwd_plain<NY, false>), dim3(ns), dim3(64), 0, ctx->stream, d,
                                                 ctx->dstrips, ns, prog_b, ctx->dbnd, ctx->bnd_stride, ctx->dpart,
                                                 ctx->spin_limit, nullptr, 0LL, nullptr)) }
-    if (ctx->p.scheme == SHCFD_SCHEME_DUAL) {
+    if (inner_loop_scheme(ctx->p.scheme)) {
         { ProfScope ps(ctx, PROF_REDUCE);
         hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, ctx->stream, d, ctx->dpart, ns); }
         if (step_internal == 1)
@@ -774,7 +795,7 @@ int shcfd_set_option(shcfd_ctx* ctx, int option, int value) {
 int shcfd_calc_next_step_implicit(shcfd_ctx* ctx, int step_internal, double* res_out, double* omega_out) {
     if (!ctx) return SHCFD_E_ARG;
     int s = ctx->p.scheme;
-    if (s != SHCFD_SCHEME_LUSGS && s != SHCFD_SCHEME_PRECON_LUSGS && s != SHCFD_SCHEME_DUAL) return SHCFD_E_STATE;
+    if (s != SHCFD_SCHEME_LUSGS && s != SHCFD_SCHEME_PRECON_LUSGS && !inner_loop_scheme(s)) return SHCFD_E_STATE;
     int rc = implicit_async(ctx, step_internal);
     if (rc) return rc;
     if (res_out || omega_out) {
@@ -920,16 +941,18 @@ static int enqueue_step(shcfd_ctx* ctx, int step) {
 
 int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double* res_last) {
     if (!ctx) return SHCFD_E_ARG;
-    if (ctx->p.scheme != SHCFD_SCHEME_DUAL) {
+    if (!inner_loop_scheme(ctx->p.scheme)) {
         int rc = enqueue_step(ctx, step);
         return rc ? rc : shcfd_check_error(ctx);
     }
     int rc = SHCFD_OK;
 #define TRY(x) do { rc = (x); if (rc) return rc; } while (0)
     switch (ctx->p.scheme) {
-    case SHCFD_SCHEME_DUAL: {
+    case SHCFD_SCHEME_NR:      // NR/2d/main.main.raw.f90:1-20, the loop of dual time
+    case SHCFD_SCHEME_PRECON:  // precon/2d/main.main.raw.f90:1-19: no set_dt, set_ABpm sets dt_mat
+    case SHCFD_SCHEME_DUAL: {  // dual/2d/main.main.raw.f90:1-20
         TRY(part_primitive(ctx));
-        TRY(shcfd_set_dt(ctx, step, nullptr));
+        if (ctx->p.scheme != SHCFD_SCHEME_PRECON) TRY(shcfd_set_dt(ctx, step, nullptr));
         TRY(shcfd_copy_state(ctx, SHCFD_F_QPP, SHCFD_F_QP));
         TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
         for (int j = 1; j < ctx->p.num_internal_loop; j++) {
@@ -945,7 +968,8 @@ int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double
         if (res_first) *res_first = sc[SCAL_RES1];
         if (res_last) *res_last = sc[SCAL_RES];
         TRY(shcfd_check_error(ctx));
-        // check_internal_loop_convergence: sch_dual.raw.f90:284-299
+        // check_internal_loop_convergence: sch_dual.raw.f90:284-299 (the same
+        // routine in sch_NR.raw.f90:216-231 and sch_precon.raw.f90:276-291)
         double rate = sc[SCAL_RES] / sc[SCAL_RES1];
         if (rate > ctx->p.res_rate_warning) {
             fprintf(stderr, "WARNING:NOT CONVERGED AT INTERNAL LOOP (step,res_rate) = %12d %11.3E\n", step, rate);
@@ -1008,7 +1032,7 @@ static void warm_allocations(shcfd_ctx* ctx) {
 int shcfd_iterate(shcfd_ctx* ctx, int first_step, int nsteps, int graph_steps) {
     if (!ctx || nsteps < 0) return SHCFD_E_ARG;
     if (graph_steps < 0) graph_steps = ctx->tuned_graph_steps;
-    if (ctx->p.scheme == SHCFD_SCHEME_DUAL) {
+    if (inner_loop_scheme(ctx->p.scheme)) {
         for (int k = 0; k < nsteps; k++) {
             int rc = shcfd_step(ctx, first_step + k, first_step - 1, nullptr, nullptr);
             if (rc) return rc;
@@ -1056,7 +1080,7 @@ int shcfd_iterate(shcfd_ctx* ctx, int first_step, int nsteps, int graph_steps) {
 // tuning does not advance the solution.  choice[4] = fast, tile rows, graph
 // steps, best ms per step (may be NULL).
 int shcfd_autotune(shcfd_ctx* ctx, int trial_steps, double* choice) {
-    if (!ctx || trial_steps < 1 || ctx->p.scheme == SHCFD_SCHEME_DUAL) return SHCFD_E_ARG;
+    if (!ctx || trial_steps < 1 || inner_loop_scheme(ctx->p.scheme)) return SHCFD_E_ARG;
     const size_t nq = (size_t)ctx->dimq * (size_t)ctx->NC;
     double* save = nullptr;
     double* ssave = nullptr;

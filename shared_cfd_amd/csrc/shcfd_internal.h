@@ -48,6 +48,18 @@ enum {
 };
 enum { IFLAG_ERR = 0, IFLAG_COUNT = 8 };
 
+// Schemes whose physical step runs the inner loop of dual time: dq/dt from
+// q, qp, qpp (set_dqdt), an omega-limited update and the residual norm res
+// (NR/2d, precon/2d and dual/2d time_*_viscous.f90).
+__host__ __device__ inline bool inner_loop_scheme(int s) {
+    return s == SHCFD_SCHEME_NR || s == SHCFD_SCHEME_PRECON || s == SHCFD_SCHEME_DUAL;
+}
+// Schemes whose set_ABpm builds the low-Mach preconditioned operator
+// (sch_precon.raw.f90, sch_dual.raw.f90).
+__host__ __device__ inline bool precond_scheme(int s) {
+    return s == SHCFD_SCHEME_PRECON || s == SHCFD_SCHEME_PRECON_LUSGS || s == SHCFD_SCHEME_DUAL;
+}
+
 __host__ __device__ inline long long cidx(const DevBlock& B, int i, int j) {
     return B.off + (long long)(j + 1) * B.pitch + (i + 1);
 }

@@ -188,7 +188,8 @@ __device__ inline void flux_balance(const DevCtx& d, const DevBlock& B, long lon
 }
 
 // Cell-centre split flux Jacobians and LU-SGS scalars of one cell:
-// LU-SGS/2d/sch_lusgs.raw.f90:26-136, preconLU-SGS/2d/sch_precon.raw.f90:28-197,
+// LU-SGS/2d/sch_lusgs.raw.f90:26-136, NR/2d/sch_NR.raw.f90:26-137,
+// precon/2d/sch_precon.raw.f90:27-197, preconLU-SGS/2d/sch_precon.raw.f90:28-197,
 // dual/2d/sch_dual.raw.f90:28-204.
 template <int NY>
 struct CellJac {
@@ -220,7 +221,7 @@ __device__ inline void cell_jacobian(const DevCtx& d, long long c, long long P, 
     double vol = FLD(d, SHCFD_F_VOL, 0, c);
     double dtv = (d.p.time_step == 0) ? d.scal[SCAL_DT_GRBL] : FLD(d, SHCFD_F_DT, 0, c);
     double nuA, nuB;
-    J.precond = (scheme == SHCFD_SCHEME_DUAL || scheme == SHCFD_SCHEME_PRECON_LUSGS);
+    J.precond = precond_scheme(scheme);
     double pscale = 1.0;
     J.phi = 0.0;
     J.dtm = 0.0;
@@ -247,7 +248,7 @@ __device__ inline void cell_jacobian(const DevCtx& d, long long c, long long P, 
                      (nuB * (FLD(d, SHCFD_F_DSJ, 0, c) + FLD(d, SHCFD_F_DSJ, 0, c - P)));
         double nr = wc[imu] / rho;
         double mx = (uds > nr) ? uds : nr;
-        if (scheme == SHCFD_SCHEME_PRECON_LUSGS) {
+        if (scheme != SHCFD_SCHEME_DUAL) {
             double dtm = (d.p.cfl * vol) / mx;
             J.dtm = dtm;
             dtv = dtm;
@@ -373,7 +374,10 @@ __device__ inline void cell_jacobian(const DevCtx& d, long long c, long long P, 
     }
 #undef AM_
 #undef BM_
-    if (scheme != SHCFD_SCHEME_DUAL) J.alpha = 1.0 / (((vol / dtv) + (dsci * nuA)) + (dscj * nuB));
+    if (scheme == SHCFD_SCHEME_NR || scheme == SHCFD_SCHEME_PRECON)  // BDF2 diagonal: sch_NR.raw.f90:134-136
+        J.alpha = 1.0 / ((((3.0 * vol) / (2.0 * dtv)) + (dsci * nuA)) + (dscj * nuB));
+    else if (scheme != SHCFD_SCHEME_DUAL)
+        J.alpha = 1.0 / (((vol / dtv) + (dsci * nuA)) + (dscj * nuB));
 }
 
 // Cells touched by one copy of section_exchange (cond/no-nV/condition.tail.f90:19-68):

@@ -170,9 +170,11 @@ program main
          call set_w(); call set_thermo_prop(i)
          call set_BC(i); call set_HV(); call set_TG(); call set_TGv()
          call calc_next_step_RK2_second()
-      case (SCHEME_DUAL)                   ! time_scheme/dual/2d/main.main.raw.f90
+      case (SCHEME_NR, SCHEME_PRECON, SCHEME_DUAL)
+         ! time_scheme/{NR,precon,dual}/2d/main.main.raw.f90: the same inner
+         ! loop; precon has no set_dt (its set_ABpm sets dt_mat)
          call set_w(); call set_thermo_prop(i)
-         call set_dt(i)
+         if (prm%scheme /= SCHEME_PRECON) call set_dt(i)
          call copy_qpp()
          call copy_qp()
          j = 1

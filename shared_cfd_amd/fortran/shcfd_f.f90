@@ -24,6 +24,7 @@ module shcfd_f
    integer(c_int), parameter :: SHCFD_F_Q = 0, SHCFD_F_QP = 1, SHCFD_F_QPP = 2, SHCFD_F_W = 3
    integer(c_int), parameter :: SHCFD_F_XH = 27, SHCFD_F_RH = 28
    integer(c_int), parameter :: SCHEME_EULER = 0, SCHEME_RK2 = 1, SCHEME_LUSGS = 2
+   integer(c_int), parameter :: SCHEME_NR = 3, SCHEME_PRECON = 4
    integer(c_int), parameter :: SCHEME_DUAL = 5, SCHEME_PRECON_LUSGS = 6
 
    interface

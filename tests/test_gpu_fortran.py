@@ -39,10 +39,12 @@ def _run_oracle(case, nsteps, out_period):
     return o, np.array(rms)
 
 
-@pytest.mark.parametrize("name,fast", [("sod", 0), ("sod", 1), ("step", 1)])
+@pytest.mark.parametrize("name,fast", [("sod", 0), ("sod", 1), ("step", 1), ("step_nr", 0)])
 def test_fortran_driver_matches_oracle(tmp_path, name, fast):
     if name == "sod":
         case, n, per = C.sod(64, 4), 20, 5
+    elif name == "step_nr":
+        case, n, per = C.forward_step(n_per_unit=40, scheme=C.SCHEME_NR, num_internal_loop=6), 3, 1
     else:
         case, n, per = C.forward_step(n_per_unit=40, num_internal_loop=6), 3, 1
     rms_d, (step, tt, blocks) = _run_driver(case, tmp_path, n, per, fast)

@@ -80,14 +80,18 @@ def test_routines(viscous, limiter, nb, ny):
 
 
 @pytest.mark.parametrize("scheme,fast", [(C.SCHEME_PRECON_LUSGS, 0), (C.SCHEME_DUAL, 0), (C.SCHEME_PRECON_LUSGS, 1),
-                                         (C.SCHEME_DUAL, 1)])
-def test_precond_routines(scheme, fast):
+                                         (C.SCHEME_DUAL, 1), (C.SCHEME_NR, 0), (C.SCHEME_NR, 1),
+                                         (C.SCHEME_PRECON, 0), (C.SCHEME_PRECON, 1)])
+def test_scheme_routines(scheme, fast):
+    local = scheme in (C.SCHEME_PRECON_LUSGS, C.SCHEME_PRECON)
+    inner = scheme in (C.SCHEME_NR, C.SCHEME_PRECON, C.SCHEME_DUAL)
     case = C.random_case(seed=5, ni=(13, 10), nj=(70, 70), viscous=1, with_cut=True, scheme=scheme,
-                         time_step=1 if scheme == C.SCHEME_PRECON_LUSGS else 0)
+                         time_step=1 if local else 0)
     o, s = _pair(case, fast)
     o.prepare(); s.prepare()
-    if scheme == C.SCHEME_DUAL:
-        o.set_dt(); s.set_dt()
+    if inner:
+        if scheme != C.SCHEME_PRECON:
+            o.set_dt(); s.set_dt()
         o.copy_field(C.F["QPP"], C.F["QP"]); s.copy_state("QPP", "QP")
         o.copy_field(C.F["QP"], C.F["Q"]); s.copy_state("QP", "Q")
         # make q differ from qp so dq/dt is not zero
@@ -98,18 +102,20 @@ def test_precond_routines(scheme, fast):
         o.set_w(); s.set_w(); o.set_thermo_prop(); s.set_thermo_prop()
     o.set_bc(); s.set_BC(); o.set_hv(); s.set_HV(); o.set_tg(); s.set_TG(); o.set_tgv(); s.set_TGv()
     o.set_abpm(); s.set_ABpm()
-    fields = ["AP", "AM", "BP", "BM", "ALPHA", "PRE1", "PRE2", "PHI"]
-    fields += ["PRESCA"] if scheme == C.SCHEME_DUAL else ["DT"]
+    fields = ["AP", "AM", "BP", "BM", "ALPHA"]
+    if scheme != C.SCHEME_NR:
+        fields += ["PRE1", "PRE2", "PHI"]
+    fields += ["PRESCA"] if scheme == C.SCHEME_DUAL else (["DT"] if local else [])
     for f in fields:
         _cmp(o, s, f, interior)
-    if scheme == C.SCHEME_DUAL:
+    if inner:
         o.set_dqdt(); s.set_dqdt()
         _cmp(o, s, "DQDT", interior)
     err_o, res_o, om_o = o.implicit(1)
     res_s, om_s = s.calc_next_step_implicit(1)
     _cmp(o, s, "DQ", interior)
     _cmp(o, s, "Q", interior)
-    if scheme == C.SCHEME_DUAL:
+    if inner:
         assert abs(res_s - res_o) <= 1e-12 * abs(res_o)
         assert abs(om_s - om_o) <= 1e-12 * abs(om_o)
 
@@ -149,6 +155,29 @@ def test_forward_step_dual(n, fast):
 def test_precon_viscous_steps(fast):
     case = C.random_case(seed=9, ni=(40,), nj=(70,), viscous=1, scheme=C.SCHEME_PRECON_LUSGS, time_step=1, mach=0.1)
     o, s, _ = _run_steps(case, 5, fast)
+    _cmp(o, s, "Q", interior, tol=1e-11)
+
+
+@pytest.mark.parametrize("scheme,fast", [(C.SCHEME_NR, 0), (C.SCHEME_NR, 1)])
+def test_forward_step_nr(scheme, fast):
+    case = C.forward_step(n_per_unit=40, scheme=scheme, num_internal_loop=6)
+    o, s, out = _run_steps(case, 2, fast)
+    for r1o, r2o, r1s, r2s in out:
+        assert abs(r1s - r1o) <= 1e-12 * abs(r1o)
+        assert abs(r2s - r2o) <= 1e-11 * abs(r2o)
+    _cmp(o, s, "Q", interior, tol=1e-11)
+
+
+@pytest.mark.parametrize("fast", [0, 1])
+def test_precon_inner_loop_steps(fast):
+    case = C.random_case(seed=9, ni=(40,), nj=(70,), viscous=1, scheme=C.SCHEME_PRECON, time_step=1, mach=0.1)
+    case.params.cfl = 0.1
+    case.params.num_internal_loop = 6
+    case.params.res_rate_warning = case.params.res_rate_error = 1e300  # 5 inner iterations do not converge
+    o, s, out = _run_steps(case, 3, fast)
+    for r1o, r2o, r1s, r2s in out:
+        assert abs(r1s - r1o) <= 1e-11 * abs(r1o)
+        assert abs(r2s - r2o) <= 1e-10 * abs(r2o)
     _cmp(o, s, "Q", interior, tol=1e-11)
 
 

@@ -322,11 +322,30 @@ def test_dual_preconditioner_algebra():
 
 
 # --------------------------------------------------------------------------- LU-SGS: brute force
-def _brute_force_check(o, dual):
+def _cell_rhs(F, I, J, kind):
+    """Right-hand side of one cell from the stored fluxes: the flux balance,
+    minus Vol dq/dt for the inner-loop schemes, with the preconditioner
+    (I + pre1 pre2^T) on the flux balance (precon) or on the whole (dual)."""
+    tgi, tgj = F("TGI") - F("TGVI"), F("TGJ") - F("TGVJ")
+    dsi, dsj = F("DSI")[..., 0], F("DSJ")[..., 0]
+    r = dsi[J, I - 1] * tgi[J, I - 1] - dsi[J, I] * tgi[J, I] + dsj[J - 1, I] * tgj[J - 1, I] - dsj[J, I] * tgj[J, I]
+    p1, p2 = F("PRE1")[J, I], F("PRE2")[J, I]
+    if kind == "precon":
+        r = r + np.dot(p2, r) * p1
+    if kind in ("dual", "nr", "precon"):
+        r = r - F("VOL")[J, I, 0] * F("DQDT")[J, I]
+    if kind == "dual":
+        r = r + F("PRESCA")[J, I][1] * np.dot(p2, r) * p1
+    return r
+
+
+def _brute_force_check(o, kind):
     """The LU-SGS update satisfies (D+L) D^-1 (D+U) Dq = R' exactly, with
     the block-lower/upper couplings dsc*A+/B+ and dsc*A-/B- of the
-    neighbouring cells, D = 1/alpha (LU-SGS) or S/alpha (dual), and
-    Dq = 0 outside the block (time_lusgs_viscous.f90, time_dual_viscous.f90)."""
+    neighbouring cells, D = 1/alpha (LU-SGS, NR, precon) or S/alpha (dual),
+    and Dq = 0 outside the block (time_lusgs_viscous.f90, time_NR_viscous.f90,
+    time_precon_viscous.f90, time_dual_viscous.f90)."""
+    dual = kind == "dual"
     case = o.case
     ni, nj = case.ni[0], case.nj[0]
     N = ni * nj * 4
@@ -359,14 +378,7 @@ def _brute_force_check(o, dual):
             if j < nj:
                 b = idx(i, j + 1)
                 Um[a:a + 4, b:b + 4] = F("DSCJ")[J + 1, I, 0] * F("BM")[J + 1, I].reshape(4, 4, order="F")
-            # flux balance
-            tgi, tgj = F("TGI") - F("TGVI"), F("TGJ") - F("TGVJ")
-            dsi, dsj = F("DSI")[..., 0], F("DSJ")[..., 0]
-            r = dsi[J, I - 1] * tgi[J, I - 1] - dsi[J, I] * tgi[J, I] + dsj[J - 1, I] * tgj[J - 1, I] - dsj[J, I] * tgj[J, I]
-            if dual:
-                r = r - F("VOL")[J, I, 0] * F("DQDT")[J, I]
-                r = r + ps[1] * np.dot(p2, r) * p1
-            Rv[a:a + 4] = r
+            Rv[a:a + 4] = _cell_rhs(F, I, J, kind)
     M = (Dm + Lm) @ np.linalg.solve(Dm, Dm + Um)
     dq = np.concatenate([F("DQ")[j + 1, i + 1] for j in range(1, nj + 1) for i in range(1, ni + 1)])
     res = M @ dq - Rv
@@ -377,7 +389,7 @@ def test_lusgs_brute_force():
     case = C.random_case(seed=12, ni=(6,), nj=(5,), scheme=C.SCHEME_LUSGS)
     o = _prep(case)
     o.implicit(0)
-    _brute_force_check(o, dual=False)
+    _brute_force_check(o, "lusgs")
 
 
 def test_dual_lusgs_brute_force():
@@ -392,19 +404,122 @@ def test_dual_lusgs_brute_force():
     o.set_abpm(); o.set_dqdt()
     q_before = _int(o, "Q").copy()
     err, res, om = o.implicit(1)
-    _brute_force_check(o, dual=True)
+    _brute_force_check(o, "dual")
     # the update is q + min(omega_max, omega) dq, and res = sum dq_E^2
     dq = _int(o, "DQ")
     np.testing.assert_allclose(res, np.sum(dq[..., 3] ** 2), rtol=1e-12)
     np.testing.assert_allclose(_int(o, "Q"), q_before + min(case.params.omega_max, om) * dq, rtol=1e-14)
 
 
+def _inner_state(o, kind):
+    """Primitives, fluxes, set_ABpm and set_dqdt at the current q; returns the
+    unsteady right-hand side of every interior cell."""
+    o.set_w(); o.set_thermo_prop(); o.set_bc(); o.set_hv(); o.set_tg(); o.set_tgv()
+    o.set_abpm(); o.set_dqdt()
+    F = lambda name: o.field(name, 0)
+    ni, nj = o.case.ni[0], o.case.nj[0]
+    return np.array([_cell_rhs(F, i + 1, j + 1, kind) for j in range(1, nj + 1) for i in range(1, ni + 1)])
+
+
+def _inner_case(scheme, seed, cfl=None):
+    case = C.random_case(seed=seed, ni=(6,), nj=(5,), scheme=scheme, viscous=1,
+                         time_step=1 if scheme == C.SCHEME_PRECON else 0, mach=0.3)
+    if cfl is not None:
+        case.params.cfl = cfl
+    case.params.omega_max = 1.0
+    o = _oracle(case)
+    o.prepare()
+    if scheme == C.SCHEME_NR:
+        o.set_dt()
+    o.copy_field(C.F["QPP"], C.F["QP"]); o.copy_field(C.F["QP"], C.F["Q"])
+    return case, o
+
+
+@pytest.mark.parametrize("scheme,kind", [(C.SCHEME_NR, "nr"), (C.SCHEME_PRECON, "precon")])
+def test_inner_loop_brute_force(scheme, kind):
+    """NR and precon inner iterations: the sweeps solve the same
+    approximate factorisation as LU-SGS with the BDF2 right-hand side, and
+    the update is q + min(omega_max, omega) dq with res = sum dq_E^2
+    (NR/2d/time_NR_viscous.f90, precon/2d/time_precon_viscous.f90)."""
+    case, o = _inner_case(scheme, 41)
+    q = o.field("Q", 0)
+    q[2:-2, 2:-2] *= 1.0 + 1e-3 * np.cos(np.arange(q.shape[1] - 4))[None, :, None]
+    _inner_state(o, kind)
+    q_before = _int(o, "Q").copy()
+    err, res, om = o.implicit(1)
+    assert err == 0
+    _brute_force_check(o, kind)
+    dq = _int(o, "DQ")
+    np.testing.assert_allclose(res, np.sum(dq[..., 3] ** 2), rtol=1e-12)
+    np.testing.assert_allclose(_int(o, "Q"), q_before + min(case.params.omega_max, om) * dq, rtol=1e-14)
+
+
+@pytest.mark.parametrize("scheme,kind", [(C.SCHEME_NR, "nr"), (C.SCHEME_PRECON, "precon")])
+def test_inner_loop_diagonal_small_dt(scheme, kind):
+    """At a tiny time step the BDF2 term 3 Vol/(2 dt) dominates the
+    operator, so one inner iteration must nearly zero the unsteady
+    residual R(q) - Vol (3q - 4qp + qpp)/(2 dt): a diagonal Vol/dt (the
+    steady LU-SGS one) would leave half of it, a wrong sign of the dq/dt
+    term would grow it (sch_NR.raw.f90:134-136, sch_precon.raw.f90:194-196)."""
+    case, o = _inner_case(scheme, 43, cfl=1e-5)
+    o.set_first_dual(0)
+    r0 = _inner_state(o, kind)
+    err, _, _ = o.implicit(1)
+    assert err == 0
+    r1 = _inner_state(o, kind)
+    assert np.linalg.norm(r1) < 1e-3 * np.linalg.norm(r0), np.linalg.norm(r1) / np.linalg.norm(r0)
+
+
+def test_set_dqdt_bdf2_exact():
+    """set_dqdt is exact for data quadratic in time (BDF2) and, on the first
+    physical step, for data linear in time (sch_NR.raw.f90:143-172)."""
+    case, o = _inner_case(C.SCHEME_NR, 45)
+    dt = o.dt_grbl()
+    assert dt > 0.0
+    q0 = o.field("Q", 0).copy()
+    a = 1e-2 * np.sin(np.arange(q0.size)).reshape(q0.shape)
+    b = 3e-3 * np.cos(np.arange(q0.size)).reshape(q0.shape)
+    qt = lambda n: q0 * (1.0 + a * n + b * n ** 2)   # q at t = n dt
+    o.field("QPP", 0)[...] = qt(0); o.field("QP", 0)[...] = qt(1); o.field("Q", 0)[...] = qt(2)
+    o.set_first_dual(0)
+    o.set_dqdt()
+    exact = interior(q0 * (a + 4.0 * b) / dt, 6, 5)
+    np.testing.assert_allclose(_int(o, "DQDT"), exact, rtol=1e-9, atol=1e-12 * np.abs(exact).max())
+    o.field("QP", 0)[...] = q0; o.field("Q", 0)[...] = q0 * (1.0 + a)
+    o.set_first_dual(1)
+    o.set_dqdt()
+    exact = interior(q0 * a / dt, 6, 5)
+    np.testing.assert_allclose(_int(o, "DQDT"), exact, rtol=1e-9, atol=1e-12 * np.abs(exact).max())
+
+
+def test_precon_scaling_eigenvalue():
+    """The precon scheme's (I + pre1 pre2^T) scales the acoustic direction
+    (Y, u, v, H) by Ur^2/c^2 with Ur = min(max(nu/dmin, |u|), c) and leaves
+    the directions orthogonal to pre2 alone (sch_precon.raw.f90:44-68)."""
+    case, o = _inner_case(C.SCHEME_PRECON, 47)
+    o.set_abpm()
+    w = _int(o, "W")
+    p1, p2 = _int(o, "PRE1"), _int(o, "PRE2")
+    rho, u, v, p = w[..., 0], w[..., 1], w[..., 2], w[..., 3]
+    c = np.sqrt(w[..., 5] * p / rho)
+    dmin = np.minimum(_int(o, "DSCI")[..., 0], _int(o, "DSCJ")[..., 0])
+    ur = np.minimum(np.maximum(w[..., 8] / rho / dmin, np.hypot(u, v)), c)
+    lam = 1.0 + np.sum(p2 * p1, axis=-1)
+    np.testing.assert_allclose(lam, ur ** 2 / c ** 2, rtol=1e-12)
+    # pre1 is parallel to (Y, u, v, H)
+    h = w[..., 6]
+    base = np.stack([np.ones_like(u), u, v, h], axis=-1)
+    np.testing.assert_allclose(p1 / base, (p1 / base)[..., :1] * np.ones(4), rtol=1e-12)
+
+
 # --------------------------------------------------------------------------- invariants of whole steps
-@pytest.mark.parametrize("scheme", [C.SCHEME_EULER, C.SCHEME_RK2, C.SCHEME_LUSGS, C.SCHEME_PRECON_LUSGS])
+@pytest.mark.parametrize("scheme", [C.SCHEME_EULER, C.SCHEME_RK2, C.SCHEME_LUSGS, C.SCHEME_PRECON_LUSGS,
+                                    C.SCHEME_NR, C.SCHEME_PRECON, C.SCHEME_DUAL])
 def test_free_stream_preservation(scheme):
     """A uniform flow on a skewed grid with consistent boundary states stays
     uniform (closed cells + SLAU consistency)."""
-    case = C.random_case(seed=21, ni=(9,), nj=(7,), scheme=scheme, time_step=1 if scheme == C.SCHEME_PRECON_LUSGS else 0)
+    local = scheme in (C.SCHEME_PRECON_LUSGS, C.SCHEME_PRECON)
+    case = C.random_case(seed=21, ni=(9,), nj=(7,), scheme=scheme, time_step=1 if local else 0)
     kap, R = 1.4, 287.0
     q0 = C.cons_from_prim(np.array(1.2), np.array(150.0), np.array(20.0), np.array(1.0e5), kap)
     case.q0[0][2:-2, 2:-2, :] = q0
