@@ -193,15 +193,34 @@ int shcfd_iterate(shcfd_ctx *ctx, int first_step, int nsteps, int graph_steps);
  * choice[4] (may be NULL) = fast sweeps, tile rows, graph steps, ms/step. */
 int shcfd_autotune(shcfd_ctx *ctx, int trial_steps, double *choice);
 
+/* ---- fixed physical time step and device snapshots (no counterpart in
+ * the reference, whose global time step always follows the CFL rule of
+ * set_dt; a time-accurate run that is differentiated or compared across
+ * perturbed inputs needs the same dt in every run, and checkpointing needs
+ * states kept on the device) ---------------------------------------------- */
+/* dt > 0 holds dt_grbl at dt in set_dt (global time step only); 0 restores
+ * the CFL rule */
+int shcfd_set_fixed_dt(shcfd_ctx *ctx, double dt);
+/* copy field (SHCFD_F_Q, _QP or _QPP, all cells) to / from snapshot slot
+ * (0 <= slot < SHCFD_MAX_SNAPSHOTS; device memory of a slot is allocated on
+ * its first save and kept until shcfd_snapshot_free or shcfd_destroy) */
+int shcfd_snapshot_save(shcfd_ctx *ctx, int slot, int field);
+int shcfd_snapshot_load(shcfd_ctx *ctx, int slot, int field);
+int shcfd_snapshot_free(shcfd_ctx *ctx);
+
 /* ---- discrete adjoint (no counterpart in the reference, which has no
- * adjoint; for shape and inflow-angle gradients of steady inviscid
- * ideal-gas solutions: nY = 1, LU-SGS, optimised implicit path) ---------- */
+ * adjoint; for shape and inflow-angle gradients of inviscid ideal-gas
+ * solutions: nY = 1, optimised implicit path; steady: LU-SGS, unsteady:
+ * the NR or dual scheme with a fixed time step) --------------------------- */
 /* what of shcfd_adjoint_upload / shcfd_adjoint_download */
 enum {
     SHCFD_ADJ_LAMBDA = 0, /* adjoint state, dimq per cell (reference cell layout) */
     SHCFD_ADJ_QBAR = 1,   /* lambda^T dR/dq + dJ/dq per cell                      */
-    SHCFD_ADJ_XBAR = 2    /* lambda^T dR/dX + dJ/dX per node: (x, y) in the cell
+    SHCFD_ADJ_XBAR = 2,   /* lambda^T dR/dX + dJ/dX per node: (x, y) in the cell
                              layout at the node's index, as node uploads use     */
+    SHCFD_ADJ_XBAR_SUM = 3,   /* unsteady: node bars summed over the steps done   */
+    SHCFD_ADJ_LAMBDA_LATER = 4 /* unsteady: lambda of the step after the last one
+                                  done (lambda^2 once step 1 is done)             */
 };
 /* objective J = sum over the j-min faces i in [i0, i1] of block (0-based
  * block, 1-based i) of dsj * (TGj_u * ax + TGj_v * ay): the pressure
@@ -225,6 +244,23 @@ int shcfd_residual_vjp(shcfd_ctx *ctx);
  * lambda, so XBAR is dJ/dX and the pool bars dJ/d(fixed states). */
 int shcfd_adjoint_solve(shcfd_ctx *ctx, int max_iter, double rtol, int check_every, double *res0, double *res,
                         int *iters);
+/* Unsteady adjoint of the dual-time (BDF2) steps of the NR or dual scheme
+ * with a fixed time step: U^n = R(q^n) - Vol (a0 q^n + a1 q^(n-1) + a2 q^(n-2))/dt
+ * = 0, with (a0, a1, a2) = (3/2, -2, 1/2) and (1, -1, 0) on the first step.
+ * For J = sum_n J^n(q^n), the adjoint steps run from the last physical step
+ * to the first; each solves
+ *   (dU^n/dq^n)^T lambda^n = -(dJ^n/dq^n + Vol/dt (2 lambda^(n+1) - lambda^(n+2)/2))^T
+ * by the transposed LU-SGS iteration (the NR operator as preconditioner)
+ * and adds lambda^nT dU^n/dX + dJ^n/dX to XBAR_SUM and lambda^nT dU^n/d(fixed
+ * states) to the pool-bar sum.  reset zeroes lambda, the later lambda and
+ * the sums.  Before each step the caller restores Q = q^n, QP = q^(n-1),
+ * QPP = q^(n-2) (shcfd_snapshot_load), sets first_dual with shcfd_set_time
+ * and the weights of J^n with shcfd_adjoint_objective (0, 0 for none). */
+int shcfd_adjoint_unsteady_reset(shcfd_ctx *ctx);
+int shcfd_adjoint_unsteady_step(shcfd_ctx *ctx, int max_iter, double rtol, int check_every, double *res0,
+                                double *res, int *iters);
+/* the pool-bar sum (npool doubles) */
+int shcfd_adjoint_unsteady_poolbar(shcfd_ctx *ctx, double *host);
 
 /* ---- per-kernel timing (no counterpart in the reference, whose timers
  * are commented-out stubs: store/debug/time_barrier.f90:1-37) ------------- */

@@ -79,6 +79,9 @@ enum { SHCFD_SIDE_IMIN = 0, SHCFD_SIDE_IMAX = 1, SHCFD_SIDE_JMIN = 2, SHCFD_SIDE
  * (geo,width,order,p1,c1,s1,pm1,p2,c2,s2,pm2), store/cond/core/mod_cond.py:86-91 */
 #define SHCFD_CUT_ROW 11
 
+/* device snapshot slots per context (shcfd_snapshot_save) */
+#define SHCFD_MAX_SNAPSHOTS 4096
+
 /* ---- field identifiers (upload/download in the reference layout) --------- */
 enum {
     SHCFD_F_Q = 0,      /* q(1:dimq)        conservative                     */

@@ -55,6 +55,7 @@ def lib():
         L.or_tt.argtypes = [vp]
         L.or_tt.restype = ctypes.c_double
         L.or_set_first_dual.argtypes = [vp, ctypes.c_int]
+        L.or_set_fixed_dt.argtypes = [vp, ctypes.c_double]
         L.or_copy_field.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.or_calc_next_step_implicit.argtypes = [vp, ctypes.c_int, dp, dp]
         L.or_prepare.argtypes = [vp]

@@ -34,6 +34,7 @@ struct or_ctx {
     double *pool;
     int npool;
     double dt_grbl, tt;
+    double dt_fixed; /* > 0: the global time step is held at this value */
     int first_dual;
     double res1;
     double *scratch; /* per-cell res contributions, max block size */
@@ -867,11 +868,13 @@ double or_set_dt(or_ctx *c) {
             }
     }
     if (c->p.time_step == 0) {
-        c->dt_grbl = tmp;
+        c->dt_grbl = (c->dt_fixed > 0.0) ? c->dt_fixed : tmp;
         c->tt = c->tt + c->dt_grbl;
     }
     return tmp;
 }
+
+void or_set_fixed_dt(or_ctx *c, double dt) { c->dt_fixed = dt; }
 
 /* ======================================================================== */
 /* flux Jacobians: LU-SGS/2d/sch_lusgs.raw.f90:1-141,                       */

@@ -56,6 +56,7 @@ void or_calc_next_step_rk2(or_ctx *c, int stage); /* RK2/time_RK2_viscous.raw.f9
 double or_energy_rms(or_ctx *c);      /* core/check_convergence.f90:17-28    */
 void or_copy_field(or_ctx *c, int dst, int src);
 double or_dt_grbl(or_ctx *c);
+void or_set_fixed_dt(or_ctx *c, double dt);
 double or_tt(or_ctx *c);
 void or_set_first_dual(or_ctx *c, int first);
 

@@ -23,7 +23,8 @@ import numpy as np
 import torch
 
 from . import airfoil as A
-from .native import ADJ_XBAR, Solver
+from . import cases as C
+from .native import ADJ_LAMBDA, ADJ_LAMBDA_LATER, ADJ_XBAR, ADJ_XBAR_SUM, Solver
 
 
 @dataclasses.dataclass
@@ -150,3 +151,165 @@ class AirfoilCoefficients(torch.autograd.Function):
         finally:
             flow.close()
         return torch.from_numpy(r.dJ_dp), torch.tensor(r.dJ_dalpha, dtype=torch.float64), None
+
+
+# ---------------------------------------------------------------------------
+# unsteady: impulsively started airfoil, BDF2 dual-time steps
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class UnsteadyPrimal:
+    steps: int
+    J: float
+    res_ratio_max: float     # largest last/first inner residual over the steps
+    seconds: float
+
+
+@dataclasses.dataclass
+class UnsteadyAdjoint:
+    steps: int
+    iterations: int          # transposed LU-SGS iterations over all steps
+    res_ratio_max: float
+    seconds: float           # reverse sweep, recomputation included
+    recomputed_steps: int
+    dJ_dp: np.ndarray
+    dJ_dalpha: float
+
+
+class UnsteadyAirfoilFlow:
+    """The airfoil started impulsively from the free stream and advanced by
+    ``nsteps`` BDF2 dual-time steps of the NR scheme (or, with ``scheme`` =
+    SCHEME_DUAL, the low-Mach preconditioned dual scheme) with a fixed time
+    step ``dt`` (``inner - 1`` inner iterations each).  J = mean over the
+    last ``window`` steps of w_l C_L + w_d C_D.
+
+    Checkpointing: with ``checkpoint_every`` = k > 1 the forward run keeps
+    the states (q^m, q^(m-1)) at every k-th step on the device; the reverse
+    sweep recomputes each segment of k steps from its checkpoint (the solver
+    is deterministic, so the recomputed states are bit-identical) and runs
+    the adjoint steps of the segment backwards.  k = 1 keeps every state."""
+
+    def __init__(self, amplitudes=None, alpha_deg=1.25, dt=0.01, inner=41, scheme=C.SCHEME_NR, cfl_tau=5.0, **mesh):
+        prm = dict(scheme=scheme, time_step=0, num_internal_loop=inner, omega_max=1.0, dqmax=1.0, cfl_tau=cfl_tau)
+        prm.update(mesh.pop("params", {}) or {})
+        self.ac = A.naca_cmesh(amplitudes=amplitudes, alpha_deg=alpha_deg, params=prm, **mesh)
+        self.solver = Solver(self.ac.case)
+        self.solver.prepare()
+        self.solver.set_fixed_dt(dt)
+        self.dt = dt
+        self.q_inf = 0.5 * self.ac.rho_inf * self.ac.v_inf ** 2
+        self.i0, self.i1 = self.ac.wall_range()
+        self.nsteps = 0
+
+    def close(self):
+        self.solver.close()
+
+    def directions(self):
+        e_l, e_d = self.ac.directions()
+        return -e_l / self.q_inf, -e_d / self.q_inf
+
+    def _force_flux(self):
+        s = self.solver
+        s.adjoint_objective(0, self.i0, self.i1, 1.0, 0.0)
+        fx = s.objective()
+        s.adjoint_objective(0, self.i0, self.i1, 0.0, 1.0)
+        fy = s.objective()
+        return np.array([fx, fy])
+
+    # ---- forward -----------------------------------------------------------
+    def run(self, nsteps, window, w_l=1.0, w_d=0.0, checkpoint_every=1) -> UnsteadyPrimal:
+        s = self.solver
+        k = int(checkpoint_every)
+        assert nsteps >= 1 and 1 <= window <= nsteps and k >= 1
+        self.nsteps, self.window, self.k = nsteps, window, k
+        self.w = (w_l, w_d)
+        self.forces = {}
+        a_l, a_d = self.directions()
+        t0 = time.perf_counter()
+        s.snapshot_save(0, "Q")
+        s.snapshot_save(1, "Q")      # q^(-1) is never read; keep the slot valid
+        J, worst = 0.0, 0.0
+        for n in range(1, nsteps + 1):
+            r1, r2 = s.step(n, 0)
+            worst = max(worst, r2 / r1 if r1 > 0 else 0.0)
+            if n > nsteps - window:
+                F = self._force_flux()
+                self.forces[n] = F
+                J += float(F @ (w_l * a_l + w_d * a_d)) / window
+            if k == 1:
+                s.snapshot_save(n, "Q")
+            elif n % k == 0 and n < nsteps:
+                s.snapshot_save(2 * (n // k), "Q")
+                s.snapshot_save(2 * (n // k) + 1, "QP")
+        s.synchronize()
+        self.J = J
+        return UnsteadyPrimal(nsteps, J, worst, time.perf_counter() - t0)
+
+    # ---- reverse -----------------------------------------------------------
+    def gradient(self, rtol=1e-10, max_iter=20000, check_every=10) -> UnsteadyAdjoint:
+        s = self.solver
+        N, k, window, dt = self.nsteps, self.k, self.window, self.dt
+        w_l, w_d = self.w
+        a_l, a_d = self.directions()
+        a = (w_l * a_l + w_d * a_d) / window
+        t0 = time.perf_counter()
+        s.adjoint_unsteady_reset()
+        its, worst, recomputed = 0, 0.0, 0
+        ncp = (N + k - 1) // k
+        seg0 = 2 * ncp                      # first segment slot (k > 1)
+
+        def slot_of(m, k0):
+            """(slot, field) holding q^m while segment (k0, ...] is live."""
+            if k == 1:
+                return m, "Q"
+            if m > k0:
+                return seg0 + (m - k0 - 1), "Q"
+            if m == k0:
+                return 2 * (k0 // k), "Q"
+            return 2 * (k0 // k) + 1, "Q"   # m = k0 - 1: the checkpoint's QP copy
+
+        starts = [0] if k == 1 else list(range(0, N, k))
+        for k0 in reversed(starts):
+            k1 = N if k == 1 else min(k0 + k, N)
+            if k > 1:
+                s.snapshot_load(2 * (k0 // k), "Q")
+                s.snapshot_load(2 * (k0 // k) + 1, "QP")
+                s.set_time(k0 * dt, 1 if k0 == 0 else 0)
+                for n in range(k0 + 1, k1 + 1):
+                    s.step(n, 0)
+                    s.snapshot_save(seg0 + (n - k0 - 1), "Q")
+                    recomputed += 1
+            for n in range(k1, k0, -1):
+                for m, fld in ((n, "Q"), (max(n - 1, 0), "QP"), (max(n - 2, 0), "QPP")):
+                    if k > 1 and m < k0 - 1:
+                        m = k0 - 1
+                    sl, _ = slot_of(m, k0)
+                    s.snapshot_load(sl, fld)
+                s.set_time(n * dt, 1 if n == 1 else 0)
+                wt = a if n > N - window else np.zeros(2)
+                s.adjoint_objective(0, self.i0, self.i1, float(wt[0]), float(wt[1]))
+                r0, r, it = s.adjoint_unsteady_step(max_iter, rtol, check_every)
+                its += it
+                worst = max(worst, r / r0 if r0 > 0 else 0.0)
+        s.synchronize()
+        secs = time.perf_counter() - t0
+        ni, nj = self.ac.ni, self.ac.nj
+        xb = s.adjoint_download(ADJ_XBAR_SUM, 0)[1:nj + 2, 1:ni + 2]
+        dJdX = np.stack([xb[..., 0].T, xb[..., 1].T])
+        dJ_dp = np.tensordot(self.ac.phi, dJdX, axes=([1, 2, 3], [0, 1, 2]))
+        # inflow angle: far-field state over all steps, the rotation of the
+        # force directions, and the initial (free-stream) state q^0 through
+        # U^1 = ... + Vol q^0/dt and U^2 = ... - Vol q^0/(2 dt)
+        al, V = self.ac.alpha, self.ac.v_inf
+        pbar = s.adjoint_unsteady_poolbar()
+        dpool = pbar[1] * (-V * np.sin(al)) + pbar[2] * (V * np.cos(al))
+        da_l = np.array([np.cos(al), np.sin(al)]) / self.q_inf
+        da_d = np.array([np.sin(al), -np.cos(al)]) / self.q_inf
+        drot = sum(float(F @ (w_l * da_l + w_d * da_d)) for F in self.forces.values()) / window
+        lam1 = s.adjoint_download(ADJ_LAMBDA, 0)[2:nj + 2, 2:ni + 2]
+        lam2 = s.adjoint_download(ADJ_LAMBDA_LATER, 0)[2:nj + 2, 2:ni + 2]
+        vol = s.download("VOL", 0)[2:nj + 2, 2:ni + 2, 0]
+        q0bar = vol[..., None] * (lam1 - 0.5 * lam2) / dt
+        rho = self.ac.rho_inf
+        dq0 = np.array([0.0, -rho * V * np.sin(al), rho * V * np.cos(al), 0.0])
+        dic = float(np.sum(q0bar * dq0))
+        return UnsteadyAdjoint(N, its, worst, secs, recomputed, dJ_dp, float(dpool + drot + dic))

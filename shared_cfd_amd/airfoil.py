@@ -98,12 +98,13 @@ def _wall_line(ni, nwake, wake_len, t):
 
 def naca_cmesh(ni=512, nj=128, nwake=64, mach=0.8, alpha_deg=1.25, t=0.12, radius=15.0, wake_len=15.0,
                first_height=2e-3, nb=32, amplitudes=None, cfl=5.0, order=2, rho_inf=1.4, p_inf=1.0,
-               kappa=1.4) -> AirfoilCase:
+               kappa=1.4, params=None) -> AirfoilCase:
     """The inviscid NACA 4-digit (thickness ``t``) airfoil case.
 
     ``amplitudes`` (2*nb values, upper then lower surface bumps, in chords)
     deform the wall; the case starts from the free stream.  LU-SGS with local
-    time steps; ``order`` 1: first-order face states, 2: MUSCL with the
+    time steps unless ``params`` (a dict of shcfd_params fields) says
+    otherwise; ``order`` 1: first-order face states, 2: MUSCL with the
     reference limiter and kappa=-1, 3: kappa=1/3."""
     assert (ni - 2 * nwake) % 2 == 0
     x, y, sgn = _wall_line(ni, nwake, wake_len, t)
@@ -162,10 +163,11 @@ def naca_cmesh(ni=512, nj=128, nwake=64, mach=0.8, alpha_deg=1.25, t=0.12, radiu
     ]
     # wake: j-min cells i=1..nwake exchange with i=ni..ni-nwake+1
     cut = [[4, nwake, -1, 1, 1, 1, -1, 1, 1, ni, -1]]
-    params = C.make_params(scheme=C.SCHEME_LUSGS, time_step=1, high_order=0 if order == 1 else 1,
-                           limiter=C.LIMITER_REFERENCE,
-                           cfl=cfl, k_mus=-1.0 if order == 2 else 1.0 / 3.0, kappa=kappa, rgas=rgas, nu=0.0,
-                           res_rate_warning=1e300, res_rate_error=1e300)
+    kw = dict(scheme=C.SCHEME_LUSGS, time_step=1, high_order=0 if order == 1 else 1, limiter=C.LIMITER_REFERENCE,
+              cfl=cfl, k_mus=-1.0 if order == 2 else 1.0 / 3.0, kappa=kappa, rgas=rgas, nu=0.0,
+              res_rate_warning=1e300, res_rate_error=1e300)
+    kw.update(params or {})
+    params = C.make_params(**kw)
     case = C.Case("naca_cmesh", [ni], [nj], 1, params, [Xn[0].copy()], [Xn[1].copy()], q,
                   np.array(rows, dtype=np.int32), pool, np.array(cut, dtype=np.int32),
                   description=f"NACA 4-digit t={t:.2f} C-mesh {ni}x{nj}, M={mach}, alpha={alpha_deg} deg, inviscid")

@@ -304,13 +304,90 @@ __global__ void __launch_bounds__(128) k_adj_cell(DevCtx d, AdjView a, int b) {
 #pragma unroll
     for (int k = 0; k < 4; k++) q[k] = T::var(FLD(d, SHCFD_F_Q, k, c), k);
     ideal_w<T>(q, d.p.kappa, d.p.rgas, B.nu, w);
+    // unsteady: the BDF2 term -Vol a0/dt q^n of U^n and the coupling to the
+    // later steps' residuals
+    const double sh = a.unsteady ? FLD(d, SHCFD_F_VOL, 0, c) * (a.unsteady == 2 ? 1.0 : 1.5) / d.scal[SCAL_DT_GRBL]
+                                 : 0.0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         double s = 0.0;
 #pragma unroll
         for (int m = 0; m < D1::dw; m++) s += a.wbar[m * d.NC + c] * w[m].d[k];
+        if (a.unsteady) s = (s + a.src[k * d.NC + c]) - sh * a.lam[k * d.NC + c];
         a.qbar[k * d.NC + c] = s;
     }
+}
+
+// start of an unsteady adjoint step: src = Vol/dt (2 lambda^(n+1) -
+// lambda^(n+2)/2) from lam (= lambda^(n+1)) and lam2 (= lambda^(n+2)), then
+// lam2 = lambda^(n+1); lam stays as the initial guess of lambda^n
+__global__ void __launch_bounds__(128) k_adj_unsteady_begin(DevCtx d, AdjView a, int b) {
+    const DevBlock B = d.blk[b];
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1, j = (int)blockIdx.y + 1;
+    if (i > B.ni || j > B.nj) return;
+    const long long c = cidx(B, i, j);
+    const double f = FLD(d, SHCFD_F_VOL, 0, c) / d.scal[SCAL_DT_GRBL];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double l1 = a.lam[k * d.NC + c], l2 = a.lam2[k * d.NC + c];
+        a.src[k * d.NC + c] = f * (2.0 * l1 - 0.5 * l2);
+        a.lam2[k * d.NC + c] = l1;
+    }
+}
+
+// d Vol / d(node) of cell (ci, cj) for its corner (ci-1+oi, cj-1+oj): Vol is
+// the signed area of triangles (A, B, D) + (B, C, D) with A = (ci-1, cj-1),
+// B = (ci, cj-1), C = (ci, cj), D = (ci-1, cj) as k_geom_a computes it
+__device__ inline void vol_node_grad(const DevCtx& d, const DevBlock& B, int ci, int cj, int oi, int oj, double& gx,
+                                     double& gy) {
+    auto X = [&](int ii, int jj) { return FLD(d, SHCFD_F_XH, 0, cidx(B, ii, jj)); };
+    auto Y = [&](int ii, int jj) { return FLD(d, SHCFD_F_RH, 0, cidx(B, ii, jj)); };
+    const double ax = X(ci - 1, cj - 1), ay = Y(ci - 1, cj - 1), bx = X(ci, cj - 1), by = Y(ci, cj - 1);
+    const double cx = X(ci, cj), cy = Y(ci, cj), dx = X(ci - 1, cj), dy = Y(ci - 1, cj);
+    // T(a, b, c) = (a1 (b2 - c2) + b1 (c2 - a2) + c1 (a2 - b2)) / 2
+    if (oi == 0 && oj == 0) {         // A: first vertex of T(A, B, D)
+        gx = 0.5 * (by - dy); gy = 0.5 * (dx - bx);
+    } else if (oi == 1 && oj == 0) {  // B: second of T(A, B, D), first of T(B, C, D)
+        gx = 0.5 * (dy - ay) + 0.5 * (cy - dy); gy = 0.5 * (ax - dx) + 0.5 * (dx - cx);
+    } else if (oi == 1 && oj == 1) {  // C: second of T(B, C, D)
+        gx = 0.5 * (dy - by); gy = 0.5 * (bx - dx);
+    } else {                          // D: third of both
+        gx = 0.5 * (ay - by) + 0.5 * (by - cy); gy = 0.5 * (bx - ax) + 0.5 * (cx - bx);
+    }
+}
+
+// end of an unsteady adjoint step: node bars of this step (XBAR of the last
+// reverse pass plus the volume term -lambda.dq/dt dVol/dX of U^n) added to
+// the sums
+__global__ void __launch_bounds__(128) k_adj_unsteady_accum(DevCtx d, AdjView a, int b) {
+    const DevBlock B = d.blk[b];
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)blockIdx.y;
+    if (i > B.ni || j > B.nj) return;
+    const long long n = cidx(B, i, j);
+    double sx = a.xbar[n], sy = a.ybar[n];
+    // the four cells around node (i, j): cell (ci, cj) sees it as corner (i-ci+1, j-cj+1)
+#pragma unroll
+    for (int oj = 0; oj < 2; oj++)
+#pragma unroll
+        for (int oi = 0; oi < 2; oi++) {
+            const int ci = i + 1 - oi, cj = j + 1 - oj;
+            if (ci < 1 || ci > B.ni || cj < 1 || cj > B.nj) continue;
+            const long long c = cidx(B, ci, cj);
+            double vb = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) vb -= a.lam[k * d.NC + c] * FLD(d, SHCFD_F_DQDT, k, c);
+            double gx, gy;
+            vol_node_grad(d, B, ci, cj, oi, oj, gx, gy);
+            sx += vb * gx;
+            sy += vb * gy;
+        }
+    a.xacc[n] += sx;
+    a.yacc[n] += sy;
+}
+
+__global__ void k_adj_pool_acc(const double* poolbar, double* poolacc, int n) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k < n) poolacc[k] += poolbar[k];
 }
 
 // ---------------------------------------------------------------------------

@@ -640,7 +640,7 @@ __global__ void __launch_bounds__(256) k_fast_post(DevCtx d, const QStrip* strip
 // the same two sweeps with operators stored at the neighbour's slot:
 //     GA'(c) = -HA(c + e_i)^T,  GB'(c) = -HB(c + e_j)^T,
 //     HA'(c) = -GA(c - e_i)^T,  HB'(c) = -GB(c - e_j)^T,
-// zero where the neighbour is outside the block.  LU-SGS (S = I) only.
+// zero where the neighbour is outside the block.  LU-SGS and NR (S = I) only.
 // ---------------------------------------------------------------------------
 // one transposed operator of neighbour cn: -(fs * M)^T in lane rows
 __device__ inline void put_transposed(double* o, const QStrip& S, int dd, int r, int pbase, const double* M, double fs,
@@ -846,7 +846,8 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
 }
 
 int FastPath::adjoint_prep(Ctx& c) {
-    if (c.p.scheme != SHCFD_SCHEME_LUSGS) return SHCFD_E_STATE;
+    const int sch = c.p.scheme;
+    if (sch != SHCFD_SCHEME_LUSGS && sch != SHCFD_SCHEME_NR && sch != SHCFD_SCHEME_DUAL) return SHCFD_E_STATE;
     if (!aG) {
         auto al = [&](void** p, size_t bytes) {
             if (hipMalloc(p, bytes) != hipSuccess) return false;
@@ -858,7 +859,13 @@ int FastPath::adjoint_prep(Ctx& c) {
                   al((void**)&ahand, sizeof(double) * 2 * hstride * (nstrips + 1));
         if (!ok) return SHCFD_E_NOMEM;
     }
-    hipLaunchKernelGGL(k_fast_adj_prep, dim3(nwork), dim3(256), 0, c.stream, c.dev(), strips, work, aG, aH);
+    // The transposed iteration is preconditioned with the LU-SGS operator of
+    // the residual it solves: for the BDF2 steps of dual time that is the NR
+    // operator (plain Jacobians, diagonal 3 Vol/(2 dt) + ...), whatever
+    // pseudo-time preconditioning the primal inner loop used.
+    DevCtx dp = c.dev();
+    if (dp.p.scheme == SHCFD_SCHEME_DUAL) dp.p.scheme = SHCFD_SCHEME_NR;
+    hipLaunchKernelGGL(k_fast_adj_prep, dim3(nwork), dim3(256), 0, c.stream, dp, strips, work, aG, aH);
     return hipGetLastError() == hipSuccess ? SHCFD_OK : SHCFD_E_HIP;
 }
 

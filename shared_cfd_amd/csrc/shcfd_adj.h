@@ -22,12 +22,19 @@ struct AdjView {
     double* ybar;
     int obj_block, obj_i0, obj_i1;  // wall faces j = 0, i in [i0, i1] of a block
     double obj_ax, obj_ay;          // objective weights of the momentum fluxes
+    // unsteady (BDF2 dual time): 0 steady, 1 a BDF2 step, 2 the first (BDF1) step
+    int unsteady;
+    double* src;       // Vol/dt (2 lambda^(n+1) - lambda^(n+2)/2) (4 per interior cell)
+    double* lam2;      // lambda of the later step (4 per interior cell)
+    double* xacc;      // node bars summed over the steps
+    double* yacc;
 };
 
 struct Adjoint {
     bool ready = false;
     double* mem = nullptr;     // one allocation for all arrays of AdjView
     double* poolbar = nullptr; // bars of the fixed boundary states (npool)
+    double* poolacc = nullptr; // their sum over unsteady steps (npool)
     double* part = nullptr;    // reduction partials
     double* objval = nullptr;  // objective value
     int npart = 0;

@@ -1,7 +1,8 @@
-// shcfd_adjoint.hip -- C-ABI of the discrete adjoint (steady, inviscid, ideal
-// gas, LU-SGS on the optimised implicit path).  The reference has no adjoint;
-// these entry points drive the reverse kernels of k_adjoint.hip and the
-// transposed sweeps of k_fast.hip.
+// shcfd_adjoint.hip -- C-ABI of the discrete adjoint (inviscid, ideal gas, on
+// the optimised implicit path: steady LU-SGS, and the BDF2 dual-time steps of
+// the NR scheme with a fixed time step).  The reference has no adjoint; these
+// entry points drive the reverse kernels of k_adjoint.hip and the transposed
+// sweeps of k_fast.hip.
 #include <cmath>
 #include <new>
 #include <vector>
@@ -25,15 +26,23 @@ __global__ void k_adj_cell(DevCtx, AdjView, int);
 __global__ void k_adj_nodes(DevCtx, AdjView, int);
 __global__ void k_adj_objective(DevCtx, AdjView, double*);
 __global__ void k_adj_norm_part(DevCtx, AdjView, int, double*);
+__global__ void k_adj_unsteady_begin(DevCtx, AdjView, int);
+__global__ void k_adj_unsteady_accum(DevCtx, AdjView, int);
+__global__ void k_adj_pool_acc(const double*, double*, int);
 // shcfd_api.hip
 bool fast_ready(Ctx* ctx);
 
+__global__ void k_adj_set_dt(DevCtx d, double dt) { d.scal[SCAL_DT_GRBL] = dt; }
+
 int Adjoint::init(Ctx& c) {
     const long long NC = c.NC;
-    const long long n = NC * (4 + 4 + 9 + 4 * 9 + 2 + 2 + 1 + 1 + 1 + 1);
+    const long long n = NC * (4 + 4 + 9 + 4 * 9 + 2 + 2 + 1 + 1 + 1 + 1 + 4 + 4 + 1 + 1);
+    const size_t npb = sizeof(double) * (c.npool > 0 ? c.npool : 1);
     npart = 256;
     if (hipMalloc((void**)&mem, sizeof(double) * n) != hipSuccess) return SHCFD_E_NOMEM;
-    if (hipMalloc((void**)&poolbar, sizeof(double) * (c.npool > 0 ? c.npool : 1)) != hipSuccess) return SHCFD_E_NOMEM;
+    if (hipMalloc((void**)&poolbar, npb) != hipSuccess) return SHCFD_E_NOMEM;
+    if (hipMalloc((void**)&poolacc, npb) != hipSuccess) return SHCFD_E_NOMEM;
+    if (hipMemset(poolacc, 0, npb) != hipSuccess) return SHCFD_E_HIP;
     if (hipMalloc((void**)&part, sizeof(double) * (npart + 1)) != hipSuccess) return SHCFD_E_NOMEM;
     objval = part + npart;
     if (hipMemset(mem, 0, sizeof(double) * n) != hipSuccess) return SHCFD_E_HIP;
@@ -50,13 +59,18 @@ int Adjoint::init(Ctx& c) {
     v.dsbar_j = take(1);
     v.xbar = take(1);
     v.ybar = take(1);
+    v.src = take(4);
+    v.lam2 = take(4);
+    v.xacc = take(1);
+    v.yacc = take(1);
+    v.unsteady = 0;
     v.obj_block = -1;
     ready = true;
     return SHCFD_OK;
 }
 
 void Adjoint::release() {
-    void* ptrs[] = {mem, poolbar, part};
+    void* ptrs[] = {mem, poolbar, poolacc, part};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     *this = Adjoint();
@@ -77,7 +91,9 @@ using namespace shcfd;
 namespace {
 
 int supported(shcfd_ctx* ctx) {
-    if (ctx->ny != 1 || ctx->p.viscous || ctx->p.thermal != SHCFD_THERMAL_IDEAL || ctx->p.scheme != SHCFD_SCHEME_LUSGS)
+    if (ctx->ny != 1 || ctx->p.viscous || ctx->p.thermal != SHCFD_THERMAL_IDEAL ||
+        (ctx->p.scheme != SHCFD_SCHEME_LUSGS && ctx->p.scheme != SHCFD_SCHEME_NR &&
+         ctx->p.scheme != SHCFD_SCHEME_DUAL))
         return SHCFD_E_STATE;
     return SHCFD_OK;
 }
@@ -160,6 +176,33 @@ int qbar_rms(shcfd_ctx* ctx, double* out) {
     return std::isfinite(*out) ? SHCFD_OK : SHCFD_E_NAN;
 }
 
+// lambda += M^-T (dR/dq^T lambda + g) from the current lambda until the RMS
+// of the bracket falls below rtol times its first value (the operators of M
+// are those of the last adjoint_prep); QBAR, XBAR and the pool bars belong
+// to the final lambda on return
+int transposed_iterations(shcfd_ctx* ctx, int max_iter, double rtol, int check_every, double* res0, double* res,
+                          int* iters) {
+    AdjView& a = ctx->adj->v;
+    double r0 = 0.0, r = 0.0;
+    int it = 0;
+    vjp_async(ctx);
+    int rc = qbar_rms(ctx, &r0);
+    r = r0;
+    while (!rc && it < max_iter && r > rtol * r0) {
+        rc = ctx->fast->adjoint_apply(*ctx, a.qbar, a.lam);
+        if (rc) break;
+        vjp_async(ctx);
+        it++;
+        if (it % check_every == 0 || it == max_iter) rc = qbar_rms(ctx, &r);
+    }
+    if (!rc) rc = qbar_rms(ctx, &r);
+    if (res0) *res0 = r0;
+    if (res) *res = r;
+    if (iters) *iters = it;
+    if (!rc) rc = shcfd_check_error(ctx);
+    return rc;
+}
+
 // host <-> device copy of K cell-indexed components of one block in the
 // reference layout (k fastest, then i = -1..ni+2, then j = -1..nj+2)
 int copy_block(shcfd_ctx* ctx, double* const* dev, int K, int block, double* host, bool to_device) {
@@ -240,6 +283,13 @@ int shcfd_adjoint_download(shcfd_ctx* ctx, int what, int block, double* host) {
         dev[0] = a.xbar;
         dev[1] = a.ybar;
         return copy_block(ctx, dev, 2, block, host, false);
+    case SHCFD_ADJ_XBAR_SUM:
+        dev[0] = a.xacc;
+        dev[1] = a.yacc;
+        return copy_block(ctx, dev, 2, block, host, false);
+    case SHCFD_ADJ_LAMBDA_LATER:
+        for (int k = 0; k < 4; k++) dev[k] = a.lam2 + (long long)k * ctx->NC;
+        return copy_block(ctx, dev, 4, block, host, false);
     default:
         return SHCFD_E_ARG;
     }
@@ -261,6 +311,7 @@ int shcfd_residual_vjp(shcfd_ctx* ctx) {
     if (!ctx) return SHCFD_E_ARG;
     int rc = ensure(ctx);
     if (!rc && ctx->adj->v.obj_block < 0) rc = SHCFD_E_STATE;
+    if (!rc) ctx->adj->v.unsteady = 0;
     if (!rc) rc = residual_state(ctx);
     if (rc) return rc;
     vjp_async(ctx);
@@ -272,31 +323,78 @@ int shcfd_adjoint_solve(shcfd_ctx* ctx, int max_iter, double rtol, int check_eve
                         int* iters) {
     if (!ctx || max_iter < 0 || check_every < 1) return SHCFD_E_ARG;
     int rc = ensure(ctx);
-    if (!rc && ctx->adj->v.obj_block < 0) rc = SHCFD_E_STATE;
+    if (!rc && (ctx->adj->v.obj_block < 0 || ctx->p.scheme != SHCFD_SCHEME_LUSGS)) rc = SHCFD_E_STATE;
     if (!rc && !fast_ready(ctx)) rc = SHCFD_E_STATE;
-    if (!rc) rc = residual_state(ctx);
+    if (rc) return rc;
+    ctx->adj->v.unsteady = 0;
+    rc = residual_state(ctx);
     if (!rc) rc = shcfd_set_dt(ctx, 0, nullptr);
     if (!rc) rc = ctx->fast->adjoint_prep(*ctx);
-    if (rc) return rc;
-    AdjView& a = ctx->adj->v;
-    double r0 = 0.0, r = 0.0;
-    int it = 0;
-    vjp_async(ctx);
-    rc = qbar_rms(ctx, &r0);
-    r = r0;
-    while (!rc && it < max_iter && r > rtol * r0) {
-        rc = ctx->fast->adjoint_apply(*ctx, a.qbar, a.lam);
-        if (rc) break;
-        vjp_async(ctx);
-        it++;
-        if (it % check_every == 0 || it == max_iter) rc = qbar_rms(ctx, &r);
-    }
-    if (!rc) rc = qbar_rms(ctx, &r);
-    if (res0) *res0 = r0;
-    if (res) *res = r;
-    if (iters) *iters = it;
-    if (!rc) rc = shcfd_check_error(ctx);
+    if (!rc) rc = transposed_iterations(ctx, max_iter, rtol, check_every, res0, res, iters);
     return rc;
+}
+
+int shcfd_adjoint_unsteady_reset(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    int rc = ensure(ctx);
+    if (rc) return rc;
+    const AdjView& a = ctx->adj->v;
+    const size_t cell = sizeof(double) * (size_t)ctx->NC;
+    hipStream_t st = ctx->stream;
+    if (hipMemsetAsync(a.lam, 0, 4 * cell, st) != hipSuccess || hipMemsetAsync(a.lam2, 0, 4 * cell, st) != hipSuccess ||
+        hipMemsetAsync(a.xacc, 0, cell, st) != hipSuccess || hipMemsetAsync(a.yacc, 0, cell, st) != hipSuccess ||
+        hipMemsetAsync(ctx->adj->poolacc, 0, sizeof(double) * (ctx->npool > 0 ? ctx->npool : 1), st) != hipSuccess)
+        return SHCFD_E_HIP;
+    return SHCFD_OK;
+}
+
+int shcfd_adjoint_unsteady_step(shcfd_ctx* ctx, int max_iter, double rtol, int check_every, double* res0,
+                                double* res, int* iters) {
+    if (!ctx || max_iter < 0 || check_every < 1) return SHCFD_E_ARG;
+    int rc = ensure(ctx);
+    if (!rc && (ctx->adj->v.obj_block < 0 || ctx->p.scheme == SHCFD_SCHEME_LUSGS || ctx->p.time_step != 0 ||
+                !(ctx->dt_fixed > 0.0)))
+        rc = SHCFD_E_STATE;
+    if (!rc && !fast_ready(ctx)) rc = SHCFD_E_STATE;
+    if (rc) return rc;
+    DevCtx d = ctx->dev();
+    AdjView& a = ctx->adj->v;
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(k_adj_set_dt, dim3(1), dim3(1), 0, st, d, ctx->dt_fixed);
+    rc = residual_state(ctx);
+    if (!rc) rc = shcfd_set_dqdt(ctx);
+    if (rc) return rc;
+    a.unsteady = ctx->first_dual ? 2 : 1;
+    for (int b = 0; b < ctx->nb; b++) {
+        const DevBlock& B = ctx->hblk[b];
+        hipLaunchKernelGGL(k_adj_unsteady_begin, dim3((unsigned)((B.ni + 127) / 128), (unsigned)B.nj), dim3(128), 0, st,
+                           d, a, b);
+    }
+    rc = ctx->fast->adjoint_prep(*ctx);
+    if (!rc) rc = transposed_iterations(ctx, max_iter, rtol, check_every, res0, res, iters);
+    if (rc) return rc;
+    for (int b = 0; b < ctx->nb; b++) {
+        const DevBlock& B = ctx->hblk[b];
+        hipLaunchKernelGGL(k_adj_unsteady_accum, dim3((unsigned)((B.ni + 1 + 127) / 128), (unsigned)(B.nj + 1)),
+                           dim3(128), 0, st, d, a, b);
+    }
+    if (ctx->npool > 0)
+        hipLaunchKernelGGL(k_adj_pool_acc, dim3((unsigned)((ctx->npool + 255) / 256)), dim3(256), 0, st,
+                           ctx->adj->poolbar, ctx->adj->poolacc, ctx->npool);
+    if (hipGetLastError() != hipSuccess) return SHCFD_E_HIP;
+    return shcfd_check_error(ctx);
+}
+
+int shcfd_adjoint_unsteady_poolbar(shcfd_ctx* ctx, double* host) {
+    if (!ctx || !host) return SHCFD_E_ARG;
+    int rc = ensure(ctx);
+    if (rc) return rc;
+    if (ctx->npool <= 0) return SHCFD_OK;
+    if (hipMemcpyAsync(host, ctx->adj->poolacc, sizeof(double) * ctx->npool, hipMemcpyDeviceToHost, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return SHCFD_E_HIP;
+    return SHCFD_OK;
 }
 
 }  // extern "C"

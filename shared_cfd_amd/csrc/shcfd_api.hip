@@ -209,6 +209,7 @@ int shcfd_destroy(shcfd_ctx* ctx) {
     drop_graph(ctx);
     if (ctx->fast) { ctx->fast->release(); delete ctx->fast; ctx->fast = nullptr; }
     adjoint_release(ctx);
+    for (double* sp : ctx->snaps) if (sp) hipFree(sp);
     for (auto& e : ctx->prof_free) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
     for (int k = 0; k < PROF_COUNT; k++)
         for (auto& e : ctx->prof_pending[k]) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
@@ -579,15 +580,24 @@ int shcfd_set_tgv(shcfd_ctx* ctx) {
     return SHCFD_OK;
 }
 
+__global__ void k_fixed_dt(DevCtx d, double dt) {
+    d.scal[SCAL_DT_GRBL] = dt;
+    d.scal[SCAL_TT] = d.scal[SCAL_TT] + dt;
+}
+
 int shcfd_set_dt(shcfd_ctx* ctx, int step, double* dt_grbl_out) {
     (void)step;
     if (!ctx) return SHCFD_E_ARG;
     {
     ProfScope ps(ctx, PROF_DT);
-    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_dt<NY>, ctx->dpart));
-    if (ctx->p.time_step == 0)
-        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, ctx->stream, ctx->dev(), ctx->dpart, ntiles(ctx), 0,
-                           (int)SCAL_DT_GRBL);
+    if (ctx->p.time_step == 0 && ctx->dt_fixed > 0.0) {
+        hipLaunchKernelGGL(k_fixed_dt, dim3(1), dim3(1), 0, ctx->stream, ctx->dev(), ctx->dt_fixed);
+    } else {
+        DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_dt<NY>, ctx->dpart));
+        if (ctx->p.time_step == 0)
+            hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, ctx->stream, ctx->dev(), ctx->dpart, ntiles(ctx),
+                               0, (int)SCAL_DT_GRBL);
+    }
     }
     LAUNCHCHK();
     if (dt_grbl_out) {
@@ -596,6 +606,46 @@ int shcfd_set_dt(shcfd_ctx* ctx, int step, double* dt_grbl_out) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
         *dt_grbl_out = s;
     }
+    return SHCFD_OK;
+}
+
+int shcfd_set_fixed_dt(shcfd_ctx* ctx, double dt) {
+    if (!ctx || !(dt >= 0.0) || (dt > 0.0 && ctx->p.time_step != 0)) return SHCFD_E_ARG;
+    ctx->dt_fixed = dt;
+    drop_graph(ctx);
+    return SHCFD_OK;
+}
+
+// device snapshots of the q-sized state fields
+static int snap_field(int field) {
+    return field == SHCFD_F_Q || field == SHCFD_F_QP || field == SHCFD_F_QPP;
+}
+
+int shcfd_snapshot_save(shcfd_ctx* ctx, int slot, int field) {
+    if (!ctx || slot < 0 || slot >= SHCFD_MAX_SNAPSHOTS || !snap_field(field)) return SHCFD_E_ARG;
+    const size_t bytes = sizeof(double) * (size_t)ctx->dimq * (size_t)ctx->NC;
+    if ((int)ctx->snaps.size() <= slot) ctx->snaps.resize(slot + 1, nullptr);
+    if (!ctx->snaps[slot] && hipMalloc((void**)&ctx->snaps[slot], bytes) != hipSuccess) {
+        ctx->snaps[slot] = nullptr;
+        return SHCFD_E_NOMEM;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->snaps[slot], ctx->f[field], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return SHCFD_OK;
+}
+
+int shcfd_snapshot_load(shcfd_ctx* ctx, int slot, int field) {
+    if (!ctx || slot < 0 || slot >= (int)ctx->snaps.size() || !ctx->snaps[slot] || !snap_field(field))
+        return SHCFD_E_ARG;
+    const size_t bytes = sizeof(double) * (size_t)ctx->dimq * (size_t)ctx->NC;
+    HIPCHK(hipMemcpyAsync(ctx->f[field], ctx->snaps[slot], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return SHCFD_OK;
+}
+
+int shcfd_snapshot_free(shcfd_ctx* ctx) {
+    if (!ctx) return SHCFD_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (double* sp : ctx->snaps) if (sp) hipFree(sp);
+    ctx->snaps.clear();
     return SHCFD_OK;
 }
 

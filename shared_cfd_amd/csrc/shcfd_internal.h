@@ -146,6 +146,8 @@ struct Ctx {
     double dt_grbl = 0.0, tt = 0.0;
     int first_dual = 1;
     double res1 = 0.0;
+    double dt_fixed = 0.0;         // > 0: global time step held at this value
+    std::vector<double*> snaps;    // device snapshots of q-sized fields (dimq*NC doubles each)
     hipStream_t stream = nullptr;
     std::string last_error;
     int sweep_impl = 0;            // 0 = plain (stored Jacobians)

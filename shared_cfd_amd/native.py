@@ -33,8 +33,10 @@ EXPORTS = [
     "shcfd_set_block_nu", "shcfd_set_block_active", "shcfd_energy_rms_blocks", "shcfd_iterate", "shcfd_autotune",
     "shcfd_adjoint_objective", "shcfd_objective", "shcfd_adjoint_upload", "shcfd_adjoint_download",
     "shcfd_adjoint_poolbar", "shcfd_residual_vjp", "shcfd_adjoint_solve",
+    "shcfd_set_fixed_dt", "shcfd_snapshot_save", "shcfd_snapshot_load", "shcfd_snapshot_free",
+    "shcfd_adjoint_unsteady_reset", "shcfd_adjoint_unsteady_step", "shcfd_adjoint_unsteady_poolbar",
 ]
-ADJ_LAMBDA, ADJ_QBAR, ADJ_XBAR = 0, 1, 2
+ADJ_LAMBDA, ADJ_QBAR, ADJ_XBAR, ADJ_XBAR_SUM, ADJ_LAMBDA_LATER = 0, 1, 2, 3, 4
 OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT, OPT_SKEW_SWEEP, OPT_TILE_ROWS = 1, 2, 3, 4
 
 # kernel classes of shcfd_profile_read (include/shcfd.h)
@@ -92,6 +94,14 @@ def lib():
         L.shcfd_residual_vjp.argtypes = [vp]
         L.shcfd_adjoint_solve.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp,
                                           ctypes.POINTER(ctypes.c_int)]
+        L.shcfd_set_fixed_dt.argtypes = [vp, ctypes.c_double]
+        L.shcfd_snapshot_save.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.shcfd_snapshot_load.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.shcfd_snapshot_free.argtypes = [vp]
+        L.shcfd_adjoint_unsteady_reset.argtypes = [vp]
+        L.shcfd_adjoint_unsteady_step.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp,
+                                                  ctypes.POINTER(ctypes.c_int)]
+        L.shcfd_adjoint_unsteady_poolbar.argtypes = [vp, dp]
         L.shcfd_set_block_nu.argtypes = [vp, dp]
         L.shcfd_set_block_active.argtypes = [vp, ip]
         L.shcfd_energy_rms_blocks.argtypes = [vp, dp]
@@ -300,7 +310,7 @@ class Solver:
         check(self.L.shcfd_adjoint_upload(self.ctx, ADJ_LAMBDA, b, _dp(a)), "adjoint_upload")
 
     def adjoint_download(self, what, b):
-        k = 2 if what == ADJ_XBAR else self.case.dimq
+        k = 2 if what in (ADJ_XBAR, ADJ_XBAR_SUM) else self.case.dimq
         out = np.zeros((self.case.nj[b] + 4, self.case.ni[b] + 4, k), dtype=np.float64)
         check(self.L.shcfd_adjoint_download(self.ctx, what, b, _dp(out)), "adjoint_download")
         return out
@@ -318,6 +328,33 @@ class Solver:
         check(self.L.shcfd_adjoint_solve(self.ctx, max_iter, rtol, check_every, ctypes.byref(r0), ctypes.byref(r),
                                          ctypes.byref(it)), "adjoint_solve")
         return r0.value, r.value, it.value
+
+    # ---- fixed time step, snapshots, unsteady adjoint ----
+    def set_fixed_dt(self, dt):
+        check(self.L.shcfd_set_fixed_dt(self.ctx, float(dt)), "set_fixed_dt")
+
+    def snapshot_save(self, slot, field="Q"):
+        check(self.L.shcfd_snapshot_save(self.ctx, slot, C.F[field]), "snapshot_save")
+
+    def snapshot_load(self, slot, field="Q"):
+        check(self.L.shcfd_snapshot_load(self.ctx, slot, C.F[field]), "snapshot_load")
+
+    def snapshot_free(self):
+        check(self.L.shcfd_snapshot_free(self.ctx), "snapshot_free")
+
+    def adjoint_unsteady_reset(self):
+        check(self.L.shcfd_adjoint_unsteady_reset(self.ctx), "adjoint_unsteady_reset")
+
+    def adjoint_unsteady_step(self, max_iter=20000, rtol=1e-10, check_every=20):
+        r0, r, it = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_int(0)
+        check(self.L.shcfd_adjoint_unsteady_step(self.ctx, max_iter, rtol, check_every, ctypes.byref(r0),
+                                                 ctypes.byref(r), ctypes.byref(it)), "adjoint_unsteady_step")
+        return r0.value, r.value, it.value
+
+    def adjoint_unsteady_poolbar(self):
+        out = np.zeros(max(1, self.case.bc_pool.size), dtype=np.float64)
+        check(self.L.shcfd_adjoint_unsteady_poolbar(self.ctx, _dp(out)), "adjoint_unsteady_poolbar")
+        return out[: self.case.bc_pool.size]
 
     # ---- per-kernel timing ----
     def profile(self, classes=None):
