@@ -45,9 +45,8 @@ __global__ void __launch_bounds__(256) k_thermo_chem(DevCtx d, const Tile* tiles
 
 // one cell of proceed_reaction; returns the Newton iterations / CHEMEQ2
 // attempts it took
-template <int NS>
-__device__ inline int reaction_cell(const DevCtx& d, const DevBlock& B, const TileCoord& t, const shcfd_mech* m,
-                                    int mode) {
+template <int NS, int MODE>
+__device__ inline int reaction_cell(const DevCtx& d, const DevBlock& B, const TileCoord& t, const shcfd_mech* m) {
     using D = Dims<NS>;
     long long c = cidx(B, t.i, t.j);
     double dt = d.scal[SCAL_DT_GRBL];
@@ -57,7 +56,7 @@ __device__ inline int reaction_cell(const DevCtx& d, const DevBlock& B, const Ti
     for (int k = 0; k < NS; k++) vrho[k] = FLD(d, SHCFD_F_Q, k, c);
     int e, att = 0;
     double T1;
-    if (mode == SHCFD_REACTION_POINT_IMPLICIT)
+    if constexpr (MODE == SHCFD_REACTION_POINT_IMPLICIT)
         e = chem::point_implicit<NS>(m, T, vrho, dt, att, T1);
     else
         e = chem::reaction_chemeq2<NS>(m, T, vrho, dt, att);
@@ -70,17 +69,17 @@ __device__ inline int reaction_cell(const DevCtx& d, const DevBlock& B, const Ti
     return att;
 }
 
-// proceed_reaction, thermal_model.f90:45-73 (mode 1: reaction_chemeq2;
-// mode 2: point-implicit backward Euler), over dt_grbl.  The per-cell
-// iteration counts are summed into SCAL_CHEM_ITERS (one atomic per wave).
-template <int NS>
-__global__ void __launch_bounds__(256) k_reaction(DevCtx d, const Tile* tiles, const shcfd_mech* __restrict__ m,
-                                                  int mode) {
+// proceed_reaction, thermal_model.f90:45-73 (MODE 1: reaction_chemeq2;
+// MODE 2: point-implicit backward Euler), over dt_grbl.  One instantiation
+// per method, so each holds only its own registers.  The per-cell iteration
+// counts are summed into SCAL_CHEM_ITERS (one atomic per wave).
+template <int NS, int MODE>
+__global__ void __launch_bounds__(256) k_reaction(DevCtx d, const Tile* tiles, const shcfd_mech* __restrict__ m) {
     TileCoord t = tile_coord(tiles);
     const DevBlock B = d.blk[t.b];
     bool inside = !(t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj);
     int att = 0;
-    if (inside) att = reaction_cell<NS>(d, B, t, m, mode);
+    if (inside) att = reaction_cell<NS, MODE>(d, B, t, m);
     double w = (double)att;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
@@ -169,7 +168,8 @@ __global__ void __launch_bounds__(256) k_chem0d(const shcfd_mech* __restrict__ m
 
 #define SHCFD_CHEM_FLOW_INST(NS)                                                        \
     template __global__ void k_thermo_chem<NS>(DevCtx, const Tile*, const shcfd_mech*); \
-    template __global__ void k_reaction<NS>(DevCtx, const Tile*, const shcfd_mech*, int);
+    template __global__ void k_reaction<NS, SHCFD_REACTION_CHEMEQ2>(DevCtx, const Tile*, const shcfd_mech*);      \
+    template __global__ void k_reaction<NS, SHCFD_REACTION_POINT_IMPLICIT>(DevCtx, const Tile*, const shcfd_mech*);
 #define SHCFD_CHEM0D_OP(NS, OP) \
     template __global__ void k_chem0d<NS, OP>(const shcfd_mech*, int, const double*, int, double*, int, int*, double);
 #define SHCFD_CHEM0D_INST(NS)                                                                     \

@@ -46,36 +46,6 @@ __device__ inline long long slot1(long long base, int dd, int L) {
     return base + 64LL * (dd & ~1) + 2 * L + (dd & 1);
 }
 
-// quad broadcast: every lane of a 4-lane group gets lane k of the group
-template <int K>
-__device__ inline double quad_bcast(double x) {
-    constexpr int ctrl = K | (K << 2) | (K << 4) | (K << 6);
-    int lo = __double2loint(x), hi = __double2hiint(x);
-    // every lane has a source within its quad: bound_ctrl set, no old value
-    lo = __builtin_amdgcn_mov_dpp(lo, ctrl, 0xF, 0xF, true);
-    hi = __builtin_amdgcn_mov_dpp(hi, ctrl, 0xF, 0xF, true);
-    return __hiloint2double(hi, lo);
-}
-// lane L receives lane L-4 (0 for L < 4) / lane L+4 (0 for L > 59)
-__device__ inline double shift_up4(double x) {
-    int lo = __double2loint(x), hi = __double2hiint(x);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xF, 0xF, true);
-        hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xF, 0xF, true);
-    }
-    return __hiloint2double(hi, lo);
-}
-__device__ inline double shift_down4(double x) {
-    int lo = __double2loint(x), hi = __double2hiint(x);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xF, 0xF, true);
-        hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xF, 0xF, true);
-    }
-    return __hiloint2double(hi, lo);
-}
-
 // 4x4 helpers (column-major)
 __device__ inline void mm4(const double* A, const double* B, double* C) {
 #pragma unroll

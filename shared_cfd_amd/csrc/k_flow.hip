@@ -396,7 +396,13 @@ __global__ void __launch_bounds__(256) k_rms_part(DevCtx d, const Tile* tiles, d
 // set_ABpm (LU-SGS, preconLU-SGS, dual) and set_dqdt
 // ---------------------------------------------------------------------------
 // MATS: 0 = no stored Jacobians, 1 = Ap/Am/Bp/Bm fields (cell SoA),
-// 2 = the strip-skewed copy read by the skewed sweeps (slot given)
+// 2 = the strip-skewed copy read by the skewed sweeps (slot given),
+// 3 = the quad-row copy of the 16-row sweeps (slot of wave 0, lane 4r of
+// the step given): lane 4r+c of wave w owns row k = 4w + c; per sweep
+// (set 0: forward, 1: backward) it reads dq + 2 pairs of doubles, pair p at
+// ((set dq2 + p) nsl + slot + 64 w + c) 2, dq2 = dq + 2: elements 0..dq-1
+// are row k of A+ (A-), dq..2dq-1 row k of B+ (B-), then (alpha, dsci),
+// (dscj, 0); rows k >= dq are zero
 template <int NY, int MATS>
 __device__ inline void set_abpm_cell(const DevCtx& d, const DevBlock& B, int i, int j, double* skew, long long nsl,
                                      long long slot) {
@@ -419,6 +425,26 @@ __device__ inline void set_abpm_cell(const DevCtx& d, const DevBlock& B, int i, 
             skew[(long long)(SKEW_AM * dq * dq + e) * nsl + slot] = J.Am[e];
             skew[(long long)(SKEW_BM * dq * dq + e) * nsl + slot] = J.Bm[e];
         }
+    if (MATS == 3) {
+        constexpr int C = (dq + 3) / 4, dq2 = dq + 2;
+        const double dsci = FLD(d, SHCFD_F_DSCI, 0, c), dscj = FLD(d, SHCFD_F_DSCJ, 0, c);
+        for (int set = 0; set < 2; set++) {
+            const double* MA = set == 0 ? J.Ap : J.Am;
+            const double* MB = set == 0 ? J.Bp : J.Bm;
+            for (int e = 0; e < 2 * dq2; e++) {
+                double* o = skew + ((long long)(set * dq2 + e / 2) * nsl + slot) * 2 + (e & 1);
+                for (int w = 0; w < C; w++)
+                    for (int cc = 0; cc < 4; cc++) {
+                        const int k = 4 * w + cc;
+                        double v;
+                        if (e < dq) v = (k < dq) ? MA[k + dq * e] : 0.0;
+                        else if (e < 2 * dq) v = (k < dq) ? MB[k + dq * (e - dq)] : 0.0;
+                        else v = e == 2 * dq ? J.alpha : (e == 2 * dq + 1 ? dsci : (e == 2 * dq + 2 ? dscj : 0.0));
+                        o[2 * (64 * w + cc)] = v;
+                    }
+            }
+        }
+    }
     FLD(d, SHCFD_F_ALPHA, 0, c) = J.alpha;
     if (J.precond) {
         FLD(d, SHCFD_F_PHI, 0, c) = J.phi;
@@ -464,6 +490,21 @@ __global__ void __launch_bounds__(256) k_set_abpm_skew(DevCtx d, const Strip* st
     const int i = t - L + 1, j = S.j0 + L;
     if (i < 1 || i > B.ni || j > B.nj) return;
     set_abpm_cell<NY, 2>(d, B, i, j, skew, nsl, sbase[s] + 64LL * t + L);
+}
+
+// set_ABpm writing the quad-row copy of the 16-row sweeps: thread = row r of
+// step t of a strip (16 steps x 16 rows per workgroup); grid = (strips,
+// ceil((max ni + 15)/16))
+template <int NY>
+__global__ void __launch_bounds__(256) k_set_abpm16(DevCtx d, const Strip16* strips, double* jac, long long nsl) {
+    const Strip16 S = strips[blockIdx.x];
+    const int t = (int)blockIdx.y * 16 + (int)(threadIdx.x >> 4), r = (int)(threadIdx.x & 15);
+    const DevBlock B = d.blk[S.b];
+    if (t >= B.ni + 15) return;
+    const int i = t - r + 1, j = S.j0 + r;
+    if (i < 1 || i > B.ni || j > B.nj) return;
+    constexpr int C = (NY + 3 + 3) / 4;
+    set_abpm_cell<NY, 3>(d, B, i, j, jac, nsl, S.base + 64LL * C * t + 4 * r);
 }
 
 template <int NY>
@@ -534,6 +575,7 @@ __global__ void __launch_bounds__(256) k_copy_q(DevCtx d, const Tile* tiles, int
     template __global__ void k_set_abpm<NY>(DevCtx, const Tile*);                                        \
     template __global__ void k_set_abpm_scal<NY>(DevCtx, const Tile*);                                   \
     template __global__ void k_set_abpm_skew<NY>(DevCtx, const Strip*, const long long*, double*, long long); \
+    template __global__ void k_set_abpm16<NY>(DevCtx, const Strip16*, double*, long long);                 \
     template __global__ void k_set_dqdt<NY>(DevCtx, const Tile*);                                        \
     template __global__ void k_explicit<NY>(DevCtx, const Tile*, int);                                   \
     template __global__ void k_copy_q<NY>(DevCtx, const Tile*, int, int);

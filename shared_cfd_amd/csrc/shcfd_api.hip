@@ -215,7 +215,8 @@ int shcfd_destroy(shcfd_ctx* ctx) {
         for (auto& e : ctx->prof_pending[k]) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
     for (int fld = 0; fld < SHCFD_F_COUNT; fld++) if (ctx->f[fld]) hipFree(ctx->f[fld]);
     void* ptrs[] = {ctx->dblk, ctx->dtiles, ctx->dstrips, ctx->dprog, ctx->dbnd, ctx->dpart, ctx->scal,
-                    ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool, ctx->dmech, ctx->dskew, ctx->dsbase};
+                    ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool, ctx->dmech, ctx->dskew, ctx->dsbase,
+                    ctx->djac16, ctx->dstrips16, ctx->dbnd16};
     for (void* p : ptrs) if (p) hipFree(p);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -509,7 +510,10 @@ int shcfd_proceed_reaction(shcfd_ctx* ctx) {
         (ctx->p.reaction != SHCFD_REACTION_CHEMEQ2 && ctx->p.reaction != SHCFD_REACTION_POINT_IMPLICIT))
         return SHCFD_E_STATE;
     ProfScope ps(ctx, PROF_REACTION);
-    DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH(k_reaction<NS>, ctx->dmech, ctx->p.reaction));
+    if (ctx->p.reaction == SHCFD_REACTION_POINT_IMPLICIT)
+        DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH((k_reaction<NS, SHCFD_REACTION_POINT_IMPLICIT>), ctx->dmech))
+    else
+        DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH((k_reaction<NS, SHCFD_REACTION_CHEMEQ2>), ctx->dmech))
     LAUNCHCHK();
     return SHCFD_OK;
 }
@@ -655,6 +659,7 @@ int shcfd_set_abpm(shcfd_ctx* ctx) {
     DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_abpm<NY>));
     LAUNCHCHK();
     ctx->skew_active = 0;
+    ctx->q16_active = 0;
     return SHCFD_OK;
 }
 
@@ -686,11 +691,79 @@ static bool skew_enabled(shcfd_ctx* ctx) {
     return true;
 }
 
+// the quad-row copy and hand-off state of the 16-row sweeps (nY > 1, every
+// implicit scheme but dual time, whose S scaling stays on the row-per-lane
+// sweeps); allocated on first use
+static bool q16_enabled(shcfd_ctx* ctx) {
+    if (!ctx->use_skew || ctx->ny == 1 || ctx->p.scheme == SHCFD_SCHEME_DUAL) return false;
+    if (ctx->djac16) return true;
+    const int dq = ctx->dimq, C = (dq + 3) / 4;
+    std::vector<Strip16> st;
+    long long nsl = 0;
+    int maxni = 0;
+    for (int b = 0; b < ctx->nb; b++) {
+        const DevBlock& B = ctx->hblk[b];
+        maxni = std::max(maxni, B.ni);
+        const int first = (int)st.size();
+        for (int j0 = 1; j0 <= B.nj; j0 += 16) {
+            Strip16 S{};
+            S.b = b;
+            S.j0 = j0;
+            S.prev = (j0 == 1) ? -1 : (int)st.size() - 1;
+            S.next = -1;
+            S.base = nsl;
+            if (S.prev >= 0) st[S.prev].next = (int)st.size();
+            nsl += 64LL * C * (B.ni + 15);
+            st.push_back(S);
+        }
+        (void)first;
+    }
+    const int ns = (int)st.size();
+    if (2 * ns > 2 * ctx->npart) return false;  // (res, omega) partials per strip
+    // hand-off row: columns -24 .. maxni + 40 (the consumers' loads run ahead)
+    const long long bstride = (long long)(maxni + 64) * dq;
+    auto al = [](void** p, size_t bytes) {
+        if (hipMalloc(p, bytes) != hipSuccess) return false;
+        return hipMemset(*p, 0, bytes) == hipSuccess;
+    };
+    bool ok = al((void**)&ctx->djac16, sizeof(double) * 4 * (dq + 2) * (size_t)nsl) &&
+              al((void**)&ctx->dstrips16, sizeof(Strip16) * ns) &&
+              al((void**)&ctx->dbnd16, sizeof(double) * (size_t)(2 * bstride * (ns + 1))) &&
+              hipMemcpy(ctx->dstrips16, st.data(), sizeof(Strip16) * ns, hipMemcpyHostToDevice) == hipSuccess;
+    (void)C;
+    if (!ok) {
+        void* ptrs[] = {ctx->djac16, ctx->dstrips16, ctx->dbnd16};
+        for (void* p : ptrs) if (p) hipFree(p);
+        ctx->djac16 = nullptr;
+        ctx->dstrips16 = nullptr;
+        ctx->dbnd16 = nullptr;
+        ctx->use_skew = 0;
+        (void)hipGetLastError();
+        return false;
+    }
+    ctx->nstrips16 = ns;
+    ctx->nsl16 = nsl;
+    ctx->bstride16 = bstride;
+    return true;
+}
+
 // set_ABpm inside a whole step: without stored Jacobians when the optimised
 // sweeps run (they need alpha, the pre-conditioner vectors/scalars and the
-// local dt), with the skewed copy for the row-per-lane sweeps at nY > 1
+// local dt); at nY > 1 with the quad-row copy for the 16-row sweeps, or (dual
+// time) the skewed copy for the row-per-lane sweeps
 static int abpm_for_step(shcfd_ctx* ctx) {
     if (!fast_enabled(ctx)) {
+        if (q16_enabled(ctx)) {
+            ProfScope ps(ctx, PROF_ABPM);
+            int maxni = 0;
+            for (const auto& B : ctx->hblk) maxni = std::max(maxni, B.ni);
+            dim3 grid((unsigned)ctx->nstrips16, (unsigned)((maxni + 15 + 15) / 16));
+            DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_set_abpm16<NY>, grid, dim3(256), 0, ctx->stream, ctx->dev(),
+                                                    ctx->dstrips16, ctx->djac16, ctx->nsl16));
+            LAUNCHCHK();
+            ctx->q16_active = 1;
+            return SHCFD_OK;
+        }
         if (!skew_enabled(ctx)) return shcfd_set_abpm(ctx);
         ProfScope ps(ctx, PROF_ABPM);
         int maxni = 0;
@@ -761,12 +834,40 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
     DevCtx d = ctx->dev();
     int* prog_f = ctx->dprog;
     int* prog_b = ctx->dprog + ns;
+    const bool q16 = ctx->q16_active && ctx->djac16 && !fast_enabled(ctx);
+    ctx->q16_active = 0;
+    const int C16 = (ctx->dimq + 3) / 4, ns16 = ctx->nstrips16;
     { ProfScope ps(ctx, PROF_RHS);
     DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_rhs<NY>, ctx->dprog, 2 * ns)); }
     if (fast_enabled(ctx)) {
         int rc = ctx->fast->implicit(*ctx, step_internal, prof_begin, prof_end);
         LAUNCHCHK();
         return rc;
+    }
+    if (q16) {
+        const dim3 blk((unsigned)(64 * C16));
+        double* hand_f = ctx->dbnd16;
+        double* hand_b = ctx->dbnd16 + ctx->bstride16 * (ns16 + 1);
+        { ProfScope ps(ctx, PROF_SWEEP_FWD);
+        hipLaunchKernelGGL(k_fill16, dim3(256), dim3(256), 0, ctx->stream, d, ctx->dstrips16, ns16, ctx->dbnd16,
+                           ctx->bstride16, ctx->dimq);
+        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep16_fwd<NY>, dim3(ns16), blk, 0, ctx->stream, d, ctx->dstrips16,
+                                                ns16, hand_f, ctx->bstride16, ctx->spin_limit, ctx->djac16,
+                                                ctx->nsl16)) }
+        { ProfScope ps(ctx, PROF_SWEEP_BWD);
+        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep16_bwd<NY>, dim3(ns16), blk, 0, ctx->stream, d, ctx->dstrips16,
+                                                ns16, hand_b, ctx->bstride16, ctx->dpart, ctx->spin_limit,
+                                                ctx->djac16, ctx->nsl16)) }
+        if (inner_loop_scheme(ctx->p.scheme)) {
+            { ProfScope ps(ctx, PROF_REDUCE);
+            hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, ctx->stream, d, ctx->dpart, ns16); }
+            if (step_internal == 1)
+                hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, ctx->stream, d, (int)SCAL_RES1, (int)SCAL_RES);
+            { ProfScope ps(ctx, PROF_UPDATE);
+            DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_update<NY>)); }
+        }
+        LAUNCHCHK();
+        return SHCFD_OK;
     }
     const bool sk = ctx->skew_active && ctx->dskew;
     ctx->skew_active = 0;

@@ -69,4 +69,45 @@ __device__ inline int load_flag(const int* p) {
 // wait until every vector-memory operation of this wave has completed
 __device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// quad broadcast: every lane of a 4-lane group gets lane k of the group
+template <int K>
+__device__ inline double quad_bcast(double x) {
+    constexpr int ctrl = K | (K << 2) | (K << 4) | (K << 6);
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    // every lane has a source within its quad: bound_ctrl set, no old value
+    lo = __builtin_amdgcn_mov_dpp(lo, ctrl, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, ctrl, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+// lane L receives lane L-4 (0 for L < 4) / lane L+4 (0 for L > 59)
+__device__ inline double shift_up4(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xF, 0xF, true);
+        hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xF, 0xF, true);
+    }
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double shift_down4(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xF, 0xF, true);
+        hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xF, 0xF, true);
+    }
+    return __hiloint2double(hi, lo);
+}
+
+// quad broadcast with the source lane given at run time (constant after
+// unrolling, so the switch folds to one DPP pair)
+__device__ inline double quad_bcast_n(double x, int k) {
+    switch (k) {
+    case 0: return quad_bcast<0>(x);
+    case 1: return quad_bcast<1>(x);
+    case 2: return quad_bcast<2>(x);
+    default: return quad_bcast<3>(x);
+    }
+}
+
 }  // namespace shcfd

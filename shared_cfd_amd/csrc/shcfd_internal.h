@@ -88,6 +88,15 @@ struct Strip {
     int next;    // strip index of the row range above (backward) or -1
 };
 
+// 16-row strip of the quad-row sweeps for nY > 1 (k_sweep16.hip): lane
+// 4r + c holds components c, c+4, c+8, ... of row j0 + r; sweep step t
+// processes column t - r + 1 of row r (forward).  Operator slots of step t
+// are base + 64 t + lane.
+struct Strip16 {
+    int b, j0, prev, next;
+    long long base;
+};
+
 // kernel classes timed by the optional HIP-event profiler
 enum {
     PROF_PRIMITIVE = 0, PROF_BC, PROF_HV, PROF_TG, PROF_TGV, PROF_DT, PROF_ABPM, PROF_DQDT, PROF_RHS,
@@ -162,6 +171,14 @@ struct Ctx {
     int tuned_graph_steps = 0;     // schedule chosen by shcfd_autotune
     // strip-skewed Jacobian copy for the row-per-lane sweeps (nY > 1 steps)
     int use_skew = 1;
+    // quad-row sweeps for nY > 1 and schemes without the dual-time S scaling
+    Strip16* dstrips16 = nullptr;
+    int nstrips16 = 0;
+    double* djac16 = nullptr;      // 2 sweeps x 2 directions x C x dq planes of nsl16 slots
+    long long nsl16 = 0;
+    double* dbnd16 = nullptr;      // hand-off rows of both sweeps: 2 x (nstrips16 + 1) x bstride16
+    long long bstride16 = 0;
+    int q16_active = 0;            // the last set_ABpm of a step wrote the quad-row copy
     int skew_active = 0;           // the last set_ABpm of a step wrote the skewed copy
     double* dskew = nullptr;
     long long skew_nsl = 0;        // slots: sum over strips of 64 (ni + 63)

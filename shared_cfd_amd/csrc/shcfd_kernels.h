@@ -436,12 +436,20 @@ __global__ void k_sweep_bwd_plain(DevCtx, const Strip*, int, int*, double*, long
                                   const double*, long long, const long long*);
 template <int NY> __global__ void k_update(DevCtx, const Tile*);
 __global__ void k_reduce_res_omega(DevCtx, const double*, int);
+// quad-row sweeps for nY > 1 (k_sweep16.hip; operators from k_set_abpm16)
+template <int NY> __global__ void k_set_abpm16(DevCtx, const Strip16*, double*, long long);
+__global__ void k_fill16(DevCtx, const Strip16*, int, double*, long long, int);
+template <int NY>
+__global__ void k_sweep16_fwd(DevCtx, const Strip16*, int, double*, long long, long long, const double*, long long);
+template <int NY>
+__global__ void k_sweep16_bwd(DevCtx, const Strip16*, int, double*, long long, double*, long long, const double*,
+                              long long);
 __global__ void k_copy_scal(DevCtx, int, int);
 // chemistry (k_chem.hip)
 enum { CHEM_OP_FEX = 0, CHEM_OP_FEXVAR = 1, CHEM_OP_JEX = 2, CHEM_OP_CALCT = 3, CHEM_OP_CHEMEQ2 = 4,
        CHEM_OP_POINT_IMPLICIT = 5 };
 template <int NS> __global__ void k_thermo_chem(DevCtx, const Tile*, const shcfd_mech*);
-template <int NS> __global__ void k_reaction(DevCtx, const Tile*, const shcfd_mech*, int);
+template <int NS, int MODE> __global__ void k_reaction(DevCtx, const Tile*, const shcfd_mech*);
 template <int NS, int OP> __global__ void k_chem0d(const shcfd_mech*, int, const double*, int, double*, int, int*, double);
 // geometry (k_geom.hip)
 __global__ void k_geom_a(DevCtx, const Tile*);
