@@ -74,6 +74,24 @@ __device__ inline void post_matrix(const CellJac<1>& J, bool dual, double* S) {
     }
 }
 
+// alpha and the dual-time vectors/scalars of a cell as set_ABpm stored them
+// (ALPHA, PRE1, PRE2, PRESCA: the values cell_jacobian gives for the same
+// state, which the stage kernels read instead of recomputing)
+struct PostScal {
+    double alpha, pre1[4], pre2[4], ps[4];
+};
+__device__ inline PostScal post_scal(const DevCtx& d, long long c) {
+    PostScal J;
+    J.alpha = FLD(d, SHCFD_F_ALPHA, 0, c);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        J.pre1[k] = FLD(d, SHCFD_F_PRE1, k, c);
+        J.pre2[k] = FLD(d, SHCFD_F_PRE2, k, c);
+        J.ps[k] = FLD(d, SHCFD_F_PRESCA, k, c);
+    }
+    return J;
+}
+
 // cell of slot (strip S, step d, row r)
 __device__ inline bool slot_cell(const QStrip& S, const DevBlock& B, int d, int r, int& i, int& j) {
     i = d - r + 1;
@@ -142,11 +160,13 @@ __global__ void __launch_bounds__(256) k_fast_prep(DevCtx d, const QStrip* strip
     for (int m = 0; m < 4; m++) {
         mm4(mats[m], Sm, T);
         double* o = m < 2 ? G : H;
+        // doubles (2p', 2p'+1) of a lane's row are adjacent: one 16-byte store
 #pragma unroll
         for (int k = 0; k < 4; k++)
 #pragma unroll
-            for (int l = 0; l < 4; l++)
-                o[op_index(S, dd, 4 * r + k, 2 * (m & 1) + (l >> 1), l & 1)] = fs[m] * T[k + 4 * l];
+            for (int lp = 0; lp < 2; lp++)
+                *reinterpret_cast<double2*>(&o[op_index(S, dd, 4 * r + k, 2 * (m & 1) + lp, 0)]) =
+                    make_double2(fs[m] * T[k + 4 * (2 * lp)], fs[m] * T[k + 4 * (2 * lp + 1)]);
     }
 }
 
@@ -422,8 +442,8 @@ __global__ void __launch_bounds__(256) k_fast_mid(DevCtx d, const QStrip* strips
     int i, j;
     if (!slot_cell(S, B, dd, r, i, j)) return;
     const long long c = cidx(B, i, j);
-    CellJac<1> J;
-    cell_jacobian<1>(d, c, B.pitch, J);
+    // the dual-time scalars and vectors set_ABpm stored for this state
+    const PostScal J = post_scal(d, c);
     double v[4], x[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) v[k] = V_sl[slot1(S.base, dd, 4 * r + k)];
@@ -560,8 +580,7 @@ __global__ void __launch_bounds__(256) k_fast_post(DevCtx d, const QStrip* strip
 #pragma unroll
         for (int k = 0; k < 4; k++) z[k] = Z_sl[slot1(S.base, dd, 4 * r + k)];
         if (dual) {
-            CellJac<1> J;
-            cell_jacobian<1>(d, c, B.pitch, J);
+            const PostScal J = post_scal(d, c);
             double dz = 0.0;
 #pragma unroll
             for (int k = 0; k < 4; k++) dz = fma(J.pre2[k], z[k], dz);

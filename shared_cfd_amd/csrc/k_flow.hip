@@ -397,12 +397,13 @@ __global__ void __launch_bounds__(256) k_rms_part(DevCtx d, const Tile* tiles, d
 // ---------------------------------------------------------------------------
 // MATS: 0 = no stored Jacobians, 1 = Ap/Am/Bp/Bm fields (cell SoA),
 // 2 = the strip-skewed copy read by the skewed sweeps (slot given),
-// 3 = the quad-row copy of the 16-row sweeps (slot of wave 0, lane 4r of
-// the step given): lane 4r+c of wave w owns row k = 4w + c; per sweep
-// (set 0: forward, 1: backward) it reads dq + 2 pairs of doubles, pair p at
-// ((set dq2 + p) nsl + slot + 64 w + c) 2, dq2 = dq + 2: elements 0..dq-1
-// are row k of A+ (A-), dq..2dq-1 row k of B+ (B-), then (alpha, dsci),
-// (dscj, 0); rows k >= dq are zero
+// 3 = the quad-row copy of the 16-row sweeps (nsl = total steps NT of all
+// strips, slot = 64 x global step + 4 r): lane 4r+c of wave w owns row
+// k = 4w + c; per sweep (set 0: forward, 1: backward) it reads dq + 2 pairs
+// of doubles, pair p of global step g at ((((set NT + g) C + w) dq2 + p) 64 +
+// lane) 2, dq2 = dq + 2, so the operators of one step of a strip are
+// contiguous: elements 0..dq-1 are row k of A+ (A-), dq..2dq-1 row k of B+
+// (B-), then (alpha, dsci), (dscj, 0); rows k >= dq are zero
 template <int NY, int MATS>
 __device__ inline void set_abpm_cell(const DevCtx& d, const DevBlock& B, int i, int j, double* skew, long long nsl,
                                      long long slot) {
@@ -432,16 +433,18 @@ __device__ inline void set_abpm_cell(const DevCtx& d, const DevBlock& B, int i, 
             const double* MA = set == 0 ? J.Ap : J.Am;
             const double* MB = set == 0 ? J.Bp : J.Bm;
             for (int e = 0; e < 2 * dq2; e++) {
-                double* o = skew + ((long long)(set * dq2 + e / 2) * nsl + slot) * 2 + (e & 1);
-                for (int w = 0; w < C; w++)
+                const long long g = slot >> 6, l0 = slot & 63;
+                for (int w = 0; w < C; w++) {
+                    double* o = skew + ((((set * nsl + g) * C + w) * dq2 + e / 2) * 64 + l0) * 2 + (e & 1);
                     for (int cc = 0; cc < 4; cc++) {
                         const int k = 4 * w + cc;
                         double v;
                         if (e < dq) v = (k < dq) ? MA[k + dq * e] : 0.0;
                         else if (e < 2 * dq) v = (k < dq) ? MB[k + dq * (e - dq)] : 0.0;
                         else v = e == 2 * dq ? J.alpha : (e == 2 * dq + 1 ? dsci : (e == 2 * dq + 2 ? dscj : 0.0));
-                        o[2 * (64 * w + cc)] = v;
+                        o[2 * cc] = v;
                     }
+                }
             }
         }
     }
@@ -503,8 +506,7 @@ __global__ void __launch_bounds__(256) k_set_abpm16(DevCtx d, const Strip16* str
     if (t >= B.ni + 15) return;
     const int i = t - r + 1, j = S.j0 + r;
     if (i < 1 || i > B.ni || j > B.nj) return;
-    constexpr int C = (NY + 3 + 3) / 4;
-    set_abpm_cell<NY, 3>(d, B, i, j, jac, nsl, S.base + 64LL * C * t + 4 * r);
+    set_abpm_cell<NY, 3>(d, B, i, j, jac, nsl, 64LL * (S.base + t) + 4 * r);
 }
 
 template <int NY>

@@ -23,9 +23,10 @@
 // topological order of the same dependency graph, so the results are
 // identical.  A lane reads its two matrix rows (2 dq doubles) and the cell's
 // alpha, dsci, dscj of each step from the quad-row copy written by
-// k_set_abpm16 as dq + 2 pairs of doubles, pair p at ((set (dq+2) + p) nsl +
-// base + 64 C t + 64 w + lane) 2, so every 16-byte load of a wave reads 1 KB
-// of consecutive bytes.  Loads run PD steps ahead of their use in
+// k_set_abpm16 as dq + 2 pairs of doubles, pair p of global step g = base + t
+// at ((((set NT + g) C + w) (dq+2) + p) 64 + lane) 2: every 16-byte load of
+// a wave reads 1 KB of consecutive bytes, and one step of a strip is one
+// contiguous block (few pages per step).  Loads run PD steps ahead of their use in
 // straight-line code (the last steps re-read a valid step), so the
 // compiler's memory waits count exactly the loads issued after the one a
 // step consumes.
@@ -77,14 +78,14 @@ struct StepIn {
     double al, fi, fj, r, q;
 };
 
-// the pair-interleaved operator rows and scalars of one step (set 0:
-// forward, 1: backward) at lane slot `ls`
+// the pair-interleaved operator rows and scalars of one step: `base` is the
+// lane's pair 0 of that step
 template <int DQ>
-__device__ inline void load_ops(const double* __restrict__ jac, long long nsl, int set, long long ls, StepIn<DQ>& in) {
+__device__ inline void load_ops(const double* __restrict__ base, StepIn<DQ>& in) {
     constexpr int dq2 = DQ + 2;
 #pragma unroll
     for (int p = 0; p < dq2; p++) {
-        const double2 v = *reinterpret_cast<const double2*>(jac + ((long long)(set * dq2 + p) * nsl + ls) * 2);
+        const double2 v = *reinterpret_cast<const double2*>(base + 128LL * p);
         const int e0 = 2 * p, e1 = 2 * p + 1;
         double* dst0 = e0 < DQ ? &in.a[e0] : (e0 < 2 * DQ ? &in.b[e0 - DQ] : (e0 == 2 * DQ ? &in.al : &in.fj));
         *dst0 = v.x;
@@ -129,14 +130,16 @@ k_sweep16_fwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long l
     const bool has_next = S.next >= 0;
     const bool mine = r == 0 && kok;  // lanes that take the neighbour strip's value
     const int nsteps = ni + 15;
-    const long long lane_slot = S.base + 64LL * w + L;
+    constexpr int dq2 = dq + 2;
+    // lane's pair 0 of step 0 (set 0); one step = C dq2 64 pairs
+    const double* jl = jac + (((S.base * C + w) * dq2) * 64 + L) * 2;
     const int kk = kok ? k : 0;
     const double* hsrc = hand + (long long)(S.prev >= 0 ? S.prev : nstrips) * hstride + (long long)HO * dq + kk;
     double* hdst = hand + (long long)s * hstride + (long long)HO * dq + kk;
     double* dqf = d.f[SHCFD_F_DQ] + (long long)kk * d.NC;
     auto load = [&](int t, StepIn<dq>& in) {
         t = t < nsteps ? t : nsteps - 1;  // past the end: re-read the last step
-        load_ops<dq>(jac, nsl, 0, lane_slot + 64LL * C * t, in);
+        load_ops<dq>(jl + 128LL * C * dq2 * t, in);
         const int i = t - r + 1;
         const int ic = i < 1 ? 1 : (i > ni ? ni : i);
         in.r = dqf[cidx(B, ic, rowok ? j : nj)];
@@ -217,7 +220,9 @@ k_sweep16_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long l
     const bool inner = inner_loop_scheme(d.p.scheme);
     const double dqmax = d.p.dqmax;
     const int nsteps = ni + 15;
-    const long long lane_slot = S.base + 64LL * w + L;
+    constexpr int dq2 = dq + 2;
+    // lane's pair 0 of step 0 of set 1 (nsl = total steps of all strips)
+    const double* jl = jac + ((((nsl + S.base) * C + w) * dq2) * 64 + L) * 2;
     const int kk = kok ? k : 0;
     const double* hsrc = hand + (long long)(S.next >= 0 ? S.next : nstrips) * hstride + (long long)HO * dq + kk;
     double* hdst = hand + (long long)s * hstride + (long long)HO * dq + kk;
@@ -226,7 +231,7 @@ k_sweep16_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long l
     auto load = [&](int t, StepIn<dq>& in) {
         t = t < nsteps ? t : nsteps - 1;  // past the end: re-read the last step
         // cell (i, j0+r) sits at forward step i + r - 1 = ni + 14 - t
-        load_ops<dq>(jac, nsl, 1, lane_slot + 64LL * C * (ni + 14 - t), in);
+        load_ops<dq>(jl + 128LL * C * dq2 * (ni + 14 - t), in);
         const int i = ni - t + 15 - r;
         const int ic = i < 1 ? 1 : (i > ni ? ni : i);
         const long long c = cidx(B, ic, rowok ? j : nj);

@@ -237,6 +237,7 @@ static int check_field(shcfd_ctx* ctx, int field, int block) {
 int shcfd_upload_field(shcfd_ctx* ctx, int field, int block, const double* host) {
     int rc = check_field(ctx, field, block);
     if (rc || !host) return SHCFD_E_ARG;
+    if (field == SHCFD_F_TGVI || field == SHCFD_F_TGVJ) ctx->tgv_zero = 0;
     const DevBlock& B = ctx->hblk[block];
     int K = field_ncomp(field, ctx->ny);
     size_t rows = (size_t)B.nj + 4, w = (size_t)B.ni + 4;
@@ -573,12 +574,17 @@ int shcfd_set_tgv(shcfd_ctx* ctx) {
     if (!ctx) return SHCFD_E_ARG;
     ProfScope ps(ctx, PROF_TGV);
     if (!ctx->p.viscous) {
-        // init_TGv: viscosity/non-viscous/sch_viscous.f90 (TGv = 0)
-        long long n = (long long)ctx->dimq * ctx->NC;
-        hipLaunchKernelGGL(k_zero_field, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[SHCFD_F_TGVI], n);
-        hipLaunchKernelGGL(k_zero_field, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[SHCFD_F_TGVJ], n);
+        // init_TGv: viscosity/non-viscous/sch_viscous.f90 (TGv = 0); nothing
+        // else writes TGv of an inviscid run, so once is enough
+        if (!ctx->tgv_zero) {
+            long long n = (long long)ctx->dimq * ctx->NC;
+            hipLaunchKernelGGL(k_zero_field, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[SHCFD_F_TGVI], n);
+            hipLaunchKernelGGL(k_zero_field, dim3(1024), dim3(256), 0, ctx->stream, ctx->f[SHCFD_F_TGVJ], n);
+            ctx->tgv_zero = 1;
+        }
     } else {
         DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_tgv<NY>));
+        ctx->tgv_zero = 0;
     }
     LAUNCHCHK();
     return SHCFD_OK;
@@ -711,9 +717,9 @@ static bool q16_enabled(shcfd_ctx* ctx) {
             S.j0 = j0;
             S.prev = (j0 == 1) ? -1 : (int)st.size() - 1;
             S.next = -1;
-            S.base = nsl;
+            S.base = nsl;  // in steps
             if (S.prev >= 0) st[S.prev].next = (int)st.size();
-            nsl += 64LL * C * (B.ni + 15);
+            nsl += B.ni + 15;
             st.push_back(S);
         }
         (void)first;
@@ -726,11 +732,10 @@ static bool q16_enabled(shcfd_ctx* ctx) {
         if (hipMalloc(p, bytes) != hipSuccess) return false;
         return hipMemset(*p, 0, bytes) == hipSuccess;
     };
-    bool ok = al((void**)&ctx->djac16, sizeof(double) * 4 * (dq + 2) * (size_t)nsl) &&
+    bool ok = al((void**)&ctx->djac16, sizeof(double) * 2 * 2 * 64 * C * (dq + 2) * (size_t)nsl) &&
               al((void**)&ctx->dstrips16, sizeof(Strip16) * ns) &&
               al((void**)&ctx->dbnd16, sizeof(double) * (size_t)(2 * bstride * (ns + 1))) &&
               hipMemcpy(ctx->dstrips16, st.data(), sizeof(Strip16) * ns, hipMemcpyHostToDevice) == hipSuccess;
-    (void)C;
     if (!ok) {
         void* ptrs[] = {ctx->djac16, ctx->dstrips16, ctx->dbnd16};
         for (void* p : ptrs) if (p) hipFree(p);

@@ -89,9 +89,9 @@ struct Strip {
 };
 
 // 16-row strip of the quad-row sweeps for nY > 1 (k_sweep16.hip): lane
-// 4r + c holds components c, c+4, c+8, ... of row j0 + r; sweep step t
-// processes column t - r + 1 of row r (forward).  Operator slots of step t
-// are base + 64 t + lane.
+// 4r + c of wave w holds component 4w + c of row j0 + r; sweep step t
+// processes column t - r + 1 of row r (forward).  base = the strip's first
+// global step in the operator copy.
 struct Strip16 {
     int b, j0, prev, next;
     long long base;
@@ -156,6 +156,7 @@ struct Ctx {
     int first_dual = 1;
     double res1 = 0.0;
     double dt_fixed = 0.0;         // > 0: global time step held at this value
+    int tgv_zero = 0;              // TGVI/TGVJ hold zeros (inviscid set_TGv done)
     std::vector<double*> snaps;    // device snapshots of q-sized fields (dimq*NC doubles each)
     hipStream_t stream = nullptr;
     std::string last_error;
@@ -174,8 +175,8 @@ struct Ctx {
     // quad-row sweeps for nY > 1 and schemes without the dual-time S scaling
     Strip16* dstrips16 = nullptr;
     int nstrips16 = 0;
-    double* djac16 = nullptr;      // 2 sweeps x 2 directions x C x dq planes of nsl16 slots
-    long long nsl16 = 0;
+    double* djac16 = nullptr;      // operator copy: 2 sweeps x nsl16 steps x C x (dq + 2) x 64 pairs
+    long long nsl16 = 0;           // total steps of all strips
     double* dbnd16 = nullptr;      // hand-off rows of both sweeps: 2 x (nstrips16 + 1) x bstride16
     long long bstride16 = 0;
     int q16_active = 0;            // the last set_ABpm of a step wrote the quad-row copy
