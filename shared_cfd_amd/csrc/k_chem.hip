@@ -86,6 +86,43 @@ __global__ void __launch_bounds__(256) k_reaction(DevCtx d, const Tile* tiles, c
     if ((threadIdx.x & 63) == 0 && w > 0.0) atomicAdd(d.scal + SCAL_CHEM_ITERS, w);
 }
 
+// proceed_reaction with the point-implicit update (MODE 2), one wavefront per
+// workgroup: the (NS+1)^2 Newton matrix of each lane lives in LDS
+// (lane-interleaved) instead of registers, so the runtime species rows of
+// the Jacobian and the LU's pivot rows are indexed directly.  Workgroup k
+// handles row k % tile_rows of tile k / tile_rows.
+template <int NS>
+__global__ void __launch_bounds__(64) k_reaction_lds(DevCtx d, const Tile* tiles, int tile_rows,
+                                                      const shcfd_mech* __restrict__ m) {
+    constexpr int N = NS + 1;
+    __shared__ double jsh[N * N * 64];
+    const Tile tl = tiles[blockIdx.x / tile_rows];
+    const int i = tl.i0 + (int)threadIdx.x, j = tl.j0 + (int)(blockIdx.x % tile_rows);
+    const DevBlock B = d.blk[tl.b];
+    int att = 0;
+    if (i >= 1 && i <= B.ni && j >= 1 && j <= B.nj) {
+        using D = Dims<NS>;
+        const long long c = cidx(B, i, j);
+        const double dt = d.scal[SCAL_DT_GRBL];
+        const double T = FLD(d, SHCFD_F_W, IW_P, c) / FLD(d, SHCFD_F_W, IW_RHO, c) / FLD(d, SHCFD_F_W, D::ir, c);
+        double vrho[NS];
+#pragma unroll
+        for (int k = 0; k < NS; k++) vrho[k] = FLD(d, SHCFD_F_Q, k, c);
+        double T1;
+        const int e = chem::point_implicit_lds<NS>(m, T, vrho, dt, att, T1, jsh + threadIdx.x);
+        if (e) {
+            atomicCAS(d.iflag + IFLAG_ERR, 0, e);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NS; k++) FLD(d, SHCFD_F_Q, k, c) = vrho[k];
+        }
+    }
+    double w = (double)att;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
+    if (threadIdx.x == 0 && w > 0.0) atomicAdd(d.scal + SCAL_CHEM_ITERS, w);
+}
+
 // batched 0-D harness: cell q's input row starts at in + q*ldi, output row at
 // out + q*ldo; status[q] receives an iteration/attempt count or -code.
 template <int NS, int op>
@@ -169,7 +206,8 @@ __global__ void __launch_bounds__(256) k_chem0d(const shcfd_mech* __restrict__ m
 #define SHCFD_CHEM_FLOW_INST(NS)                                                        \
     template __global__ void k_thermo_chem<NS>(DevCtx, const Tile*, const shcfd_mech*); \
     template __global__ void k_reaction<NS, SHCFD_REACTION_CHEMEQ2>(DevCtx, const Tile*, const shcfd_mech*);      \
-    template __global__ void k_reaction<NS, SHCFD_REACTION_POINT_IMPLICIT>(DevCtx, const Tile*, const shcfd_mech*);
+    template __global__ void k_reaction<NS, SHCFD_REACTION_POINT_IMPLICIT>(DevCtx, const Tile*, const shcfd_mech*); \
+    template __global__ void k_reaction_lds<NS>(DevCtx, const Tile*, int, const shcfd_mech*);
 #define SHCFD_CHEM0D_OP(NS, OP) \
     template __global__ void k_chem0d<NS, OP>(const shcfd_mech*, int, const double*, int, double*, int, int*, double);
 #define SHCFD_CHEM0D_INST(NS)                                                                     \

@@ -512,7 +512,9 @@ int shcfd_proceed_reaction(shcfd_ctx* ctx) {
         return SHCFD_E_STATE;
     ProfScope ps(ctx, PROF_REACTION);
     if (ctx->p.reaction == SHCFD_REACTION_POINT_IMPLICIT)
-        DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH((k_reaction<NS, SHCFD_REACTION_POINT_IMPLICIT>), ctx->dmech))
+        DISPATCH_FLOW_NS(ctx->ny, hipLaunchKernelGGL(k_reaction_lds<NS>, dim3((unsigned)(ntiles(ctx) * ctx->tile_rows)),
+                                                     dim3(64), 0, ctx->stream, ctx->dev(), ctx->dtiles,
+                                                     ctx->tile_rows, ctx->dmech))
     else
         DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH((k_reaction<NS, SHCFD_REACTION_CHEMEQ2>), ctx->dmech))
     LAUNCHCHK();

@@ -416,9 +416,53 @@ SHCFD_HD void fex_var(const shcfd_mech* m, const double (&n)[NS + 1], double (&d
     dm[NS] = fmin2(dT, 0.0);
 }
 
-// Jex: FJ.f90:454-708.  J[a][b] = d(dndt_a)/d(n_b) (row a, column b).
-template <int NS>
-SHCFD_HD void jex(const shcfd_mech* m, const double (&n)[NS + 1], double (&J)[NS + 1][NS + 1], double (&dndt)[NS + 1]) {
+// Storage of the (NS+1)^2 Jacobian for jex_impl and the LU: registers
+// (indices compile-time after unrolling; a runtime row is selected by
+// comparison) or per-lane LDS (lane-interleaved, runtime row indexing).
+template <int N>
+struct RegMat {
+    double (&J)[N][N];
+    SHCFD_HD double get(int a, int b) const { return J[a][b]; }
+    SHCFD_HD void set(int a, int b, double v) { J[a][b] = v; }
+    template <int NR>
+    SHCFD_HD void sub_row(int it, const double (&v)[N]) {
+#pragma unroll
+        for (int a = 0; a < NR; a++)
+            if (a == it) {
+#pragma unroll
+                for (int b = 0; b < N; b++) J[a][b] = J[a][b] - v[b];
+            }
+    }
+    template <int NR>
+    SHCFD_HD void add_row(int it, const double (&v)[N]) {
+#pragma unroll
+        for (int a = 0; a < NR; a++)
+            if (a == it) {
+#pragma unroll
+                for (int b = 0; b < N; b++) J[a][b] = J[a][b] + v[b];
+            }
+    }
+};
+template <int N>
+struct LdsMat {
+    double* p;  // this lane's element (0,0); element (a,b) at p[(a N + b) 64]
+    __device__ double get(int a, int b) const { return p[(a * N + b) * 64]; }
+    __device__ void set(int a, int b, double v) { p[(a * N + b) * 64] = v; }
+    template <int NR>
+    __device__ void sub_row(int it, const double (&v)[N]) {
+#pragma unroll
+        for (int b = 0; b < N; b++) p[(it * N + b) * 64] = p[(it * N + b) * 64] - v[b];
+    }
+    template <int NR>
+    __device__ void add_row(int it, const double (&v)[N]) {
+#pragma unroll
+        for (int b = 0; b < N; b++) p[(it * N + b) * 64] = p[(it * N + b) * 64] + v[b];
+    }
+};
+
+// Jex: FJ.f90:454-708.  J(a,b) = d(dndt_a)/d(n_b) (row a, column b).
+template <int NS, class Mat>
+SHCFD_HD void jex_impl(const shcfd_mech* m, const double (&n)[NS + 1], Mat& J, double (&dndt)[NS + 1]) {
     constexpr int N = NS + 1;
     double T = n[NS], logT = dlog(T), Tinv = 1.0 / T, logPRT = dlog(PST / RU) - logT;
     double sn = 0.0;
@@ -432,7 +476,7 @@ SHCFD_HD void jex(const shcfd_mech* m, const double (&n)[NS + 1], double (&J)[NS
     for (int a = 0; a < N; a++) {
         dndt[a] = 0.0;
 #pragma unroll
-        for (int b = 0; b < N; b++) J[a][b] = 0.0;
+        for (int b = 0; b < N; b++) J.set(a, b, 0.0);
     }
     double vlogM = 0.0, vM = 0.0, dlnkdlnM = 0.0;
     for (int i = 0; i < m->nr; i++) {
@@ -469,29 +513,19 @@ SHCFD_HD void jex(const shcfd_mech* m, const double (&n)[NS + 1], double (&J)[NS
         for (int k = 0; k < m->num_nu[i][0]; k++) {
             int it = m->ind_nu[i][0][k];
             sel_sub(dndt, it, Dr);
-#pragma unroll
-            for (int a = 0; a < NS; a++)
-                if (a == it) {
-#pragma unroll
-                    for (int b = 0; b < N; b++) J[a][b] = J[a][b] - drdn[b];
-                }
+            J.template sub_row<NS>(it, drdn);
         }
         for (int k = 0; k < m->num_nu[i][1]; k++) {
             int it = m->ind_nu[i][1][k];
             sel_add(dndt, it, Dr);
-#pragma unroll
-            for (int a = 0; a < NS; a++)
-                if (a == it) {
-#pragma unroll
-                    for (int b = 0; b < N; b++) J[a][b] = J[a][b] + drdn[b];
-                }
+            J.template add_row<NS>(it, drdn);
         }
     }
     // FJ.f90:680-697
     double de = 0.0, cv = 0.0, dcv = 0.0, t1 = 0.0, t2 = 0.0;
 #pragma unroll
     for (int j = 0; j < NS; j++) {
-        t1 = t1 + S.vert[j] * J[j][NS];
+        t1 = t1 + S.vert[j] * J.get(j, NS);
         de = de + S.vert[j] * dndt[j];
         dcv = dcv + S.vcvr[j] * dndt[j];
         cv = cv + S.vcvr[j] * n[j];
@@ -499,14 +533,20 @@ SHCFD_HD void jex(const shcfd_mech* m, const double (&n)[NS + 1], double (&J)[NS
     }
     double dTdt = -(de * T / cv);
     dndt[NS] = dTdt;
-    J[NS][NS] = -((T * t1 + dcv + dTdt * t2) / cv);
+    J.set(NS, NS, -((T * t1 + dcv + dTdt * t2) / cv));
 #pragma unroll
     for (int j = 0; j < NS; j++) {
         double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < NS; k++) s = s + S.vert[k] * J[k][j];
-        J[NS][j] = -((T * s + dTdt * S.vcvr[j]) / cv);
+        for (int k = 0; k < NS; k++) s = s + S.vert[k] * J.get(k, j);
+        J.set(NS, j, -((T * s + dTdt * S.vcvr[j]) / cv));
     }
+}
+
+template <int NS>
+SHCFD_HD void jex(const shcfd_mech* m, const double (&n)[NS + 1], double (&J)[NS + 1][NS + 1], double (&dndt)[NS + 1]) {
+    RegMat<NS + 1> R{J};
+    jex_impl<NS>(m, n, R, dndt);
 }
 
 // rho_rel2abs: sub_chem.f90:1036-1085 (vrho relative in, kg/m^3 out)
@@ -875,6 +915,133 @@ SHCFD_HD int point_implicit(const shcfd_mech* m, double T, double (&vrho)[NS], d
             f[a] = (y0[a] - y[a]) + dt * f[a];
         }
         if (lu_solve<N>(J, f) != 0) {
+            iters = it;
+            return SHCFD_E_CHEM;
+        }
+        done = true;
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            double yn = fmax2(y[j] + f[j], N_EPS);
+            double d = yn - y[j];
+            if ((d < 0.0 ? -d : d) > PI_RTOL * y[j] + atol) done = false;
+            y[j] = yn;
+        }
+        double Tn = y[NS] + f[NS];
+        double dT = Tn - y[NS];
+        if ((dT < 0.0 ? -dT : dT) > PI_RTOL * y[NS]) done = false;
+        y[NS] = Tn;
+        if (!(y[NS] > 0.0)) {
+            iters = it;
+            return SHCFD_E_CHEM;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NS; j++) vrho[j] = y[j] * m->mw[j] * 1e3;
+    iters = it;
+    T_out = y[NS];
+    return 0;
+}
+
+// lu_solve with the matrix in per-lane LDS: the same pivot choice, swaps and
+// operation order as lu_solve, with runtime row indices
+template <int N>
+__device__ int lu_solve_lds(LdsMat<N>& A, double (&b)[N]) {
+    int info = 0;
+    int piv[N];
+    for (int k = 0; k < N - 1; k++) {
+        int l = k;
+        double amax = A.get(k, k) < 0.0 ? -A.get(k, k) : A.get(k, k);
+        for (int i = k + 1; i < N; i++) {
+            const double ai = A.get(i, k);
+            const double a = ai < 0.0 ? -ai : ai;
+            if (a > amax) {
+                amax = a;
+                l = i;
+            }
+        }
+        piv[k] = l;
+        const double pv = A.get(l, k);
+        if (pv == 0.0) {
+            info = k + 1;
+            continue;
+        }
+        if (l != k) A.set(l, k, A.get(k, k));
+        A.set(k, k, pv);
+        const double t = -1.0 / pv;
+        for (int i = k + 1; i < N; i++) A.set(i, k, t * A.get(i, k));
+        for (int j = k + 1; j < N; j++) {
+            const double tj = A.get(l, j);
+            if (l != k) A.set(l, j, A.get(k, j));
+            A.set(k, j, tj);
+            for (int i = k + 1; i < N; i++) A.set(i, j, A.get(i, j) + tj * A.get(i, k));
+        }
+    }
+    piv[N - 1] = N - 1;
+    if (A.get(N - 1, N - 1) == 0.0) info = N;
+    if (info) return info;
+#pragma unroll
+    for (int k = 0; k < N - 1; k++) {
+        const int l = piv[k];
+        double t = b[k];
+#pragma unroll
+        for (int i = k + 1; i < N; i++)
+            if (i == l) t = b[i];
+#pragma unroll
+        for (int i = k + 1; i < N; i++)
+            if (i == l) b[i] = b[k];
+        b[k] = t;
+#pragma unroll
+        for (int i = k + 1; i < N; i++) b[i] = b[i] + t * A.get(i, k);
+    }
+#pragma unroll
+    for (int kb = 0; kb < N; kb++) {
+        const int k = N - 1 - kb;
+        b[k] = b[k] / A.get(k, k);
+        const double t = -b[k];
+#pragma unroll
+        for (int i = 0; i < k; i++) b[i] = b[i] + t * A.get(i, k);
+    }
+    return 0;
+}
+
+// point_implicit with the Newton matrix in per-lane LDS (Jl: this lane's
+// element (0,0), stride 64 doubles): the same iteration and arithmetic
+template <int NS>
+__device__ int point_implicit_lds(const shcfd_mech* m, double T, double (&vrho)[NS], double dt, int& iters,
+                                  double& T_out, double* Jl) {
+    constexpr int N = NS + 1;
+    LdsMat<N> J{Jl};
+    double y0[N], y[N];
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+        y0[j] = vrho[j] * m->inv_mw[j] * 1e-3;
+        y0[j] = fmax2(y0[j], N_EPS);
+        y[j] = y0[j];
+    }
+    y0[NS] = T;
+    y[NS] = T;
+    double atol = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; j++) atol = atol + y0[j];
+    atol = atol * PI_ATOL;
+    int it = 0;
+    bool done = false;
+    while (!done) {
+        if (it >= PI_MAXIT) {
+            iters = it;
+            return SHCFD_E_CHEM;
+        }
+        it = it + 1;
+        double f[N], dJ[N];
+        fex<NS>(m, y, f);
+        jex_impl<NS>(m, y, J, dJ);
+        for (int a = 0; a < N; a++) {
+            for (int b = 0; b < N; b++) J.set(a, b, -(dt * J.get(a, b)));
+            J.set(a, a, J.get(a, a) + 1.0);
+        }
+#pragma unroll
+        for (int a = 0; a < N; a++) f[a] = (y0[a] - y[a]) + dt * f[a];
+        if (lu_solve_lds<N>(J, f) != 0) {
             iters = it;
             return SHCFD_E_CHEM;
         }

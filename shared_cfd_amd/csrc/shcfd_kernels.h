@@ -450,6 +450,7 @@ enum { CHEM_OP_FEX = 0, CHEM_OP_FEXVAR = 1, CHEM_OP_JEX = 2, CHEM_OP_CALCT = 3, 
        CHEM_OP_POINT_IMPLICIT = 5 };
 template <int NS> __global__ void k_thermo_chem(DevCtx, const Tile*, const shcfd_mech*);
 template <int NS, int MODE> __global__ void k_reaction(DevCtx, const Tile*, const shcfd_mech*);
+template <int NS> __global__ void k_reaction_lds(DevCtx, const Tile*, int, const shcfd_mech*);
 template <int NS, int OP> __global__ void k_chem0d(const shcfd_mech*, int, const double*, int, double*, int, int*, double);
 // geometry (k_geom.hip)
 __global__ void k_geom_a(DevCtx, const Tile*);
