@@ -48,6 +48,12 @@ enum {
     SHCFD_SCHEME_PRECON_LUSGS = 6
 };
 
+/* ---- geometry ("dimension" of checkout.inp, checkout_flow.py) ----------- */
+enum {
+    SHCFD_DIM_PLANAR = 0,        /* store/dim/2d                                              */
+    SHCFD_DIM_AXISYMMETRIC = 1   /* store/dim/q2d: x axial, r radial (r >= 0), per radian     */
+};
+
 /* ---- MUSCL limiter ------------------------------------------------------- */
 enum {
     SHCFD_LIMITER_REFERENCE = 0,  /* s=2DuDl/(Du^2+Dl^2+1e-300), muscl.f90:32 */
@@ -125,6 +131,8 @@ enum {
     SHCFD_F_BP,
     SHCFD_F_BM,
     SHCFD_F_DQ,         /* Dq(1:dimq) of the last implicit update            */
+    SHCFD_F_AREA,       /* Area: planar cell area (axisymmetric; 0 in 2-D)   */
+    SHCFD_F_GEOJACC,    /* geojacc(1:2,1:2) cell-centre metrics (axisym.)    */
     SHCFD_F_COUNT
 };
 
@@ -220,6 +228,7 @@ typedef struct shcfd_params {
     double kappa;          /* ratio of specific heats                         */
     double rgas;           /* gas constant                                    */
     double nu;             /* kinematic viscosity (mu = rho*nu)               */
+    int dimension;         /* SHCFD_DIM_*                                     */
 } shcfd_params;
 
 #ifdef __cplusplus

@@ -81,7 +81,7 @@ static int field_ncomp(int fld, int ny) {
         return ny;
     case SHCFD_F_VNI: case SHCFD_F_VNJ: case SHCFD_F_VNCI: case SHCFD_F_VNCJ:
         return 2;
-    case SHCFD_F_GEOJACI: case SHCFD_F_GEOJACJ: case SHCFD_F_PRESCA:
+    case SHCFD_F_GEOJACI: case SHCFD_F_GEOJACJ: case SHCFD_F_GEOJACC: case SHCFD_F_PRESCA:
         return 4;
     case SHCFD_F_AP: case SHCFD_F_AM: case SHCFD_F_BP: case SHCFD_F_BM:
         return dimq * dimq;
@@ -203,6 +203,7 @@ static double dis_point(double x1, double y1, double x2, double y2) {
 }
 
 void or_init_geometry(or_ctx *c) {
+    const int ax = c->p.dimension == SHCFD_DIM_AXISYMMETRIC;
     for (int b = 0; b < c->nb; b++) {
         int ni = c->ni[b], nj = c->nj[b];
 #define XH(i, j) SC(SHCFD_F_XH, i, j)
@@ -229,20 +230,33 @@ void or_init_geometry(or_ctx *c) {
             XC(i, nj + 1) = 2.0 * XC(i, nj) - XC(i, nj - 1);
             RC(i, nj + 1) = 2.0 * RC(i, nj) - RC(i, nj - 1);
         }
-        /* set_Area_Vol, geometry.f90:107-122 (Area = 0 in 2-D) */
+        /* set_Area_Vol, dim/2d/geometry.f90:107-122 (Area = 0 in 2-D);
+         * dim/q2d/geometry.f90:107-122 (Area planar, Vol per radian) */
         for (int j = 1; j <= nj; j++)
             for (int i = 1; i <= ni; i++) {
                 double aabd = area_triangle(XH(i - 1, j - 1), RH(i - 1, j - 1), XH(i, j - 1), RH(i, j - 1), XH(i - 1, j), RH(i - 1, j));
                 double abcd = area_triangle(XH(i, j - 1), RH(i, j - 1), XH(i, j), RH(i, j), XH(i - 1, j), RH(i - 1, j));
-                SC(SHCFD_F_VOL, i, j) = aabd + abcd;
+                if (ax) {
+                    SC(SHCFD_F_AREA, i, j) = aabd + abcd;
+                    SC(SHCFD_F_VOL, i, j) = ((((RH(i - 1, j - 1) + RH(i, j - 1)) + RH(i - 1, j)) * aabd) / 3.0) +
+                                            ((((RH(i, j - 1) + RH(i, j)) + RH(i - 1, j)) * abcd) / 3.0);
+                } else {
+                    SC(SHCFD_F_AREA, i, j) = 0.0;
+                    SC(SHCFD_F_VOL, i, j) = aabd + abcd;
+                }
             }
-        /* set_dsij, geometry.f90:123-140 */
+        /* set_dsij, dim/2d/geometry.f90:123-140; dim/q2d/geometry.f90:123-141
+         * (times the mean node r) */
         for (int j = 0; j <= nj; j++)
-            for (int i = 1; i <= ni; i++)
-                SC(SHCFD_F_DSJ, i, j) = dis_point(XH(i - 1, j), RH(i - 1, j), XH(i, j), RH(i, j));
+            for (int i = 1; i <= ni; i++) {
+                double ds = dis_point(XH(i - 1, j), RH(i - 1, j), XH(i, j), RH(i, j));
+                SC(SHCFD_F_DSJ, i, j) = ax ? (ds * (RH(i - 1, j) + RH(i, j))) / 2.0 : ds;
+            }
         for (int j = 1; j <= nj; j++)
-            for (int i = 0; i <= ni; i++)
-                SC(SHCFD_F_DSI, i, j) = dis_point(XH(i, j - 1), RH(i, j - 1), XH(i, j), RH(i, j));
+            for (int i = 0; i <= ni; i++) {
+                double ds = dis_point(XH(i, j - 1), RH(i, j - 1), XH(i, j), RH(i, j));
+                SC(SHCFD_F_DSI, i, j) = ax ? (ds * (RH(i, j - 1) + RH(i, j))) / 2.0 : ds;
+            }
         /* set_vn, geometry.f90:141-169 */
         for (int j = 1; j <= nj; j++)
             for (int i = 0; i <= ni; i++) {
@@ -266,7 +280,10 @@ void or_init_geometry(or_ctx *c) {
 }
 
 void or_set_vnc_dsc(or_ctx *c) {
-    /* time_scheme/LU-SGS/2d/sch_lusgs.raw.f90:143-181 */
+    /* time_scheme/LU-SGS/2d/sch_lusgs.raw.f90:143-181; LU-SGS/q2d/
+     * sch_lusgs.raw.f90:151-186 (dsci, dscj times the mean r of the two
+     * mid-face points, after the normals) */
+    const int ax = c->p.dimension == SHCFD_DIM_AXISYMMETRIC;
     for (int b = 0; b < c->nb; b++) {
         int ni = c->ni[b], nj = c->nj[b];
         for (int j = 1; j <= nj; j++)
@@ -277,12 +294,16 @@ void or_set_vnc_dsc(or_ctx *c) {
                 double d1 = (XH(i - 1, j - 1) + XH(i - 1, j)) * 0.5, d2 = (RH(i - 1, j - 1) + RH(i - 1, j)) * 0.5;
                 double dsci = sqrt(((c1 - a1) * (c1 - a1)) + ((c2 - a2) * (c2 - a2)));
                 double dscj = sqrt(((b1 - d1) * (b1 - d1)) + ((b2 - d2) * (b2 - d2)));
-                SC(SHCFD_F_DSCI, i, j) = dsci;
-                SC(SHCFD_F_DSCJ, i, j) = dscj;
                 VEC(SHCFD_F_VNCI, 0, i, j) = (b1 - d1) / dscj;
                 VEC(SHCFD_F_VNCI, 1, i, j) = (b2 - d2) / dscj;
                 VEC(SHCFD_F_VNCJ, 0, i, j) = (c1 - a1) / dsci;
                 VEC(SHCFD_F_VNCJ, 1, i, j) = (c2 - a2) / dsci;
+                if (ax) {
+                    dsci = (dsci * (c2 + a2)) * 0.5;
+                    dscj = (dscj * (b2 + d2)) * 0.5;
+                }
+                SC(SHCFD_F_DSCI, i, j) = dsci;
+                SC(SHCFD_F_DSCJ, i, j) = dscj;
             }
     }
 }
@@ -315,6 +336,20 @@ void or_set_geojac(or_ctx *c) {
                 VEC(SHCFD_F_GEOJACJ, 1, i, j) = -dydxi / det;
                 VEC(SHCFD_F_GEOJACJ, 3, i, j) = dxdxi / det;
             }
+        if (c->p.dimension == SHCFD_DIM_AXISYMMETRIC)
+            /* cell-centred metrics: viscosity/viscous/no-nV/q2d/sch_viscous.f90:10-28 */
+            for (int j = 1; j <= nj; j++)
+                for (int i = 1; i <= ni; i++) {
+                    double dxdxi = (XC(i + 1, j) - XC(i - 1, j)) * 0.5;
+                    double dydxi = (RC(i + 1, j) - RC(i - 1, j)) * 0.5;
+                    double dxdeta = (XC(i, j + 1) - XC(i, j - 1)) * 0.5;
+                    double dydeta = (RC(i, j + 1) - RC(i, j - 1)) * 0.5;
+                    double det = (dxdxi * dydeta) - (dydxi * dxdeta);
+                    VEC(SHCFD_F_GEOJACC, 0, i, j) = dydeta / det;
+                    VEC(SHCFD_F_GEOJACC, 2, i, j) = -dxdeta / det;
+                    VEC(SHCFD_F_GEOJACC, 1, i, j) = -dydxi / det;
+                    VEC(SHCFD_F_GEOJACC, 3, i, j) = dxdxi / det;
+                }
     }
 }
 #undef XH
@@ -816,6 +851,9 @@ void or_set_tgv(or_ctx *c) {
                         hidwdr = hidwdr + (dwdr[k] * lhi[k]);
                     }
                     double divu = dudx + dvdr;
+                    /* q2d: + v / r(i,j), the centre radius of the face's lower-index
+                     * cell (viscous/no-nV/q2d/sch_viscous.f90:173-174, :262-263) */
+                    if (c->p.dimension == SHCFD_DIM_AXISYMMETRIC) divu = divu + (v / SC(SHCFD_F_R, i, j));
                     double t_xx = -(((2.0 / 3.0) * mu) * divu) + ((2.0 * mu) * dudx);
                     double t_rr = -(((2.0 / 3.0) * mu) * divu) + ((2.0 * mu) * dvdr);
                     double t_xr = mu * (dvdx + dudr);
@@ -974,6 +1012,14 @@ void or_set_abpm(or_ctx *c) {
                     /* sch_lusgs.raw.f90:38-41, sch_NR.raw.f90:40-41 */
                     nuA = fabs(unA) + cs;
                     nuB = fabs(unB) + cs;
+                    if (c->p.dimension == SHCFD_DIM_AXISYMMETRIC) {
+                        /* dpdq, kept in pre2: LU-SGS/q2d/sch_lusgs.raw.f90:43-49,
+                         * NR/q2d/sch_NR.raw.f90:43-49 */
+                        for (int k = 0; k < ny; k++) VEC(SHCFD_F_PRE2, k, i, j) = ((0.5 * gmmad) * Vs) + dhi[k];
+                        VEC(SHCFD_F_PRE2, ny, i, j) = -(gmmad * u);
+                        VEC(SHCFD_F_PRE2, ny + 1, i, j) = -(gmmad * v);
+                        VEC(SHCFD_F_PRE2, ny + 2, i, j) = gmmad;
+                    }
                 } else {
                     /* sch_precon.raw.f90:44-59 / sch_dual.raw.f90:44-59 */
                     double dmin = (dsci < dscj) ? dsci : dscj;
@@ -1096,15 +1142,40 @@ void or_set_dqdt(or_ctx *c) {
 /* NR/2d/time_NR_viscous.f90:1-102, precon/2d/time_precon_viscous.f90:1-103, */
 /* dual/2d/time_dual_viscous.f90:1-125                                      */
 /* ======================================================================== */
-/* flux balance of one cell: time_lusgs_viscous.f90:12-22 (Area = 0 in 2-D) */
+/* Sq + Svq of the radial momentum of an axisymmetric cell: the pressure
+ * (dim/q2d/Sq.f90:1-18) minus the hoop stress of viscous flow
+ * (viscosity/viscous/no-nV/q2d/sch_viscous.f90:290-319) */
+static double radial_source(or_ctx *c, int b, int i, int j) {
+    double svq = 0.0;
+    if (c->p.viscous) {
+        double v = W(IW_V, i, j), mu = W(IW_MU(c), i, j), r = SC(SHCFD_F_R, i, j);
+        double dudxi = (W(IW_U, i + 1, j) - W(IW_U, i - 1, j)) * 0.5;
+        double dvdxi = (W(IW_V, i + 1, j) - W(IW_V, i - 1, j)) * 0.5;
+        double dudeta = (W(IW_U, i, j + 1) - W(IW_U, i, j - 1)) * 0.5;
+        double dvdeta = (W(IW_V, i, j + 1) - W(IW_V, i, j - 1)) * 0.5;
+        double dudx = (dudxi * VEC(SHCFD_F_GEOJACC, 0, i, j)) + (dudeta * VEC(SHCFD_F_GEOJACC, 1, i, j));
+        double dvdr = (dvdxi * VEC(SHCFD_F_GEOJACC, 2, i, j)) + (dvdeta * VEC(SHCFD_F_GEOJACC, 3, i, j));
+        double divu = (dudx + dvdr) + (v / r);
+        double t_tt = -(((2.0 / 3.0) * mu) * divu) + (((2.0 * mu) * v) / r);
+        svq = -t_tt;
+    }
+    return W(IW_P, i, j) + svq;
+}
+
+/* flux balance of one cell: time_lusgs_viscous.f90:12-22 (Area = 0 in 2-D);
+ * LU-SGS/q2d/time_lusgs_viscous.f90:15-20: + Area*(Sq+Svq), nonzero on the
+ * radial momentum (component ny+1, 0-based) only */
 static inline void flux_balance(or_ctx *c, int b, int i, int j, double *out) {
     int dq = c->dimq;
+    const int ax = c->p.dimension == SHCFD_DIM_AXISYMMETRIC;
+    const double area = ax ? SC(SHCFD_F_AREA, i, j) : 0.0;
+    const double src = ax ? radial_source(c, b, i, j) : 0.0;
     for (int k = 0; k < dq; k++) {
         double s = ((((SC(SHCFD_F_DSI, i - 1, j) * (VEC(SHCFD_F_TGI, k, i - 1, j) - VEC(SHCFD_F_TGVI, k, i - 1, j))) -
                       (SC(SHCFD_F_DSI, i, j) * (VEC(SHCFD_F_TGI, k, i, j) - VEC(SHCFD_F_TGVI, k, i, j)))) +
                      (SC(SHCFD_F_DSJ, i, j - 1) * (VEC(SHCFD_F_TGJ, k, i, j - 1) - VEC(SHCFD_F_TGVJ, k, i, j - 1)))) -
                     (SC(SHCFD_F_DSJ, i, j) * (VEC(SHCFD_F_TGJ, k, i, j) - VEC(SHCFD_F_TGVJ, k, i, j))));
-        out[k] = s + 0.0; /* + Area*(Sq+Svq), Area = 0 */
+        out[k] = ax ? s + (area * (k == c->ny + 1 ? src : 0.0)) : s + 0.0;
     }
 }
 
@@ -1129,6 +1200,7 @@ int or_calc_next_step_implicit(or_ctx *c, int step_internal, double *res_out, do
     /* inner loop of dual time: dq/dt term, res and the omega-limited update */
     int inner = dual || scheme == SHCFD_SCHEME_NR || scheme == SHCFD_SCHEME_PRECON;
     int prec = scheme == SHCFD_SCHEME_PRECON_LUSGS || scheme == SHCFD_SCHEME_PRECON;
+    const int ax = c->p.dimension == SHCFD_DIM_AXISYMMETRIC;
     double omega = 1e300, res = 0.0;
     int err = 0;
     for (int b = 0; b < c->nb; b++) {
@@ -1180,6 +1252,16 @@ int or_calc_next_step_implicit(or_ctx *c, int step_internal, double *res_out, do
                     double ps4 = VEC(SHCFD_F_PRESCA, 3, i, j), ips1 = 1.0 / VEC(SHCFD_F_PRESCA, 0, i, j);
                     for (int k = 0; k < dq; k++) DQ(k, i, j) = DQ(k, i, j) + (ps4 * d) * VEC(SHCFD_F_PRE1, k, i, j);
                     for (int k = 0; k < dq; k++) DQ(k, i, j) = ips1 * DQ(k, i, j);
+                }
+                if (ax) {
+                    /* implicit pressure source: LU-SGS/q2d/time_lusgs_viscous.f90:38-44,
+                     * NR/q2d/time_NR_viscous.f90:43-49 (dpdq kept in pre2) */
+                    const double *dpdq = &VEC(SHCFD_F_PRE2, 0, i, j);
+                    double tmp = 0.0;
+                    for (int l = 0; l <= ny; l++) tmp = tmp + (DQ(l, i, j) * dpdq[l]);
+                    tmp = tmp + (DQ(ny + 2, i, j) * dpdq[ny + 2]);
+                    tmp = DQ(ny + 1, i, j) + ((al * SC(SHCFD_F_AREA, i, j)) * tmp);
+                    DQ(ny + 1, i, j) = tmp / (1.0 - ((al * SC(SHCFD_F_AREA, i, j)) * dpdq[ny + 1]));
                 }
             }
         }

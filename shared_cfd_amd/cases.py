@@ -30,12 +30,13 @@ SCHEME_EULER, SCHEME_RK2, SCHEME_LUSGS, SCHEME_NR, SCHEME_PRECON, SCHEME_DUAL, S
 LIMITER_REFERENCE, LIMITER_MINMOD = 0, 1
 BC_EXTRAP, BC_REFLECT, BC_NOSLIP, BC_FIXED, BC_INFLOW_TUV, BC_OUTFLOW_P, BC_SLIP_NORMAL = range(7)
 SIDE_IMIN, SIDE_IMAX, SIDE_JMIN, SIDE_JMAX = range(4)
+DIM_PLANAR, DIM_AXISYMMETRIC = 0, 1
 
 FIELDS = [
     "Q", "QP", "QPP", "W", "VHI", "DHI", "WHLI", "WHRI", "WHLJ", "WHRJ",
     "TGI", "TGJ", "TGVI", "TGVJ", "VOL", "DSI", "DSJ", "VNI", "VNJ", "DSCI", "DSCJ",
     "VNCI", "VNCJ", "GEOJACI", "GEOJACJ", "X", "R", "XH", "RH", "DT", "ALPHA", "PHI",
-    "PRESCA", "PRE1", "PRE2", "DQDT", "AP", "AM", "BP", "BM", "DQ",
+    "PRESCA", "PRE1", "PRE2", "DQDT", "AP", "AM", "BP", "BM", "DQ", "AREA", "GEOJACC",
 ]
 F = {name: idx for idx, name in enumerate(FIELDS)}
 
@@ -51,7 +52,7 @@ def field_ncomp(name: str, ny: int) -> int:
         return ny
     if name in ("VNI", "VNJ", "VNCI", "VNCJ"):
         return 2
-    if name in ("GEOJACI", "GEOJACJ", "PRESCA"):
+    if name in ("GEOJACI", "GEOJACJ", "GEOJACC", "PRESCA"):
         return 4
     if name in ("AP", "AM", "BP", "BM"):
         return dimq * dimq
@@ -69,7 +70,7 @@ class Params(ctypes.Structure):
         ("cfl_tau", ctypes.c_double), ("omega_max", ctypes.c_double), ("omega_min", ctypes.c_double),
         ("dqmax", ctypes.c_double), ("res_rate_warning", ctypes.c_double),
         ("res_rate_error", ctypes.c_double), ("kappa", ctypes.c_double), ("rgas", ctypes.c_double),
-        ("nu", ctypes.c_double),
+        ("nu", ctypes.c_double), ("dimension", ctypes.c_int),
     ]
 
 
@@ -77,7 +78,7 @@ def make_params(**kw) -> Params:
     d = dict(scheme=SCHEME_LUSGS, time_step=0, high_order=1, limiter=LIMITER_REFERENCE, viscous=0,
              thermal=0, num_internal_loop=21, reaction=0, phi_mus=1.0, k_mus=-1.0, cfl=0.5, cfl_tau=1e-2,
              omega_max=0.1, omega_min=1e-3, dqmax=1e-2, res_rate_warning=1e-5, res_rate_error=1e-3,
-             kappa=1.4, rgas=287.0, nu=1.6e-5)
+             kappa=1.4, rgas=287.0, nu=1.6e-5, dimension=DIM_PLANAR)
     d.update(kw)
     p = Params()
     for k, v in d.items():
@@ -252,7 +253,7 @@ def write_case(case: "Case", path: str, max_step: int, out_period: int = 1, min_
         vals = [p.phi_mus, p.k_mus, p.cfl, p.cfl_tau, p.omega_max, p.omega_min, p.dqmax,
                 min(p.res_rate_warning, 1e300), min(p.res_rate_error, 1e300), p.kappa, p.rgas, p.nu]
         f.write(" ".join(repr(float(v)) for v in vals) + "\n")
-        f.write(f"{p.reaction} {int(case.w0 is not None)}\n")
+        f.write(f"{p.reaction} {int(case.w0 is not None)} {p.dimension}\n")
         if p.thermal == 2:
             if mech_files is None:
                 raise ValueError("write_case: the Chemkin thermal model needs mech_files")
@@ -379,9 +380,12 @@ def split_case_i(case: "Case", k: int, orient: str = "ii") -> "Case":
 
 # ---- random smooth states on skewed grids (parity tests) -----------------------------
 def random_case(seed=0, ni=(13,), nj=(9,), ny=1, scheme=SCHEME_LUSGS, viscous=0, limiter=0,
-                time_step=0, k_mus=1.0 / 3.0, high_order=1, mach=0.5, with_cut=False) -> Case:
+                time_step=0, k_mus=1.0 / 3.0, high_order=1, mach=0.5, with_cut=False,
+                dimension=DIM_PLANAR) -> Case:
     """Smooth random perturbations of a uniform flow on skewed curvilinear
-    blocks; all sides zero-gradient except a FIXED inflow and a slip wall."""
+    blocks; all sides zero-gradient except a FIXED inflow and a slip wall.
+    Axisymmetric (``dimension=DIM_AXISYMMETRIC``): the grid is lifted off the
+    axis to r >= 0.3 (y is the radius)."""
     rng = np.random.default_rng(seed)
     kappa, rgas = 1.4, 287.0
     xhs, rhs, qs = [], [], []
@@ -391,6 +395,8 @@ def random_case(seed=0, ni=(13,), nj=(9,), ny=1, scheme=SCHEME_LUSGS, viscous=0,
         t = np.linspace(0, 1, n_j + 1)[None, :]
         xh = x_off + s * 1.0 + 0.05 * np.sin(2 * np.pi * t) * (1 - s) * 0.5 + 0.02 * rng.standard_normal((n_i + 1, n_j + 1)) / max(n_i, n_j)
         rh = t * 0.8 + 0.04 * np.sin(np.pi * s) + 0.02 * rng.standard_normal((n_i + 1, n_j + 1)) / max(n_i, n_j)
+        if dimension == DIM_AXISYMMETRIC:
+            rh = rh + 0.3
         if with_cut and b > 0:
             xh[0, :] = xhs[-1][-1, :]
             rh[0, :] = rhs[-1][-1, :]
@@ -434,7 +440,7 @@ def random_case(seed=0, ni=(13,), nj=(9,), ny=1, scheme=SCHEME_LUSGS, viscous=0,
     params = make_params(scheme=scheme, time_step=time_step, high_order=high_order, limiter=limiter,
                          viscous=viscous, cfl=0.8, k_mus=k_mus, kappa=kappa, rgas=rgas, nu=1.6e-5 * 1e3,
                          num_internal_loop=4, cfl_tau=2.0, omega_max=1.0, omega_min=1e-6, dqmax=0.5,
-                         res_rate_warning=1e300, res_rate_error=1e300)
+                         res_rate_warning=1e300, res_rate_error=1e300, dimension=dimension)
     return Case(f"random{seed}", list(ni), list(nj), ny, params, xhs, rhs, qs,
                 np.array(rows, dtype=np.int32), pool,
                 np.array(cut, dtype=np.int32).reshape(-1, 11), description="random smooth test state")

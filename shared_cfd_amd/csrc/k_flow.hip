@@ -449,6 +449,18 @@ __device__ inline void set_abpm_cell(const DevCtx& d, const DevBlock& B, int i, 
         }
     }
     FLD(d, SHCFD_F_ALPHA, 0, c) = J.alpha;
+    if (d.p.dimension == SHCFD_DIM_AXISYMMETRIC) {
+        // dp/dq of the implicit pressure source, kept in pre2 (unused by the
+        // axisymmetric schemes): LU-SGS/q2d/sch_lusgs.raw.f90:43-49,
+        // NR/q2d/sch_NR.raw.f90:43-49
+        const double u = FLD(d, SHCFD_F_W, IW_U, c), v = FLD(d, SHCFD_F_W, IW_V, c);
+        const double gmmad = FLD(d, SHCFD_F_W, dq + 1, c) - 1.0;
+        const double Vs = (u * u) + (v * v);
+        for (int k = 0; k < NY; k++) FLD(d, SHCFD_F_PRE2, k, c) = ((0.5 * gmmad) * Vs) + FLD(d, SHCFD_F_DHI, k, c);
+        FLD(d, SHCFD_F_PRE2, NY, c) = -(gmmad * u);
+        FLD(d, SHCFD_F_PRE2, NY + 1, c) = -(gmmad * v);
+        FLD(d, SHCFD_F_PRE2, NY + 2, c) = gmmad;
+    }
     if (J.precond) {
         FLD(d, SHCFD_F_PHI, 0, c) = J.phi;
         for (int k = 0; k < dq; k++) {

@@ -2,6 +2,15 @@
 // dim/2d/geometry.f90:79-169 (centres, Vol, dsi/dsj, vni/vnj),
 // LU-SGS/2d/sch_lusgs.raw.f90:143-181 (dsci/dscj, vnci/vncj),
 // viscosity/viscous/no-nV/2d/sch_viscous.f90:1-54 (geojaci/geojacj).
+// Axisymmetric grids (dim/q2d/geometry.f90:107-141, LU-SGS/q2d/
+// sch_lusgs.raw.f90:151-186, viscous/no-nV/q2d/sch_viscous.f90:1-29): every
+// volume and face measure is per radian about the x axis -- Vol is the sum
+// over the cell's two triangles of the triangle area times its mean vertex r
+// (exact for the revolved triangle), dsi/dsj are face lengths times the mean
+// r of their two nodes, dsci/dscj carry the mean r of the two mid-face points
+// they join, Area keeps the planar cell area (weight of the pressure and hoop
+// stress sources) and geojacc holds the cell-centred metrics of the hoop
+// stress.
 #include "shcfd_device.h"
 #include "shcfd_kernels.h"
 
@@ -25,27 +34,40 @@ __global__ void __launch_bounds__(256) k_geom_a(DevCtx d, const Tile* tiles) {
     const DevBlock B = d.blk[t.b];
     const int i = t.i, j = t.j, ni = B.ni, nj = B.nj;
     const long long c = cidx(B, i, j);
+    const bool ax = d.p.dimension == SHCFD_DIM_AXISYMMETRIC;
     if (i >= 1 && i <= ni && j >= 1 && j <= nj) {
         XC(i, j) = 0.25 * (((XH(i - 1, j - 1) + XH(i, j - 1)) + XH(i - 1, j)) + XH(i, j));
         RC(i, j) = 0.25 * (((RH(i - 1, j - 1) + RH(i, j - 1)) + RH(i - 1, j)) + RH(i, j));
         double aabd = area_triangle(XH(i - 1, j - 1), RH(i - 1, j - 1), XH(i, j - 1), RH(i, j - 1), XH(i - 1, j), RH(i - 1, j));
         double abcd = area_triangle(XH(i, j - 1), RH(i, j - 1), XH(i, j), RH(i, j), XH(i - 1, j), RH(i - 1, j));
-        FLD(d, SHCFD_F_VOL, 0, c) = aabd + abcd;
+        if (ax) {
+            FLD(d, SHCFD_F_AREA, 0, c) = aabd + abcd;
+            FLD(d, SHCFD_F_VOL, 0, c) = ((((RH(i - 1, j - 1) + RH(i, j - 1)) + RH(i - 1, j)) * aabd) / 3.0) +
+                                        ((((RH(i, j - 1) + RH(i, j)) + RH(i - 1, j)) * abcd) / 3.0);
+        } else {
+            FLD(d, SHCFD_F_AREA, 0, c) = 0.0;
+            FLD(d, SHCFD_F_VOL, 0, c) = aabd + abcd;
+        }
         double a1 = (XH(i - 1, j - 1) + XH(i, j - 1)) * 0.5, a2 = (RH(i - 1, j - 1) + RH(i, j - 1)) * 0.5;
         double b1 = (XH(i, j - 1) + XH(i, j)) * 0.5, b2 = (RH(i, j - 1) + RH(i, j)) * 0.5;
         double c1 = (XH(i - 1, j) + XH(i, j)) * 0.5, c2 = (RH(i - 1, j) + RH(i, j)) * 0.5;
         double d1 = (XH(i - 1, j - 1) + XH(i - 1, j)) * 0.5, d2 = (RH(i - 1, j - 1) + RH(i - 1, j)) * 0.5;
         double dsci = sqrt(((c1 - a1) * (c1 - a1)) + ((c2 - a2) * (c2 - a2)));
         double dscj = sqrt(((b1 - d1) * (b1 - d1)) + ((b2 - d2) * (b2 - d2)));
-        FLD(d, SHCFD_F_DSCI, 0, c) = dsci;
-        FLD(d, SHCFD_F_DSCJ, 0, c) = dscj;
         FLD(d, SHCFD_F_VNCI, 0, c) = (b1 - d1) / dscj;
         FLD(d, SHCFD_F_VNCI, 1, c) = (b2 - d2) / dscj;
         FLD(d, SHCFD_F_VNCJ, 0, c) = (c1 - a1) / dsci;
         FLD(d, SHCFD_F_VNCJ, 1, c) = (c2 - a2) / dsci;
+        if (ax) {
+            dsci = (dsci * (c2 + a2)) * 0.5;
+            dscj = (dscj * (b2 + d2)) * 0.5;
+        }
+        FLD(d, SHCFD_F_DSCI, 0, c) = dsci;
+        FLD(d, SHCFD_F_DSCJ, 0, c) = dscj;
     }
     if (i >= 1 && i <= ni && j <= nj) {  // j-face (i, j), j in [0, nj]
-        FLD(d, SHCFD_F_DSJ, 0, c) = dis_point(XH(i - 1, j), RH(i - 1, j), XH(i, j), RH(i, j));
+        double ds = dis_point(XH(i - 1, j), RH(i - 1, j), XH(i, j), RH(i, j));
+        FLD(d, SHCFD_F_DSJ, 0, c) = ax ? (ds * (RH(i - 1, j) + RH(i, j))) / 2.0 : ds;
         double nx = -(RH(i, j) - RH(i - 1, j));
         double nr = XH(i, j) - XH(i - 1, j);
         double sd = sqrt((nx * nx) + (nr * nr));
@@ -53,7 +75,8 @@ __global__ void __launch_bounds__(256) k_geom_a(DevCtx d, const Tile* tiles) {
         FLD(d, SHCFD_F_VNJ, 1, c) = nr / sd;
     }
     if (i <= ni && j >= 1 && j <= nj) {  // i-face (i, j), i in [0, ni]
-        FLD(d, SHCFD_F_DSI, 0, c) = dis_point(XH(i, j - 1), RH(i, j - 1), XH(i, j), RH(i, j));
+        double ds = dis_point(XH(i, j - 1), RH(i, j - 1), XH(i, j), RH(i, j));
+        FLD(d, SHCFD_F_DSI, 0, c) = ax ? (ds * (RH(i, j - 1) + RH(i, j))) / 2.0 : ds;
         double nx = RH(i, j) - RH(i, j - 1);
         double nr = -(XH(i, j) - XH(i, j - 1));
         double sd = sqrt((nx * nx) + (nr * nr));
@@ -111,6 +134,18 @@ __global__ void __launch_bounds__(256) k_geojac(DevCtx d, const Tile* tiles) {
         FLD(d, SHCFD_F_GEOJACJ, 2, c) = -dxdeta / det;
         FLD(d, SHCFD_F_GEOJACJ, 1, c) = -dydxi / det;
         FLD(d, SHCFD_F_GEOJACJ, 3, c) = dxdxi / det;
+    }
+    if (d.p.dimension == SHCFD_DIM_AXISYMMETRIC && i >= 1 && i <= ni && j >= 1 && j <= nj) {
+        // cell-centred metrics of the hoop stress (viscous/no-nV/q2d/sch_viscous.f90:10-28)
+        double dxdxi = (XC(i + 1, j) - XC(i - 1, j)) * 0.5;
+        double dydxi = (RC(i + 1, j) - RC(i - 1, j)) * 0.5;
+        double dxdeta = (XC(i, j + 1) - XC(i, j - 1)) * 0.5;
+        double dydeta = (RC(i, j + 1) - RC(i, j - 1)) * 0.5;
+        double det = (dxdxi * dydeta) - (dydxi * dxdeta);
+        FLD(d, SHCFD_F_GEOJACC, 0, c) = dydeta / det;
+        FLD(d, SHCFD_F_GEOJACC, 2, c) = -dxdeta / det;
+        FLD(d, SHCFD_F_GEOJACC, 1, c) = -dydxi / det;
+        FLD(d, SHCFD_F_GEOJACC, 3, c) = dxdxi / det;
     }
 }
 

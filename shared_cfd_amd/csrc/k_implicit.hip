@@ -135,6 +135,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
     const int j = S.j0 + lane;
     const bool rowok = j <= nj;
     const bool dual = d.p.scheme == SHCFD_SCHEME_DUAL;
+    const bool ax = d.p.dimension == SHCFD_DIM_AXISYMMETRIC;
     const bool has_prev = S.prev >= 0, has_next = S.next >= 0;
     double a_in[dq], bo[dq];
 #pragma unroll
@@ -174,6 +175,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
                 for (int k = 0; k < dq; k++) x[k] = x[k] + (ps4 * dd) * FLD(d, SHCFD_F_PRE1, k, c);
 #pragma unroll
                 for (int k = 0; k < dq; k++) x[k] = ips1 * x[k];
+            }
+            if (ax) {
+                // implicit pressure source on the radial momentum:
+                // LU-SGS/q2d/time_lusgs_viscous.f90:38-44, NR/q2d/time_NR_viscous.f90:43-49
+                double tmp = 0.0;
+#pragma unroll
+                for (int l = 0; l <= NY; l++) tmp = tmp + (x[l] * FLD(d, SHCFD_F_PRE2, l, c));
+                tmp = tmp + (x[NY + 2] * FLD(d, SHCFD_F_PRE2, NY + 2, c));
+                const double aa = al * FLD(d, SHCFD_F_AREA, 0, c);
+                tmp = x[NY + 1] + (aa * tmp);
+                x[NY + 1] = tmp / (1.0 - (aa * FLD(d, SHCFD_F_PRE2, NY + 1, c)));
             }
 #pragma unroll
             for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = x[k];

@@ -92,6 +92,7 @@ namespace {
 
 int supported(shcfd_ctx* ctx) {
     if (ctx->ny != 1 || ctx->p.viscous || ctx->p.thermal != SHCFD_THERMAL_IDEAL ||
+        ctx->p.dimension != SHCFD_DIM_PLANAR ||
         (ctx->p.scheme != SHCFD_SCHEME_LUSGS && ctx->p.scheme != SHCFD_SCHEME_NR &&
          ctx->p.scheme != SHCFD_SCHEME_DUAL))
         return SHCFD_E_STATE;

@@ -29,7 +29,7 @@ int field_ncomp(int fld, int ny) {
         return ny;
     case SHCFD_F_VNI: case SHCFD_F_VNJ: case SHCFD_F_VNCI: case SHCFD_F_VNCJ:
         return 2;
-    case SHCFD_F_GEOJACI: case SHCFD_F_GEOJACJ: case SHCFD_F_PRESCA:
+    case SHCFD_F_GEOJACI: case SHCFD_F_GEOJACJ: case SHCFD_F_GEOJACC: case SHCFD_F_PRESCA:
         return 4;
     case SHCFD_F_AP: case SHCFD_F_AM: case SHCFD_F_BP: case SHCFD_F_BM:
         return dimq * dimq;
@@ -113,8 +113,19 @@ int shcfd_version(void) { return 100; }
 // the scheme / time-step pairs the reference's checkout accepts
 // (store/checkout/checkout_flow.py:120-185): precon and preconLU-SGS need
 // local time steps, dual time a global one
+// checkout_flow.py:106-185: the choices the reference accepts.  Axisymmetric
+// cases run the explicit schemes, LU-SGS and the NR inner loop
+// (time_scheme/{LU-SGS,NR}/q2d); dual time is refused there as in the
+// reference, and so are the two preconditioned schemes, whose q2d sweeps
+// assign to the whole Dq array through a case-insensitive name clash and then
+// read an unset pressure increment (preconLU-SGS/q2d/time_precon_viscous.f90
+// and precon/q2d/time_precon_viscous.f90, forward sweep).
 static bool params_valid(const shcfd_params* p) {
     if (p->time_step != 0 && p->time_step != 1) return false;
+    if (p->dimension != SHCFD_DIM_PLANAR && p->dimension != SHCFD_DIM_AXISYMMETRIC) return false;
+    if (p->dimension == SHCFD_DIM_AXISYMMETRIC &&
+        (p->scheme == SHCFD_SCHEME_DUAL || p->scheme == SHCFD_SCHEME_PRECON || p->scheme == SHCFD_SCHEME_PRECON_LUSGS))
+        return false;
     switch (p->scheme) {
     case SHCFD_SCHEME_EULER:
     case SHCFD_SCHEME_RK2:
@@ -225,6 +236,7 @@ int shcfd_destroy(shcfd_ctx* ctx) {
 
 int shcfd_set_params(shcfd_ctx* ctx, const shcfd_params* params) {
     if (!ctx || !params || !params_valid(params)) return SHCFD_E_ARG;
+    if (params->dimension != ctx->p.dimension) return SHCFD_E_STATE;  // the metrics belong to the geometry
     ctx->p = *params;
     return SHCFD_OK;
 }
@@ -674,7 +686,7 @@ int shcfd_set_abpm(shcfd_ctx* ctx) {
 // the strip-skewed Jacobian copy is used inside whole steps for nY > 1
 // (where the optimised dimq = 4 sweeps do not apply); allocated on first use
 static bool skew_enabled(shcfd_ctx* ctx) {
-    if (!ctx->use_skew || ctx->ny == 1) return false;
+    if (!ctx->use_skew || ctx->ny == 1 || ctx->p.dimension != SHCFD_DIM_PLANAR) return false;
     if (ctx->dskew) return true;
     const int ns = (int)ctx->hstrips.size();
     std::vector<long long> sb(ns);
@@ -703,7 +715,8 @@ static bool skew_enabled(shcfd_ctx* ctx) {
 // implicit scheme but dual time, whose S scaling stays on the row-per-lane
 // sweeps); allocated on first use
 static bool q16_enabled(shcfd_ctx* ctx) {
-    if (!ctx->use_skew || ctx->ny == 1 || ctx->p.scheme == SHCFD_SCHEME_DUAL) return false;
+    if (!ctx->use_skew || ctx->ny == 1 || ctx->p.scheme == SHCFD_SCHEME_DUAL || ctx->p.dimension != SHCFD_DIM_PLANAR)
+        return false;
     if (ctx->djac16) return true;
     const int dq = ctx->dimq, C = (dq + 3) / 4;
     std::vector<Strip16> st;
@@ -815,7 +828,7 @@ static void prof_end(Ctx* c, int id) {
 }
 
 static bool fast_enabled(shcfd_ctx* ctx) {
-    if (!ctx->use_fast || ctx->ny != 1) return false;
+    if (!ctx->use_fast || ctx->ny != 1 || ctx->p.dimension != SHCFD_DIM_PLANAR) return false;
     if (!ctx->fast) {
         ctx->fast = new FastPath();
         if (ctx->fast->init(*ctx) != SHCFD_OK) {

@@ -85,7 +85,10 @@ __device__ inline void slau_face(const double* wl, const double* wr, double n1, 
 }
 
 // viscous flux of one face: viscosity/viscous/no-nV/2d/sch_viscous.f90:106-178
-// (DIR 0: i-face (i,j) between i and i+1; DIR 1: j-face between j and j+1)
+// (DIR 0: i-face (i,j) between i and i+1; DIR 1: j-face between j and j+1).
+// Axisymmetric: the dilatation gains v/r with the face-averaged v over the
+// centre radius of cell (i,j), the lower-index cell of the face, as
+// viscous/no-nV/q2d/sch_viscous.f90:173-174 and :262-263 write it.
 template <int NY, int DIR>
 __device__ inline void viscous_face(const DevCtx& d, const DevBlock& B, int i, int j, double* out) {
     constexpr int dq = NY + 3, ig = dq + 1, ir = dq + 3, imu = dq + 4;
@@ -157,6 +160,7 @@ __device__ inline void viscous_face(const DevCtx& d, const DevBlock& B, int i, i
         hidwdr = hidwdr + (dwdr[k] * lhi[k]);
     }
     double divu = dudx + dvdr;
+    if (d.p.dimension == SHCFD_DIM_AXISYMMETRIC) divu = divu + (v / FLD(d, SHCFD_F_R, 0, c));
     double t_xx = -(((2.0 / 3.0) * mu) * divu) + ((2.0 * mu) * dudx);
     double t_rr = -(((2.0 / 3.0) * mu) * divu) + ((2.0 * mu) * dvdr);
     double t_xr = mu * (dvdx + dudr);
@@ -171,19 +175,48 @@ __device__ inline void viscous_face(const DevCtx& d, const DevBlock& B, int i, i
     out[NY + 2] = ((((u * t_xx) + (v * t_xr)) - qx) * n1) + ((((u * t_xr) + (v * t_rr)) - qr) * n2);
 }
 
-// flux balance of one cell: LU-SGS/2d/time_lusgs_viscous.f90:12-22 (Area=0)
+// radial-momentum source Sq + Svq of an axisymmetric cell: the pressure
+// (dim/q2d/Sq.f90:1-18) minus the hoop stress t_tt of viscous flow
+// (viscosity/viscous/no-nV/q2d/sch_viscous.f90:290-319; 0 when inviscid)
+template <int NY>
+__device__ inline double radial_source(const DevCtx& d, const DevBlock& B, long long c) {
+    constexpr int imu = NY + 3 + 4;
+    const long long P = B.pitch;
+    auto Wf = [&](int k, long long cc) { return FLD(d, SHCFD_F_W, k, cc); };
+    double svq = 0.0;
+    if (d.p.viscous) {
+        const double v = Wf(IW_V, c), mu = Wf(imu, c), r = FLD(d, SHCFD_F_R, 0, c);
+        double dudxi = (Wf(IW_U, c + 1) - Wf(IW_U, c - 1)) * 0.5;
+        double dvdxi = (Wf(IW_V, c + 1) - Wf(IW_V, c - 1)) * 0.5;
+        double dudeta = (Wf(IW_U, c + P) - Wf(IW_U, c - P)) * 0.5;
+        double dvdeta = (Wf(IW_V, c + P) - Wf(IW_V, c - P)) * 0.5;
+        double dudx = (dudxi * FLD(d, SHCFD_F_GEOJACC, 0, c)) + (dudeta * FLD(d, SHCFD_F_GEOJACC, 1, c));
+        double dvdr = (dvdxi * FLD(d, SHCFD_F_GEOJACC, 2, c)) + (dvdeta * FLD(d, SHCFD_F_GEOJACC, 3, c));
+        double divu = (dudx + dvdr) + (v / r);
+        double t_tt = -(((2.0 / 3.0) * mu) * divu) + (((2.0 * mu) * v) / r);
+        svq = -t_tt;
+    }
+    return Wf(IW_P, c) + svq;
+}
+
+// flux balance of one cell: LU-SGS/2d/time_lusgs_viscous.f90:12-22 (Area = 0
+// in 2-D), LU-SGS/q2d/time_lusgs_viscous.f90:15-20 (Area*(Sq+Svq), nonzero on
+// the radial momentum only)
 template <int NY>
 __device__ inline void flux_balance(const DevCtx& d, const DevBlock& B, long long c, double* s) {
     const long long P = B.pitch;
     double di0 = FLD(d, SHCFD_F_DSI, 0, c - 1), di1 = FLD(d, SHCFD_F_DSI, 0, c);
     double dj0 = FLD(d, SHCFD_F_DSJ, 0, c - P), dj1 = FLD(d, SHCFD_F_DSJ, 0, c);
+    const bool ax = d.p.dimension == SHCFD_DIM_AXISYMMETRIC;
+    const double area = ax ? FLD(d, SHCFD_F_AREA, 0, c) : 0.0;
+    const double src = ax ? radial_source<NY>(d, B, c) : 0.0;
 #pragma unroll
     for (int k = 0; k < NY + 3; k++) {
         double v = ((((di0 * (FLD(d, SHCFD_F_TGI, k, c - 1) - FLD(d, SHCFD_F_TGVI, k, c - 1))) -
                       (di1 * (FLD(d, SHCFD_F_TGI, k, c) - FLD(d, SHCFD_F_TGVI, k, c)))) +
                      (dj0 * (FLD(d, SHCFD_F_TGJ, k, c - P) - FLD(d, SHCFD_F_TGVJ, k, c - P)))) -
                     (dj1 * (FLD(d, SHCFD_F_TGJ, k, c) - FLD(d, SHCFD_F_TGVJ, k, c))));
-        s[k] = v + 0.0;
+        s[k] = ax ? v + (area * (k == NY + 1 ? src : 0.0)) : v + 0.0;
     }
 }
 

@@ -33,7 +33,7 @@ program main
                prm%num_internal_loop, fast
    read(10, *) prm%phi_mus, prm%k_mus, prm%cfl, prm%cfl_tau, prm%omega_max, prm%omega_min, prm%dqmax, &
                prm%res_rate_warning, prm%res_rate_error, prm%kappa, prm%rgas, prm%nu
-   read(10, *) prm%reaction, has_w
+   read(10, *) prm%reaction, has_w, prm%dimension
    if (prm%thermal == 2) then
       read(10, '(a)') chem_inp
       read(10, '(a)') mw_inp

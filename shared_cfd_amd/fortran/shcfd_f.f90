@@ -18,6 +18,7 @@ module shcfd_f
       integer(c_int) :: num_internal_loop, reaction
       real(c_double) :: phi_mus, k_mus, cfl, cfl_tau, omega_max, omega_min, dqmax
       real(c_double) :: res_rate_warning, res_rate_error, kappa, rgas, nu
+      integer(c_int) :: dimension   ! 0 two-dimensional, 1 axi-symmetric (checkout.inp "dimension")
    end type shcfd_params_t
 
    ! field identifiers (shcfd_types.h)
