@@ -141,6 +141,25 @@ int shcfd_set_mechanism(shcfd_ctx *ctx, const shcfd_mech *mech);
  * (the Fortran driver). */
 int shcfd_load_mechanism(shcfd_ctx *ctx, const char *chem_inp, const char *mw_inp, const char *trans_inp, double rtol,
                          double atol);
+
+/* ---- cold flow with NASA-9 thermodynamics (thermal model 5) ---------------
+ * The reference's init_pack_cold_flow (store/therm_lib/NASA/core/
+ * sub_chem.f90:380-387): read_cheminp (:2-67) of chem.inp, set_therm_data
+ * (:68-166) of the CEA thermo.inp, set_trans_data (:167-245) of the CEA
+ * trans.inp (optional: NULL gives mu = 0) and read_fo_composition (:247-300)
+ * of control_chem.inp, with the stream mole ratios normalised to mol/kg
+ * (calc_ini, :420-441).  pt (may be NULL) receives the stream conditions
+ * (po, To, pf, Tf).  Host code; on error a message goes to errbuf. */
+int shcfd_nasa9_parse(const char *chem_inp, const char *thermo_inp, const char *trans_inp,
+                      const char *control_chem_inp, shcfd_nasa9 *out, double *pt, char *errbuf, int errlen);
+/* make tables the context's cold-flow data; the context must have nY = 2
+ * and thermal = SHCFD_THERMAL_COLD_NASA9.  set_thermo_prop then runs
+ * store/therm_lib/NASA/cold/thermal_model.f90:1-26 per cell. */
+int shcfd_set_nasa9(shcfd_ctx *ctx, const shcfd_nasa9 *tables);
+/* the static state of the fuel (which = 0) or oxidizer (which = 1) stream at
+ * (p, T): initialize_cold_flow / set_static_qw (sub_chem.f90:388-488):
+ * q(1:5), w(1:10), vhi(1:2).  Host code. */
+int shcfd_nasa9_stream(const shcfd_nasa9 *tables, int which, double p, double T, double *q, double *w, double *vhi);
 /* proceed_reaction: store/therm_lib/chemkin/flow/reactive/thermal_model.f90:45-73
  * (reaction_chemeq2 per cell over dt_grbl; or the point-implicit update,
  * params.reaction) */

@@ -139,8 +139,35 @@ enum {
 /* ---- thermal models ("thermal model" of checkout.inp) -------------------- */
 enum {
     SHCFD_THERMAL_IDEAL = 0,    /* store/therm_lib/ideal                        */
-    SHCFD_THERMAL_CHEMKIN = 2   /* store/therm_lib/chemkin (NASA-7, reactive)   */
+    SHCFD_THERMAL_CHEMKIN = 2,  /* store/therm_lib/chemkin (NASA-7, reactive)   */
+    SHCFD_THERMAL_COLD_NASA9 = 5 /* store/therm_lib/NASA/cold: frozen mixing of a
+                                   fuel and an oxidizer stream (nY = 2), NASA-9 */
 };
+
+/* ---- NASA-9 tables of the cold-flow model ----------------------------------
+ * What read_cheminp / set_therm_data / set_trans_data / read_fo_composition
+ * produce for it (store/therm_lib/NASA/core/sub_chem.f90:1-300, module chem
+ * of mod_chem.raw.f90:17-40), 0-based: per species up to 6 temperature
+ * sections of the CEA thermo.inp polynomials (co[0..6] = a1..a7, co[7] = b1,
+ * co[8] = b2), the CEA trans.inp viscosity fits of the species that have one
+ * (ln mu[micropoise] = A ln T + B/T + C/T^2 + D, in trans.inp order, tr2th =
+ * species index) and the two streams in mol/kg. */
+#define SHCFD_NASA9_MAX_NS 8
+#define SHCFD_NASA9_MAX_SEC 6
+#define SHCFD_NASA9_MAX_TSEC 3
+typedef struct shcfd_nasa9 {
+    int ns, nt;                                               /* species, transport entries          */
+    int nsec[SHCFD_NASA9_MAX_NS];                             /* num_sctn                            */
+    int nsec_trans[SHCFD_NASA9_MAX_NS];                       /* num_sctn_trans per transport entry   */
+    int tr2th[SHCFD_NASA9_MAX_NS];                            /* transport entry -> species           */
+    double mw[SHCFD_NASA9_MAX_NS];                            /* g/mol                               */
+    double trange[SHCFD_NASA9_MAX_NS][SHCFD_NASA9_MAX_SEC][2];
+    double co[SHCFD_NASA9_MAX_NS][SHCFD_NASA9_MAX_SEC][9];
+    double trange_trans[SHCFD_NASA9_MAX_NS][SHCFD_NASA9_MAX_TSEC][2];
+    double trans[SHCFD_NASA9_MAX_NS][SHCFD_NASA9_MAX_TSEC][4];
+    double nf[SHCFD_NASA9_MAX_NS];                            /* fuel stream, mol/kg                 */
+    double no[SHCFD_NASA9_MAX_NS];                            /* oxidizer stream, mol/kg             */
+} shcfd_nasa9;
 /* ---- chemistry source per physical step (the point-implicit slot of
  * store/time_scheme/{LU-SGS/2d,RK2}/main.main.point_implicit.f90) --------- */
 enum {

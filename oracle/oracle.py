@@ -75,6 +75,8 @@ def lib():
         L.or_chem_rho_rel2abs.argtypes = [mp, ctypes.c_double, ctypes.c_double, dp, dp, dp]
         L.or_chem_rho_rel2abs.restype = None
         L.or_chem_calc_t.argtypes = [mp, dp, ctypes.c_double, ctypes.c_double, dp, dp]
+        L.or_cold_flow.argtypes = [mp, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp]
+        L.or_set_nasa9.argtypes = [vp, mp]
         L.or_chem_chemeq2.argtypes = [mp, dp, ctypes.c_double, ctypes.c_double, ip]
         L.or_chem_reaction_chemeq2.argtypes = [mp, ctypes.c_double, dp, ctypes.c_double, ip]
         L.or_chem_point_implicit.argtypes = [mp, ctypes.c_double, dp, ctypes.c_double, ip, dp]
@@ -134,6 +136,11 @@ class Oracle:
             self.mech = OrMech.from_buffer_copy(bytes(case.mech))
             if L.or_set_mechanism(self.ctx, ctypes.byref(self.mech)) != 0:
                 raise RuntimeError("or_set_mechanism failed")
+        self.nasa9 = None
+        if getattr(case, "nasa9", None) is not None:
+            self.nasa9 = type(case.nasa9).from_buffer_copy(bytes(case.nasa9))  # the oracle's own copy
+            if L.or_set_nasa9(self.ctx, ctypes.byref(self.nasa9)) != 0:
+                raise RuntimeError("or_set_nasa9 failed")
         if getattr(case, "w0", None) is not None:
             for b in range(case.nblocks):
                 self.field("W", b)[...] = case.w0[b]
@@ -209,6 +216,15 @@ def chem_rho_rel2abs(m, p, T, vrho_rel):
     rho, E = ctypes.c_double(), ctypes.c_double()
     lib().or_chem_rho_rel2abs(ctypes.byref(m), p, T, _dp(v), ctypes.byref(rho), ctypes.byref(E))
     return v, rho.value, E.value
+
+
+def cold_flow(t, y0, y1, E, T0):
+    """(T, out(7) = MWave, kappa, mu, DHi(2), vhi(2), iterations) of the
+    oracle's cold_flow restatement."""
+    T = ctypes.c_double(T0)
+    out = np.zeros(7)
+    it = lib().or_cold_flow(ctypes.byref(t), y0, y1, E, ctypes.byref(T), _dp(out))
+    return T.value, out, it
 
 
 def chem_calc_t(m, vrho, rho, E, T0):

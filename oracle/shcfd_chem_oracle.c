@@ -765,3 +765,138 @@ int or_chem_point_implicit(const shcfd_mech *m, double T, double *vrho, double d
     if (T_out) *T_out = y[ns];
     return 0;
 }
+
+/* ======================================================================== */
+/* NASA-9 frozen mixing of two streams: cold_flow,                          */
+/* store/therm_lib/NASA/core/sub_chem.f90:489-610, with check_section_number */
+/* (func_chem.f90:41-62), calc_vThrt (:64-90), calc_vTcpr (:91-116),         */
+/* calc_vTmu (:143-157) and check_section_number_trans (:173-192).           */
+/* ======================================================================== */
+#define N9_RU 8.3144621 /* NASA/core/mod_chem.raw.f90:6 */
+#define N9_OMEGA 0.5
+#define N9_EPS 1e-9
+
+static int n9_sec(const shcfd_nasa9 *m, int s, double T) {
+    int l;
+    if (m->trange[s][0][0] > T) return 0;
+    for (l = 0; l < m->nsec[s]; l++)
+        if (m->trange[s][l][1] > T) return l;
+    return m->nsec[s] - 1;
+}
+
+static int n9_sec_trans(const shcfd_nasa9 *m, int t, double T) {
+    int l, num = -1;
+    if (m->trange_trans[t][0][0] > T) num = 0;
+    for (l = 0; l < m->nsec_trans[t]; l++)
+        if (m->trange_trans[t][l][1] > T) { num = l; break; }
+    return num < 0 ? m->nsec_trans[t] - 1 : num;
+}
+
+/* h/RT = sum co_k vh_k, cp/R = sum co_k vc_k; the powers of T by the
+ * reference's running product from 1/T^2 */
+static void n9_basis(double T, double logT, double *vh, double *vc) {
+    double p[7];
+    int k;
+    p[0] = 1.0 / (T * T);
+    for (k = 1; k < 7; k++) p[k] = p[k - 1] * T;
+    vh[0] = -p[0];
+    vh[1] = p[1] * logT;
+    vh[2] = p[2];
+    vh[3] = p[3] / 2.0;
+    vh[4] = p[4] / 3.0;
+    vh[5] = p[5] / 4.0;
+    vh[6] = p[6] / 5.0;
+    vh[7] = p[1];
+    vh[8] = 0.0;
+    for (k = 0; k < 7; k++) vc[k] = p[k];
+    vc[7] = 0.0;
+    vc[8] = 0.0;
+}
+
+static void n9_sums(const shcfd_nasa9 *m, const double *n, double T, double *ert, double *cvr) {
+    double vh[9], vc[9];
+    int i, j;
+    n9_basis(T, or_log(T), vh, vc);
+    *ert = 0.0;
+    *cvr = 0.0;
+    for (i = 0; i < m->ns; i++) {
+        const double *co = m->co[i][n9_sec(m, i, T)];
+        double a = -1.0, b = -1.0;
+        for (j = 0; j < 9; j++) {
+            a = a + co[j] * vh[j];
+            b = b + co[j] * vc[j];
+        }
+        *ert = *ert + a * n[i];
+        *cvr = *cvr + b * n[i];
+    }
+}
+
+/* out = (MWave, kappa, mu, DHi(1:2), vhi(1:2)); returns the iterations */
+int or_cold_flow(const shcfd_nasa9 *m, double y0, double y1, double E, double *Tio, double *out) {
+    double n[SHCFD_NASA9_MAX_NS], mun[SHCFD_NASA9_MAX_NS], dhs[SHCFD_NASA9_MAX_NS], vhs[SHCFD_NASA9_MAX_NS];
+    double vh[9], vc[9], vmu[4];
+    double sn = 0.0, ert, cvr, T = *Tio, T_old, kappa, logT, rt, mu;
+    int i, j, k = 0;
+    for (i = 0; i < m->ns; i++) {
+        n[i] = y0 * m->nf[i] + y1 * m->no[i];
+        sn = sn + n[i];
+    }
+    out[0] = 1e3 / sn;
+    n9_sums(m, n, T, &ert, &cvr);
+    T_old = T * 0.5;
+    while (fabs(E / N9_RU - ert * T) > cvr * N9_EPS * 1e-2 && fabs(T - T_old) > N9_EPS && k < 100) {
+        T_old = T;
+        T = T + N9_OMEGA * (E / N9_RU - ert * T) / cvr;
+        n9_sums(m, n, T, &ert, &cvr);
+        k = k + 1;
+    }
+    kappa = 1.0 + sn / cvr;
+    out[1] = kappa;
+    logT = or_log(T);
+    n9_basis(T, logT, vh, vc);
+    for (i = 0; i < m->ns; i++) {
+        const double *co = m->co[i][n9_sec(m, i, T)];
+        double a = 0.0;
+        for (j = 0; j < 9; j++) a = a + co[j] * vh[j];
+        dhs[i] = 1.0 * (kappa - (kappa - 1.0) * a);
+        vhs[i] = 1.0 * a;
+    }
+    rt = N9_RU * T;
+    {
+        double d0 = 0.0, d1 = 0.0, h0 = 0.0, h1 = 0.0;
+        for (i = 0; i < m->ns; i++) {
+            d0 = d0 + m->nf[i] * dhs[i];
+            d1 = d1 + m->no[i] * dhs[i];
+            h0 = h0 + m->nf[i] * vhs[i];
+            h1 = h1 + m->no[i] * vhs[i];
+        }
+        out[3] = d0 * rt;
+        out[4] = d1 * rt;
+        out[5] = h0 * rt;
+        out[6] = h1 * rt;
+    }
+    vmu[2] = 1.0 / (T * T);
+    vmu[0] = logT;
+    vmu[1] = vmu[2] * T;
+    vmu[3] = 1.0;
+    for (i = 0; i < m->nt; i++) {
+        const double *a = m->trans[i][n9_sec_trans(m, i, T)];
+        double s = 0.0;
+        for (j = 0; j < 4; j++) s = s + a[j] * vmu[j];
+        mun[i] = or_exp(s);
+    }
+    mu = 0.0;
+    for (i = 0; i < m->nt; i++) {
+        double denom = 0.0, MWi = m->mw[m->tr2th[i]];
+        for (j = 0; j < m->nt; j++) {
+            double MWj = m->mw[m->tr2th[j]];
+            double x = 1.0 + sqrt(mun[i] / mun[j]) * sqrt(sqrt(MWj / MWi));
+            double phi = 0.25 * (x * x) * sqrt(2.0 * MWj / (MWi + MWj));
+            denom = denom + n[m->tr2th[j]] * phi;
+        }
+        mu = mu + n[m->tr2th[i]] * mun[i] / denom;
+    }
+    out[2] = mu * 1e-7;
+    *Tio = T;
+    return k;
+}

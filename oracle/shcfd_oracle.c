@@ -40,6 +40,8 @@ struct or_ctx {
     double *scratch; /* per-cell res contributions, max block size */
     double *bnu;     /* kinematic viscosity per block (default: params nu) */
     shcfd_mech mech; /* Chemkin tables (thermal = SHCFD_THERMAL_CHEMKIN) */
+    shcfd_nasa9 nasa9; /* cold-flow tables (thermal = SHCFD_THERMAL_COLD_NASA9) */
+    int has_nasa9;
     int has_mech;
     int chem_err;    /* first chemistry failure since the step began (or_step clears it) */
 };
@@ -446,9 +448,45 @@ int or_proceed_reaction(or_ctx *c) {
     return err;
 }
 
+int or_set_nasa9(or_ctx *c, const shcfd_nasa9 *m) {
+    if (c->ny != 2) return SHCFD_E_ARG;
+    c->nasa9 = *m;
+    c->has_nasa9 = 1;
+    return 0;
+}
+
+/* store/therm_lib/NASA/cold/thermal_model.f90:1-26 */
+static void set_thermo_prop_cold(or_ctx *c) {
+    const double R_uni = 8.3144621e3; /* core/prmtr.f90:4 */
+    for (int b = 0; b < c->nb; b++) {
+        int ni = c->ni[b], nj = c->nj[b];
+#pragma omp parallel for
+        for (int j = 1; j <= nj; j++)
+            for (int i = 1; i <= ni; i++) {
+                double out[7];
+                double ei = Q(4, i, j) / W(IW_RHO, i, j) - 0.5 * (W(IW_U, i, j) * W(IW_U, i, j) + W(IW_V, i, j) * W(IW_V, i, j));
+                double T = W(IW_P, i, j) / (W(IW_RHO, i, j) * W(IW_R(c), i, j));
+                or_cold_flow(&c->nasa9, W(IW_Y(0), i, j), W(IW_Y(1), i, j), ei, &T, out);
+                W(IW_P, i, j) = W(IW_RHO, i, j) * (R_uni / out[0]) * T;
+                W(IW_G(c), i, j) = out[1];
+                W(IW_H(c), i, j) = (Q(4, i, j) + W(IW_P, i, j)) / W(IW_RHO, i, j);
+                W(IW_MU(c), i, j) = out[2];
+                W(IW_R(c), i, j) = R_uni / out[0];
+                DHI(0, i, j) = out[3];
+                DHI(1, i, j) = out[4];
+                VHI(0, i, j) = out[5];
+                VHI(1, i, j) = out[6];
+            }
+    }
+}
+
 void or_set_thermo_prop(or_ctx *c) {
     if (c->p.thermal == SHCFD_THERMAL_CHEMKIN) {
         set_thermo_prop_chemkin(c);
+        return;
+    }
+    if (c->p.thermal == SHCFD_THERMAL_COLD_NASA9) {
+        set_thermo_prop_cold(c);
         return;
     }
     /* therm_lib/ideal/thermal_model.raw.f90:7-33 */

@@ -84,6 +84,12 @@ double or_exp(double x);
 double or_log(double x);
 void or_dgesl(const double *a, int n, const int *ipvt, double *b);
 
+/* ---- NASA-9 cold flow (shcfd_chem_oracle.c) ------------------------------
+ * out = (MWave, kappa, mu, DHi(1:2), vhi(1:2)); *T: start value in, solution
+ * out; returns the Newton iterations (NASA/core/sub_chem.f90:489-610) */
+int or_cold_flow(const shcfd_nasa9 *m, double y0, double y1, double E, double *T, double *out);
+int or_set_nasa9(or_ctx *c, const shcfd_nasa9 *m);
+
 #ifdef __cplusplus
 }
 #endif

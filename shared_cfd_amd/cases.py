@@ -108,6 +108,7 @@ class Case:
     mech: Optional[object] = None           # chem.Mech for the Chemkin thermal model
     w0: Optional[List[np.ndarray]] = None   # initial w per block (T start of calc_T)
     block_nu: Optional[List[float]] = None  # per-block kinematic viscosity (batches)
+    nasa9: Optional[object] = None          # nasa9.Nasa9 tables of the cold-flow thermal model
 
     @property
     def nblocks(self) -> int:

@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "../../include/shcfd.h"
+#include "shcfd_nasa9.h"
 
 namespace {
 
@@ -426,7 +427,228 @@ void parse(const char* chem_inp, const char* mw_inp, const char* trans_inp, shcf
     }
 }
 
+// ---------------------------------------------------------------------------
+// NASA-9 tables of the cold-flow model: read_cheminp (NASA/core/
+// sub_chem.f90:2-67: the element and species lists of chem.inp),
+// set_therm_data (:68-166: CEA thermo.inp -- comment lines '!', a title line,
+// the default-ranges line, then per species a name line (A18), a line
+// (i2,8x,5(a2,f6.2),2x,f13.7) with the section count and the molecular
+// weight, and per section the range (2(1x,f10.3)) and the coefficients
+// (5D16.9 / 2D16.8,16x,2D16.8); reading stops at the first "END"),
+// set_trans_data (:167-245, as for the Chemkin model) and
+// read_fo_composition (:247-300: pressure, temperature and mole ratios of
+// the oxidizer, then of the fuel, from control_chem.inp), normalised to
+// mol/kg as initialize_cold_flow does (calc_ini, :420-441).
+// ---------------------------------------------------------------------------
+void parse_nasa9(const char* chem_inp, const char* thermo_inp, const char* trans_inp, const char* control_inp,
+                 shcfd_nasa9& m, double* pt) {
+    std::memset(&m, 0, sizeof(m));
+    std::vector<std::string> spc;
+    {
+        Lines L(chem_inp);
+        std::string line = L.next();  // "elements"
+        std::string buf;
+        auto word = [&]() {
+            if (buf.empty()) buf = L.next();
+            if (buf.empty()) throw ParseError{"chem.inp: unexpected end of file"};
+            return head_token(buf);
+        };
+        for (;;) {
+            std::string w = word();
+            if (w == "end" || w == "END") break;
+        }
+        line = buf.empty() ? L.next() : buf;  // "species"
+        buf.clear();
+        for (;;) {
+            std::string w = word();
+            if (w == "end" || w == "END") break;
+            spc.push_back(w);
+        }
+    }
+    if (spc.empty() || spc.size() > SHCFD_NASA9_MAX_NS) throw ParseError{"chem.inp: 1 to 8 species expected"};
+    m.ns = (int)spc.size();
+    auto find = [&](const std::string& n) {
+        for (size_t i = 0; i < spc.size(); i++)
+            if (spc[i] == n) return (int)i;
+        return -1;
+    };
+    {
+        std::ifstream T(thermo_inp);
+        if (!T) throw ParseError{std::string("cannot open ") + thermo_inp};
+        std::string s;
+        do {
+            if (!std::getline(T, s)) throw ParseError{"thermo.inp: no data"};
+        } while (!s.empty() && s[0] == '!');
+        std::getline(T, s);  // default ranges
+        int found = 0;
+        std::vector<int> seen(spc.size(), 0);
+        while (found < m.ns && std::getline(T, s)) {
+            s = fixline(s, 200);
+            std::string name = trim(s.substr(0, std::min<size_t>(18, s.size())));
+            if (s.compare(0, 3, "END") == 0) break;
+            std::string l2;
+            if (!std::getline(T, l2)) throw ParseError{"thermo.inp: truncated entry " + name};
+            l2 = fixline(l2, 200);
+            const int nsec = ifield(l2, 0, 2);
+            const int j = find(name);
+            if (j < 0 || seen[j]) {
+                for (int k = 0; k < 3 * nsec; k++) std::getline(T, s);
+                continue;
+            }
+            if (nsec < 1 || nsec > SHCFD_NASA9_MAX_SEC) throw ParseError{"thermo.inp: bad section count for " + name};
+            seen[j] = 1;
+            found++;
+            m.nsec[j] = nsec;
+            m.mw[j] = field(l2, 52, 13);
+            for (int l = 0; l < nsec; l++) {
+                std::string a, b, c;
+                if (!std::getline(T, a) || !std::getline(T, b) || !std::getline(T, c))
+                    throw ParseError{"thermo.inp: truncated entry " + name};
+                a = fixline(a, 200);
+                b = fixline(b, 200);
+                c = fixline(c, 200);
+                m.trange[j][l][0] = field(a, 1, 10);
+                m.trange[j][l][1] = field(a, 12, 10);
+                for (int k = 0; k < 5; k++) m.co[j][l][k] = field(b, 16 * k, 16);
+                m.co[j][l][5] = field(c, 0, 16);
+                m.co[j][l][6] = field(c, 16, 16);
+                m.co[j][l][7] = field(c, 48, 16);
+                m.co[j][l][8] = field(c, 64, 16);
+            }
+        }
+        if (found != m.ns) throw ParseError{"Some species have not found in thermo.inp."};
+    }
+    if (trans_inp) {
+        std::ifstream T(trans_inp);
+        if (!T) throw ParseError{std::string("cannot open ") + trans_inp};
+        std::string s;
+        std::getline(T, s);
+        while (std::getline(T, s)) {
+            s = fixline(s, 300);
+            if (!s.empty() && s[0] == '!') continue;
+            if (s.compare(0, 3, "END") == 0 || s.compare(0, 3, "end") == 0) break;
+            std::string sname = rtrim(s.substr(0, std::min<size_t>(15, s.size())));
+            std::string comment = s.size() > 16 ? trim(s.substr(16, 15)) : std::string();
+            int iv = ifield(s, 35, 1), ic = ifield(s, 37, 1);
+            int k = find(trim(sname));
+            if (!comment.empty() || k < 0) {
+                for (int i = 0; i < iv + ic; i++) std::getline(T, s);
+                continue;
+            }
+            if (m.nt >= SHCFD_NASA9_MAX_NS || iv > SHCFD_NASA9_MAX_TSEC || iv < 1)
+                throw ParseError{"trans.inp: too many entries"};
+            int t = m.nt++;
+            m.tr2th[t] = k;
+            m.nsec_trans[t] = iv;
+            for (int i = 0; i < iv; i++) {
+                if (!std::getline(T, s)) throw ParseError{"ERROR : ODD FORMAT AT trans.inp"};
+                s = fixline(s, 300);
+                if (s.size() < 2 || s[1] != 'V') throw ParseError{"ERROR : ODD FORMAT AT trans.inp"};
+                m.trange_trans[t][i][0] = field(s, 2, 7);
+                m.trange_trans[t][i][1] = field(s, 11, 7);
+                for (int j = 0; j < 4; j++) m.trans[t][i][j] = field(s, 20 + 15 * j, 15);
+            }
+            for (int i = 0; i < ic; i++) {
+                if (!std::getline(T, s)) throw ParseError{"ERROR : ODD FORMAT AT trans.inp"};
+                if (s.size() < 2 || s[1] != 'C') throw ParseError{"ERROR : ODD FORMAT AT trans.inp"};
+            }
+        }
+    }
+    {
+        std::ifstream C(control_inp);
+        if (!C) throw ParseError{std::string("cannot open ") + control_inp};
+        std::string s;
+        double* comp[2] = {m.no, m.nf};
+        double p_t[4] = {0.0, 0.0, 0.0, 0.0};
+        std::getline(C, s);  // title
+        for (int st = 0; st < 2; st++) {
+            for (int k = 0; k < 2; k++) {
+                if (!std::getline(C, s)) throw ParseError{"control_chem.inp: truncated"};
+                s = fixline(s, 200);
+                p_t[2 * st + k] = field(s, 25, 15);
+            }
+            std::getline(C, s);  // "... Composition(mole ratio):"
+            for (;;) {
+                if (!std::getline(C, s)) throw ParseError{"control_chem.inp: truncated composition"};
+                s = trim(fixline(s, 200));
+                if (s.compare(0, 3, "end") == 0) break;
+                std::string rest = s;
+                std::string name = head_token(rest);
+                if (rest.empty()) throw ParseError{"Bad format at control_chem.inp while reading Compositions"};
+                const int j = find(name);
+                if (j < 0) throw ParseError{"control_chem.inp: unknown species " + name};
+                comp[st][j] = to_real(rest, "control_chem.inp");
+            }
+        }
+        for (int st = 0; st < 2; st++) {  // mole ratios -> mol/kg (calc_ini)
+            double sm = 0.0;
+            for (int j = 0; j < m.ns; j++) sm = sm + (comp[st][j] * m.mw[j]);
+            for (int j = 0; j < m.ns; j++) comp[st][j] = (comp[st][j] / sm) * 1e3;
+        }
+        if (pt) {  // (po, To, pf, Tf)
+            for (int k = 0; k < 4; k++) pt[k] = p_t[k];
+        }
+    }
+}
+
 }  // namespace
+
+extern "C" int shcfd_nasa9_parse(const char* chem_inp, const char* thermo_inp, const char* trans_inp,
+                                 const char* control_chem_inp, shcfd_nasa9* out, double* pt, char* errbuf,
+                                 int errlen) {
+    if (!chem_inp || !thermo_inp || !control_chem_inp || !out) return SHCFD_E_ARG;
+    try {
+        parse_nasa9(chem_inp, thermo_inp, trans_inp, control_chem_inp, *out, pt);
+    } catch (const ParseError& e) {
+        if (errbuf && errlen > 0) std::snprintf(errbuf, (size_t)errlen, "%s", e.msg.c_str());
+        return SHCFD_E_ARG;
+    }
+    return SHCFD_OK;
+}
+
+// initialize_cold_flow for one stream (NASA/core/sub_chem.f90:388-488):
+// E = Ru T sum n_j (h_j/(Ru T) - 1) at (p, T) (calc_ini, from E = 0), then
+// cold_flow from T, rho = p / (Ru 1e3 / MW T) and set_static_qw: q = (Y rho,
+// 0, 0, rho E), w = (rho, 0, 0, p, Y, kappa, E + p/rho, p/(rho T), mu), vhi.
+extern "C" int shcfd_nasa9_stream(const shcfd_nasa9* m, int which, double p, double T, double* q, double* w,
+                                  double* vhi) {
+    if (!m || (which != 0 && which != 1) || !(p > 0.0) || !(T > 0.0) || !q || !w || !vhi) return SHCFD_E_ARG;
+    if (m->ns < 1 || m->ns > SHCFD_NASA9_MAX_NS) return SHCFD_E_ARG;
+    using namespace shcfd::nasa9;
+    const double* n = which == 0 ? m->nf : m->no;
+    double vh[9];
+    v_thrt(T, shcfd::chem::dlog(T), vh);
+    double E = 0.0;
+    for (int j = 0; j < m->ns; j++) {
+        const double* co = m->co[j][section(m, j, T)];
+        double tmp = -1.0;
+        for (int k = 0; k < 9; k++) tmp = tmp + (vh[k] * co[k]);
+        E = E + (tmp * n[j]);
+    }
+    E = (E * RU) * T;
+    const double Y[2] = {which == 0 ? 1.0 : 0.0, which == 0 ? 0.0 : 1.0};
+    double Tn = T;
+    ColdOut o;
+    cold_flow(m, Y[0], Y[1], E, Tn, o);
+    const double rho = p / (((RU * 1e3) / o.mwave) * Tn);
+    for (int i = 0; i < 2; i++) {
+        q[i] = Y[i] * rho;
+        w[4 + i] = Y[i];
+        vhi[i] = o.vhi[i];
+    }
+    q[2] = 0.0;
+    q[3] = 0.0;
+    q[4] = rho * E;
+    w[0] = rho;
+    w[1] = 0.0;
+    w[2] = 0.0;
+    w[3] = p;
+    w[6] = o.kappa;
+    w[7] = E + (p / rho);
+    w[8] = p / (rho * Tn);
+    w[9] = o.mu;
+    return SHCFD_OK;
+}
 
 extern "C" int shcfd_mech_parse(const char* chem_inp, const char* mw_inp, const char* trans_inp, shcfd_mech* out,
                                 char* errbuf, int errlen) {

@@ -227,7 +227,7 @@ int shcfd_destroy(shcfd_ctx* ctx) {
     for (int fld = 0; fld < SHCFD_F_COUNT; fld++) if (ctx->f[fld]) hipFree(ctx->f[fld]);
     void* ptrs[] = {ctx->dblk, ctx->dtiles, ctx->dstrips, ctx->dprog, ctx->dbnd, ctx->dpart, ctx->scal,
                     ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool, ctx->dmech, ctx->dskew, ctx->dsbase,
-                    ctx->djac16, ctx->dstrips16, ctx->dbnd16};
+                    ctx->djac16, ctx->dstrips16, ctx->dbnd16, ctx->dnasa9};
     for (void* p : ptrs) if (p) hipFree(p);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -500,6 +500,13 @@ int shcfd_set_thermo_prop(shcfd_ctx* ctx, int step) {
         LAUNCHCHK();
         return SHCFD_OK;
     }
+    if (ctx->p.thermal == SHCFD_THERMAL_COLD_NASA9) {
+        if (!ctx->dnasa9) return SHCFD_E_STATE;
+        ProfScope ps(ctx, PROF_THERMO_CHEM);
+        TILE_LAUNCH(k_thermo_cold, ctx->dnasa9);
+        LAUNCHCHK();
+        return SHCFD_OK;
+    }
     if (ctx->p.thermal != SHCFD_THERMAL_IDEAL) return SHCFD_E_STATE;
     ProfScope ps(ctx, PROF_PRIMITIVE);
     DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_thermo_ideal<NY>));
@@ -514,6 +521,22 @@ int shcfd_set_mechanism(shcfd_ctx* ctx, const shcfd_mech* mech) {
     if (!ctx->dmech) HIPCHK(hipMalloc((void**)&ctx->dmech, sizeof(shcfd_mech)));
     HIPCHK(hipMemcpy(ctx->dmech, mech, sizeof(shcfd_mech), hipMemcpyHostToDevice));
     ctx->has_mech = 1;
+    return SHCFD_OK;
+}
+
+// the cold-flow model's tables (init_pack_cold_flow, NASA/core/sub_chem.f90:
+// 380-387): frozen mixing of two streams, nY = 2
+int shcfd_set_nasa9(shcfd_ctx* ctx, const shcfd_nasa9* tables) {
+    if (!ctx || !tables || ctx->ny != 2 || ctx->p.thermal != SHCFD_THERMAL_COLD_NASA9) return SHCFD_E_ARG;
+    const shcfd_nasa9& m = *tables;
+    if (m.ns < 1 || m.ns > SHCFD_NASA9_MAX_NS || m.nt < 0 || m.nt > m.ns) return SHCFD_E_ARG;
+    for (int i = 0; i < m.ns; i++)
+        if (m.nsec[i] < 1 || m.nsec[i] > SHCFD_NASA9_MAX_SEC || !(m.mw[i] > 0.0)) return SHCFD_E_ARG;
+    for (int t = 0; t < m.nt; t++)
+        if (m.nsec_trans[t] < 1 || m.nsec_trans[t] > SHCFD_NASA9_MAX_TSEC || m.tr2th[t] < 0 || m.tr2th[t] >= m.ns)
+            return SHCFD_E_ARG;
+    if (!ctx->dnasa9) HIPCHK(hipMalloc((void**)&ctx->dnasa9, sizeof(shcfd_nasa9)));
+    HIPCHK(hipMemcpy(ctx->dnasa9, tables, sizeof(shcfd_nasa9), hipMemcpyHostToDevice));
     return SHCFD_OK;
 }
 

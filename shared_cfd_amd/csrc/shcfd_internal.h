@@ -164,6 +164,7 @@ struct Ctx {
     long long spin_limit = 1LL << 23;
     // Chemkin mechanism (module chem of the reference), device copy
     shcfd_mech* dmech = nullptr;
+    shcfd_nasa9* dnasa9 = nullptr;  // cold-flow tables (SHCFD_THERMAL_COLD_NASA9)
     int has_mech = 0;
     int tile_rows = TJ;            // rows per tile of the per-cell kernels (1, 2 or 4)
     // captured whole-step graphs (shcfd_iterate)

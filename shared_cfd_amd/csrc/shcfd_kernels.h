@@ -490,4 +490,7 @@ __global__ void k_geom_a(DevCtx, const Tile*);
 __global__ void k_geom_ghost(DevCtx, int);
 __global__ void k_geojac(DevCtx, const Tile*);
 
+// k_nasa9.hip
+__global__ void k_thermo_cold(DevCtx d, const Tile* tiles, const shcfd_nasa9* __restrict__ m);
+
 }  // namespace shcfd

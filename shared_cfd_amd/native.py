@@ -35,6 +35,7 @@ EXPORTS = [
     "shcfd_adjoint_poolbar", "shcfd_residual_vjp", "shcfd_adjoint_solve",
     "shcfd_set_fixed_dt", "shcfd_snapshot_save", "shcfd_snapshot_load", "shcfd_snapshot_free",
     "shcfd_adjoint_unsteady_reset", "shcfd_adjoint_unsteady_step", "shcfd_adjoint_unsteady_poolbar",
+    "shcfd_nasa9_parse", "shcfd_set_nasa9", "shcfd_nasa9_stream",
 ]
 ADJ_LAMBDA, ADJ_QBAR, ADJ_XBAR, ADJ_XBAR_SUM, ADJ_LAMBDA_LATER = 0, 1, 2, 3, 4
 OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT, OPT_SKEW_SWEEP, OPT_TILE_ROWS = 1, 2, 3, 4
@@ -156,6 +157,10 @@ class Solver:
 
             chem._lib()
             check(L.shcfd_set_mechanism(self.ctx, ctypes.byref(case.mech)), "shcfd_set_mechanism")
+        if case.nasa9 is not None:
+            from . import nasa9 as _n9
+
+            check(_n9._lib().shcfd_set_nasa9(self.ctx, ctypes.byref(case.nasa9)), "shcfd_set_nasa9")
         if case.w0 is not None and load_q:
             for b in range(case.nblocks):
                 self.upload("W", b, case.w0[b])
