@@ -57,6 +57,42 @@ __global__ void __launch_bounds__(256) k_thermo_ideal(DevCtx d, const Tile* tile
     FLD(d, SHCFD_F_VHI, 0, c) = ((kap / (kap - 1.0)) * rg) * T;
 }
 
+// set_w followed by the ideal-gas set_thermo_prop inside a whole step: one
+// pass over the cells, the same expressions as k_set_w and k_thermo_ideal
+template <int NY>
+__global__ void __launch_bounds__(256) k_primitive_ideal(DevCtx d, const Tile* tiles) {
+    using D = Dims<NY>;
+    TileCoord t = tile_coord(tiles);
+    const DevBlock B = d.blk[t.b];
+    if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) return;
+    long long c = cidx(B, t.i, t.j);
+    double q[D::dq];
+#pragma unroll
+    for (int k = 0; k < D::dq; k++) q[k] = FLD(d, SHCFD_F_Q, k, c);
+    double rho = 0.0;
+#pragma unroll
+    for (int k = 0; k < NY; k++) rho = rho + q[k];
+    const double inv = 1.0 / rho;
+    const double u = q[NY] * inv, v = q[NY + 1] * inv;
+    FLD(d, SHCFD_F_W, IW_RHO, c) = rho;
+    FLD(d, SHCFD_F_W, IW_U, c) = u;
+    FLD(d, SHCFD_F_W, IW_V, c) = v;
+#pragma unroll
+    for (int k = 0; k < NY; k++) FLD(d, SHCFD_F_W, IW_Y(k), c) = q[k] * inv;
+    const double kap = d.p.kappa, rg = d.p.rgas, nu = B.nu;
+    const double qe = q[NY + 2];
+    const double ei = qe / rho - 0.5 * ((u * u) + (v * v));
+    const double T = (ei * (kap - 1.0)) / rg;
+    const double p = (rho * rg) * T;
+    FLD(d, SHCFD_F_W, IW_P, c) = p;
+    FLD(d, SHCFD_F_W, D::ig, c) = kap;
+    FLD(d, SHCFD_F_W, D::ih, c) = (qe + p) / rho;
+    FLD(d, SHCFD_F_W, D::imu, c) = rho * nu;
+    FLD(d, SHCFD_F_W, D::ir, c) = rg;
+    FLD(d, SHCFD_F_DHI, 0, c) = 0.0;
+    FLD(d, SHCFD_F_VHI, 0, c) = ((kap / (kap - 1.0)) * rg) * T;
+}
+
 // ---------------------------------------------------------------------------
 // boundary conditions
 // ---------------------------------------------------------------------------
@@ -269,6 +305,56 @@ __global__ void __launch_bounds__(256) k_set_tg(DevCtx d, const Tile* tiles) {
     if (i >= 1 && i <= B.ni && j <= B.nj) {
 #pragma unroll
         for (int k = 0; k < D::dw; k++) { wl[k] = FLD(d, SHCFD_F_WHLJ, k, c); wr[k] = FLD(d, SHCFD_F_WHRJ, k, c); }
+        slau_face<NY>(wl, wr, FLD(d, SHCFD_F_VNJ, 0, c), FLD(d, SHCFD_F_VNJ, 1, c), tg);
+#pragma unroll
+        for (int k = 0; k < D::dq; k++) FLD(d, SHCFD_F_TGJ, k, c) = tg[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// set_HV followed by set_TG inside a whole step: the face states of one face
+// stay in registers and go straight into the SLAU flux (the same expressions
+// as k_set_hv and k_set_tg, so TGi/TGj are identical); wHl/wHr are not
+// stored.  The per-routine entry points keep the two separate kernels.
+// ---------------------------------------------------------------------------
+template <int NY, int DIR>
+__device__ inline void face_states(const DevCtx& d, long long c, long long s, double hp, double kk, int lim,
+                                   int muscl, double* wl, double* wr) {
+    (void)DIR;
+#pragma unroll
+    for (int k = 0; k < NY + 8; k++) {
+        double w0 = FLD(d, SHCFD_F_W, k, c), w1 = FLD(d, SHCFD_F_W, k, c + s);
+        if (muscl) {
+            double du0, dl0, du1, dl1;
+            muscl_delta(w1 - w0, w0 - FLD(d, SHCFD_F_W, k, c - s), kk, lim, du0, dl0);
+            muscl_delta(FLD(d, SHCFD_F_W, k, c + 2 * s) - w1, w1 - w0, kk, lim, du1, dl1);
+            wl[k] = w0 + hp * du0;
+            wr[k] = w1 - hp * dl1;
+        } else {
+            wl[k] = w0;
+            wr[k] = w1;
+        }
+    }
+}
+
+template <int NY>
+__global__ void __launch_bounds__(256) k_set_hv_tg(DevCtx d, const Tile* tiles) {
+    using D = Dims<NY>;
+    TileCoord t = tile_coord(tiles);
+    const DevBlock B = d.blk[t.b];
+    const int i = t.i, j = t.j;
+    const double hp = d.p.phi_mus * 0.5, kk = d.p.k_mus;
+    const int lim = d.p.limiter, muscl = d.p.high_order;
+    const long long c = cidx(B, i, j);
+    double wl[D::dw], wr[D::dw], tg[D::dq];
+    if (i <= B.ni && j >= 1 && j <= B.nj) {
+        face_states<NY, 0>(d, c, 1, hp, kk, lim, muscl, wl, wr);
+        slau_face<NY>(wl, wr, FLD(d, SHCFD_F_VNI, 0, c), FLD(d, SHCFD_F_VNI, 1, c), tg);
+#pragma unroll
+        for (int k = 0; k < D::dq; k++) FLD(d, SHCFD_F_TGI, k, c) = tg[k];
+    }
+    if (i >= 1 && i <= B.ni && j <= B.nj) {
+        face_states<NY, 1>(d, c, B.pitch, hp, kk, lim, muscl, wl, wr);
         slau_face<NY>(wl, wr, FLD(d, SHCFD_F_VNJ, 0, c), FLD(d, SHCFD_F_VNJ, 1, c), tg);
 #pragma unroll
         for (int k = 0; k < D::dq; k++) FLD(d, SHCFD_F_TGJ, k, c) = tg[k];
@@ -578,11 +664,13 @@ __global__ void __launch_bounds__(256) k_copy_q(DevCtx d, const Tile* tiles, int
 #define INST(NY)                                                                                         \
     template __global__ void k_set_w<NY>(DevCtx, const Tile*);                                           \
     template __global__ void k_thermo_ideal<NY>(DevCtx, const Tile*);                                    \
+    template __global__ void k_primitive_ideal<NY>(DevCtx, const Tile*);                                 \
     template __global__ void k_bc_cut<NY>(DevCtx, const int*, int, int);                                 \
     template __global__ void k_bc_side<NY>(DevCtx, const int*, int, const double*, int);                 \
     template __global__ void k_bc_corner<NY>(DevCtx);                                                    \
     template __global__ void k_set_hv<NY>(DevCtx, const Tile*);                                          \
     template __global__ void k_set_tg<NY>(DevCtx, const Tile*);                                          \
+    template __global__ void k_set_hv_tg<NY>(DevCtx, const Tile*);                                       \
     template __global__ void k_set_tgv<NY>(DevCtx, const Tile*);                                         \
     template __global__ void k_set_dt<NY>(DevCtx, const Tile*, double*);                                 \
     template __global__ void k_rms_part<NY>(DevCtx, const Tile*, double*);                               \

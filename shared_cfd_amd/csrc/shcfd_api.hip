@@ -1046,17 +1046,29 @@ int shcfd_energy_rms(shcfd_ctx* ctx, double* rms_out) {
 // ---------------------------------------------------------------------------
 // whole steps
 // ---------------------------------------------------------------------------
+// set_w + set_thermo_prop (ideal gas: one fused pass)
 static int part_primitive(shcfd_ctx* ctx) {
+    if (ctx->p.thermal == SHCFD_THERMAL_IDEAL) {
+        ProfScope ps(ctx, PROF_PRIMITIVE);
+        DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_primitive_ideal<NY>));
+        LAUNCHCHK();
+        return SHCFD_OK;
+    }
     int rc = shcfd_set_w(ctx);
     if (!rc) rc = shcfd_set_thermo_prop(ctx, 0);
     return rc;
 }
+// set_BC, set_HV + set_TG (one fused kernel: the face states are not
+// stored), set_TGv
 static int part_main(shcfd_ctx* ctx) {
     int rc = shcfd_set_bc(ctx, 0);
-    if (!rc) rc = shcfd_set_hv(ctx);
-    if (!rc) rc = shcfd_set_tg(ctx);
-    if (!rc) rc = shcfd_set_tgv(ctx);
-    return rc;
+    if (rc) return rc;
+    {
+        ProfScope ps(ctx, PROF_HV);
+        DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_hv_tg<NY>));
+        LAUNCHCHK();
+    }
+    return shcfd_set_tgv(ctx);
 }
 
 int shcfd_prepare(shcfd_ctx* ctx) {
