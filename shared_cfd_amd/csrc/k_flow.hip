@@ -209,9 +209,8 @@ __global__ void k_bc_side(DevCtx d, const int* bc, int nbc, const double* pool, 
 
 // set_corners: cond/no-nV/condition.tail.f90:331-356
 template <int NY>
-__global__ void k_bc_corner(DevCtx d) {
+__device__ inline void bc_corner_cell(const DevCtx& d, int gid) {
     using D = Dims<NY>;
-    int gid = blockIdx.x * blockDim.x + threadIdx.x;
     int b = gid >> 2;
     if (b >= d.nb) return;
     const DevBlock B = d.blk[b];
@@ -225,6 +224,10 @@ __global__ void k_bc_corner(DevCtx d) {
     for (int k = 0; k < NY; k++)
         FLD(d, SHCFD_F_VHI, k, cc) =
             (1.0 / 3.0) * ((FLD(d, SHCFD_F_VHI, k, c0) + FLD(d, SHCFD_F_VHI, k, cj)) + FLD(d, SHCFD_F_VHI, k, ci));
+}
+template <int NY>
+__global__ void k_bc_corner(DevCtx d) {
+    bc_corner_cell<NY>(d, (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -347,6 +350,12 @@ __global__ void __launch_bounds__(256) k_set_hv_tg(DevCtx d, const Tile* tiles) 
     const int lim = d.p.limiter, muscl = d.p.high_order;
     const long long c = cidx(B, i, j);
     double wl[D::dw], wr[D::dw], tg[D::dq];
+    {
+        // set_corners (cond/no-nV/condition.tail.f90:331-356): the corner ghosts are
+        // read by no face of this kernel, only by later ones
+        const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+        if (g < 4 * d.nb) bc_corner_cell<NY>(d, g);
+    }
     if (i <= B.ni && j >= 1 && j <= B.nj) {
         face_states<NY, 0>(d, c, 1, hp, kk, lim, muscl, wl, wr);
         slau_face<NY>(wl, wr, FLD(d, SHCFD_F_VNI, 0, c), FLD(d, SHCFD_F_VNI, 1, c), tg);

@@ -26,10 +26,13 @@ namespace shcfd {
 // RHS: time_lusgs_viscous.f90:12-22, time_precon_viscous.f90:12-21
 // (preconLU-SGS), time_NR_viscous.f90:16-25, time_precon_viscous.f90:16-27
 // (precon), time_dual_viscous.f90:16-28.  Tile 0 also clears the sweep
-// progress flags.
+// progress flags.  With dqdt_inline set (whole steps of the inner-loop
+// schemes) the cell's dq/dt of set_dqdt (sch_dual.raw.f90:211-240) is formed
+// here, with set_dqdt's expressions, stored and used, instead of in a pass
+// of its own.
 // ---------------------------------------------------------------------------
 template <int NY>
-__global__ void __launch_bounds__(256) k_rhs(DevCtx d, const Tile* tiles, int* prog, int nprog) {
+__global__ void __launch_bounds__(256) k_rhs(DevCtx d, const Tile* tiles, int* prog, int nprog, int dqdt_inline) {
     using D = Dims<NY>;
     constexpr int dq = D::dq;
     if (blockIdx.x == 0)
@@ -41,6 +44,16 @@ __global__ void __launch_bounds__(256) k_rhs(DevCtx d, const Tile* tiles, int* p
     double s[dq];
     flux_balance<NY>(d, B, c, s);
     const int scheme = d.p.scheme;
+    if (dqdt_inline) {
+        const double dtv = (d.p.time_step == 0) ? d.scal[SCAL_DT_GRBL] : FLD(d, SHCFD_F_DT, 0, c);
+        const bool first = d.scal[SCAL_FIRST_DUAL] != 0.0;
+#pragma unroll
+        for (int k = 0; k < dq; k++) {
+            const double q = FLD(d, SHCFD_F_Q, k, c), qp = FLD(d, SHCFD_F_QP, k, c);
+            FLD(d, SHCFD_F_DQDT, k, c) =
+                first ? (q - qp) / dtv : (((3.0 * q) - (4.0 * qp)) + FLD(d, SHCFD_F_QPP, k, c)) / (2.0 * dtv);
+        }
+    }
     if (scheme == SHCFD_SCHEME_DUAL) {
         double vol = FLD(d, SHCFD_F_VOL, 0, c);
         double x[dq];
@@ -384,7 +397,7 @@ __global__ void __launch_bounds__(256) k_update(DevCtx d, const Tile* tiles) {
 }
 
 #define INST(NY)                                                                                               \
-    template __global__ void k_rhs<NY>(DevCtx, const Tile*, int*, int);                                        \
+    template __global__ void k_rhs<NY>(DevCtx, const Tile*, int*, int, int);                                        \
     template __global__ void k_sweep_fwd_plain<NY, false>(DevCtx, const Strip*, int*, double*, long long, long long, \
                                                           const double*, long long, const long long*);          \
     template __global__ void k_sweep_fwd_plain<NY, true>(DevCtx, const Strip*, int*, double*, long long, long long,  \

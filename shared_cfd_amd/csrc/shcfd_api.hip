@@ -568,9 +568,9 @@ int shcfd_chem_iterations(shcfd_ctx* ctx, double* total, int reset) {
     return SHCFD_OK;
 }
 
-int shcfd_set_bc(shcfd_ctx* ctx, int step) {
-    (void)step;
-    if (!ctx) return SHCFD_E_ARG;
+// section_exchange and the BC segments; set_corners when `corners` (whole
+// steps leave the corners to the fused face-flux kernel, which reads none)
+static int bc_async(shcfd_ctx* ctx, bool corners) {
     DevCtx d = ctx->dev();
     ProfScope ps(ctx, PROF_BC);
     if (ctx->ncut > 0) {
@@ -585,10 +585,17 @@ int shcfd_set_bc(shcfd_ctx* ctx, int step) {
         DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_bc_side<NY>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                                                 ctx->stream, d, ctx->dbc, ctx->nbc, ctx->dpool, ctx->bc_maxlen));
     }
-    DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_bc_corner<NY>, dim3((4 * ctx->nb + 255) / 256), dim3(256), 0,
-                                            ctx->stream, d));
+    if (corners)
+        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_bc_corner<NY>, dim3((4 * ctx->nb + 255) / 256), dim3(256), 0,
+                                                ctx->stream, d));
     LAUNCHCHK();
     return SHCFD_OK;
+}
+
+int shcfd_set_bc(shcfd_ctx* ctx, int step) {
+    (void)step;
+    if (!ctx) return SHCFD_E_ARG;
+    return bc_async(ctx, true);
 }
 
 int shcfd_set_hv(shcfd_ctx* ctx) {
@@ -881,7 +888,8 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
     ctx->q16_active = 0;
     const int C16 = (ctx->dimq + 3) / 4, ns16 = ctx->nstrips16;
     { ProfScope ps(ctx, PROF_RHS);
-    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_rhs<NY>, ctx->dprog, 2 * ns)); }
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_rhs<NY>, ctx->dprog, 2 * ns, ctx->dqdt_inline)); }
+    ctx->dqdt_inline = 0;
     if (fast_enabled(ctx)) {
         int rc = ctx->fast->implicit(*ctx, step_internal, prof_begin, prof_end);
         LAUNCHCHK();
@@ -1059,9 +1067,9 @@ static int part_primitive(shcfd_ctx* ctx) {
     return rc;
 }
 // set_BC, set_HV + set_TG (one fused kernel: the face states are not
-// stored), set_TGv
+// stored; it also fills the corner ghosts, set_corners), set_TGv
 static int part_main(shcfd_ctx* ctx) {
-    int rc = shcfd_set_bc(ctx, 0);
+    int rc = bc_async(ctx, false);
     if (rc) return rc;
     {
         ProfScope ps(ctx, PROF_HV);
@@ -1165,7 +1173,7 @@ int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double
             TRY(part_primitive(ctx));
             TRY(part_main(ctx));
             TRY(abpm_for_step(ctx));
-            TRY(shcfd_set_dqdt(ctx));
+            ctx->dqdt_inline = 1;  // set_dqdt, formed in the RHS pass
             TRY(implicit_async(ctx, j));
         }
         double sc[SCAL_COUNT];

@@ -180,6 +180,7 @@ struct Ctx {
     long long nsl16 = 0;           // total steps of all strips
     double* dbnd16 = nullptr;      // hand-off rows of both sweeps: 2 x (nstrips16 + 1) x bstride16
     long long bstride16 = 0;
+    int dqdt_inline = 0;           // the next implicit update forms dq/dt in its RHS pass
     int q16_active = 0;            // the last set_ABpm of a step wrote the quad-row copy
     int skew_active = 0;           // the last set_ABpm of a step wrote the skewed copy
     double* dskew = nullptr;

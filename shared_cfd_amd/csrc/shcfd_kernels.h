@@ -462,7 +462,7 @@ __global__ void k_zero_field(double*, long long);
 __global__ void k_reduce_final(DevCtx, const double*, int, int, int);
 
 // implicit update (k_implicit.hip)
-template <int NY> __global__ void k_rhs(DevCtx, const Tile*, int*, int);
+template <int NY> __global__ void k_rhs(DevCtx, const Tile*, int*, int, int);
 template <int NY, bool SKEW>
 __global__ void k_sweep_fwd_plain(DevCtx, const Strip*, int*, double*, long long, long long, const double*, long long,
                                   const long long*);
