@@ -160,6 +160,11 @@ int shcfd_set_nasa9(shcfd_ctx *ctx, const shcfd_nasa9 *tables);
  * (p, T): initialize_cold_flow / set_static_qw (sub_chem.f90:388-488):
  * q(1:5), w(1:10), vhi(1:2).  Host code. */
 int shcfd_nasa9_stream(const shcfd_nasa9 *tables, int which, double p, double T, double *q, double *w, double *vhi);
+/* init_pack_cold_flow for hosts that do not hold the tables themselves (the
+ * Fortran driver): shcfd_nasa9_parse of the four files (trans_inp may be
+ * NULL or ""), then shcfd_set_nasa9. */
+int shcfd_load_nasa9(shcfd_ctx *ctx, const char *chem_inp, const char *thermo_inp, const char *trans_inp,
+                     const char *control_chem_inp);
 /* proceed_reaction: store/therm_lib/chemkin/flow/reactive/thermal_model.f90:45-73
  * (reaction_chemeq2 per cell over dt_grbl; or the point-implicit update,
  * params.reaction) */

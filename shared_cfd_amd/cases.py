@@ -242,7 +242,8 @@ def write_case(case: "Case", path: str, max_step: int, out_period: int = 1, min_
                fast: bool = True, mech_files=None) -> None:
     """case.inp for shared_cfd_amd/main_shcfd (the Fortran main program).
     ``mech_files`` = (chem.inp, MW.inp, trans.inp or None) for the Chemkin
-    thermal model."""
+    thermal model, (chem.inp, thermo.inp, trans.inp or None,
+    control_chem.inp) for the cold-flow model."""
     p = case.params
     with open(path, "w") as f:
         f.write("SHCFD-CASE\n")
@@ -263,6 +264,11 @@ def write_case(case: "Case", path: str, max_step: int, out_period: int = 1, min_
                 f.write(os.path.abspath(fn) if fn != "-" else "-")
                 f.write("\n")
             f.write(f"{case.mech.rtol!r} {case.mech.atol!r}\n")
+        elif p.thermal == 5:
+            if mech_files is None:
+                raise ValueError("write_case: the cold-flow model needs mech_files")
+            for fn in mech_files:  # chem.inp, thermo.inp, trans.inp or None, control_chem.inp
+                f.write((os.path.abspath(fn) if fn else "-") + "\n")
         f.write(f"{float(max_step)!r} {min_rms!r} {out_period}\n")
         f.write(f"{case.bc_rows.shape[0]} {case.bc_pool.size}\n")
         for r in case.bc_rows:

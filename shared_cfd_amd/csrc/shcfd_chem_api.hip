@@ -107,6 +107,24 @@ int shcfd_load_mechanism(shcfd_ctx* ctx, const char* chem_inp, const char* mw_in
     return rc;
 }
 
+// init_pack_cold_flow for hosts that do not hold the tables (the Fortran
+// driver): shcfd_nasa9_parse of the four files, then shcfd_set_nasa9
+int shcfd_load_nasa9(shcfd_ctx* ctx, const char* chem_inp, const char* thermo_inp, const char* trans_inp,
+                     const char* control_chem_inp) {
+    if (!ctx || !chem_inp || !thermo_inp || !control_chem_inp) return SHCFD_E_ARG;
+    shcfd_nasa9* t = new (std::nothrow) shcfd_nasa9;
+    if (!t) return SHCFD_E_NOMEM;
+    char err[256];
+    int rc = shcfd_nasa9_parse(chem_inp, thermo_inp, (trans_inp && trans_inp[0]) ? trans_inp : nullptr,
+                               control_chem_inp, t, nullptr, err, 256);
+    if (rc == SHCFD_OK)
+        rc = shcfd_set_nasa9(ctx, t);
+    else
+        std::fprintf(stderr, "shcfd_load_nasa9: %s\n", err);
+    delete t;
+    return rc;
+}
+
 int shcfd_chem_eval(const shcfd_mech* m, int op, int n, const double* in, double* out, int* status, double dt) {
     if (!m || n < 0 || !in || !out || (m->ns != 8 && m->ns != 9) || m->nr < 1 || m->nr > SHCFD_CHEM_MAX_NR)
         return SHCFD_E_ARG;

@@ -15,7 +15,7 @@ program main
    integer, allocatable :: ni(:), nj(:), bc(:), cut(:)
    double precision, allocatable :: pool(:), a(:), nodes(:,:), qin(:), w(:), pool_out(:)
    character(len=32) :: magic
-   character(len=1024) :: chem_inp, mw_inp, trans_inp
+   character(len=1024) :: chem_inp, mw_inp, trans_inp, thermo_inp, control_inp
    double precision :: rtol, atol
    integer :: has_w
    integer(c_int) :: fast
@@ -40,6 +40,12 @@ program main
       read(10, '(a)') trans_inp
       if (trim(trans_inp) == "-") trans_inp = ""
       read(10, *) rtol, atol
+   else if (prm%thermal == 5) then
+      read(10, '(a)') chem_inp
+      read(10, '(a)') thermo_inp
+      read(10, '(a)') trans_inp
+      read(10, '(a)') control_inp
+      if (trim(trans_inp) == "-") trans_inp = ""
    end if
    read(10, *) max_step, min_RMS, OutPeriod
    read(10, *) nbc, npool
@@ -104,6 +110,7 @@ program main
       deallocate(nodes, a, qin)
    end do
    if (prm%thermal == 2) call init_therm(chem_inp, mw_inp, trans_inp, rtol, atol)
+   if (prm%thermal == 5) call init_pack_cold_flow(chem_inp, thermo_inp, trans_inp, control_inp)
    close(10)
    call shcfd_check(shcfd_set_bc_table(ctx, bc, int(nbc, c_int), pool, int(npool, c_int)), "set_bc_table")
    call shcfd_check(shcfd_set_cut_table(ctx, cut, int(ncut, c_int)), "set_cut_table")

@@ -153,6 +153,12 @@ module shcfd_f
          character(kind=c_char), intent(in) :: chem_inp(*), mw_inp(*), trans_inp(*)
          real(c_double), value :: rtol, atol
       end function
+      integer(c_int) function shcfd_load_nasa9(ctx, chem_inp, thermo_inp, trans_inp, control_inp) &
+            bind(C, name="shcfd_load_nasa9")
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: ctx
+         character(kind=c_char), intent(in) :: chem_inp(*), thermo_inp(*), trans_inp(*), control_inp(*)
+      end function
       integer(c_int) function shcfd_proceed_reaction_c(ctx) bind(C, name="shcfd_proceed_reaction")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
@@ -219,6 +225,12 @@ contains
       double precision, intent(in) :: rtol, atol
       call shcfd_check(shcfd_load_mechanism(ctx, trim(chem_inp)//c_null_char, trim(mw_inp)//c_null_char, &
                                             trim(trans_inp)//c_null_char, rtol, atol), "init_therm")
+   end subroutine
+   subroutine init_pack_cold_flow(chem_inp, thermo_inp, trans_inp, control_inp)  ! NASA/core/sub_chem.f90:380-387
+      character(*), intent(in) :: chem_inp, thermo_inp, trans_inp, control_inp
+      call shcfd_check(shcfd_load_nasa9(ctx, trim(chem_inp)//c_null_char, trim(thermo_inp)//c_null_char, &
+                                        trim(trans_inp)//c_null_char, trim(control_inp)//c_null_char), &
+                       "init_pack_cold_flow")
    end subroutine
    subroutine set_BC(step)                  ! store/cond/no-nV/condition.tail.f90
       integer, intent(in) :: step

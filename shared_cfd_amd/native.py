@@ -35,7 +35,7 @@ EXPORTS = [
     "shcfd_adjoint_poolbar", "shcfd_residual_vjp", "shcfd_adjoint_solve",
     "shcfd_set_fixed_dt", "shcfd_snapshot_save", "shcfd_snapshot_load", "shcfd_snapshot_free",
     "shcfd_adjoint_unsteady_reset", "shcfd_adjoint_unsteady_step", "shcfd_adjoint_unsteady_poolbar",
-    "shcfd_nasa9_parse", "shcfd_set_nasa9", "shcfd_nasa9_stream",
+    "shcfd_nasa9_parse", "shcfd_set_nasa9", "shcfd_nasa9_stream", "shcfd_load_nasa9",
 ]
 ADJ_LAMBDA, ADJ_QBAR, ADJ_XBAR, ADJ_XBAR_SUM, ADJ_LAMBDA_LATER = 0, 1, 2, 3, 4
 OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT, OPT_SKEW_SWEEP, OPT_TILE_ROWS = 1, 2, 3, 4

@@ -81,3 +81,25 @@ def test_fortran_driver_reacting(tmp_path):
     q_o = o.field("Q", 0)[2:14, 2:26]
     q_d = blocks[0][1:13, 1:25, :case.dimq]
     np.testing.assert_allclose(q_d, q_o, rtol=1e-10, atol=1e-10 * np.max(np.abs(q_o)))
+
+
+@pytest.mark.parametrize("scheme", [C.SCHEME_LUSGS, C.SCHEME_PRECON_LUSGS])
+def test_fortran_driver_cold_flow(tmp_path, scheme):
+    """Cold flow with NASA-9 thermodynamics (init_pack_cold_flow) on a small
+    fuel/oxidizer layer with the tables of the reference's sample no28."""
+    from shared_cfd_amd import nasa9 as N
+
+    g = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    files = (os.path.join(g, "no28_chem.inp"), os.path.join(g, "nasa9_thermo.inp"),
+             os.path.join(g, "nasa9_trans.inp"), os.path.join(g, "no28_control_chem.inp"))
+    t, pt = N.parse(*files)
+    case = N.mixing_case(t, pt, ni=24, nj=12, seed=2, scheme=scheme)
+    n = 4
+    C.write_case(case, str(tmp_path / "case.inp"), max_step=n, out_period=1, fast=True, mech_files=files)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "normal end" in r.stdout, r.stdout + r.stderr
+    step, tt, blocks = C.read_result_bin(str(tmp_path / "result.bin"), case)
+    o, _ = _run_oracle(case, n, 1)
+    q_o = o.field("Q", 0)[2:14, 2:26]
+    q_d = blocks[0][1:13, 1:25, :case.dimq]
+    np.testing.assert_allclose(q_d, q_o, rtol=1e-11, atol=1e-11 * np.max(np.abs(q_o)))
