@@ -320,10 +320,11 @@ __global__ void __launch_bounds__(256) k_set_tg(DevCtx d, const Tile* tiles) {
 // as k_set_hv and k_set_tg, so TGi/TGj are identical); wHl/wHr are not
 // stored.  The per-routine entry points keep the two separate kernels.
 // ---------------------------------------------------------------------------
-template <int NY, int DIR>
+// left/right states of the face between cell c and c + s (s = 1: i-face,
+// s = pitch: j-face), as k_set_hv forms them
+template <int NY>
 __device__ inline void face_states(const DevCtx& d, long long c, long long s, double hp, double kk, int lim,
                                    int muscl, double* wl, double* wr) {
-    (void)DIR;
 #pragma unroll
     for (int k = 0; k < NY + 8; k++) {
         double w0 = FLD(d, SHCFD_F_W, k, c), w1 = FLD(d, SHCFD_F_W, k, c + s);
@@ -357,13 +358,13 @@ __global__ void __launch_bounds__(256) k_set_hv_tg(DevCtx d, const Tile* tiles) 
         if (g < 4 * d.nb) bc_corner_cell<NY>(d, g);
     }
     if (i <= B.ni && j >= 1 && j <= B.nj) {
-        face_states<NY, 0>(d, c, 1, hp, kk, lim, muscl, wl, wr);
+        face_states<NY>(d, c, 1, hp, kk, lim, muscl, wl, wr);
         slau_face<NY>(wl, wr, FLD(d, SHCFD_F_VNI, 0, c), FLD(d, SHCFD_F_VNI, 1, c), tg);
 #pragma unroll
         for (int k = 0; k < D::dq; k++) FLD(d, SHCFD_F_TGI, k, c) = tg[k];
     }
     if (i >= 1 && i <= B.ni && j <= B.nj) {
-        face_states<NY, 1>(d, c, B.pitch, hp, kk, lim, muscl, wl, wr);
+        face_states<NY>(d, c, B.pitch, hp, kk, lim, muscl, wl, wr);
         slau_face<NY>(wl, wr, FLD(d, SHCFD_F_VNJ, 0, c), FLD(d, SHCFD_F_VNJ, 1, c), tg);
 #pragma unroll
         for (int k = 0; k < D::dq; k++) FLD(d, SHCFD_F_TGJ, k, c) = tg[k];
