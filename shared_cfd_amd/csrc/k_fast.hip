@@ -828,7 +828,10 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
     if (inner_loop_scheme(c.p.scheme)) {
         hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, st, d, part, nwork);
         if (step_internal == 1) hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, st, d, (int)SCAL_RES1, (int)SCAL_RES);
-        hipLaunchKernelGGL(k_update<1>, dim3((unsigned)c.htiles.size()), dim3(64 * c.tile_rows), 0, st, d, c.dtiles);
+        if (c.defer_update)
+            c.update_pending = 1;  // applied by the next primitive pass
+        else
+            hipLaunchKernelGGL(k_update<1>, dim3((unsigned)c.htiles.size()), dim3(64 * c.tile_rows), 0, st, d, c.dtiles);
     }
     pe(&c, PROF_UPDATE);
     return hipGetLastError() == hipSuccess ? SHCFD_OK : SHCFD_E_HIP;

@@ -58,9 +58,11 @@ __global__ void __launch_bounds__(256) k_thermo_ideal(DevCtx d, const Tile* tile
 }
 
 // set_w followed by the ideal-gas set_thermo_prop inside a whole step: one
-// pass over the cells, the same expressions as k_set_w and k_thermo_ideal
+// pass over the cells, the same expressions as k_set_w and k_thermo_ideal.
+// apply_update: first the previous inner iteration's q += omega Dq, with
+// k_update's expression (time_dual_viscous.f90:113-124)
 template <int NY>
-__global__ void __launch_bounds__(256) k_primitive_ideal(DevCtx d, const Tile* tiles) {
+__global__ void __launch_bounds__(256) k_primitive_ideal(DevCtx d, const Tile* tiles, int apply_update) {
     using D = Dims<NY>;
     TileCoord t = tile_coord(tiles);
     const DevBlock B = d.blk[t.b];
@@ -69,6 +71,14 @@ __global__ void __launch_bounds__(256) k_primitive_ideal(DevCtx d, const Tile* t
     double q[D::dq];
 #pragma unroll
     for (int k = 0; k < D::dq; k++) q[k] = FLD(d, SHCFD_F_Q, k, c);
+    if (apply_update && B.active) {
+        const double om = dmin(d.p.omega_max, d.scal[SCAL_OMEGA]);
+#pragma unroll
+        for (int k = 0; k < D::dq; k++) {
+            q[k] = q[k] + om * FLD(d, SHCFD_F_DQ, k, c);
+            FLD(d, SHCFD_F_Q, k, c) = q[k];
+        }
+    }
     double rho = 0.0;
 #pragma unroll
     for (int k = 0; k < NY; k++) rho = rho + q[k];
@@ -674,7 +684,7 @@ __global__ void __launch_bounds__(256) k_copy_q(DevCtx d, const Tile* tiles, int
 #define INST(NY)                                                                                         \
     template __global__ void k_set_w<NY>(DevCtx, const Tile*);                                           \
     template __global__ void k_thermo_ideal<NY>(DevCtx, const Tile*);                                    \
-    template __global__ void k_primitive_ideal<NY>(DevCtx, const Tile*);                                 \
+    template __global__ void k_primitive_ideal<NY>(DevCtx, const Tile*, int);                                 \
     template __global__ void k_bc_cut<NY>(DevCtx, const int*, int, int);                                 \
     template __global__ void k_bc_side<NY>(DevCtx, const int*, int, const double*, int);                 \
     template __global__ void k_bc_corner<NY>(DevCtx);                                                    \

@@ -914,8 +914,12 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
             hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, ctx->stream, d, ctx->dpart, ns16); }
             if (step_internal == 1)
                 hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, ctx->stream, d, (int)SCAL_RES1, (int)SCAL_RES);
-            { ProfScope ps(ctx, PROF_UPDATE);
-            DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_update<NY>)); }
+            if (ctx->defer_update) {
+                ctx->update_pending = 1;
+            } else {
+                ProfScope ps(ctx, PROF_UPDATE);
+                DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_update<NY>));
+            }
         }
         LAUNCHCHK();
         return SHCFD_OK;
@@ -945,8 +949,12 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
         hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, ctx->stream, d, ctx->dpart, ns); }
         if (step_internal == 1)
             hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, ctx->stream, d, (int)SCAL_RES1, (int)SCAL_RES);
-        { ProfScope ps(ctx, PROF_UPDATE);
-        DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_update<NY>)); }
+        if (ctx->defer_update) {
+            ctx->update_pending = 1;
+        } else {
+            ProfScope ps(ctx, PROF_UPDATE);
+            DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_update<NY>));
+        }
     }
     LAUNCHCHK();
     return SHCFD_OK;
@@ -1054,15 +1062,30 @@ int shcfd_energy_rms(shcfd_ctx* ctx, double* rms_out) {
 // ---------------------------------------------------------------------------
 // whole steps
 // ---------------------------------------------------------------------------
-// set_w + set_thermo_prop (ideal gas: one fused pass)
+// q += omega Dq of an inner iteration whose update was deferred
+static int apply_pending_update(shcfd_ctx* ctx) {
+    if (!ctx->update_pending) return SHCFD_OK;
+    ctx->update_pending = 0;
+    ProfScope ps(ctx, PROF_UPDATE);
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_update<NY>));
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+// set_w + set_thermo_prop (ideal gas: one fused pass, which also applies a
+// deferred inner-iteration update)
 static int part_primitive(shcfd_ctx* ctx) {
     if (ctx->p.thermal == SHCFD_THERMAL_IDEAL) {
+        const int apply = ctx->update_pending;
+        ctx->update_pending = 0;
         ProfScope ps(ctx, PROF_PRIMITIVE);
-        DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_primitive_ideal<NY>));
+        DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_primitive_ideal<NY>, apply));
         LAUNCHCHK();
         return SHCFD_OK;
     }
-    int rc = shcfd_set_w(ctx);
+    int rc = apply_pending_update(ctx);
+    if (rc) return rc;
+    rc = shcfd_set_w(ctx);
     if (!rc) rc = shcfd_set_thermo_prop(ctx, 0);
     return rc;
 }
@@ -1174,8 +1197,12 @@ int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double
             TRY(part_main(ctx));
             TRY(abpm_for_step(ctx));
             ctx->dqdt_inline = 1;  // set_dqdt, formed in the RHS pass
-            TRY(implicit_async(ctx, j));
+            ctx->defer_update = 1;  // q += omega Dq, in the next primitive pass
+            rc = implicit_async(ctx, j);
+            ctx->defer_update = 0;
+            if (rc) return rc;
         }
+        TRY(apply_pending_update(ctx));
         double sc[SCAL_COUNT];
         HIPCHK(hipStreamSynchronize(ctx->stream));
         HIPCHK(hipMemcpy(sc, ctx->scal, sizeof(sc), hipMemcpyDeviceToHost));

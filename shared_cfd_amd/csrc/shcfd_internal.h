@@ -181,6 +181,8 @@ struct Ctx {
     double* dbnd16 = nullptr;      // hand-off rows of both sweeps: 2 x (nstrips16 + 1) x bstride16
     long long bstride16 = 0;
     int dqdt_inline = 0;           // the next implicit update forms dq/dt in its RHS pass
+    int defer_update = 0;          // the next inner iteration leaves q += omega Dq to the next primitive pass
+    int update_pending = 0;        // q += omega Dq is still to be applied
     int q16_active = 0;            // the last set_ABpm of a step wrote the quad-row copy
     int skew_active = 0;           // the last set_ABpm of a step wrote the skewed copy
     double* dskew = nullptr;

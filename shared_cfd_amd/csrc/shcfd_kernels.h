@@ -445,7 +445,7 @@ template <int NY> __global__ void k_bc_side(DevCtx, const int*, int, const doubl
 template <int NY> __global__ void k_bc_corner(DevCtx);
 template <int NY> __global__ void k_set_hv(DevCtx, const Tile*);
 template <int NY> __global__ void k_set_tg(DevCtx, const Tile*);
-template <int NY> __global__ void k_primitive_ideal(DevCtx, const Tile*);
+template <int NY> __global__ void k_primitive_ideal(DevCtx, const Tile*, int);
 template <int NY> __global__ void k_set_hv_tg(DevCtx, const Tile*);
 template <int NY> __global__ void k_set_tgv(DevCtx, const Tile*);
 template <int NY> __global__ void k_set_dt(DevCtx, const Tile*, double*);
