@@ -249,7 +249,10 @@ static int check_field(shcfd_ctx* ctx, int field, int block) {
 int shcfd_upload_field(shcfd_ctx* ctx, int field, int block, const double* host) {
     int rc = check_field(ctx, field, block);
     if (rc || !host) return SHCFD_E_ARG;
-    if (field == SHCFD_F_TGVI || field == SHCFD_F_TGVJ) ctx->tgv_zero = 0;
+    if (field == SHCFD_F_TGVI || field == SHCFD_F_TGVJ) {
+        ctx->tgv_zero = 0;
+        drop_graph(ctx);  // captured launches may skip the viscous fluxes
+    }
     const DevBlock& B = ctx->hblk[block];
     int K = field_ncomp(field, ctx->ny);
     size_t rows = (size_t)B.nj + 4, w = (size_t)B.ni + 4;

@@ -32,6 +32,7 @@ struct DevCtx {
     double* f[SHCFD_F_COUNT];
     double* scal;     // device scalars, see SCAL_* below
     int* iflag;       // device int flags (error codes, counters)
+    int tgv_zero;     // TGVI/TGVJ hold zeros: the flux balance need not read them
     shcfd_params p;
 };
 
@@ -195,6 +196,7 @@ struct Ctx {
         d.blk = dblk;
         for (int i = 0; i < SHCFD_F_COUNT; i++) d.f[i] = f[i];
         d.scal = scal; d.iflag = iflag; d.p = p;
+        d.tgv_zero = tgv_zero;
         return d;
     }
 };
