@@ -52,3 +52,18 @@ def test_cold_flow_steps(scheme):
         assert e == 0
         s.step(n, 0)
     assert_close(interior(s.download("Q", 0), 64, 70), interior(o.field("Q", 0), 64, 70), 1e-11, "Q")
+
+
+def test_cold_flow_axisymmetric_steps():
+    """The fuel/oxidizer layer as an annular jet (grid lifted off the axis,
+    axisymmetric metrics and sources), LU-SGS with viscous fluxes."""
+    t, pt = _tables()
+    case = N.mixing_case(t, pt, ni=48, nj=70, seed=7)
+    case.params.dimension = C.DIM_AXISYMMETRIC
+    case.rh = [r + 0.01 for r in case.rh]
+    o, s = _pair(case)
+    o.prepare(); s.prepare()
+    for n in range(1, 5):
+        assert o.step()[0] == 0
+        s.step(n, 0)
+    assert_close(interior(s.download("Q", 0), 48, 70), interior(o.field("Q", 0), 48, 70), 1e-11, "Q")
