@@ -210,3 +210,22 @@ def test_quad_row_sweeps_match_field_sweeps(scheme):
         s.close()
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+def test_routines_after_whole_steps():
+    """Whole steps use fused kernels (primitive + thermo, MUSCL + SLAU with the
+    corner fill, dq/dt in the RHS, the deferred inner update); the state they
+    leave serves the per-routine entry points as the reference's routines
+    would: face states, fluxes and corner ghosts after two dual-time steps."""
+    case = C.random_case(seed=9, ni=(13, 10), nj=(70, 70), viscous=1, with_cut=True, scheme=C.SCHEME_DUAL)
+    o, s, _ = _run_steps(case, 2, fast=True)
+    o.set_w(); s.set_w(); o.set_thermo_prop(); s.set_thermo_prop()
+    o.set_bc(); s.set_BC()
+    _cmp(o, s, "W", lambda a, ni, nj: a, tol=1e-11)
+    o.set_hv(); s.set_HV()
+    for f, sel in (("WHLI", iface), ("WHRI", iface), ("WHLJ", jface), ("WHRJ", jface)):
+        _cmp(o, s, f, sel, tol=1e-11)
+    o.set_tg(); s.set_TG(); o.set_tgv(); s.set_TGv()
+    _cmp(o, s, "TGI", iface, tol=1e-10)
+    _cmp(o, s, "TGVJ", jface, tol=1e-9)
+    _cmp(o, s, "DQDT", interior, tol=1e-10)
