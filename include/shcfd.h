@@ -192,6 +192,22 @@ int shcfd_chem_boundary_state(const shcfd_mech *mech, double p, double T, const 
  * Synchronous; allocates and frees its own device buffers. */
 int shcfd_chem_eval(const shcfd_mech *mech, int op, int nstates, const double *in, double *out, int *status,
                     double dt);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, code between graph captures, harnesses that
+ * forbid hipMalloc): shcfd_chem_eval with the same arguments, rows, results
+ * and error behaviour, but it calls no hipMalloc or hipFree.  It stages the
+ * mechanism table, the input and output rows and the status words in one
+ * fixed device array per GPU of SHCFD_CHEM_SCRATCH_BYTES, held until the
+ * process ends: a call needs
+ *   256-aligned sizeof(shcfd_mech) + 256-aligned 8*n*row_in
+ *   + 256-aligned 8*n*row_out + 256-aligned 4*n
+ * bytes and returns SHCFD_E_TOO_LARGE (nothing computed) when that exceeds
+ * the array.  Synchronous, on the null stream of the current device.  All
+ * calls on one GPU share its array: calls that overlap on one GPU (from
+ * several host threads) race and must be serialised by the caller. */
+#define SHCFD_CHEM_SCRATCH_BYTES (16u << 20)
+int shcfd_chem_eval_noalloc(const shcfd_mech *mech, int op, int nstates, const double *in, double *out,
+                            int *status, double dt);
 
 /* ---- batches of independent blocks (no counterpart in the reference, whose
  * kinematic viscosity is one compile-time constant, thermal_model.raw.f90:4,
@@ -199,7 +215,8 @@ int shcfd_chem_eval(const shcfd_mech *mech, int op, int nstates, const double *i
 /* per-block kinematic viscosity of the ideal gas (mu = rho nu); shcfd_create
  * sets every block to params->nu */
 int shcfd_set_block_nu(shcfd_ctx *ctx, const double *nu);
-/* per-block flag: 0 freezes a block (its implicit update leaves q unchanged) */
+/* per-block flag: 0 freezes a block: the implicit and explicit updates and
+ * the chemistry source leave its q unchanged */
 int shcfd_set_block_active(shcfd_ctx *ctx, const int *active);
 /* the energy-change RMS of check_convergence (check_convergence.f90:17-28)
  * for each block separately */
@@ -207,9 +224,14 @@ int shcfd_energy_rms_blocks(shcfd_ctx *ctx, double *rms);
 /* nsteps physical steps (numbered from first_step) with no host
  * synchronisation between steps; graph_steps > 0 captures that many steps
  * into one HIP graph and replays it, 0 launches eagerly, < 0 uses the
- * schedule chosen by shcfd_autotune.  The schemes with a dual-time inner
- * loop (NR, precon, dual), which check the inner residual on the host after
- * every step, run step by step through shcfd_step. */
+ * schedule chosen by shcfd_autotune.  For the schemes with a dual-time inner
+ * loop (NR, precon, dual) a graph holds whole physical steps (every inner
+ * iteration, its reductions and updates); each step stores its first and
+ * last inner residual on the device, and after each replay the host applies
+ * check_internal_loop_convergence (sch_dual.raw.f90:284-299) to the steps in
+ * order.  A failing step is then reported after the whole replay, i.e. with
+ * the state of the replay's last step; eager launches (graph_steps = 0) stop
+ * at the failing step, as the reference does. */
 int shcfd_iterate(shcfd_ctx *ctx, int first_step, int nsteps, int graph_steps);
 /* times candidate schedules (optimised or per-routine sweeps, tile rows of
  * the per-cell kernels, eager or graph launches) over trial_steps steps each

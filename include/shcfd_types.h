@@ -34,7 +34,9 @@ enum {
     SHCFD_E_TNEWTON = 7,       /* T-from-e Newton iteration limit            */
     SHCFD_E_CHEM = 8,          /* chemistry integrator failure               */
     SHCFD_E_SPIN = 9,          /* sweep wavefront flag wait exceeded bound   */
-    SHCFD_E_STATE = 10         /* call made in the wrong state / scheme      */
+    SHCFD_E_STATE = 10,        /* call made in the wrong state / scheme      */
+    SHCFD_E_TOO_LARGE = 11     /* request exceeds the fixed device scratch of
+                                  a no-allocation entry point              */
 };
 
 /* ---- time schemes (checkout.inp "time scheme", checkout_flow.py:134-185) -- */

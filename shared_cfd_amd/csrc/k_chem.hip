@@ -77,7 +77,7 @@ template <int NS, int MODE>
 __global__ void __launch_bounds__(256) k_reaction(DevCtx d, const Tile* tiles, const shcfd_mech* __restrict__ m) {
     TileCoord t = tile_coord(tiles);
     const DevBlock B = d.blk[t.b];
-    bool inside = !(t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj);
+    bool inside = !(t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) && B.active;  // frozen blocks keep their state
     int att = 0;
     if (inside) att = reaction_cell<NS, MODE>(d, B, t, m);
     double w = (double)att;
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(64) k_reaction_lds(DevCtx d, const Tile* tiles
     const int i = tl.i0 + (int)threadIdx.x, j = tl.j0 + (int)(blockIdx.x % tile_rows);
     const DevBlock B = d.blk[tl.b];
     int att = 0;
-    if (i >= 1 && i <= B.ni && j >= 1 && j <= B.nj) {
+    if (i >= 1 && i <= B.ni && j >= 1 && j <= B.nj && B.active) {  // frozen blocks keep their state
         using D = Dims<NS>;
         const long long c = cidx(B, i, j);
         const double dt = d.scal[SCAL_DT_GRBL];

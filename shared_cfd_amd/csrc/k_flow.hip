@@ -652,7 +652,8 @@ __global__ void __launch_bounds__(256) k_explicit(DevCtx d, const Tile* tiles, i
     using D = Dims<NY>;
     TileCoord t = tile_coord(tiles);
     const DevBlock B = d.blk[t.b];
-    if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) return;
+    // a frozen block (active = 0) keeps its state
+    if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj || !B.active) return;
     const long long c = cidx(B, t.i, t.j);
     double s[D::dq];
     flux_balance<NY>(d, B, c, s);

@@ -104,6 +104,7 @@ const char* shcfd_strerror(int code) {
     case SHCFD_E_CHEM: return "chemistry integrator failure";
     case SHCFD_E_SPIN: return "sweep wavefront wait exceeded its bound";
     case SHCFD_E_STATE: return "call not valid for this scheme/state";
+    case SHCFD_E_TOO_LARGE: return "request exceeds the fixed device scratch of a no-allocation entry point";
     default: return "unknown error";
     }
 }
@@ -227,7 +228,7 @@ int shcfd_destroy(shcfd_ctx* ctx) {
     for (int fld = 0; fld < SHCFD_F_COUNT; fld++) if (ctx->f[fld]) hipFree(ctx->f[fld]);
     void* ptrs[] = {ctx->dblk, ctx->dtiles, ctx->dstrips, ctx->dprog, ctx->dbnd, ctx->dpart, ctx->scal,
                     ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool, ctx->dmech, ctx->dskew, ctx->dsbase,
-                    ctx->djac16, ctx->dstrips16, ctx->dbnd16, ctx->dnasa9};
+                    ctx->djac16, ctx->dstrips16, ctx->dbnd16, ctx->dnasa9, ctx->dhist};
     for (void* p : ptrs) if (p) hipFree(p);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -237,6 +238,7 @@ int shcfd_destroy(shcfd_ctx* ctx) {
 int shcfd_set_params(shcfd_ctx* ctx, const shcfd_params* params) {
     if (!ctx || !params || !params_valid(params)) return SHCFD_E_ARG;
     if (params->dimension != ctx->p.dimension) return SHCFD_E_STATE;  // the metrics belong to the geometry
+    drop_graph(ctx);  // a captured graph holds the parameters by value
     ctx->p = *params;
     return SHCFD_OK;
 }
@@ -305,6 +307,9 @@ int shcfd_set_cut_table(shcfd_ctx* ctx, const int* rows, int nrows) {
                 if (!inb(bd, di, dj) || !inb(bs, si, sj)) return SHCFD_E_ARG;
             }
     }
+    // a captured graph holds the table's device address and row count
+    drop_graph(ctx);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctx->dcut) { hipFree(ctx->dcut); ctx->dcut = nullptr; }
     ctx->hcut.assign(rows, rows + SHCFD_CUT_ROW * nrows);
     ctx->ncut = nrows;
@@ -340,6 +345,8 @@ int shcfd_set_bc_table(shcfd_ctx* ctx, const int* rows, int nrows, const double*
         }
         maxlen = std::max(maxlen, e - s + 1);
     }
+    drop_graph(ctx);  // a captured graph holds the table and pool addresses
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctx->dbc) { hipFree(ctx->dbc); ctx->dbc = nullptr; }
     if (ctx->dpool) { hipFree(ctx->dpool); ctx->dpool = nullptr; }
     ctx->hbc.assign(rows, rows + SHCFD_BC_ROW * nrows);
@@ -373,6 +380,7 @@ int shcfd_init_geometry(shcfd_ctx* ctx) {
 }
 
 __global__ void k_set_scal(DevCtx d, int slot, double v) { d.scal[slot] = v; }
+__global__ void k_store_scal(DevCtx d, int slot, double* dst) { *dst = d.scal[slot]; }
 
 int shcfd_set_time(shcfd_ctx* ctx, double tt, int first_dual) {
     if (!ctx) return SHCFD_E_ARG;
@@ -973,6 +981,7 @@ int shcfd_set_option(shcfd_ctx* ctx, int option, int value) {
     case SHCFD_OPT_SPIN_LIMIT:
         if (value < 1) return SHCFD_E_ARG;
         ctx->spin_limit = value;
+        drop_graph(ctx);  // captured sweep launches hold the bound by value
         return SHCFD_OK;
     case SHCFD_OPT_SKEW_SWEEP:
         ctx->use_skew = value ? 1 : 0;
@@ -1179,55 +1188,71 @@ static int enqueue_step(shcfd_ctx* ctx, int step) {
     return rc;
 }
 
+// The launches of one physical step of an inner-loop scheme (NR, precon,
+// dual): NR/2d/main.main.raw.f90:1-20, precon/2d/main.main.raw.f90:1-19 (no
+// set_dt: set_ABpm sets dt_mat), dual/2d/main.main.raw.f90:1-20.  Only kernel
+// launches on the context stream, so the whole step -- every inner iteration,
+// its (res, omega) reductions and the deferred update -- can be captured into
+// one graph.  hist (may be NULL) receives (res1, res) of the step, for the
+// host's check_internal_loop_convergence after a graph replay.  The step ends
+// by clearing the BDF1 flag (first_dual) on the device.
+static int enqueue_inner_step(shcfd_ctx* ctx, double* hist) {
+    int rc = SHCFD_OK;
+#define TRY(x) do { rc = (x); if (rc) return rc; } while (0)
+    TRY(part_primitive(ctx));
+    if (ctx->p.scheme != SHCFD_SCHEME_PRECON) TRY(shcfd_set_dt(ctx, 0, nullptr));
+    TRY(shcfd_copy_state(ctx, SHCFD_F_QPP, SHCFD_F_QP));
+    TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
+    for (int j = 1; j < ctx->p.num_internal_loop; j++) {
+        TRY(part_primitive(ctx));
+        TRY(part_main(ctx));
+        TRY(abpm_for_step(ctx));
+        ctx->dqdt_inline = 1;   // set_dqdt, formed in the RHS pass
+        ctx->defer_update = 1;  // q += omega Dq, in the next primitive pass
+        rc = implicit_async(ctx, j);
+        ctx->defer_update = 0;
+        if (rc) return rc;
+    }
+    TRY(apply_pending_update(ctx));
+#undef TRY
+    if (hist) {
+        hipLaunchKernelGGL(k_store_scal, dim3(1), dim3(1), 0, ctx->stream, ctx->dev(), (int)SCAL_RES1, hist);
+        hipLaunchKernelGGL(k_store_scal, dim3(1), dim3(1), 0, ctx->stream, ctx->dev(), (int)SCAL_RES, hist + 1);
+    }
+    hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(1), 0, ctx->stream, ctx->dev(), (int)SCAL_FIRST_DUAL, 0.0);
+    LAUNCHCHK();
+    return SHCFD_OK;
+}
+
+// check_internal_loop_convergence: sch_dual.raw.f90:284-299 (the same routine
+// in sch_NR.raw.f90:216-231 and sch_precon.raw.f90:276-291)
+static int check_inner_rate(shcfd_ctx* ctx, int step, int step_res, double res1, double res) {
+    if (res != res || res1 != res1) return SHCFD_E_NAN;
+    const double rate = res / res1;
+    if (rate > ctx->p.res_rate_warning) {
+        fprintf(stderr, "WARNING:NOT CONVERGED AT INTERNAL LOOP (step,res_rate) = %12d %11.3E\n", step, rate);
+        if (rate > ctx->p.res_rate_error && step != step_res + 1) return SHCFD_E_RESRATE;
+    }
+    return SHCFD_OK;
+}
+
 int shcfd_step(shcfd_ctx* ctx, int step, int step_res, double* res_first, double* res_last) {
     if (!ctx) return SHCFD_E_ARG;
     if (!inner_loop_scheme(ctx->p.scheme)) {
         int rc = enqueue_step(ctx, step);
         return rc ? rc : shcfd_check_error(ctx);
     }
-    int rc = SHCFD_OK;
-#define TRY(x) do { rc = (x); if (rc) return rc; } while (0)
-    switch (ctx->p.scheme) {
-    case SHCFD_SCHEME_NR:      // NR/2d/main.main.raw.f90:1-20, the loop of dual time
-    case SHCFD_SCHEME_PRECON:  // precon/2d/main.main.raw.f90:1-19: no set_dt, set_ABpm sets dt_mat
-    case SHCFD_SCHEME_DUAL: {  // dual/2d/main.main.raw.f90:1-20
-        TRY(part_primitive(ctx));
-        if (ctx->p.scheme != SHCFD_SCHEME_PRECON) TRY(shcfd_set_dt(ctx, step, nullptr));
-        TRY(shcfd_copy_state(ctx, SHCFD_F_QPP, SHCFD_F_QP));
-        TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
-        for (int j = 1; j < ctx->p.num_internal_loop; j++) {
-            TRY(part_primitive(ctx));
-            TRY(part_main(ctx));
-            TRY(abpm_for_step(ctx));
-            ctx->dqdt_inline = 1;  // set_dqdt, formed in the RHS pass
-            ctx->defer_update = 1;  // q += omega Dq, in the next primitive pass
-            rc = implicit_async(ctx, j);
-            ctx->defer_update = 0;
-            if (rc) return rc;
-        }
-        TRY(apply_pending_update(ctx));
-        double sc[SCAL_COUNT];
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        HIPCHK(hipMemcpy(sc, ctx->scal, sizeof(sc), hipMemcpyDeviceToHost));
-        if (res_first) *res_first = sc[SCAL_RES1];
-        if (res_last) *res_last = sc[SCAL_RES];
-        TRY(shcfd_check_error(ctx));
-        // check_internal_loop_convergence: sch_dual.raw.f90:284-299 (the same
-        // routine in sch_NR.raw.f90:216-231 and sch_precon.raw.f90:276-291)
-        double rate = sc[SCAL_RES] / sc[SCAL_RES1];
-        if (rate > ctx->p.res_rate_warning) {
-            fprintf(stderr, "WARNING:NOT CONVERGED AT INTERNAL LOOP (step,res_rate) = %12d %11.3E\n", step, rate);
-            if (rate > ctx->p.res_rate_error && step != step_res + 1) return SHCFD_E_RESRATE;
-        }
-        hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(1), 0, ctx->stream, ctx->dev(), (int)SCAL_FIRST_DUAL, 0.0);
-        ctx->first_dual = 0;
-        break;
-    }
-    default:
-        return SHCFD_E_STATE;
-    }
-#undef TRY
-    return shcfd_check_error(ctx);
+    int rc = enqueue_inner_step(ctx, nullptr);
+    if (rc) return rc;
+    ctx->first_dual = 0;
+    double sc[SCAL_COUNT];
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(sc, ctx->scal, sizeof(sc), hipMemcpyDeviceToHost));
+    if (res_first) *res_first = sc[SCAL_RES1];
+    if (res_last) *res_last = sc[SCAL_RES];
+    rc = shcfd_check_error(ctx);
+    if (rc) return rc;
+    return check_inner_rate(ctx, step, step_res, sc[SCAL_RES1], sc[SCAL_RES]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1268,31 +1293,40 @@ int shcfd_energy_rms_blocks(shcfd_ctx* ctx, double* rms) {
     return SHCFD_OK;
 }
 
-// allocate everything a step uses, outside any graph capture
+// allocate everything a step uses, outside any graph capture: the optimised
+// nY = 1 path, else the quad-row copy of the 16-row sweeps (nY > 1, all
+// implicit schemes but dual time), else the strip-skewed copy (nY > 1 dual
+// time, or when the quad-row copy could not be allocated)
 static void warm_allocations(shcfd_ctx* ctx) {
-    if (!fast_enabled(ctx)) skew_enabled(ctx);
+    const int s = ctx->p.scheme;
+    const bool implicit = s == SHCFD_SCHEME_LUSGS || s == SHCFD_SCHEME_PRECON_LUSGS || inner_loop_scheme(s);
+    if (!implicit || fast_enabled(ctx)) return;
+    if (!q16_enabled(ctx)) skew_enabled(ctx);
 }
 
 int shcfd_iterate(shcfd_ctx* ctx, int first_step, int nsteps, int graph_steps) {
     if (!ctx || nsteps < 0) return SHCFD_E_ARG;
     if (graph_steps < 0) graph_steps = ctx->tuned_graph_steps;
-    if (inner_loop_scheme(ctx->p.scheme)) {
-        for (int k = 0; k < nsteps; k++) {
-            int rc = shcfd_step(ctx, first_step + k, first_step - 1, nullptr, nullptr);
-            if (rc) return rc;
-        }
-        return SHCFD_OK;
-    }
+    const bool inner = inner_loop_scheme(ctx->p.scheme);
+    const int step_res = first_step - 1;
     warm_allocations(ctx);
     int done = 0;
     if (graph_steps > 0 && nsteps >= graph_steps) {
         if (ctx->graph_steps != graph_steps) {
             drop_graph(ctx);
+            if (inner && ctx->nhist < graph_steps) {
+                if (ctx->dhist) hipFree(ctx->dhist);
+                ctx->dhist = nullptr;
+                ctx->nhist = 0;
+                HIPCHK(hipMalloc((void**)&ctx->dhist, sizeof(double) * 2 * graph_steps));
+                ctx->nhist = graph_steps;
+            }
             const unsigned prof = ctx->prof;
             ctx->prof = 0;  // no event records inside the graph
             HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             int rc = SHCFD_OK;
-            for (int k = 0; k < graph_steps && !rc; k++) rc = enqueue_step(ctx, first_step + k);
+            for (int k = 0; k < graph_steps && !rc; k++)
+                rc = inner ? enqueue_inner_step(ctx, ctx->dhist + 2 * k) : enqueue_step(ctx, first_step + k);
             hipGraph_t g = nullptr;
             hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
             ctx->prof = prof;
@@ -1308,10 +1342,26 @@ int shcfd_iterate(shcfd_ctx* ctx, int first_step, int nsteps, int graph_steps) {
             }
             ctx->graph_steps = graph_steps;
         }
-        for (; done + graph_steps <= nsteps; done += graph_steps) HIPCHK(hipGraphLaunch(ctx->graph, ctx->stream));
+        for (; done + graph_steps <= nsteps; done += graph_steps) {
+            HIPCHK(hipGraphLaunch(ctx->graph, ctx->stream));
+            if (!inner) continue;
+            // the host checks of each replayed step, in step order
+            ctx->first_dual = 0;
+            std::vector<double> h(2 * (size_t)graph_steps);
+            HIPCHK(hipMemcpyAsync(h.data(), ctx->dhist, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            int rc = shcfd_check_error(ctx);
+            if (rc) return rc;
+            for (int k = 0; k < graph_steps; k++) {
+                rc = check_inner_rate(ctx, first_step + done + k, step_res, h[2 * k], h[2 * k + 1]);
+                if (rc) return rc;
+            }
+        }
     }
     for (; done < nsteps; done++) {
-        int rc = enqueue_step(ctx, first_step + done);
+        int rc = inner ? shcfd_step(ctx, first_step + done, step_res, nullptr, nullptr)
+                       : enqueue_step(ctx, first_step + done);
         if (rc) return rc;
     }
     return shcfd_check_error(ctx);
@@ -1324,8 +1374,9 @@ int shcfd_iterate(shcfd_ctx* ctx, int first_step, int nsteps, int graph_steps) {
 // tuning does not advance the solution.  choice[4] = fast, tile rows, graph
 // steps, best ms per step (may be NULL).
 int shcfd_autotune(shcfd_ctx* ctx, int trial_steps, double* choice) {
-    if (!ctx || trial_steps < 1 || inner_loop_scheme(ctx->p.scheme)) return SHCFD_E_ARG;
+    if (!ctx || trial_steps < 1) return SHCFD_E_ARG;
     const size_t nq = (size_t)ctx->dimq * (size_t)ctx->NC;
+    const int first_dual = ctx->first_dual;
     double* save = nullptr;
     double* ssave = nullptr;
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1343,6 +1394,7 @@ int shcfd_autotune(shcfd_ctx* ctx, int trial_steps, double* choice) {
         for (int k = 0; k < 3; k++)
             HIPCHK(hipMemcpy(ctx->f[fields[k]], save + k * nq, sizeof(double) * nq, hipMemcpyDeviceToDevice));
         HIPCHK(hipMemcpy(ctx->scal, ssave, sizeof(double) * SCAL_COUNT, hipMemcpyDeviceToDevice));
+        ctx->first_dual = first_dual;
         return SHCFD_OK;
     };
     hipEvent_t e0, e1;

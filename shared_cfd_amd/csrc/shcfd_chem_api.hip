@@ -2,6 +2,7 @@
 // states and the batched 0-D harness on the GPU.
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -77,7 +78,76 @@ int boundary_state(const shcfd_mech* m, double p, double T, const double* vrho_r
 
 }  // namespace
 
+// ---- device scratch of shcfd_chem_eval_noalloc -------------------------------
+// One file-scope array per GPU, held until the process ends.  A call stages
+// the mechanism table, the input rows, the output rows and the status words
+// in it, each part starting on a 256-byte boundary.  Host code never uses the
+// array's name as an address: the per-device address comes from
+// hipGetSymbolAddress, taken once per device on the first call made with that
+// device current.
+static __device__ __align__(256) unsigned char scratch[SHCFD_CHEM_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(unsigned char** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return SHCFD_E_HIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return SHCFD_E_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return SHCFD_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+}  // namespace
+
 extern "C" {
+
+int shcfd_chem_eval_noalloc(const shcfd_mech* m, int op, int n, const double* in, double* out, int* status,
+                            double dt) {
+    if (!m || n < 0 || !in || !out || (m->ns != 8 && m->ns != 9) || m->nr < 1 || m->nr > SHCFD_CHEM_MAX_NR)
+        return SHCFD_E_ARG;
+    const int ldi = row_in(op, m->ns), ldo = row_out(op, m->ns);
+    if (ldi < 0) return SHCFD_E_ARG;
+    if (n == 0) return SHCFD_OK;
+    const size_t o_mech = 0;
+    const size_t o_in = o_mech + align256(sizeof(shcfd_mech));
+    const size_t o_out = o_in + align256(sizeof(double) * (size_t)n * ldi);
+    const size_t o_st = o_out + align256(sizeof(double) * (size_t)n * ldo);
+    const size_t need = o_st + align256(sizeof(int) * (size_t)n);
+    if (need > (size_t)SHCFD_CHEM_SCRATCH_BYTES) return SHCFD_E_TOO_LARGE;
+    unsigned char* base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc) return rc;
+    shcfd_mech* dm = reinterpret_cast<shcfd_mech*>(base + o_mech);
+    double* din = reinterpret_cast<double*>(base + o_in);
+    double* dout = reinterpret_cast<double*>(base + o_out);
+    int* dst = reinterpret_cast<int*>(base + o_st);
+    // every output row and status word is written by the kernel, so nothing
+    // left by an earlier call needs clearing
+    if (hipMemcpy(dm, m, sizeof(shcfd_mech), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(din, in, sizeof(double) * (size_t)n * ldi, hipMemcpyHostToDevice) != hipSuccess)
+        return SHCFD_E_HIP;
+    if (m->ns == 8)
+        launch0d<8>(op, dm, n, din, ldi, dout, ldo, dst, dt);
+    else
+        launch0d<9>(op, dm, n, din, ldi, dout, ldo, dst, dt);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(0) != hipSuccess ||
+        hipMemcpy(out, dout, sizeof(double) * (size_t)n * ldo, hipMemcpyDeviceToHost) != hipSuccess ||
+        (status && hipMemcpy(status, dst, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess))
+        return SHCFD_E_HIP;
+    return SHCFD_OK;
+}
 
 int shcfd_chem_boundary_state(const shcfd_mech* m, double p, double T, const double* vrho_rel, double* w,
                               double* vhi, double* dhi) {

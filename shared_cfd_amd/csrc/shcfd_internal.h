@@ -19,7 +19,7 @@
 namespace shcfd {
 
 struct DevBlock {
-    int ni, nj, pitch, active;  // active = 0: the implicit update leaves q unchanged
+    int ni, nj, pitch, active;  // active = 0: the updates and the chemistry leave q unchanged
     long long off;              // index of cell (-1,-1)
     double nu;                  // kinematic viscosity of the ideal gas (mu = rho nu)
 };
@@ -184,6 +184,8 @@ struct Ctx {
     int dqdt_inline = 0;           // the next implicit update forms dq/dt in its RHS pass
     int defer_update = 0;          // the next inner iteration leaves q += omega Dq to the next primitive pass
     int update_pending = 0;        // q += omega Dq is still to be applied
+    double* dhist = nullptr;       // (res1, res) per physical step of an inner-loop graph
+    int nhist = 0;
     int q16_active = 0;            // the last set_ABpm of a step wrote the quad-row copy
     int skew_active = 0;           // the last set_ABpm of a step wrote the skewed copy
     double* dskew = nullptr;

@@ -29,7 +29,7 @@ EXPORTS = [
     "shcfd_calc_next_step_rk2", "shcfd_copy_state", "shcfd_energy_rms", "shcfd_prepare", "shcfd_step",
     "shcfd_check_error", "shcfd_profile_enable", "shcfd_profile_read", "shcfd_set_option",
     "shcfd_mech_parse", "shcfd_set_mechanism", "shcfd_load_mechanism", "shcfd_proceed_reaction",
-    "shcfd_chem_boundary_state", "shcfd_chem_eval", "shcfd_chem_iterations",
+    "shcfd_chem_boundary_state", "shcfd_chem_eval", "shcfd_chem_eval_noalloc", "shcfd_chem_iterations",
     "shcfd_set_block_nu", "shcfd_set_block_active", "shcfd_energy_rms_blocks", "shcfd_iterate", "shcfd_autotune",
     "shcfd_adjoint_objective", "shcfd_objective", "shcfd_adjoint_upload", "shcfd_adjoint_download",
     "shcfd_adjoint_poolbar", "shcfd_residual_vjp", "shcfd_adjoint_solve",
