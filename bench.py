@@ -128,6 +128,73 @@ def bench_reacting(a):
     return out
 
 
+def bench_sod(a):
+    """C1: Sod shock tube, 400x4 cells, MUSCL order 2 with minmod, LU-SGS,
+    CFL 0.5, 200 steps: a graph of 20 steps is captured and replayed for
+    steps 1-20 (untimed), then replayed 9 times for steps 21-200 (timed)."""
+    from shared_cfd_amd import cases as C
+    from shared_cfd_amd.native import Solver
+
+    case = C.sod(400, 4, limiter=C.LIMITER_MINMOD, scheme=C.SCHEME_LUSGS, cfl=0.5)
+    s = Solver(case)
+    s.prepare()
+    s.iterate(1, 20, 20)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.iterate(21, 180, 20)
+    s.synchronize()
+    t1 = time.perf_counter()
+    _, tt = s.get_time()
+    s.close()
+    return {"workload": case.description + ", MUSCL 2 + minmod, LU-SGS, CFL 0.5, 200 steps (steps 21-200 timed)",
+            "cells": case.ncells, "steps_timed": 180, "seconds": round(t1 - t0, 5),
+            "value": round(case.ncells * 180 / (t1 - t0) / 1e6, 3), "unit": "Mcell-updates/s", "t_end": tt}
+
+
+def bench_batched_key(a):
+    """C3 as an extra key of the default line: the 256-block batch (graph +
+    autotuned schedule) for a few steps, against the same blocks one by one
+    with the optimised kernels (graph of steps per block) and with the
+    per-routine kernels (eager)."""
+    from shared_cfd_amd import cases as C
+    from shared_cfd_amd.native import Solver
+
+    case = C.batched_plates(n=256, seed=0)
+    cells = case.ncells
+    s = Solver(case)
+    s.prepare()
+    choice = s.autotune(2)
+    s.iterate(1, 8, -1)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.iterate(9, a.batch_steps, -1)
+    s.synchronize()
+    t1 = time.perf_counter()
+    s.close()
+    batched = cells * a.batch_steps / (t1 - t0) / 1e6
+    seq = {}
+    for name, fast, gs in (("sequential_optimised", True, 4), ("sequential_plain", False, 0)):
+        tt, nc = 0.0, 0
+        for b in range(case.nblocks):
+            sub = C.sub_case(case, [b])
+            p = Solver(sub, fast=fast)
+            p.prepare()
+            p.iterate(1, max(gs, 1), gs)          # warm-up (and capture)
+            p.synchronize()
+            tb0 = time.perf_counter()
+            p.iterate(2, a.seq_steps, gs)
+            p.synchronize()
+            tt += time.perf_counter() - tb0
+            nc += sub.ncells * a.seq_steps
+            p.close()
+        seq[name] = {"value": round(nc / tt / 1e6, 3), "unit": "Mcell-updates/s", "instances": case.nblocks,
+                     "steps": a.seq_steps}
+    return {"workload": case.description, "cells": cells, "blocks": case.nblocks, "steps": a.batch_steps,
+            "value": round(batched, 3), "unit": "Mcell-updates/s", "autotuned": choice, **seq,
+            "speedup_vs_sequential_plain": round(batched / seq["sequential_plain"]["value"], 2),
+            "speedup_vs_sequential_optimised": round(batched / seq["sequential_optimised"]["value"], 2)}
+
+
 def bench_batched(a):
     """C3: 256 independent laminar plate/channel blocks of uneven sizes in one
     context (one launch per kernel over all blocks, autotuned schedule), against
@@ -277,6 +344,9 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=None,
                     help="CPU sample length (default: 15 physical steps; reacting: 30 steps)")
     ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--prof-steps", type=int, default=2, help="eager profiled steps for kernel durations")
+    ap.add_argument("--no-extra", action="store_true", help="skip the Sod, batched and reacting keys")
+    ap.add_argument("--batch-steps", type=int, default=16)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -308,42 +378,53 @@ def main(argv=None):
     cells = case.ncells
     s = Solver(case)
     s.prepare()
-    step = 0
-    for _ in range(a.warmup):
-        step += 1
-        s.step(step, 0)
-    # short all-class profile to find the dominant kernel
+    step = 1
+    # warm-up: physical steps replayed from a graph of one step (captured here)
+    s.iterate(step, max(a.warmup, 1), 1)
+    step += max(a.warmup, 1)
+    # kernel durations: HIP events on the library's stream around every
+    # launch, over eager steps (the launches a graph replay makes are the same
+    # kernels with the same arguments); first all classes, to find the
+    # dominant one, then that class alone
     s.profile(None)
-    step += 1
     s.step(step, 0)
+    step += 1
     prof_all = s.profile_read()
     dominant = max((k for k in prof_all if alg_bytes_per_cell(k) > 0), key=lambda k: prof_all[k][0])
     s.profile([dominant])
-    s.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step += 1
+    for _ in range(a.prof_steps):
         s.step(step, 0)
-    s.synchronize()
-    t1 = time.perf_counter()
+        step += 1
     prof = s.profile_read()
     s.profile([])
+    # timed region: K physical steps as graph replays (one physical step of
+    # 20 inner iterations per graph; the host checks each step's inner
+    # residual ratio after its replay, as check_internal_loop_convergence does)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.iterate(step, a.steps, 1)
+    s.synchronize()
+    t1 = time.perf_counter()
+    step += a.steps
     dt = t1 - t0
     value = cells * inner * a.steps / dt / 1e6
     ms_dom, n_dom = prof[dominant]
     avg_s = ms_dom / n_dom / 1e3
     bytes_launch = alg_bytes_per_cell(dominant) * cells
     achieved = bytes_launch / avg_s / 1e9
-    # measured HBM bytes per launch of the dominant kernel (rocprofv3 --pmc
-    # FETCH_SIZE / WRITE_SIZE runs on this workload, copy-calibrated)
+    # HBM bytes per launch of the dominant kernel from rocprofv3 --pmc
+    # FETCH_SIZE / WRITE_SIZE runs on this workload (copy-calibrated); a
+    # profile-sourced figure, not measured in this run
     traffic = None
     tr_path = os.path.join(ROOT, "profiles", "forward_step_258k_traffic.json")
     if a.n_per_unit == 320 and os.path.exists(tr_path):
         with open(tr_path) as f:
             traffic = json.load(f).get(dominant)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dominant,
-                "avg_launch_us": round(avg_s * 1e6, 3),
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of this workload "
+                                  "(profiles/forward_step_258k_traffic.json), not measured in this run",
+                "kernel": dominant, "avg_launch_us": round(avg_s * 1e6, 3), "launches_timed": n_dom,
                 "alg_bytes_per_launch": bytes_launch,
                 "share_of_step": {k: round(v[0] / sum(x[0] for x in prof_all.values()), 4) for k, v in prof_all.items()}}
 
@@ -388,6 +469,16 @@ def main(argv=None):
                    "cells": cells, "inner_iterations_per_step": inner, "blocks": case.nblocks},
         "roofline": roofline, "cpu_baseline": cpu, "plain_gpu": plain,
     }
+    # the other workloads, measured in the same run: Sod (400x4, 200 steps),
+    # the 256-block batch with its sequential controls, and the reacting
+    # mixing layer (1024x512, 9 species / 21 reactions)
+    if not a.no_extra:
+        s.close()
+        out["sod"] = bench_sod(a)
+        out["batched"] = bench_batched_key(a)
+        ra = argparse.Namespace(**vars(a))
+        ra.steps, ra.warmup, ra.ni, ra.nj, ra.cpu_steps = 3, 1, 1024, 512, 3
+        out["reacting"] = bench_reacting(ra)
     # the metric's second figure: adjoint/primal cost at 1e-10 (airfoil
     # shape-gradient workload; the finite-difference check runs only with
     # --workload adjoint)

@@ -118,56 +118,68 @@ __device__ inline long long op_index(const QStrip& S, int dd, int L, int p, int 
     return (S.base / 64 + dd) * 512 + p * 128 + 2 * L + e;
 }
 
-__global__ void __launch_bounds__(256) k_fast_prep(DevCtx d, const QStrip* strips, const int2* work, double* R_sl,
-                                                   double* G, double* H, double* hand, int nstrips, long long hstride) {
-    // refill the hand-off rows of both sweeps: the sentinel for the columns
-    // 1..ni of the strip's block, zero elsewhere
-    {
-        const long long stride = (long long)gridDim.x * blockDim.x;
-        const long long nhand = 2LL * (nstrips + 1) * hstride;
-        const double sv = __longlong_as_double((long long)SENTINEL);
-        for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nhand; k += stride) {
-            const int row = (int)((k / hstride) % (nstrips + 1));
-            const long long col = (k % hstride) / 4 - HOFF;
-            const bool live = row < nstrips && col >= 1 && col <= d.blk[strips[row].b].ni;
-            hand[k] = live ? sv : 0.0;
-        }
-    }
-    const int2 wk = work[blockIdx.x];
-    const QStrip S = strips[wk.x];
-    const DevBlock B = d.blk[S.b];
-    const int r = threadIdx.x & 15;
-    const int dd = wk.y + (threadIdx.x >> 4);
+// the folded operators of one cell in its skewed slots: (GA, GB) into G
+// (forward sweep, with R) or (HA, HB) into H (backward sweep)
+__device__ inline void prep_cell(const DevCtx& d, const QStrip& S, const DevBlock& B, int dd, int r, double* R_sl,
+                                 double* G, double* H, bool fwd) {
     int i, j;
     if (!slot_cell(S, B, dd, r, i, j)) return;
     const long long c = cidx(B, i, j);
     const bool dual = d.p.scheme == SHCFD_SCHEME_DUAL;
     CellJac<1> J;
     cell_jacobian<1>(d, c, B.pitch, J);
-    FLD(d, SHCFD_F_ALPHA, 0, c) = J.alpha;
-    if (J.precond && !dual) FLD(d, SHCFD_F_DT, 0, c) = J.dtm;
     double Sm[16], T[16];
     post_matrix(J, dual, Sm);
     const double dsci = FLD(d, SHCFD_F_DSCI, 0, c), dscj = FLD(d, SHCFD_F_DSCJ, 0, c);
     const double fa = dsci * J.alpha, fb = dscj * J.alpha;
-    // R (component c in lane 4r+c)
+    if (fwd) {
+        // R (component c in lane 4r+c)
 #pragma unroll
-    for (int k = 0; k < 4; k++) R_sl[slot1(S.base, dd, 4 * r + k)] = FLD(d, SHCFD_F_DQ, k, c);
-    // rows of the four operators: lane 4r+k holds row k (4 doubles)
-    const double* mats[4] = {J.Ap, J.Bp, J.Am, J.Bm};
-    const double fs[4] = {fa, fb, fa, fb};
+        for (int k = 0; k < 4; k++) R_sl[slot1(S.base, dd, 4 * r + k)] = FLD(d, SHCFD_F_DQ, k, c);
+    }
+    // rows of the two operators: lane 4r+k holds row k (4 doubles)
+    const double* mats[2] = {fwd ? J.Ap : J.Am, fwd ? J.Bp : J.Bm};
+    const double fs[2] = {fa, fb};
+    double* o = fwd ? G : H;
 #pragma unroll
-    for (int m = 0; m < 4; m++) {
+    for (int m = 0; m < 2; m++) {
         mm4(mats[m], Sm, T);
-        double* o = m < 2 ? G : H;
         // doubles (2p', 2p'+1) of a lane's row are adjacent: one 16-byte store
 #pragma unroll
         for (int k = 0; k < 4; k++)
 #pragma unroll
             for (int lp = 0; lp < 2; lp++)
-                *reinterpret_cast<double2*>(&o[op_index(S, dd, 4 * r + k, 2 * (m & 1) + lp, 0)]) =
+                *reinterpret_cast<double2*>(&o[op_index(S, dd, 4 * r + k, 2 * m + lp, 0)]) =
                     make_double2(fs[m] * T[k + 4 * (2 * lp)], fs[m] * T[k + 4 * (2 * lp + 1)]);
     }
+}
+
+// refill of the hand-off rows of both sweeps: the sentinel for the columns
+// 1..ni of each strip's block, zero elsewhere (grid-stride over all rows)
+__device__ inline void hand_refill(const DevCtx& d, const QStrip* strips, double* hand, int nstrips, long long hstride) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long nhand = 2LL * (nstrips + 1) * hstride;
+    const double sv = __longlong_as_double((long long)SENTINEL);
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nhand; k += stride) {
+        const int row = (int)((k / hstride) % (nstrips + 1));
+        const long long col = (k % hstride) / 4 - HOFF;
+        const bool live = row < nstrips && col >= 1 && col <= d.blk[strips[row].b].ni;
+        hand[k] = live ? sv : 0.0;
+    }
+}
+
+// prep of the forward sweep: R and (GA, GB) per cell, the hand-off rows.
+// Workgroup = 16 steps x 16 rows of one quad strip; thread = one cell.  The
+// backward operators (HA, HB) are formed by the forward sweep's launch on
+// the CUs its strips leave idle (k_fast_fwd).
+__global__ void __launch_bounds__(256) k_fast_prep(DevCtx d, const QStrip* strips, const int2* work, double* R_sl,
+                                                   double* G, double* hand, int nstrips, long long hstride) {
+    hand_refill(d, strips, hand, nstrips, hstride);
+    const int2 wk = work[blockIdx.x];
+    const QStrip S = strips[wk.x];
+    const DevBlock B = d.blk[S.b];
+    const int dd = wk.y + (int)(threadIdx.x >> 4), r = (int)(threadIdx.x & 15);
+    prep_cell(d, S, B, dd, r, R_sl, G, nullptr, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -370,7 +382,25 @@ __device__ inline void fwd_step(const Chunk& c, int t, int u, int dd, double& a,
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_fast_fwd(DevCtx d, const QStrip* strips, int nstrips, const double* R_sl, const double* G, double* V_sl,
-           double* hand, long long hstride, long long spin_limit) {
+           double* hand, long long hstride, long long spin_limit, const int2* work, int nwork, double* H) {
+    if ((int)blockIdx.x >= nstrips) {
+        // workgroups past the strips form the backward operators (HA, HB) of
+        // the work items, concurrently with the sweep: 16 steps x 16 rows per
+        // item, two cells per thread.  They wait on nothing, and the strips
+        // (lower workgroup ids) are dispatched before them.
+        const int nh = (int)gridDim.x - nstrips;
+        for (int w = (int)blockIdx.x - nstrips; w < nwork; w += nh) {
+            const int2 wk = work[w];
+            const QStrip S = strips[wk.x];
+            const DevBlock B = d.blk[S.b];
+#pragma unroll 1
+            for (int h = 0; h < 2; h++) {
+                const int t = (int)threadIdx.x + 128 * h;
+                prep_cell(d, S, B, wk.y + (t >> 4), t & 15, nullptr, nullptr, H, false);
+            }
+        }
+        return;
+    }
     const int s = blockIdx.x;
     const int L = threadIdx.x & 63;
     const QStrip S = strips[s];
@@ -722,15 +752,7 @@ __global__ void __launch_bounds__(256) k_fast_adj_post(DevCtx d, const QStrip* s
 // refill of the hand-off rows alone (the adjoint sweeps do not run k_fast_prep)
 __global__ void __launch_bounds__(256) k_fast_hand_fill(DevCtx d, const QStrip* strips, double* hand, int nstrips,
                                                         long long hstride) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long nhand = 2LL * (nstrips + 1) * hstride;
-    const double sv = __longlong_as_double((long long)SENTINEL);
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nhand; k += stride) {
-        const int row = (int)((k / hstride) % (nstrips + 1));
-        const long long col = (k % hstride) / 4 - HOFF;
-        const bool live = row < nstrips && col >= 1 && col <= d.blk[strips[row].b].ni;
-        hand[k] = live ? sv : 0.0;
-    }
+    hand_refill(d, strips, hand, nstrips, hstride);
 }
 
 }  // namespace shcfd
@@ -804,13 +826,22 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
     hipStream_t st = c.stream;
     double* hand_f = hand;
     double* hand_b = hand + hstride * (nstrips + 1);
+    // Workgroups of the forward launch past the strips form (HA, HB) on the
+    // CUs the strips leave idle.  The launch asks for 28 KB of dynamic LDS
+    // beside the sweep's 54 KB, so that no CU holds two of its workgroups:
+    // an operator workgroup next to a strip would take issue slots and
+    // vector-memory bandwidth from the strip's waves (measured on the 258k-cell
+    // forward step: 195 us per forward sweep with them side by side, 178 us
+    // apart, 167 us with (HA, HB) formed in prep instead, where prep costs
+    // 14 us more).
+    constexpr unsigned FWD_PAD_LDS = 28672;
+    const int nhwg = std::min(nwork, 512);
     pb(&c, PROF_PREP);
-    hipLaunchKernelGGL(k_fast_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, R, G, H, hand, nstrips,
-                       hstride);
+    hipLaunchKernelGGL(k_fast_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, R, G, hand, nstrips, hstride);
     pe(&c, PROF_PREP);
     pb(&c, PROF_SWEEP_FWD);
-    hipLaunchKernelGGL(k_fast_fwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, R, G, V, hand_f, hstride,
-                       c.spin_limit);
+    hipLaunchKernelGGL(k_fast_fwd, dim3(nstrips + nhwg), dim3(128), FWD_PAD_LDS, st, d, strips, nstrips, R, G, V,
+                       hand_f, hstride, c.spin_limit, work, nwork, H);
     pe(&c, PROF_SWEEP_FWD);
     const double* Xin = V;
     if (dual) {
@@ -867,7 +898,7 @@ int FastPath::adjoint_apply(Ctx& c, const double* r_cell, double* lam) {
     hipLaunchKernelGGL(k_fast_adj_rhs, dim3(nwork), dim3(256), 0, st, d, strips, work, r_cell, aR);
     hipLaunchKernelGGL(k_fast_hand_fill, dim3(256), dim3(256), 0, st, d, strips, ahand, nstrips, hstride);
     hipLaunchKernelGGL(k_fast_fwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, aR, aG, aV, ahand, hstride,
-                       c.spin_limit);
+                       c.spin_limit, work, 0, nullptr);
     hipLaunchKernelGGL(k_fast_bwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, aV, aH, aZ,
                        ahand + hstride * (nstrips + 1), hstride, c.spin_limit);
     hipLaunchKernelGGL(k_fast_adj_post, dim3(nwork), dim3(256), 0, st, d, strips, work, aZ, lam);

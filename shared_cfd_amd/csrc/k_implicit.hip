@@ -29,10 +29,14 @@ namespace shcfd {
 // progress flags.  With dqdt_inline set (whole steps of the inner-loop
 // schemes) the cell's dq/dt of set_dqdt (sch_dual.raw.f90:211-240) is formed
 // here, with set_dqdt's expressions, stored and used, instead of in a pass
-// of its own.
+// of its own.  With abpm_inline set (whole steps on the optimised sweeps,
+// which need no stored Jacobians) the cell's set_ABpm scalars are formed and
+// stored here first, by the same code as the set_ABpm pass, which then does
+// not run.
 // ---------------------------------------------------------------------------
 template <int NY>
-__global__ void __launch_bounds__(256) k_rhs(DevCtx d, const Tile* tiles, int* prog, int nprog, int dqdt_inline) {
+__global__ void __launch_bounds__(256) k_rhs(DevCtx d, const Tile* tiles, int* prog, int nprog, int dqdt_inline,
+                                             int abpm_inline) {
     using D = Dims<NY>;
     constexpr int dq = D::dq;
     if (blockIdx.x == 0)
@@ -40,6 +44,7 @@ __global__ void __launch_bounds__(256) k_rhs(DevCtx d, const Tile* tiles, int* p
     TileCoord t = tile_coord(tiles);
     const DevBlock B = d.blk[t.b];
     if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) return;
+    if (abpm_inline) set_abpm_cell<NY, 0>(d, B, t.i, t.j, nullptr, 0, 0);
     const long long c = cidx(B, t.i, t.j);
     double s[dq];
     flux_balance<NY>(d, B, c, s);
@@ -397,7 +402,7 @@ __global__ void __launch_bounds__(256) k_update(DevCtx d, const Tile* tiles) {
 }
 
 #define INST(NY)                                                                                               \
-    template __global__ void k_rhs<NY>(DevCtx, const Tile*, int*, int, int);                                        \
+    template __global__ void k_rhs<NY>(DevCtx, const Tile*, int*, int, int, int);                                        \
     template __global__ void k_sweep_fwd_plain<NY, false>(DevCtx, const Strip*, int*, double*, long long, long long, \
                                                           const double*, long long, const long long*);          \
     template __global__ void k_sweep_fwd_plain<NY, true>(DevCtx, const Strip*, int*, double*, long long, long long,  \

@@ -836,9 +836,9 @@ static int abpm_for_step(shcfd_ctx* ctx) {
         ctx->skew_active = 1;
         return SHCFD_OK;
     }
-    ProfScope ps(ctx, PROF_ABPM);
-    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_set_abpm_scal<NY>));
-    LAUNCHCHK();
+    // the optimised sweeps need only set_ABpm's scalars: the RHS pass of the
+    // implicit update forms and stores them
+    ctx->abpm_inline = 1;
     return SHCFD_OK;
 }
 
@@ -899,8 +899,9 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
     ctx->q16_active = 0;
     const int C16 = (ctx->dimq + 3) / 4, ns16 = ctx->nstrips16;
     { ProfScope ps(ctx, PROF_RHS);
-    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_rhs<NY>, ctx->dprog, 2 * ns, ctx->dqdt_inline)); }
+    DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_rhs<NY>, ctx->dprog, 2 * ns, ctx->dqdt_inline, ctx->abpm_inline)); }
     ctx->dqdt_inline = 0;
+    ctx->abpm_inline = 0;
     if (fast_enabled(ctx)) {
         int rc = ctx->fast->implicit(*ctx, step_internal, prof_begin, prof_end);
         LAUNCHCHK();

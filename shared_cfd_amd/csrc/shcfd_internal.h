@@ -182,6 +182,7 @@ struct Ctx {
     double* dbnd16 = nullptr;      // hand-off rows of both sweeps: 2 x (nstrips16 + 1) x bstride16
     long long bstride16 = 0;
     int dqdt_inline = 0;           // the next implicit update forms dq/dt in its RHS pass
+    int abpm_inline = 0;           // the next implicit update forms set_ABpm's scalars in its RHS pass
     int defer_update = 0;          // the next inner iteration leaves q += omega Dq to the next primitive pass
     int update_pending = 0;        // q += omega Dq is still to be applied
     double* dhist = nullptr;       // (res1, res) per physical step of an inner-loop graph

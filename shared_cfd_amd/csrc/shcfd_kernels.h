@@ -466,7 +466,7 @@ __global__ void k_zero_field(double*, long long);
 __global__ void k_reduce_final(DevCtx, const double*, int, int, int);
 
 // implicit update (k_implicit.hip)
-template <int NY> __global__ void k_rhs(DevCtx, const Tile*, int*, int, int);
+template <int NY> __global__ void k_rhs(DevCtx, const Tile*, int*, int, int, int);
 template <int NY, bool SKEW>
 __global__ void k_sweep_fwd_plain(DevCtx, const Strip*, int*, double*, long long, long long, const double*, long long,
                                   const long long*);
@@ -498,5 +498,91 @@ __global__ void k_geojac(DevCtx, const Tile*);
 
 // k_nasa9.hip
 __global__ void k_thermo_cold(DevCtx d, const Tile* tiles, const shcfd_nasa9* __restrict__ m);
+
+// ---------------------------------------------------------------------------
+// set_ABpm of one cell (LU-SGS, preconLU-SGS, NR, precon, dual): the stored
+// scalars (alpha; phi, pre1, pre2 and pre_sca or the local dt of the
+// preconditioned schemes) and, by MATS, the split Jacobians
+// ---------------------------------------------------------------------------
+// MATS: 0 = no stored Jacobians, 1 = Ap/Am/Bp/Bm fields (cell SoA),
+// 2 = the strip-skewed copy read by the skewed sweeps (slot given),
+// 3 = the quad-row copy of the 16-row sweeps (nsl = total steps NT of all
+// strips, slot = 64 x global step + 4 r): lane 4r+c of wave w owns row
+// k = 4w + c; per sweep (set 0: forward, 1: backward) it reads dq + 2 pairs
+// of doubles, pair p of global step g at ((((set NT + g) C + w) dq2 + p) 64 +
+// lane) 2, dq2 = dq + 2, so the operators of one step of a strip are
+// contiguous: elements 0..dq-1 are row k of A+ (A-), dq..2dq-1 row k of B+
+// (B-), then (alpha, dsci), (dscj, 0); rows k >= dq are zero
+template <int NY, int MATS>
+__device__ inline void set_abpm_cell(const DevCtx& d, const DevBlock& B, int i, int j, double* skew, long long nsl,
+                                     long long slot) {
+    using D = Dims<NY>;
+    constexpr int dq = D::dq;
+    const long long c = cidx(B, i, j), P = B.pitch;
+    CellJac<NY> J;
+    cell_jacobian<NY>(d, c, P, J);
+    if (MATS == 1)
+        for (int e = 0; e < dq * dq; e++) {
+            FLD(d, SHCFD_F_AP, e, c) = J.Ap[e];
+            FLD(d, SHCFD_F_AM, e, c) = J.Am[e];
+            FLD(d, SHCFD_F_BP, e, c) = J.Bp[e];
+            FLD(d, SHCFD_F_BM, e, c) = J.Bm[e];
+        }
+    if (MATS == 2)
+        for (int e = 0; e < dq * dq; e++) {
+            skew[(long long)(SKEW_AP * dq * dq + e) * nsl + slot] = J.Ap[e];
+            skew[(long long)(SKEW_BP * dq * dq + e) * nsl + slot] = J.Bp[e];
+            skew[(long long)(SKEW_AM * dq * dq + e) * nsl + slot] = J.Am[e];
+            skew[(long long)(SKEW_BM * dq * dq + e) * nsl + slot] = J.Bm[e];
+        }
+    if (MATS == 3) {
+        constexpr int C = (dq + 3) / 4, dq2 = dq + 2;
+        const double dsci = FLD(d, SHCFD_F_DSCI, 0, c), dscj = FLD(d, SHCFD_F_DSCJ, 0, c);
+        for (int set = 0; set < 2; set++) {
+            const double* MA = set == 0 ? J.Ap : J.Am;
+            const double* MB = set == 0 ? J.Bp : J.Bm;
+            for (int e = 0; e < 2 * dq2; e++) {
+                const long long g = slot >> 6, l0 = slot & 63;
+                for (int w = 0; w < C; w++) {
+                    double* o = skew + ((((set * nsl + g) * C + w) * dq2 + e / 2) * 64 + l0) * 2 + (e & 1);
+                    for (int cc = 0; cc < 4; cc++) {
+                        const int k = 4 * w + cc;
+                        double v;
+                        if (e < dq) v = (k < dq) ? MA[k + dq * e] : 0.0;
+                        else if (e < 2 * dq) v = (k < dq) ? MB[k + dq * (e - dq)] : 0.0;
+                        else v = e == 2 * dq ? J.alpha : (e == 2 * dq + 1 ? dsci : (e == 2 * dq + 2 ? dscj : 0.0));
+                        o[2 * cc] = v;
+                    }
+                }
+            }
+        }
+    }
+    FLD(d, SHCFD_F_ALPHA, 0, c) = J.alpha;
+    if (d.p.dimension == SHCFD_DIM_AXISYMMETRIC) {
+        // dp/dq of the implicit pressure source, kept in pre2 (unused by the
+        // axisymmetric schemes): LU-SGS/q2d/sch_lusgs.raw.f90:43-49,
+        // NR/q2d/sch_NR.raw.f90:43-49
+        const double u = FLD(d, SHCFD_F_W, IW_U, c), v = FLD(d, SHCFD_F_W, IW_V, c);
+        const double gmmad = FLD(d, SHCFD_F_W, dq + 1, c) - 1.0;
+        const double Vs = (u * u) + (v * v);
+        for (int k = 0; k < NY; k++) FLD(d, SHCFD_F_PRE2, k, c) = ((0.5 * gmmad) * Vs) + FLD(d, SHCFD_F_DHI, k, c);
+        FLD(d, SHCFD_F_PRE2, NY, c) = -(gmmad * u);
+        FLD(d, SHCFD_F_PRE2, NY + 1, c) = -(gmmad * v);
+        FLD(d, SHCFD_F_PRE2, NY + 2, c) = gmmad;
+    }
+    if (J.precond) {
+        FLD(d, SHCFD_F_PHI, 0, c) = J.phi;
+        for (int k = 0; k < dq; k++) {
+            FLD(d, SHCFD_F_PRE1, k, c) = J.pre1[k];
+            FLD(d, SHCFD_F_PRE2, k, c) = J.pre2[k];
+        }
+        if (d.p.scheme == SHCFD_SCHEME_DUAL) {
+            for (int k = 0; k < 4; k++) FLD(d, SHCFD_F_PRESCA, k, c) = J.ps[k];
+        } else {
+            FLD(d, SHCFD_F_DT, 0, c) = J.dtm;
+        }
+    }
+}
+
 
 }  // namespace shcfd

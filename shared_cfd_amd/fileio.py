@@ -103,14 +103,19 @@ def read_records(path: str, endian: str = ">") -> List[bytes]:
     return recs
 
 
-def write_records(path: str, records: Sequence[bytes], endian: str = ">") -> None:
-    limit = 2 ** 31 - 9
+def write_records(path: str, records: Sequence[bytes], endian: str = ">", limit: int = 2 ** 31 - 9) -> None:
+    """Sequential unformatted records as gfortran writes them: a record
+    longer than `limit` bytes is split into subrecords; a subrecord's head
+    marker is negative when another subrecord follows it and its tail marker
+    is negative when another one precedes it."""
     with open(path, "wb") as f:
         for r in records:
             parts = [r[k:k + limit] for k in range(0, len(r), limit)] or [b""]
+            last = len(parts) - 1
             for k, p in enumerate(parts):
-                n = len(p) if k == len(parts) - 1 else -len(p)
-                f.write(struct.pack(endian + "i", n) + p + struct.pack(endian + "i", n))
+                head = -len(p) if k < last else len(p)
+                tail = -len(p) if k > 0 else len(p)
+                f.write(struct.pack(endian + "i", head) + p + struct.pack(endian + "i", tail))
 
 
 # ---------------------------------------------------------------------------

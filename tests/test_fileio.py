@@ -218,3 +218,22 @@ def test_reads_reference_restart():
     assert (step, tt) == (fx["step"], fx["tt"])
     for j, i, c in fx["cells"]:
         assert np.array_equal(blocks[0][j, i], np.array(c))
+
+
+def test_subrecord_marker_signs(tmp_path):
+    """Records split into subrecords carry gfortran's markers: head negative
+    when a subrecord follows, tail negative when one precedes."""
+    import struct
+
+    path = str(tmp_path / "sub.bin")
+    rec = bytes(range(250)) * 4          # 1000 bytes -> 400 + 400 + 200
+    F.write_records(path, [rec, b"xyz"], limit=400)
+    data = open(path, "rb").read()
+    marks, pos = [], 0
+    while pos < len(data):
+        (h,) = struct.unpack(">i", data[pos:pos + 4])
+        (t,) = struct.unpack(">i", data[pos + 4 + abs(h):pos + 8 + abs(h)])
+        marks.append((h, t))
+        pos += 8 + abs(h)
+    assert marks == [(-400, 400), (-400, -400), (200, -200), (3, 3)]
+    assert F.read_records(path) == [rec, b"xyz"]
