@@ -76,8 +76,19 @@ enum {
     SHCFD_BC_FIXED = 3,      /* g1 = g2 = given w(1:dimw) and vhi(1:nY)        */
     SHCFD_BC_INFLOW_TUV = 4, /* T,u,v given, p from interior (ideal gas)        */
     SHCFD_BC_OUTFLOW_P = 5,  /* as EXTRAP, p replaced by a given value in g1    */
-    SHCFD_BC_SLIP_NORMAL = 6 /* slip wall for a curved side: velocity mirrored
+    SHCFD_BC_SLIP_NORMAL = 6, /* slip wall for a curved side: velocity mirrored
                                 about the face normal (addition)             */
+    SHCFD_BC_INFLOW_MIX_P = 7, /* inflow of a given mixture at a given T, p from
+                                the interior: g1 = pool w(1:dimw) and
+                                vhi(1:nY), rho = (p MW)/(R_uni T), H = pool H
+                                (static) + |V|^2/2; pool: w, vhi, MW,
+                                R_uni T (sample/no27.chemkin_reaction/
+                                condition.f90:53-66)                        */
+    SHCFD_BC_INJECT_WALL = 8 /* wall injecting a given mixture at a given T:
+                                as INFLOW_MIX_P, with the tangential velocity
+                                negated and the normal one 2 vw - interior
+                                (u on i-sides, v on j-sides); pool: w, vhi,
+                                MW, R_uni T, vw (condition.f90:120-127)     */
 };
 /* sides */
 enum { SHCFD_SIDE_IMIN = 0, SHCFD_SIDE_IMAX = 1, SHCFD_SIDE_JMIN = 2, SHCFD_SIDE_JMAX = 3 };

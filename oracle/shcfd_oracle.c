@@ -620,6 +620,27 @@ static void bc_segment(or_ctx *c, const int *row) {
             for (int k = 0; k < c->dimw; k++) W(k, ig1, jg1) = pv[k];
             for (int k = 0; k < c->ny; k++) VHI(k, ig1, jg1) = pv[c->dimw + k];
             break;
+        case SHCFD_BC_INFLOW_MIX_P:
+        case SHCFD_BC_INJECT_WALL: {
+            /* sample/no27.chemkin_reaction/condition.f90:53-66 (oxidizer
+             * inflow: w = wo, rho = p*MWo/(R_uni*To), p from the interior,
+             * H = wo(H) + |V|^2/2) and :120-127 (fuel-injecting wall:
+             * w = wf, u = -u(1), v = 0.02*2 - v(1)) */
+            int dw = c->dimw, ny = c->ny;
+            for (int k = 0; k < dw; k++) W(k, ig1, jg1) = pv[k];
+            for (int k = 0; k < ny; k++) VHI(k, ig1, jg1) = pv[dw + k];
+            double pin = W(IW_P, ii, jj);
+            W(IW_RHO, ig1, jg1) = (pin * pv[dw + ny]) / pv[dw + ny + 1];
+            if (kind == SHCFD_BC_INJECT_WALL) {
+                int kn = (side <= SHCFD_SIDE_IMAX) ? IW_U : IW_V, kt = (side <= SHCFD_SIDE_IMAX) ? IW_V : IW_U;
+                W(kt, ig1, jg1) = -W(kt, ii, jj);
+                W(kn, ig1, jg1) = (pv[dw + ny + 2] * 2.0) - W(kn, ii, jj);
+            }
+            W(IW_P, ig1, jg1) = pin;
+            double u = W(IW_U, ig1, jg1), v = W(IW_V, ig1, jg1);
+            W(IW_H(c), ig1, jg1) = pv[IW_H(c)] + 0.5 * ((u * u) + (v * v));
+            break;
+        }
         case SHCFD_BC_INFLOW_TUV: {
             /* sample/no05.boundary_layer_ideal/condition.f90, i=1/2 block */
             double kap = c->p.kappa, rg = c->p.rgas;

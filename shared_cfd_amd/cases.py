@@ -28,7 +28,8 @@ import numpy as np
 # ---- mirrors of include/shcfd_types.h ----------------------------------------
 SCHEME_EULER, SCHEME_RK2, SCHEME_LUSGS, SCHEME_NR, SCHEME_PRECON, SCHEME_DUAL, SCHEME_PRECON_LUSGS = 0, 1, 2, 3, 4, 5, 6
 LIMITER_REFERENCE, LIMITER_MINMOD = 0, 1
-BC_EXTRAP, BC_REFLECT, BC_NOSLIP, BC_FIXED, BC_INFLOW_TUV, BC_OUTFLOW_P, BC_SLIP_NORMAL = range(7)
+(BC_EXTRAP, BC_REFLECT, BC_NOSLIP, BC_FIXED, BC_INFLOW_TUV, BC_OUTFLOW_P, BC_SLIP_NORMAL, BC_INFLOW_MIX_P,
+ BC_INJECT_WALL) = range(9)
 SIDE_IMIN, SIDE_IMAX, SIDE_JMIN, SIDE_JMAX = range(4)
 DIM_PLANAR, DIM_AXISYMMETRIC = 0, 1
 
@@ -529,6 +530,70 @@ def mixing_layer(mech, ni=1024, nj=512, seed=0, scheme=SCHEME_LUSGS, reaction=2,
                 description=f"reacting mixing layer {ni}x{nj}, {ns} species / {mech.nr} reactions, "
                             f"fuel {T_fuel:g} K / oxidiser {T_ox:g} K, p {p0:g} Pa",
                 mech=mech, w0=[w0])
+
+
+# ---- the reference's sample no27 (H2 injected into an air boundary layer) ------------
+R_UNI = 8.3144621e3   # J/(kmol K), chem_var of the reference's Chemkin model
+
+
+def no27_cold(mech, state, species=("N2", "O2", "H2", "OH", "O", "H", "HO2", "H2O"), u_in=0.603, v_wall=0.02,
+              p_ox=1e5, T_ox=300.0, p_f=1e5, T_f=300.0, cfl=0.5) -> Case:
+    """sample/no27.chemkin_reaction as its cold-flow initialisation runs it
+    (checkout.ini.inp, control.ini.inp): one 223 x 99 block (ideal_bl.x),
+    preconditioned LU-SGS with local time steps, viscous, Chemkin thermo
+    without reactions, MUSCL off, CFL 0.5.  ``state`` holds the grid and the
+    restart state (tests/golden/no27_state.npz); ``mech`` is the parsed
+    no27 mechanism with its transport data.
+
+    Boundaries of the sample's condition.f90 at the steps of that run (the
+    ignition spot it ramps in after step 40000 is off): i-min oxidizer
+    inflow (N2:O2 = 4:1 by moles at T_ox, u = u_in, p from the interior,
+    :53-66), i-max and j-max outflow at 1e5 Pa (:69-79, :135-145), j-min a
+    slip wall for i <= 54 (:93-99) and a wall injecting fuel (H2 at T_f,
+    v = v_wall) beyond (:104-131)."""
+    from . import chem
+
+    ns = mech.ns
+    xh, rh = np.asarray(state["x"], dtype=np.float64), np.asarray(state["y"], dtype=np.float64)
+    ni, nj = xh.shape[0] - 1, xh.shape[1] - 1
+    q = np.asarray(state["q"], dtype=np.float64)
+    assert q.shape == (nj, ni, ns + 3)
+    rho = q[..., :ns].sum(-1)
+    dimw = ns + 8
+    w0 = ref_zeros(dimw, ni, nj)
+    w0[2:nj + 2, 2:ni + 2, 0] = rho
+    w0[2:nj + 2, 2:ni + 2, 3] = state["p"]
+    w0[2:nj + 2, 2:ni + 2, ns + 6] = state["rgas"]
+
+    def stream(p, T, moles):
+        rel = np.zeros(ns)
+        for name, n in moles.items():
+            k = species.index(name)
+            rel[k] = n * mech.mw[k]
+        w, vhi, _ = chem.boundary_state(mech, p, T, rel)
+        mw = R_UNI / w[ns + 6]
+        return w, vhi, mw
+
+    wo, vhio, mwo = stream(p_ox, T_ox, {"N2": 4.0, "O2": 1.0})
+    wf, vhif, mwf = stream(p_f, T_f, {"H2": 2.0})
+    wo = wo.copy()
+    wo[1], wo[2] = u_in, 0.0
+    pool_o = list(wo) + list(vhio) + [mwo, R_UNI * T_ox]
+    pool_f = list(wf) + list(vhif) + [mwf, R_UNI * T_f, v_wall]
+    off_f = len(pool_o)
+    off_p = off_f + len(pool_f)
+    pool = np.array(pool_o + pool_f + [1e5], dtype=np.float64)
+    rows = [[1, SIDE_IMIN, 1, nj, BC_INFLOW_MIX_P, 0], [1, SIDE_IMAX, 1, nj, BC_OUTFLOW_P, off_p],
+            [1, SIDE_JMIN, 1, 54, BC_REFLECT, 0], [1, SIDE_JMIN, 55, ni, BC_INJECT_WALL, off_f],
+            [1, SIDE_JMAX, 1, ni, BC_OUTFLOW_P, off_p]]
+    params = make_params(scheme=SCHEME_PRECON_LUSGS, time_step=1, high_order=1, phi_mus=0.0, k_mus=1.0 / 3.0,
+                         limiter=LIMITER_REFERENCE, viscous=1, thermal=2, reaction=0, cfl=cfl, kappa=1.4,
+                         rgas=287.0, nu=0.0, num_internal_loop=2)
+    return Case("no27_cold", [ni], [nj], ns, params, [xh], [rh], [_fill_q(ni, nj, q)],
+                np.array(rows, dtype=np.int32), pool, np.zeros((0, 11), dtype=np.int32),
+                description=f"sample no27 (H2 injection into air), {ni}x{nj} cells, {ns} species, cold flow, "
+                            "preconditioned LU-SGS, local dt, viscous, first order",
+                mech=mech, w0=[w0], meta={"step": int(state["step"])})
 
 
 # ---- batch of independent laminar plate / channel blocks -------------------------

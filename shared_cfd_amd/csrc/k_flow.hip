@@ -182,6 +182,24 @@ __global__ void k_bc_side(DevCtx d, const int* bc, int nbc, const double* pool, 
         for (int k = 0; k < NY; k++) vh[k] = ((kap / (kap - 1.0)) * rg) * T;
         break;
     }
+    case SHCFD_BC_INFLOW_MIX_P:
+    case SHCFD_BC_INJECT_WALL: {
+        // sample/no27.chemkin_reaction/condition.f90:53-66, :120-127
+#pragma unroll
+        for (int k = 0; k < D::dw; k++) wv[k] = pv[k];
+#pragma unroll
+        for (int k = 0; k < NY; k++) vh[k] = pv[D::dw + k];
+        const double pin = FLD(d, SHCFD_F_W, IW_P, ci);
+        wv[IW_RHO] = (pin * pv[D::dw + NY]) / pv[D::dw + NY + 1];
+        if (kind == SHCFD_BC_INJECT_WALL) {
+            const int kn = (side <= SHCFD_SIDE_IMAX) ? IW_U : IW_V, kt = (side <= SHCFD_SIDE_IMAX) ? IW_V : IW_U;
+            wv[kt] = -FLD(d, SHCFD_F_W, kt, ci);
+            wv[kn] = (pv[D::dw + NY + 2] * 2.0) - FLD(d, SHCFD_F_W, kn, ci);
+        }
+        wv[IW_P] = pin;
+        wv[D::ih] = pv[D::ih] + 0.5 * ((wv[IW_U] * wv[IW_U]) + (wv[IW_V] * wv[IW_V]));
+        break;
+    }
     default: {
 #pragma unroll
         for (int k = 0; k < D::dw; k++) wv[k] = FLD(d, SHCFD_F_W, k, ci);
@@ -404,11 +422,13 @@ __global__ void __launch_bounds__(256) k_set_tgv(DevCtx d, const Tile* tiles) {
     }
 }
 
+#if SHCFD_INST_GROUP == 0
 __global__ void k_zero_field(double* f, long long n) {
     long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
     for (; gid < n; gid += stride) f[gid] = 0.0;
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // set_dt: time_step/{local,global}/set_dt.f90 -- dt_mat per cell and a
@@ -466,6 +486,7 @@ __global__ void __launch_bounds__(256) k_set_dt(DevCtx d, const Tile* tiles, dou
 
 // final fixed-order reduction of per-tile partials (one workgroup)
 // mode 0: min -> scal[slot] (and tt += for dt); mode 1: sum -> scal[slot]
+#if SHCFD_INST_GROUP == 0
 __global__ void __launch_bounds__(256) k_reduce_final(DevCtx d, const double* part, int n, int mode, int slot) {
     __shared__ double sh[8];
     double v = (mode == 0) ? 1e300 : 0.0;
@@ -479,6 +500,7 @@ __global__ void __launch_bounds__(256) k_reduce_final(DevCtx d, const double* pa
         if (mode == 0 && slot == SCAL_DT_GRBL) d.scal[SCAL_TT] = d.scal[SCAL_TT] + r;
     }
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // energy RMS: core/check_convergence.f90:17-28 (per-tile partial sums)

@@ -364,6 +364,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 }
 
 // final (res, omega) reduction over strips; one workgroup; sets error flags
+#if SHCFD_INST_GROUP == 0
 __global__ void __launch_bounds__(256) k_reduce_res_omega(DevCtx d, const double* part, int n) {
     __shared__ double shr[4], sho[4];
     int chunk = (n + blockDim.x - 1) / blockDim.x;
@@ -386,8 +387,11 @@ __global__ void __launch_bounds__(256) k_reduce_res_omega(DevCtx d, const double
         if (O < d.p.omega_min) set_error(d, SHCFD_E_OMEGA);
     }
 }
+#endif
 
+#if SHCFD_INST_GROUP == 0
 __global__ void k_copy_scal(DevCtx d, int dst, int src) { d.scal[dst] = d.scal[src]; }
+#endif
 
 // q += min(omega_max, omega) * Δq : time_dual_viscous.f90:113-124, time_NR_viscous.f90:90-101
 template <int NY>

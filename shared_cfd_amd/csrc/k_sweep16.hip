@@ -100,6 +100,7 @@ __device__ inline void load_ops(const double* __restrict__ base, StepIn<DQ>& in)
 // hand-off rows of both sweeps: the sentinel for the columns 1..ni of the
 // strip's block, zero elsewhere and in the final row (read by the strips
 // that have no neighbour)
+#if SHCFD_INST_GROUP == 0
 __global__ void __launch_bounds__(256) k_fill16(DevCtx d, const Strip16* strips, int nstrips, double* hand,
                                                 long long hstride, int dq) {
     const long long n = 2LL * (nstrips + 1) * hstride;
@@ -111,12 +112,14 @@ __global__ void __launch_bounds__(256) k_fill16(DevCtx d, const Strip16* strips,
         hand[e] = live ? sv : 0.0;
     }
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // forward sweep: x = alpha (R + A+(i-1) x(i-1) dsci + B+(j-1) x(j-1) dscj)
 // ---------------------------------------------------------------------------
+// one wave per 4 components: up to 4 waves for dq <= 16, 5 above
 template <int NY>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(NY + 3 <= 16 ? 256 : 320) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_sweep16_fwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long long hstride, long long spin_limit,
               const double* __restrict__ jac, long long nsl) {
     constexpr int dq = NY + 3, C = (dq + 3) / 4, DQP = 4 * C;
@@ -203,7 +206,7 @@ k_sweep16_fwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long l
 // it waits on (the one above) has a lower workgroup id.
 // ---------------------------------------------------------------------------
 template <int NY>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(NY + 3 <= 16 ? 256 : 320) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_sweep16_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long long hstride, double* part,
               long long spin_limit, const double* __restrict__ jac, long long nsl) {
     constexpr int dq = NY + 3, C = (dq + 3) / 4, DQP = 4 * C;
@@ -317,9 +320,7 @@ k_sweep16_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long l
                                                const double*, long long);                                     \
     template __global__ void k_sweep16_bwd<NY>(DevCtx, const Strip16*, int, double*, long long, double*,        \
                                                long long, const double*, long long);
-INST(1)
-INST(2)
-INST(9)
+SHCFD_FOR_EACH_NY(INST)
 #undef INST
 
 }  // namespace shcfd

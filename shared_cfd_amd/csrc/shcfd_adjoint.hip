@@ -96,6 +96,9 @@ int supported(shcfd_ctx* ctx) {
         (ctx->p.scheme != SHCFD_SCHEME_LUSGS && ctx->p.scheme != SHCFD_SCHEME_NR &&
          ctx->p.scheme != SHCFD_SCHEME_DUAL))
         return SHCFD_E_STATE;
+    // the reverse boundary kernel covers the kinds up to the curved slip wall
+    for (int r = 0; r < ctx->nbc; r++)
+        if (ctx->hbc[SHCFD_BC_ROW * r + 4] > SHCFD_BC_SLIP_NORMAL) return SHCFD_E_STATE;
     return SHCFD_OK;
 }
 

@@ -56,20 +56,50 @@ int field_ncomp(int fld, int ny) {
         }                                                                                     \
     } while (0)
 
+// species counts 1..SHCFD_MAX_NY (kernels instantiated per group, see
+// shcfd_kernels.h)
 #define DISPATCH_NY(NYV, CALL)           \
     switch (NYV) {                       \
     case 1: { constexpr int NY = 1; CALL; } break; \
     case 2: { constexpr int NY = 2; CALL; } break; \
+    case 3: { constexpr int NY = 3; CALL; } break; \
+    case 4: { constexpr int NY = 4; CALL; } break; \
+    case 5: { constexpr int NY = 5; CALL; } break; \
+    case 6: { constexpr int NY = 6; CALL; } break; \
+    case 7: { constexpr int NY = 7; CALL; } break; \
+    case 8: { constexpr int NY = 8; CALL; } break; \
     case 9: { constexpr int NY = 9; CALL; } break; \
+    case 10: { constexpr int NY = 10; CALL; } break; \
+    case 11: { constexpr int NY = 11; CALL; } break; \
+    case 12: { constexpr int NY = 12; CALL; } break; \
+    case 13: { constexpr int NY = 13; CALL; } break; \
+    case 14: { constexpr int NY = 14; CALL; } break; \
+    case 15: { constexpr int NY = 15; CALL; } break; \
+    case 16: { constexpr int NY = 16; CALL; } break; \
     default: return SHCFD_E_ARG;         \
     }
 #define DISPATCH_FLOW_NS(NSV, CALL)       \
     switch (NSV) {                       \
+    case 1: { constexpr int NS = 1; CALL; } break; \
+    case 2: { constexpr int NS = 2; CALL; } break; \
+    case 3: { constexpr int NS = 3; CALL; } break; \
+    case 4: { constexpr int NS = 4; CALL; } break; \
+    case 5: { constexpr int NS = 5; CALL; } break; \
+    case 6: { constexpr int NS = 6; CALL; } break; \
+    case 7: { constexpr int NS = 7; CALL; } break; \
+    case 8: { constexpr int NS = 8; CALL; } break; \
     case 9: { constexpr int NS = 9; CALL; } break; \
+    case 10: { constexpr int NS = 10; CALL; } break; \
+    case 11: { constexpr int NS = 11; CALL; } break; \
+    case 12: { constexpr int NS = 12; CALL; } break; \
+    case 13: { constexpr int NS = 13; CALL; } break; \
+    case 14: { constexpr int NS = 14; CALL; } break; \
+    case 15: { constexpr int NS = 15; CALL; } break; \
+    case 16: { constexpr int NS = 16; CALL; } break; \
     default: return SHCFD_E_ARG;         \
     }
 
-static bool ny_supported(int ny) { return ny == 1 || ny == 2 || ny == 9; }
+static bool ny_supported(int ny) { return ny >= 1 && ny <= SHCFD_MAX_NY; }
 
 static void drop_graph(shcfd::Ctx* ctx) {
     if (ctx->graph) hipGraphExecDestroy(ctx->graph);
@@ -330,9 +360,11 @@ int shcfd_set_bc_table(shcfd_ctx* ctx, const int* rows, int nrows, const double*
     for (int r = 0; r < nrows; r++) {
         const int* t = rows + SHCFD_BC_ROW * r;
         int b = t[0], side = t[1], s = t[2], e = t[3], kind = t[4], off = t[5];
-        if (b < 1 || b > ctx->nb || side < 0 || side > 3 || kind < 0 || kind > SHCFD_BC_SLIP_NORMAL || off < 0)
+        if (b < 1 || b > ctx->nb || side < 0 || side > 3 || kind < 0 || kind > SHCFD_BC_INJECT_WALL || off < 0)
             return SHCFD_E_ARG;
-        int need = kind == SHCFD_BC_FIXED ? ctx->dimw + ctx->ny : kind == SHCFD_BC_INFLOW_TUV ? 3 : kind == SHCFD_BC_OUTFLOW_P ? 1 : 0;
+        const int mix = ctx->dimw + ctx->ny;
+        int need = kind == SHCFD_BC_FIXED ? mix : kind == SHCFD_BC_INFLOW_TUV ? 3 : kind == SHCFD_BC_OUTFLOW_P ? 1
+                 : kind == SHCFD_BC_INFLOW_MIX_P ? mix + 2 : kind == SHCFD_BC_INJECT_WALL ? mix + 3 : 0;
         if (need > 0 && off + need > npool) return SHCFD_E_ARG;
         const DevBlock& B = ctx->hblk[b - 1];
         int n = side <= 1 ? B.nj : B.ni;
@@ -557,10 +589,12 @@ int shcfd_proceed_reaction(shcfd_ctx* ctx) {
         (ctx->p.reaction != SHCFD_REACTION_CHEMEQ2 && ctx->p.reaction != SHCFD_REACTION_POINT_IMPLICIT))
         return SHCFD_E_STATE;
     ProfScope ps(ctx, PROF_REACTION);
-    if (ctx->p.reaction == SHCFD_REACTION_POINT_IMPLICIT)
+    if (ctx->p.reaction == SHCFD_REACTION_POINT_IMPLICIT && ctx->ny <= SHCFD_LDS_NEWTON_MAX_NS)
         DISPATCH_FLOW_NS(ctx->ny, hipLaunchKernelGGL(k_reaction_lds<NS>, dim3((unsigned)(ntiles(ctx) * ctx->tile_rows)),
                                                      dim3(64), 0, ctx->stream, ctx->dev(), ctx->dtiles,
                                                      ctx->tile_rows, ctx->dmech))
+    else if (ctx->p.reaction == SHCFD_REACTION_POINT_IMPLICIT)
+        DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH((k_reaction<NS, SHCFD_REACTION_POINT_IMPLICIT>), ctx->dmech))
     else
         DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH((k_reaction<NS, SHCFD_REACTION_CHEMEQ2>), ctx->dmech))
     LAUNCHCHK();

@@ -48,6 +48,23 @@ void launch0d(int op, const shcfd_mech* dm, int n, const double* din, int ldi, d
     }
 }
 
+// the 0-D launch for the mechanism's species count (1..SHCFD_MAX_NY)
+void launch0d_ns(int ns, int op, const shcfd_mech* dm, int n, const double* din, int ldi, double* dout, int ldo,
+                 int* dst, double dt) {
+    switch (ns) {
+#define SHCFD_NS_CASE(N) case N: launch0d<N>(op, dm, n, din, ldi, dout, ldo, dst, dt); break;
+    SHCFD_NS_CASE(1) SHCFD_NS_CASE(2) SHCFD_NS_CASE(3) SHCFD_NS_CASE(4) SHCFD_NS_CASE(5) SHCFD_NS_CASE(6)
+    SHCFD_NS_CASE(7) SHCFD_NS_CASE(8) SHCFD_NS_CASE(9) SHCFD_NS_CASE(10) SHCFD_NS_CASE(11) SHCFD_NS_CASE(12)
+    SHCFD_NS_CASE(13) SHCFD_NS_CASE(14) SHCFD_NS_CASE(15) SHCFD_NS_CASE(16)
+#undef SHCFD_NS_CASE
+    default: break;
+    }
+}
+
+bool mech_ok(const shcfd_mech* m) {
+    return m->ns >= 1 && m->ns <= SHCFD_MAX_NY && m->nr >= 1 && m->nr <= SHCFD_CHEM_MAX_NR;
+}
+
 template <int NS>
 int boundary_state(const shcfd_mech* m, double p, double T, const double* vrho_rel, double* w, double* vhi,
                    double* dhi) {
@@ -115,8 +132,7 @@ extern "C" {
 
 int shcfd_chem_eval_noalloc(const shcfd_mech* m, int op, int n, const double* in, double* out, int* status,
                             double dt) {
-    if (!m || n < 0 || !in || !out || (m->ns != 8 && m->ns != 9) || m->nr < 1 || m->nr > SHCFD_CHEM_MAX_NR)
-        return SHCFD_E_ARG;
+    if (!m || n < 0 || !in || !out || !mech_ok(m)) return SHCFD_E_ARG;
     const int ldi = row_in(op, m->ns), ldo = row_out(op, m->ns);
     if (ldi < 0) return SHCFD_E_ARG;
     if (n == 0) return SHCFD_OK;
@@ -138,10 +154,7 @@ int shcfd_chem_eval_noalloc(const shcfd_mech* m, int op, int n, const double* in
     if (hipMemcpy(dm, m, sizeof(shcfd_mech), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(din, in, sizeof(double) * (size_t)n * ldi, hipMemcpyHostToDevice) != hipSuccess)
         return SHCFD_E_HIP;
-    if (m->ns == 8)
-        launch0d<8>(op, dm, n, din, ldi, dout, ldo, dst, dt);
-    else
-        launch0d<9>(op, dm, n, din, ldi, dout, ldo, dst, dt);
+    launch0d_ns(m->ns, op, dm, n, din, ldi, dout, ldo, dst, dt);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(0) != hipSuccess ||
         hipMemcpy(out, dout, sizeof(double) * (size_t)n * ldo, hipMemcpyDeviceToHost) != hipSuccess ||
         (status && hipMemcpy(status, dst, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess))
@@ -153,8 +166,11 @@ int shcfd_chem_boundary_state(const shcfd_mech* m, double p, double T, const dou
                               double* vhi, double* dhi) {
     if (!m || !vrho_rel || !w || !vhi || !(p > 0.0) || !(T > 0.0)) return SHCFD_E_ARG;
     switch (m->ns) {
-    case 8: return boundary_state<8>(m, p, T, vrho_rel, w, vhi, dhi);
-    case 9: return boundary_state<9>(m, p, T, vrho_rel, w, vhi, dhi);
+#define SHCFD_NS_CASE(N) case N: return boundary_state<N>(m, p, T, vrho_rel, w, vhi, dhi);
+    SHCFD_NS_CASE(1) SHCFD_NS_CASE(2) SHCFD_NS_CASE(3) SHCFD_NS_CASE(4) SHCFD_NS_CASE(5) SHCFD_NS_CASE(6)
+    SHCFD_NS_CASE(7) SHCFD_NS_CASE(8) SHCFD_NS_CASE(9) SHCFD_NS_CASE(10) SHCFD_NS_CASE(11) SHCFD_NS_CASE(12)
+    SHCFD_NS_CASE(13) SHCFD_NS_CASE(14) SHCFD_NS_CASE(15) SHCFD_NS_CASE(16)
+#undef SHCFD_NS_CASE
     default: return SHCFD_E_ARG;
     }
 }
@@ -196,8 +212,7 @@ int shcfd_load_nasa9(shcfd_ctx* ctx, const char* chem_inp, const char* thermo_in
 }
 
 int shcfd_chem_eval(const shcfd_mech* m, int op, int n, const double* in, double* out, int* status, double dt) {
-    if (!m || n < 0 || !in || !out || (m->ns != 8 && m->ns != 9) || m->nr < 1 || m->nr > SHCFD_CHEM_MAX_NR)
-        return SHCFD_E_ARG;
+    if (!m || n < 0 || !in || !out || !mech_ok(m)) return SHCFD_E_ARG;
     int ldi = row_in(op, m->ns), ldo = row_out(op, m->ns);
     if (ldi < 0) return SHCFD_E_ARG;
     if (n == 0) return SHCFD_OK;
@@ -214,10 +229,7 @@ int shcfd_chem_eval(const shcfd_mech* m, int op, int n, const double* in, double
                hipMemcpy(din, in, sizeof(double) * (size_t)n * ldi, hipMemcpyHostToDevice) != hipSuccess) {
         rc = SHCFD_E_HIP;
     } else {
-        if (m->ns == 8)
-            launch0d<8>(op, dm, n, din, ldi, dout, ldo, dst, dt);
-        else
-            launch0d<9>(op, dm, n, din, ldi, dout, ldo, dst, dt);
+        launch0d_ns(m->ns, op, dm, n, din, ldi, dout, ldo, dst, dt);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
             hipMemcpy(out, dout, sizeof(double) * (size_t)n * ldo, hipMemcpyDeviceToHost) != hipSuccess ||
             (status && hipMemcpy(status, dst, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess))

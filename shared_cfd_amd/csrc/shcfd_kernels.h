@@ -4,12 +4,35 @@
 #include "shcfd_device.h"
 #include "shcfd_ad.h"
 
-// species counts with compiled kernels (1 = ideal gas)
-#define SHCFD_FOR_EACH_NY(X) X(1) X(2) X(9)
-// species counts with compiled chemistry (0-D harness); flows with
-// chemistry need NY = NS in both lists
-#define SHCFD_FOR_EACH_NS(X) X(8) X(9)
-#define SHCFD_FOR_EACH_FLOW_NS(X) X(9)
+// Species counts 1..SHCFD_MAX_NY have compiled kernels (1 = ideal gas), the
+// same counts for the flow, the Chemkin chemistry and the 0-D harness.  A
+// source whose kernels take the species count as a template parameter is
+// compiled once per instantiation group (SHCFD_INST_GROUP = 0 ..
+// SHCFD_INST_GROUPS-1, given by the build); each object instantiates its
+// group's counts, and group 0 also holds the source's kernels that have no
+// species parameter.
+#define SHCFD_MAX_NY 16
+#define SHCFD_INST_GROUPS 5
+#ifndef SHCFD_INST_GROUP
+#define SHCFD_INST_GROUP 0
+#endif
+#if SHCFD_INST_GROUP == 0
+#define SHCFD_FOR_EACH_NY(X) X(1) X(2)
+#elif SHCFD_INST_GROUP == 1
+#define SHCFD_FOR_EACH_NY(X) X(3) X(4) X(5) X(6)
+#elif SHCFD_INST_GROUP == 2
+#define SHCFD_FOR_EACH_NY(X) X(7) X(8) X(9) X(10)
+#elif SHCFD_INST_GROUP == 3
+#define SHCFD_FOR_EACH_NY(X) X(11) X(12) X(13)
+#else
+#define SHCFD_FOR_EACH_NY(X) X(14) X(15) X(16)
+#endif
+#define SHCFD_FOR_EACH_NS(X) SHCFD_FOR_EACH_NY(X)
+#define SHCFD_FOR_EACH_FLOW_NS(X) SHCFD_FOR_EACH_NY(X)
+// the point-implicit flow kernel keeps each lane's (NS+1)^2 Newton matrix in
+// LDS up to this species count (64 lanes x 121 doubles = 62 KB); above it the
+// register kernel runs
+#define SHCFD_LDS_NEWTON_MAX_NS 10
 
 namespace shcfd {
 

@@ -60,3 +60,32 @@ def mixture_states(m, n, seed, p=1.01325e5, tmin=700.0, tmax=2200.0):
         vrho, rho, E = O.chem_rho_rel2abs(m, p, T, rel)
         out.append((vrho, rho, E, T))
     return out
+
+
+def no27_plus_inert(outdir, k):
+    """The sample no27 mechanism with k inert species appended, N2-like
+    copies named N2A, N2B, ... (same NASA-7 record and element counts), for
+    flows and 0-D states with 8 + k species."""
+    names = ["N2" + chr(ord("A") + i) for i in range(k)]
+    lines = open(NO27).read().split("\n")
+    out = []
+    in_thermo = False
+    n2 = None
+    for idx, ln in enumerate(lines):
+        if ln.strip() == "N2 O2 H2 OH O H HO2 H2O":
+            ln = ln + " " + " ".join(names)
+        if ln.strip() == "thermo all":
+            in_thermo = True
+        if in_thermo and ln.startswith("N2 "):
+            n2 = lines[idx:idx + 4]
+        if in_thermo and ln.strip() == "end":
+            for nm in names:
+                out.append(nm.ljust(18) + n2[0][18:])
+                out.extend(n2[1:])
+            in_thermo = False
+        out.append(ln)
+    os.makedirs(outdir, exist_ok=True)
+    p = os.path.join(outdir, "chem.inp")
+    with open(p, "w") as f:
+        f.write("\n".join(out))
+    return p, ["N2", "O2", "H2", "OH", "O", "H", "HO2", "H2O"] + names

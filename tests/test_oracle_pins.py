@@ -21,6 +21,7 @@ dropped factor) that the same check rejects, so the pin is not vacuous.
   manual) evaluated here in numpy from the mechanism tables.
 """
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -317,3 +318,37 @@ def test_inflow_tuv_and_outflow_p_ghost_states():
         g, it = w[2:nj + 2, gi], w[2:nj + 2, ni + 1]
         np.testing.assert_allclose(g[:, 3], pout, rtol=1e-15)
         np.testing.assert_allclose(g[:, [0, 1, 2]], it[:, [0, 1, 2]], rtol=1e-15)
+
+
+# --------------------------------------------------------------------------- the reference's own solution
+def _no27(**kw):
+    st = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "no27_state.npz"))
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    m = chem.parse(os.path.join(golden, "no27_chem.inp"), os.path.join(golden, "no27_trans.inp"))
+    return C.no27_cold(m, st, **kw)
+
+
+def _rms_steps(case, n):
+    o = Oracle(case)
+    o.prepare()
+    out = []
+    for _ in range(n):
+        assert o.step()[0] == 0
+        out.append(o.energy_rms())
+    return out
+
+
+def test_reference_no27_state_is_a_near_fixed_point():
+    """The state the reference wrote after 40000 steps of its no27
+    cold-flow initialisation (8 species, viscous, preconditioned LU-SGS,
+    local time steps, first order; sample/no27.chemkin_reaction/restart.bin,
+    checkout.ini.inp, control.ini.inp), continued by the oracle with the
+    sample's boundary conditions (condition.f90): the energy RMS of
+    check_convergence (store/core/check_convergence.f90:22-28) is small and
+    keeps falling, as a continuation of a converging run.  A planted mistake
+    -- the fuel stream 10 K too warm -- moves the state off that fixed point
+    and raises the RMS by more than four orders of magnitude."""
+    r = _rms_steps(_no27(), 10)
+    assert r[0] < 3e-5
+    assert all(b < a for a, b in zip(r, r[1:]))
+    assert _rms_steps(_no27(T_f=310.0), 1)[0] > 1e4 * r[0]
