@@ -186,9 +186,12 @@ int shcfd_chem_boundary_state(const shcfd_mech *mech, double p, double T, const 
  *   SHCFD_CHEM_FEXVAR   in n(ns),T          out p(N), m(N)       Fex_var, FJ.f90:301-453
  *   SHCFD_CHEM_JEX      in n(ns),T          out dndn(N,N) col-major  Jex, FJ.f90:454-708
  *   SHCFD_CHEM_CALCT    in vrho(ns),rho,E,T0  out T,kappa,MWave,mu,DHi(ns),vhi(ns)  calc_T
- *   SHCFD_CHEM_CHEMEQ2  in vrho(ns),T       out vrho(ns)   reaction_chemeq2 over dt
+ *   SHCFD_CHEM_CHEMEQ2  in vrho(ns),T       out vrho(ns)   reaction_chemeq2 over dt, with its DVODE
+ *                       hand-over after 10 attempts (sub_chem.f90:1684-1819)
  *   SHCFD_CHEM_POINT_IMPLICIT  in vrho(ns),T  out vrho(ns),T1  point-implicit (backward-Euler) update over dt
- * status[i] (may be NULL): Newton iterations / CHEMEQ2 attempts, or -code.
+ *   SHCFD_CHEM_DVODE    in vrho(ns),T       out vrho(ns)   reaction (DVODE over dt), sub_chem.f90:1236-1315
+ * status[i] (may be NULL): Newton iterations / CHEMEQ2 attempts (11: DVODE
+ * finished the interval) / DVODE steps, or -code.
  * Synchronous; allocates and frees its own device buffers. */
 int shcfd_chem_eval(const shcfd_mech *mech, int op, int nstates, const double *in, double *out, int *status,
                     double dt);

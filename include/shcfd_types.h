@@ -213,7 +213,7 @@ enum {
 /* 0-D chemistry operations of shcfd_chem_eval */
 enum {
     SHCFD_CHEM_FEX = 0, SHCFD_CHEM_FEXVAR = 1, SHCFD_CHEM_JEX = 2, SHCFD_CHEM_CALCT = 3, SHCFD_CHEM_CHEMEQ2 = 4,
-    SHCFD_CHEM_POINT_IMPLICIT = 5
+    SHCFD_CHEM_POINT_IMPLICIT = 5, SHCFD_CHEM_DVODE = 6
 };
 typedef struct shcfd_mech {
     int ne, ns, nr, nt;                                  /* elements, species, reactions, transport species */

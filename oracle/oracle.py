@@ -14,12 +14,20 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_DVODE = None
+# the reference's own DVODE, compiled from store/therm_lib/chemkin/core/dvode.f
+# by `make ref` wherever the reference's sources are present
+DVODE_REF = os.path.join(_HERE, "_ref", "libdvode_ref.so")
+REFERENCE = "/root/reference"
 
 
 def build(force: bool = False) -> str:
     so = os.path.join(_HERE, "liboracle.so")
     if force or not os.path.exists(so):
         subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+    if os.path.isdir(REFERENCE) and (force or not os.path.exists(DVODE_REF)):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref", "REF=" + REFERENCE] + (["-B"] if force else []),
+                       check=True)
     return so
 
 
@@ -87,8 +95,27 @@ def lib():
         L.or_exp.restype = ctypes.c_double
         L.or_log.argtypes = [ctypes.c_double]
         L.or_log.restype = ctypes.c_double
+        L.or_chem_reaction_dvode.argtypes = [mp, ctypes.c_double, dp, ctypes.c_double, ip]
+        L.or_set_dvode.argtypes = [vp]
+        L.or_set_dvode.restype = None
+        L.or_dvode_calls.restype = ctypes.c_long
+        if os.path.exists(DVODE_REF):
+            global _DVODE
+            _DVODE = ctypes.CDLL(DVODE_REF)
+            L.or_set_dvode(ctypes.cast(_DVODE.dvode_, ctypes.c_void_p))
         _LIB = L
     return _LIB
+
+
+def have_dvode() -> bool:
+    """True when the reference's DVODE (oracle/_ref) is loaded: CHEMEQ2's
+    hand-over and the DVODE reaction need it."""
+    return bool(lib().or_have_dvode())
+
+
+def dvode_calls() -> int:
+    """Number of DVODE integrations the oracle has run in this process."""
+    return int(lib().or_dvode_calls())
 
 
 def _ip(a):
@@ -240,6 +267,14 @@ def chem_reaction_chemeq2(m, T, vrho, dt):
     att = ctypes.c_int(0)
     e = lib().or_chem_reaction_chemeq2(ctypes.byref(m), T, _dp(v), dt, ctypes.byref(att))
     return e, v, att.value
+
+
+def chem_reaction_dvode(m, T, vrho, dt):
+    """reaction (sub_chem.f90:1236-1315): (error, vrho, DVODE steps)."""
+    v = np.array(vrho, dtype=np.float64)
+    st = ctypes.c_int(0)
+    e = lib().or_chem_reaction_dvode(ctypes.byref(m), T, _dp(v), dt, ctypes.byref(st))
+    return e, v, st.value
 
 
 def chem_point_implicit(m, T, vrho, dt):

@@ -12,6 +12,7 @@
  * -ffp-contract=off.
  */
 #include <math.h>
+#include <pthread.h>
 #include <string.h>
 
 #include "shcfd_oracle.h"
@@ -563,11 +564,102 @@ int or_chem_calc_t(const shcfd_mech *m, const double *vrho, double rho, double E
     return k;
 }
 
+/* ---- DVODE ---------------------------------------------------------------
+ * The reference's own stiff integrator (store/therm_lib/chemkin/core/
+ * dvode.f), compiled from its source into oracle/_ref/libdvode_ref.so by
+ * `make ref` and handed in by the loader (or_set_dvode).  Its state lives in
+ * COMMON blocks, so calls are serialised.  Fex and Jex (the callbacks of
+ * sub_chem.f90:1755-1757) are this file's or_chem_fex and or_chem_jex. */
+typedef void (*or_dv_f)(const int *, const double *, const double *, double *, double *, int *);
+typedef void (*or_dv_jac)(const int *, const double *, const double *, const int *, const int *, double *,
+                          const int *, double *, int *);
+typedef void (*or_dv_fn)(or_dv_f, const int *, double *, double *, const double *, const int *, const double *,
+                         const double *, const int *, int *, const int *, double *, const int *, int *, const int *,
+                         or_dv_jac, const int *, double *, int *);
+static or_dv_fn g_dvode;
+static pthread_mutex_t g_dvode_lock = PTHREAD_MUTEX_INITIALIZER;
+static const shcfd_mech *g_dvode_mech;
+static long g_dvode_calls;
+
+void or_set_dvode(void *fn) { g_dvode = (or_dv_fn)fn; }
+int or_have_dvode(void) { return g_dvode != NULL; }
+long or_dvode_calls(void) { return g_dvode_calls; }
+
+static void dv_fex(const int *neq, const double *t, const double *y, double *ydot, double *rpar, int *ipar) {
+    or_chem_fex(g_dvode_mech, y, ydot);
+}
+static void dv_jex(const int *neq, const double *t, const double *y, const int *ml, const int *mu, double *pd,
+                   const int *nrowpd, double *rpar, int *ipar) {
+    or_chem_jex(g_dvode_mech, y, pd);
+}
+
+/* The DVODE loops of chemeq2 (sub_chem.f90:1751-1770: ATOL fixed) and of
+ * reaction (:1281-1305: ATOL atol0 on the first call, atol_restart after a
+ * restart): ITOL = 1, ITASK = 1, IOPT = 0, MF = 21, integrate y from *tt to
+ * tout, restarting with ISTATE = 1 after ISTATE = -1 (500 steps) at most
+ * max_recalc = 50 times.  Returns 0, SHCFD_E_CHEM (the reference stops or
+ * reports failure), or SHCFD_E_STATE when the reference's DVODE is not
+ * loaded.  *steps (may be NULL) accumulates IWORK(11) = NST. */
+static int or_dvode_loop(const shcfd_mech *m, double *y, double *tt, double tout, double rtol, double atol0,
+                         double atol_restart, int *steps) {
+    if (!g_dvode) return SHCFD_E_STATE;
+    enum { NMAX = SHCFD_CHEM_MAX_NS + 1 };
+    int neq = m->ns + 1, itol = 1, itask = 1, iopt = 0, mf = 21, istate = 1, nrecalc = 0, e = 0;
+    int lrw = 22 + 9 * neq + 2 * neq * neq, liw = 30 + neq; /* LRW, LIW: mod_chem.raw.f90:24-25 */
+    double rwork[22 + 9 * NMAX + 2 * NMAX * NMAX], rpar[1], atol = atol0;
+    int iwork[30 + NMAX], ipar[1];
+    pthread_mutex_lock(&g_dvode_lock);
+    g_dvode_mech = m;
+    for (;;) {
+        memset(iwork, 0, sizeof(iwork));
+        g_dvode(dv_fex, &neq, y, tt, &tout, &itol, &rtol, &atol, &itask, &istate, &iopt, rwork, &lrw, iwork, &liw,
+                dv_jex, &mf, rpar, ipar);
+        if (steps) *steps += iwork[10];
+        if (istate > 0) break;
+        if (istate != -1) { /* "odd istate" */
+            e = SHCFD_E_CHEM;
+            break;
+        }
+        istate = 1;
+        atol = atol_restart;
+        nrecalc = nrecalc + 1;
+        if (nrecalc > 50) { /* "Exceed max_recalc." */
+            e = SHCFD_E_CHEM;
+            break;
+        }
+    }
+    g_dvode_calls++;
+    pthread_mutex_unlock(&g_dvode_lock);
+    return e;
+}
+
+/* reaction: sub_chem.f90:1236-1315, the per-cell DVODE integration over
+ * [0, tout] (vrho kg/m^3 in/out); on failure vrho is untouched. */
+int or_chem_reaction_dvode(const shcfd_mech *m, double T, double *vrho, double tout, int *steps) {
+    int ns = m->ns;
+    double n[SHCFD_CHEM_MAX_NS + 1], tt = 0.0;
+    for (int j = 0; j < ns; j++) {
+        n[j] = vrho[j] * m->inv_mw[j] * 1e-3;
+        n[j] = fmax(n[j], N_EPS);
+    }
+    if (steps) *steps = 0;
+    if (T < 0.0 || tout < 0.0) return SHCFD_E_ARG;
+    n[ns] = T;
+    int e = or_dvode_loop(m, n, &tt, tout, m->rtol, 0.0, m->atol, steps);
+    if (e) return e;
+    for (int j = 0; j < ns; j++) {
+        n[j] = fmax(n[j], N_EPS);
+        vrho[j] = n[j] * m->mw[j] * 1e3;
+    }
+    return 0;
+}
+
 /* chemeq2: sub_chem.f90:1714-1819.  y[0..ns-1] mol/cm^3, y[ns] = T,
- * integrated from 0 to tout.  The reference switches to DVODE once more
- * than 10 attempts have been made over the interval (:1751-1770); that
- * integrator is not restated, so such cells return SHCFD_E_CHEM, as does
- * the reference's "not converted at CHEMEQ2" stop (:1810). */
+ * integrated from 0 to tout.  Once more than 10 attempts have been made
+ * over the interval, the rest of it, from the start of the current
+ * substep, goes to DVODE (:1751-1770; SHCFD_E_STATE when the reference's
+ * DVODE is not loaded).  The reference's "not converted at CHEMEQ2" stop
+ * (:1810) returns SHCFD_E_CHEM. */
 int or_chem_chemeq2(const shcfd_mech *m, double *y, double tout, double rtol, int *attempts) {
     int ns = m->ns, N = ns + 1;
     double y0[SHCFD_CHEM_MAX_NS + 1], yp[SHCFD_CHEM_MAX_NS + 1], p[SHCFD_CHEM_MAX_NS + 1], q[SHCFD_CHEM_MAX_NS + 1],
@@ -583,7 +675,7 @@ int or_chem_chemeq2(const shcfd_mech *m, double *y, double tout, double rtol, in
             countersum = countersum + 1;
             if (countersum > 10) {
                 if (attempts) *attempts = countersum;
-                return SHCFD_E_CHEM;
+                return or_dvode_loop(m, y, &tt, tout, rtol, m->atol, m->atol, NULL); /* then dt = 0: done */
             }
             or_chem_fex_var(m, y, q0, p0);
             for (int i = 0; i < ns; i++) {

@@ -78,6 +78,10 @@ void or_chem_rho_rel2abs(const shcfd_mech *m, double p, double T, double *vrho, 
 int or_chem_calc_t(const shcfd_mech *m, const double *vrho, double rho, double E, double *T, double *out);
 int or_chem_chemeq2(const shcfd_mech *m, double *y, double tout, double rtol, int *attempts);
 int or_chem_reaction_chemeq2(const shcfd_mech *m, double T, double *vrho, double tout, int *attempts);
+int or_chem_reaction_dvode(const shcfd_mech *m, double T, double *vrho, double tout, int *steps);
+void or_set_dvode(void *fn);
+int or_have_dvode(void);
+long or_dvode_calls(void);
 int or_chem_point_implicit(const shcfd_mech *m, double T, double *vrho, double dt, int *iters, double *T_out);
 int or_dgefa(double *a, int n, int *ipvt);
 double or_exp(double x);

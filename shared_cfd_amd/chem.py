@@ -23,7 +23,7 @@ MAX_NE, MAX_NS, MAX_NR, MAX_TSEC = 8, 16, 48, 5
 RXN_NONE, RXN_REV, RXN_LOW, RXN_TROE, RXN_M, RXN_M_REV = range(6)
 THERMAL_IDEAL, THERMAL_CHEMKIN = 0, 2
 REACTION_NONE, REACTION_CHEMEQ2, REACTION_POINT_IMPLICIT = 0, 1, 2
-FEX, FEXVAR, JEX, CALCT, CHEMEQ2, POINT_IMPLICIT = range(6)
+FEX, FEXVAR, JEX, CALCT, CHEMEQ2, POINT_IMPLICIT, DVODE = range(7)
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -106,7 +106,7 @@ def boundary_state(m: Mech, p: float, T: float, vrho_rel):
 def row_sizes(op: int, ns: int):
     n1 = ns + 1
     return {FEX: (n1, n1), FEXVAR: (n1, 2 * n1), JEX: (n1, n1 * n1), CALCT: (ns + 3, 4 + 2 * ns),
-            CHEMEQ2: (n1, ns), POINT_IMPLICIT: (n1, n1)}[op]
+            CHEMEQ2: (n1, ns), POINT_IMPLICIT: (n1, n1), DVODE: (n1, ns)}[op]
 
 
 def chem_eval(m: Mech, op: int, states, dt: float = 0.0):

@@ -8,6 +8,7 @@
 // through uniform (scalar) loads.
 #include "shcfd_chem.h"
 #include "shcfd_device.h"
+#include "shcfd_dvode.h"
 #include "shcfd_kernels.h"
 
 namespace shcfd {
@@ -44,9 +45,12 @@ __global__ void __launch_bounds__(256) k_thermo_chem(DevCtx d, const Tile* tiles
 }
 
 // one cell of proceed_reaction; returns the Newton iterations / CHEMEQ2
-// attempts it took
+// attempts it took.  A CHEMEQ2 cell handed to DVODE is appended to the list
+// defer (cell index, time reached, n(ns), T) and finished by
+// k_reaction_dvode.
 template <int NS, int MODE>
-__device__ inline int reaction_cell(const DevCtx& d, const DevBlock& B, const TileCoord& t, const shcfd_mech* m) {
+__device__ inline int reaction_cell(const DevCtx& d, const DevBlock& B, const TileCoord& t, const shcfd_mech* m,
+                                    double* defer, int cap) {
     using D = Dims<NS>;
     long long c = cidx(B, t.i, t.j);
     double dt = d.scal[SCAL_DT_GRBL];
@@ -55,11 +59,26 @@ __device__ inline int reaction_cell(const DevCtx& d, const DevBlock& B, const Ti
 #pragma unroll
     for (int k = 0; k < NS; k++) vrho[k] = FLD(d, SHCFD_F_Q, k, c);
     int e, att = 0;
-    double T1;
-    if constexpr (MODE == SHCFD_REACTION_POINT_IMPLICIT)
+    if constexpr (MODE == SHCFD_REACTION_POINT_IMPLICIT) {
+        double T1;
         e = chem::point_implicit<NS>(m, T, vrho, dt, att, T1);
-    else
-        e = chem::reaction_chemeq2<NS>(m, T, vrho, dt, att);
+    } else {
+        double n[NS + 1], tt;
+        e = chem::reaction_chemeq2<NS>(m, T, vrho, dt, att, n, tt);
+        if (e == chem::CHEM_DEFER) {
+            const int k = atomicAdd(d.iflag + IFLAG_DEFER, 1);
+            if (k < cap) {
+                double* r = defer + (long long)k * (NS + 3);
+                r[0] = (double)c;
+                r[1] = tt;
+#pragma unroll
+                for (int i = 0; i <= NS; i++) r[2 + i] = n[i];
+            } else {
+                atomicCAS(d.iflag + IFLAG_ERR, 0, SHCFD_E_CHEM);
+            }
+            return att;
+        }
+    }
     if (e) {
         atomicCAS(d.iflag + IFLAG_ERR, 0, e);
         return att;
@@ -74,17 +93,55 @@ __device__ inline int reaction_cell(const DevCtx& d, const DevBlock& B, const Ti
 // per method, so each holds only its own registers.  The per-cell iteration
 // counts are summed into SCAL_CHEM_ITERS (one atomic per wave).
 template <int NS, int MODE>
-__global__ void __launch_bounds__(256) k_reaction(DevCtx d, const Tile* tiles, const shcfd_mech* __restrict__ m) {
+__global__ void __launch_bounds__(256) k_reaction(DevCtx d, const Tile* tiles, const shcfd_mech* __restrict__ m,
+                                                  double* defer, int cap) {
     TileCoord t = tile_coord(tiles);
     const DevBlock B = d.blk[t.b];
     bool inside = !(t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) && B.active;  // frozen blocks keep their state
     int att = 0;
-    if (inside) att = reaction_cell<NS, MODE>(d, B, t, m);
+    if (inside) att = reaction_cell<NS, MODE>(d, B, t, m, defer, cap);
     double w = (double)att;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
     if ((threadIdx.x & 63) == 0 && w > 0.0) atomicAdd(d.scal + SCAL_CHEM_ITERS, w);
 }
+
+// the DVODE hand-over of the CHEMEQ2 cells listed by k_reaction
+// (sub_chem.f90:1751-1770): one listed cell per lane, grid-stride over the
+// list; the DVODE steps are added to SCAL_CHEM_ITERS.  The integrator's
+// history and matrices are per-lane private arrays; the pass is separate so
+// that the CHEMEQ2 kernel keeps its register budget.
+template <int NS>
+__global__ void __launch_bounds__(64) k_reaction_dvode(DevCtx d, const shcfd_mech* __restrict__ m,
+                                                       const double* __restrict__ defer, int cap) {
+    const int cnt = min(d.iflag[IFLAG_DEFER], cap);
+    const double dt = d.scal[SCAL_DT_GRBL];
+    double steps = 0.0;
+    for (int k = blockIdx.x * 64 + (int)threadIdx.x; k < cnt; k += gridDim.x * 64) {
+        const double* r = defer + (long long)k * (NS + 3);
+        const long long c = (long long)r[0];
+        double n[NS + 1];
+        for (int i = 0; i <= NS; i++) n[i] = r[2 + i];
+        int st = 0;
+        const int e = chem::chemeq2_dvode<NS>(m, n, r[1], dt, m->rtol, m->atol, st);
+        steps += (double)st;
+        if (e) {
+            atomicCAS(d.iflag + IFLAG_ERR, 0, e);
+            continue;
+        }
+        double vrho[NS];
+        chem::n_to_vrho<NS>(m, n, vrho);
+#pragma unroll
+        for (int i = 0; i < NS; i++) FLD(d, SHCFD_F_Q, i, c) = vrho[i];
+    }
+    if (steps > 0.0) atomicAdd(d.scal + SCAL_CHEM_ITERS, steps);
+}
+
+#if SHCFD_INST_GROUP == 0
+__global__ void k_set_iflag(DevCtx d, int slot, int v) {
+    if (threadIdx.x == 0) d.iflag[slot] = v;
+}
+#endif
 
 // proceed_reaction with the point-implicit update (MODE 2), one wavefront per
 // workgroup: the (NS+1)^2 Newton matrix of each lane lives in LDS
@@ -189,31 +246,40 @@ __global__ void __launch_bounds__(256) k_chem0d(const shcfd_mech* __restrict__ m
 #pragma unroll
         for (int k = 0; k < NS; k++) y[k] = vrho[k];
         y[NS] = T1;
-    } else {
-        // in: vrho(NS), T; out: vrho(NS)
+    } else if constexpr (op == CHEM_OP_DVODE) {
+        // in: vrho(NS), T; out: vrho(NS); status: DVODE steps
         double vrho[NS];
-#pragma unroll
+        for (int k = 0; k < NS; k++) vrho[k] = x[k];
+        int steps = 0;
+        int e = chem::reaction_dvode<NS>(m, x[NS], vrho, dt, steps);
+        st = e ? -e : steps;
+        for (int k = 0; k < NS; k++) y[k] = vrho[k];
+    } else {
+        // in: vrho(NS), T; out: vrho(NS); status: CHEMEQ2 attempts (11: DVODE finished the interval)
+        double vrho[NS];
         for (int k = 0; k < NS; k++) vrho[k] = x[k];
         int att = 0;
-        int e = chem::reaction_chemeq2<NS>(m, x[NS], vrho, dt, att);
+        int e = chem::reaction_chemeq2_full<NS>(m, x[NS], vrho, dt, att);
         st = e ? -e : att;
-#pragma unroll
         for (int k = 0; k < NS; k++) y[k] = vrho[k];
     }
     status[q] = st;
 }
 
-#define SHCFD_CHEM_FLOW_INST(NS)                                                        \
-    template __global__ void k_thermo_chem<NS>(DevCtx, const Tile*, const shcfd_mech*); \
-    template __global__ void k_reaction<NS, SHCFD_REACTION_CHEMEQ2>(DevCtx, const Tile*, const shcfd_mech*);      \
-    template __global__ void k_reaction<NS, SHCFD_REACTION_POINT_IMPLICIT>(DevCtx, const Tile*, const shcfd_mech*); \
+#define SHCFD_CHEM_FLOW_INST(NS)                                                                                \
+    template __global__ void k_thermo_chem<NS>(DevCtx, const Tile*, const shcfd_mech*);                         \
+    template __global__ void k_reaction<NS, SHCFD_REACTION_CHEMEQ2>(DevCtx, const Tile*, const shcfd_mech*,     \
+                                                                    double*, int);                              \
+    template __global__ void k_reaction<NS, SHCFD_REACTION_POINT_IMPLICIT>(DevCtx, const Tile*,                 \
+                                                                           const shcfd_mech*, double*, int);    \
+    template __global__ void k_reaction_dvode<NS>(DevCtx, const shcfd_mech*, const double*, int);               \
     template __global__ void k_reaction_lds<NS>(DevCtx, const Tile*, int, const shcfd_mech*);
 #define SHCFD_CHEM0D_OP(NS, OP) \
     template __global__ void k_chem0d<NS, OP>(const shcfd_mech*, int, const double*, int, double*, int, int*, double);
 #define SHCFD_CHEM0D_INST(NS)                                                                     \
     SHCFD_CHEM0D_OP(NS, CHEM_OP_FEX) SHCFD_CHEM0D_OP(NS, CHEM_OP_FEXVAR) SHCFD_CHEM0D_OP(NS, CHEM_OP_JEX) \
     SHCFD_CHEM0D_OP(NS, CHEM_OP_CALCT) SHCFD_CHEM0D_OP(NS, CHEM_OP_CHEMEQ2)                          \
-    SHCFD_CHEM0D_OP(NS, CHEM_OP_POINT_IMPLICIT)
+    SHCFD_CHEM0D_OP(NS, CHEM_OP_POINT_IMPLICIT) SHCFD_CHEM0D_OP(NS, CHEM_OP_DVODE)
 SHCFD_FOR_EACH_FLOW_NS(SHCFD_CHEM_FLOW_INST)
 SHCFD_FOR_EACH_NS(SHCFD_CHEM0D_INST)
 

@@ -2,6 +2,7 @@
 // movement, tables, and the per-routine / per-step drivers that enqueue the
 // HIP kernels on the context's stream.
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -258,7 +259,7 @@ int shcfd_destroy(shcfd_ctx* ctx) {
     for (int fld = 0; fld < SHCFD_F_COUNT; fld++) if (ctx->f[fld]) hipFree(ctx->f[fld]);
     void* ptrs[] = {ctx->dblk, ctx->dtiles, ctx->dstrips, ctx->dprog, ctx->dbnd, ctx->dpart, ctx->scal,
                     ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool, ctx->dmech, ctx->dskew, ctx->dsbase,
-                    ctx->djac16, ctx->dstrips16, ctx->dbnd16, ctx->dnasa9, ctx->dhist};
+                    ctx->djac16, ctx->dstrips16, ctx->dbnd16, ctx->dnasa9, ctx->dhist, ctx->ddefer};
     for (void* p : ptrs) if (p) hipFree(p);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -583,20 +584,45 @@ int shcfd_set_nasa9(shcfd_ctx* ctx, const shcfd_nasa9* tables) {
     return SHCFD_OK;
 }
 
+// the list CHEMEQ2 cells are handed to DVODE through: room for every
+// interior cell, (ns + 3) doubles each; allocated outside graph capture
+static int defer_buffer(shcfd_ctx* ctx) {
+    if (ctx->ddefer) return SHCFD_OK;
+    long long cells = 0;
+    for (int b = 0; b < ctx->nb; b++) cells += (long long)ctx->hblk[b].ni * ctx->hblk[b].nj;
+    if (cells > INT_MAX) return SHCFD_E_ARG;
+    HIPCHK(hipMalloc((void**)&ctx->ddefer, sizeof(double) * (size_t)cells * (size_t)(ctx->ny + 3)));
+    ctx->defer_cap = (int)cells;
+    return SHCFD_OK;
+}
+
 int shcfd_proceed_reaction(shcfd_ctx* ctx) {
     if (!ctx) return SHCFD_E_ARG;
     if (!ctx->has_mech || ctx->p.thermal != SHCFD_THERMAL_CHEMKIN ||
         (ctx->p.reaction != SHCFD_REACTION_CHEMEQ2 && ctx->p.reaction != SHCFD_REACTION_POINT_IMPLICIT))
         return SHCFD_E_STATE;
+    if (ctx->p.reaction == SHCFD_REACTION_CHEMEQ2) {
+        const int rc = defer_buffer(ctx);
+        if (rc) return rc;
+    }
     ProfScope ps(ctx, PROF_REACTION);
     if (ctx->p.reaction == SHCFD_REACTION_POINT_IMPLICIT && ctx->ny <= SHCFD_LDS_NEWTON_MAX_NS)
         DISPATCH_FLOW_NS(ctx->ny, hipLaunchKernelGGL(k_reaction_lds<NS>, dim3((unsigned)(ntiles(ctx) * ctx->tile_rows)),
                                                      dim3(64), 0, ctx->stream, ctx->dev(), ctx->dtiles,
                                                      ctx->tile_rows, ctx->dmech))
     else if (ctx->p.reaction == SHCFD_REACTION_POINT_IMPLICIT)
-        DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH((k_reaction<NS, SHCFD_REACTION_POINT_IMPLICIT>), ctx->dmech))
-    else
-        DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH((k_reaction<NS, SHCFD_REACTION_CHEMEQ2>), ctx->dmech))
+        DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH((k_reaction<NS, SHCFD_REACTION_POINT_IMPLICIT>), ctx->dmech, nullptr, 0))
+    else {
+        // CHEMEQ2, then DVODE for the cells it hands over (sub_chem.f90:1751-1770):
+        // the list is emptied, filled by the CHEMEQ2 pass and drained by a
+        // grid-stride pass that integrates one listed cell per lane
+        hipLaunchKernelGGL(k_set_iflag, dim3(1), dim3(64), 0, ctx->stream, ctx->dev(), (int)IFLAG_DEFER, 0);
+        DISPATCH_FLOW_NS(ctx->ny, TILE_LAUNCH((k_reaction<NS, SHCFD_REACTION_CHEMEQ2>), ctx->dmech, ctx->ddefer,
+                                              ctx->defer_cap))
+        const unsigned g = (unsigned)std::min(((long long)ctx->defer_cap + 63) / 64, 1024LL);
+        DISPATCH_FLOW_NS(ctx->ny, hipLaunchKernelGGL(k_reaction_dvode<NS>, dim3(g), dim3(64), 0, ctx->stream,
+                                                     ctx->dev(), ctx->dmech, ctx->ddefer, ctx->defer_cap))
+    }
     LAUNCHCHK();
     return SHCFD_OK;
 }
@@ -1333,6 +1359,7 @@ int shcfd_energy_rms_blocks(shcfd_ctx* ctx, double* rms) {
 // implicit schemes but dual time), else the strip-skewed copy (nY > 1 dual
 // time, or when the quad-row copy could not be allocated)
 static void warm_allocations(shcfd_ctx* ctx) {
+    if (ctx->p.thermal == SHCFD_THERMAL_CHEMKIN && ctx->p.reaction == SHCFD_REACTION_CHEMEQ2) defer_buffer(ctx);
     const int s = ctx->p.scheme;
     const bool implicit = s == SHCFD_SCHEME_LUSGS || s == SHCFD_SCHEME_PRECON_LUSGS || inner_loop_scheme(s);
     if (!implicit || fast_enabled(ctx)) return;

@@ -697,13 +697,19 @@ SHCFD_HD int calc_t(const shcfd_mech* m, const double* vrho, double rho, double 
     return k;
 }
 
+// chemeq2's return value when it hands the rest of the interval to DVODE
+constexpr int CHEM_DEFER = 1 << 20;
+
 // chemeq2: sub_chem.f90:1714-1819.  y[0..NS) mol/cm^3, y[NS] = T, from 0 to
-// tout.  The reference hands the rest of the interval to DVODE once more
-// than 10 attempts were made (:1751-1770); that integrator is not part of
-// this library, so such a cell returns SHCFD_E_CHEM, as does the
-// reference's "not converted at CHEMEQ2" stop (:1810).
+// tout.  Once more than 10 attempts were made the reference hands the rest
+// of the interval, from the start of the current substep, to DVODE
+// (:1751-1770): this returns CHEM_DEFER with y holding that substep's
+// initial state and tt_out its time, and the caller continues with
+// chemeq2_dvode (shcfd_dvode.h).  The reference's "not converted at
+// CHEMEQ2" stop (:1810) returns SHCFD_E_CHEM.
 template <int NS>
-SHCFD_HD int chemeq2(const shcfd_mech* m, double (&y)[NS + 1], double tout, double rtol, int& attempts) {
+SHCFD_HD int chemeq2(const shcfd_mech* m, double (&y)[NS + 1], double tout, double rtol, int& attempts,
+                     double& tt_out) {
     constexpr int N = NS + 1;
     double y0[N], yp[N], p[N], q[N], p0[N], q0[N];
     double tt = 0.0, dt = tout - tt, ratio = 1e300;
@@ -719,7 +725,8 @@ SHCFD_HD int chemeq2(const shcfd_mech* m, double (&y)[NS + 1], double tout, doub
             countersum = countersum + 1;
             if (countersum > 10) {
                 attempts = countersum;
-                return SHCFD_E_CHEM;
+                tt_out = tt;
+                return CHEM_DEFER;
             }
             fex_var<NS>(m, y, q0, p0);
 #pragma unroll
@@ -761,25 +768,38 @@ SHCFD_HD int chemeq2(const shcfd_mech* m, double (&y)[NS + 1], double tout, doub
     return 0;
 }
 
-// reaction_chemeq2: sub_chem.f90:1684-1713 (vrho kg/m^3 in/out)
+// the conversions of reaction_chemeq2 (sub_chem.f90:1696-1700, :1707-1711)
 template <int NS>
-SHCFD_HD int reaction_chemeq2(const shcfd_mech* m, double T, double (&vrho)[NS], double tout, int& attempts) {
-    double n[NS + 1];
+SHCFD_HD void vrho_to_n(const shcfd_mech* m, const double (&vrho)[NS], double (&n)[NS + 1]) {
 #pragma unroll
     for (int j = 0; j < NS; j++) {
         n[j] = vrho[j] * m->inv_mw[j] * 1e-3;
         n[j] = fmax2(n[j], N_EPS);
     }
-    attempts = 0;
-    if (T < 0.0 || tout < 0.0) return SHCFD_E_ARG;
-    n[NS] = T;
-    int e = chemeq2<NS>(m, n, tout, m->rtol, attempts);
-    if (e) return e;
+}
+template <int NS>
+SHCFD_HD void n_to_vrho(const shcfd_mech* m, double (&n)[NS + 1], double (&vrho)[NS]) {
 #pragma unroll
     for (int j = 0; j < NS; j++) {
         n[j] = fmax2(n[j], N_EPS);
         vrho[j] = n[j] * m->mw[j] * 1e3;
     }
+}
+
+// reaction_chemeq2: sub_chem.f90:1684-1713 (vrho kg/m^3 in/out).  On
+// CHEM_DEFER, vrho is untouched and n, tt hold the state DVODE continues
+// from (chemeq2_dvode in shcfd_dvode.h, then n_to_vrho).
+template <int NS>
+SHCFD_HD int reaction_chemeq2(const shcfd_mech* m, double T, double (&vrho)[NS], double tout, int& attempts,
+                              double (&n)[NS + 1], double& tt) {
+    vrho_to_n<NS>(m, vrho, n);
+    attempts = 0;
+    tt = 0.0;
+    if (T < 0.0 || tout < 0.0) return SHCFD_E_ARG;
+    n[NS] = T;
+    int e = chemeq2<NS>(m, n, tout, m->rtol, attempts, tt);
+    if (e) return e;
+    n_to_vrho<NS>(m, n, vrho);
     return 0;
 }
 

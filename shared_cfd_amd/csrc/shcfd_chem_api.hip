@@ -18,7 +18,7 @@ int row_in(int op, int ns) {
     switch (op) {
     case SHCFD_CHEM_FEX: case SHCFD_CHEM_FEXVAR: case SHCFD_CHEM_JEX: return ns + 1;
     case SHCFD_CHEM_CALCT: return ns + 3;
-    case SHCFD_CHEM_CHEMEQ2: case SHCFD_CHEM_POINT_IMPLICIT: return ns + 1;
+    case SHCFD_CHEM_CHEMEQ2: case SHCFD_CHEM_POINT_IMPLICIT: case SHCFD_CHEM_DVODE: return ns + 1;
     default: return -1;
     }
 }
@@ -28,7 +28,7 @@ int row_out(int op, int ns) {
     case SHCFD_CHEM_FEXVAR: return 2 * (ns + 1);
     case SHCFD_CHEM_JEX: return (ns + 1) * (ns + 1);
     case SHCFD_CHEM_CALCT: return 4 + 2 * ns;
-    case SHCFD_CHEM_CHEMEQ2: return ns;
+    case SHCFD_CHEM_CHEMEQ2: case SHCFD_CHEM_DVODE: return ns;
     case SHCFD_CHEM_POINT_IMPLICIT: return ns + 1;
     default: return -1;
     }
@@ -44,6 +44,7 @@ void launch0d(int op, const shcfd_mech* dm, int n, const double* din, int ldi, d
     case SHCFD_CHEM_JEX: hipLaunchKernelGGL((k_chem0d<NS, CHEM_OP_JEX>), g, b, 0, 0, dm, n, din, ldi, dout, ldo, dst, dt); break;
     case SHCFD_CHEM_CALCT: hipLaunchKernelGGL((k_chem0d<NS, CHEM_OP_CALCT>), g, b, 0, 0, dm, n, din, ldi, dout, ldo, dst, dt); break;
     case SHCFD_CHEM_CHEMEQ2: hipLaunchKernelGGL((k_chem0d<NS, CHEM_OP_CHEMEQ2>), g, b, 0, 0, dm, n, din, ldi, dout, ldo, dst, dt); break;
+    case SHCFD_CHEM_DVODE: hipLaunchKernelGGL((k_chem0d<NS, CHEM_OP_DVODE>), g, b, 0, 0, dm, n, din, ldi, dout, ldo, dst, dt); break;
     default: hipLaunchKernelGGL((k_chem0d<NS, CHEM_OP_POINT_IMPLICIT>), g, b, 0, 0, dm, n, din, ldi, dout, ldo, dst, dt); break;
     }
 }

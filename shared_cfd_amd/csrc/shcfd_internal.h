@@ -47,7 +47,7 @@ enum {
     SCAL_CHEM_ITERS = 7,  // Newton iterations / CHEMEQ2 attempts summed over cells
     SCAL_COUNT = 16
 };
-enum { IFLAG_ERR = 0, IFLAG_COUNT = 8 };
+enum { IFLAG_ERR = 0, IFLAG_DEFER = 1, IFLAG_COUNT = 8 };  // IFLAG_DEFER: records in the DVODE hand-over list
 
 // Schemes whose physical step runs the inner loop of dual time: dq/dt from
 // q, qp, qpp (set_dqdt), an omega-limited update and the residual norm res
@@ -187,6 +187,8 @@ struct Ctx {
     int update_pending = 0;        // q += omega Dq is still to be applied
     double* dhist = nullptr;       // (res1, res) per physical step of an inner-loop graph
     int nhist = 0;
+    double* ddefer = nullptr;      // CHEMEQ2 cells handed to DVODE: (cell, t, n(ns), T) per record
+    int defer_cap = 0;             // records: one per interior cell
     int q16_active = 0;            // the last set_ABpm of a step wrote the quad-row copy
     int skew_active = 0;           // the last set_ABpm of a step wrote the skewed copy
     double* dskew = nullptr;

@@ -509,9 +509,11 @@ __global__ void k_sweep16_bwd(DevCtx, const Strip16*, int, double*, long long, d
 __global__ void k_copy_scal(DevCtx, int, int);
 // chemistry (k_chem.hip)
 enum { CHEM_OP_FEX = 0, CHEM_OP_FEXVAR = 1, CHEM_OP_JEX = 2, CHEM_OP_CALCT = 3, CHEM_OP_CHEMEQ2 = 4,
-       CHEM_OP_POINT_IMPLICIT = 5 };
+       CHEM_OP_POINT_IMPLICIT = 5, CHEM_OP_DVODE = 6 };
 template <int NS> __global__ void k_thermo_chem(DevCtx, const Tile*, const shcfd_mech*);
-template <int NS, int MODE> __global__ void k_reaction(DevCtx, const Tile*, const shcfd_mech*);
+template <int NS, int MODE> __global__ void k_reaction(DevCtx, const Tile*, const shcfd_mech*, double*, int);
+template <int NS> __global__ void k_reaction_dvode(DevCtx, const shcfd_mech*, const double*, int);
+__global__ void k_set_iflag(DevCtx, int, int);
 template <int NS> __global__ void k_reaction_lds(DevCtx, const Tile*, int, const shcfd_mech*);
 template <int NS, int OP> __global__ void k_chem0d(const shcfd_mech*, int, const double*, int, double*, int, int*, double);
 // geometry (k_geom.hip)

@@ -62,6 +62,25 @@ def mixture_states(m, n, seed, p=1.01325e5, tmin=700.0, tmax=2200.0):
     return out
 
 
+def ignition_states(m, n, seed, p=1.01325e5, tmin=1000.0, tmax=2200.0):
+    """(vrho kg/m^3, T) of H2/air mixtures seeded with radicals, for the
+    no27 species order (N2, O2, H2, OH, O, H, HO2, H2O) and any species
+    after those at small amounts: states whose ignition over 1e-7..1e-5 s
+    needs more than 10 CHEMEQ2 attempts."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        rel = np.empty(m.ns)
+        rel[:8] = [0.70, 0.20, rng.uniform(0.02, 0.3), *(10.0 ** rng.uniform(-6, -3, 4)), 10.0 ** rng.uniform(-4, -1)]
+        rel[8:] = 1e-3 * rng.random(m.ns - 8)
+        T = rng.uniform(tmin, tmax)
+        vrho, rho, E = O.chem_rho_rel2abs(m, p, T, rel)
+        out.append((vrho, T))
+    return out
+
+
 def no27_plus_inert(outdir, k):
     """The sample no27 mechanism with k inert species appended, N2-like
     copies named N2A, N2B, ... (same NASA-7 record and element counts), for
