@@ -48,8 +48,13 @@ enum {
                                     split Jacobians in the strip-skewed layout that
                                     the row-per-lane sweeps read coalesced; 0: the
                                     Ap/Am/Bp/Bm fields                              */
-    SHCFD_OPT_TILE_ROWS = 4      /* rows of the 64-wide cell tiles of the per-cell
+    SHCFD_OPT_TILE_ROWS = 4,     /* rows of the 64-wide cell tiles of the per-cell
                                     kernels: 1, 2 or 4 (default)                    */
+    SHCFD_OPT_JVP_SWEEP = 5      /* 1 (default): LU-SGS and NR at nY > 1 apply the
+                                    split Jacobians in closed form from the cell
+                                    state inside the 16-row sweeps (no operator
+                                    copy); 0: the sweeps read a copy of the
+                                    matrices (bit-identical to the field sweeps)   */
 };
 int shcfd_set_option(shcfd_ctx *ctx, int option, int value);
 int shcfd_version(void);

@@ -315,11 +315,363 @@ k_sweep16_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long l
     }
 }
 
+// ---------------------------------------------------------------------------
+// Jacobian-free quad-row sweeps: LU-SGS and NR at nY > 1.
+//
+// The split Jacobians of these schemes are A+- = Atilde/2 +- (nuA/2) I with
+// Atilde = n_x A + n_y B (LU-SGS/2d/sch_lusgs.raw.f90:112-129; NR likewise),
+// A and B the species-extended Euler flux Jacobians of :39-110.  Their
+// product with x = (x_k (k < nY), X, V, E) follows in closed form from those
+// entries:
+//     (Atilde x)_k    = U x_k + Y_k e                    (k < nY)
+//     (Atilde x)_nY   = U X + u e + n_x dp
+//     (Atilde x)_nY+1 = U V + v e + n_y dp
+//     (Atilde x)_nY+2 = U E + H e + U dp
+// with U = n_x u + n_y v, s = sum_k x_k, e = n_x X + n_y V - U s and
+// dp = sum_k DHi_k x_k + (gamma-1) ((u^2+v^2)/2 s - u X - v V + E).
+// A step therefore needs the cell's state (2 nY + 13 numbers) instead of
+// the 2 dq^2 entries of the two matrices: every lane reads one pair (c1, DHi)
+// (c1 = Y_k, u, v or H for its component) and the 13 cell scalars come
+// through LDS, written by the lanes of components 0..7 from their prefetched
+// share.  The sums s and dp are taken in the same order by every lane.  The
+// algebra is exact; the rounding differs from the matrix products
+// (agreement with them to ~1e-13 relative per sweep).
+// ---------------------------------------------------------------------------
+
+namespace {
+constexpr int JPD = 11;   // prefetch distance of the cell data, in steps (JPD + 1 = 12 steps per round)
+// a round of JPD + 1 steps must cycle whole rings: the hand-off loads (PH),
+// the scalar slots (3) and the LDS exchange buffers (2)
+static_assert((JPD + 1) % PH == 0 && (JPD + 1) % 6 == 0, "round length");
+constexpr int JSC = 16;   // cell scalars per (step, row): u v g H nxi nyi nxj nyj nuA/2 nuB/2 alpha dsci dscj
+enum { JS_U = 0, JS_V, JS_G, JS_H, JS_NXI, JS_NYI, JS_NXJ, JS_NYJ, JS_HNA, JS_HNB, JS_AL, JS_DSCI, JS_DSCJ };
+
+template <int CTRL>
+__device__ inline double quad_perm(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+// (x0 + x1) + (x2 + x3) of the four lanes of each quad, in every lane
+__device__ inline double quad_sum(double x) {
+    const double y = x + quad_perm<1 | (0 << 2) | (3 << 4) | (2 << 6)>(x);   // quad_perm [1,0,3,2]
+    return y + quad_perm<2 | (3 << 2) | (0 << 4) | (1 << 6)>(y);             // quad_perm [2,3,0,1]
+}
+
+// per-lane inputs of one step of the Jacobian-free sweeps
+struct JStepIn {
+    double2 cp;    // (c1, DHi) of the lane's component
+    double2 sp;    // the lane's share of the next step's cell scalars
+    double r, q;
+};
+
+// the lane's outgoing terms dsci A+-(x) and dscj B+-(x) (sgn = +1 forward,
+// -1 backward) from its row's x (xa, in LDS order), the quad partial sums of
+// DHi x (pq) and the cell scalars (sc)
+template <int NY, int C>
+__device__ inline void jvp_out(const double* xa, const double* pq, const double* sc, int k, double c1, double sgn,
+                               double& a_out, double& b_out) {
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < NY; l++) s = s + xa[l];
+    double qd = pq[0];
+#pragma unroll
+    for (int w = 1; w < C; w++) qd = qd + pq[w];
+    const double X = xa[NY], V = xa[NY + 1], E = xa[NY + 2];
+    const double u = sc[JS_U], v = sc[JS_V], g = sc[JS_G];
+    const double nxi = sc[JS_NXI], nyi = sc[JS_NYI], nxj = sc[JS_NXJ], nyj = sc[JS_NYJ];
+    const double Ui = (nxi * u) + (nyi * v), Uj = (nxj * u) + (nyj * v);
+    const double ei = ((nxi * X) + (nyi * V)) - (Ui * s);
+    const double ej = ((nxj * X) + (nyj * V)) - (Uj * s);
+    const double dp = qd + g * (((((0.5 * ((u * u) + (v * v))) * s) - (u * X)) - (v * V)) + E);
+    const double xk = xa[k];
+    const double cai = k < NY ? 0.0 : (k == NY ? nxi : (k == NY + 1 ? nyi : Ui));
+    const double caj = k < NY ? 0.0 : (k == NY ? nxj : (k == NY + 1 ? nyj : Uj));
+    const double oA = ((Ui * xk) + (c1 * ei)) + (cai * dp);
+    const double oB = ((Uj * xk) + (c1 * ej)) + (caj * dp);
+    a_out = sc[JS_DSCI] * ((0.5 * oA) + sgn * (sc[JS_HNA] * xk));
+    b_out = sc[JS_DSCJ] * ((0.5 * oB) + sgn * (sc[JS_HNB] * xk));
+}
+}  // namespace
+
+// the cell data of the Jacobian-free sweeps, in their strip-step layout:
+// the pair (c1, DHi) of (global step g, wave w, lane L) at cp[(g C + w) 64 + L]
+// and scalar p of (step g, row r) at sc[(g 16 + r) JSC + p].  Slots of no
+// cell stay zero (the arrays are cleared once at allocation).  Also stores
+// alpha, as set_ABpm does.
+template <int NY>
+__global__ void __launch_bounds__(256) k_set_jd16(DevCtx d, const Strip16* strips, double2* cp, double* sc) {
+    using D = Dims<NY>;
+    constexpr int dq = D::dq, C = (dq + 3) / 4;
+    const Strip16 S = strips[blockIdx.x];
+    const int t = (int)blockIdx.y * 16 + (int)(threadIdx.x >> 4), r = (int)(threadIdx.x & 15);
+    const DevBlock B = d.blk[S.b];
+    if (t >= B.ni + 15) return;
+    const int i = t - r + 1, j = S.j0 + r;
+    if (i < 1 || i > B.ni || j > B.nj) return;
+    const long long c = cidx(B, i, j), g = S.base + t;
+    // the scalars of cell_jacobian for the non-preconditioned schemes
+    // (sch_lusgs.raw.f90:26-38, :131-133; sch_NR.raw.f90:134-136)
+    const double rho = FLD(d, SHCFD_F_W, 0, c), u = FLD(d, SHCFD_F_W, IW_U, c), v = FLD(d, SHCFD_F_W, IW_V, c);
+    const double p = FLD(d, SHCFD_F_W, IW_P, c), gamm = FLD(d, SHCFD_F_W, D::ig, c), H = FLD(d, SHCFD_F_W, D::ih, c);
+    const double cs = sqrt((gamm * p) / rho);
+    const double n1a = FLD(d, SHCFD_F_VNCI, 0, c), n2a = FLD(d, SHCFD_F_VNCI, 1, c);
+    const double n1b = FLD(d, SHCFD_F_VNCJ, 0, c), n2b = FLD(d, SHCFD_F_VNCJ, 1, c);
+    const double unA = (n1a * u) + (n2a * v), unB = (n1b * u) + (n2b * v);
+    const double nuA = fabs(unA) + cs, nuB = fabs(unB) + cs;
+    const double dsci = FLD(d, SHCFD_F_DSCI, 0, c), dscj = FLD(d, SHCFD_F_DSCJ, 0, c);
+    const double vol = FLD(d, SHCFD_F_VOL, 0, c);
+    const double dtv = (d.p.time_step == 0) ? d.scal[SCAL_DT_GRBL] : FLD(d, SHCFD_F_DT, 0, c);
+    const double alpha = (d.p.scheme == SHCFD_SCHEME_NR)
+                             ? 1.0 / ((((3.0 * vol) / (2.0 * dtv)) + (dsci * nuA)) + (dscj * nuB))
+                             : 1.0 / (((vol / dtv) + (dsci * nuA)) + (dscj * nuB));
+    FLD(d, SHCFD_F_ALPHA, 0, c) = alpha;
+#pragma unroll
+    for (int w = 0; w < C; w++)
+#pragma unroll
+        for (int cc = 0; cc < 4; cc++) {
+            const int k = 4 * w + cc;
+            double c1 = 0.0, dh = 0.0;
+            if (k < NY) {
+                c1 = FLD(d, SHCFD_F_W, 4 + k, c);
+                dh = FLD(d, SHCFD_F_DHI, k, c);
+            } else if (k == NY) {
+                c1 = u;
+            } else if (k == NY + 1) {
+                c1 = v;
+            } else if (k == NY + 2) {
+                c1 = H;
+            }
+            cp[(g * C + w) * 64 + 4 * r + cc] = make_double2(c1, dh);
+        }
+    double* o = sc + (g * 16 + r) * JSC;
+    o[JS_U] = u;
+    o[JS_V] = v;
+    o[JS_G] = gamm - 1.0;
+    o[JS_H] = H;
+    o[JS_NXI] = n1a;
+    o[JS_NYI] = n2a;
+    o[JS_NXJ] = n1b;
+    o[JS_NYJ] = n2b;
+    o[JS_HNA] = nuA * 0.5;
+    o[JS_HNB] = nuB * 0.5;
+    o[JS_AL] = alpha;
+    o[JS_DSCI] = dsci;
+    o[JS_DSCJ] = dscj;
+}
+
+// forward sweep, Jacobian-free: x = alpha (R + A+(i-1) x(i-1) dsci + B+(j-1) x(j-1) dscj)
+template <int NY>
+__global__ void __launch_bounds__(NY + 3 <= 16 ? 256 : 320) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_sweep16j_fwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long long hstride, long long spin_limit,
+               const double2* __restrict__ jcp, const double* __restrict__ jsc) {
+    constexpr int dq = NY + 3, C = (dq + 3) / 4, DQP = 4 * C;
+    __shared__ double xs[2][16][DQP];
+    __shared__ double pq[2][16][C];
+    __shared__ __align__(16) double scs[3][16][JSC];
+    const int s = blockIdx.x;
+    const int w = (int)(threadIdx.x >> 6), L = (int)(threadIdx.x & 63), r = L >> 2, k = 4 * w + (L & 3);
+    const Strip16 S = strips[s];
+    const DevBlock B = d.blk[S.b];
+    const int ni = B.ni, nj = B.nj, j = S.j0 + r;
+    const bool rowok = j <= nj, kok = k < dq;
+    const bool has_next = S.next >= 0;
+    const bool mine = r == 0 && kok;
+    const int nsteps = ni + 15;
+    const int sp_slot = 4 * w + (L & 3);   // the scalar pair this lane carries (< 8 only)
+    const int kk = kok ? k : 0;
+    const double* hsrc = hand + (long long)(S.prev >= 0 ? S.prev : nstrips) * hstride + (long long)HO * dq + kk;
+    double* hdst = hand + (long long)s * hstride + (long long)HO * dq + kk;
+    double* dqf = d.f[SHCFD_F_DQ] + (long long)kk * d.NC;
+    auto scal_pair = [&](int t) {
+        t = t < nsteps ? t : nsteps - 1;
+        return sp_slot < 8 ? *reinterpret_cast<const double2*>(jsc + ((S.base + t) * 16 + r) * JSC + 2 * sp_slot)
+                           : make_double2(0.0, 0.0);
+    };
+    auto load = [&](int t, JStepIn& in) {
+        const int tc = t < nsteps ? t : nsteps - 1;  // past the end: re-read the last step
+        in.cp = jcp[((S.base + tc) * C + w) * 64 + L];
+        in.sp = scal_pair(t + 1);
+        const int i = tc - r + 1;
+        const int ic = i < 1 ? 1 : (i > ni ? ni : i);
+        in.r = dqf[cidx(B, ic, rowok ? j : nj)];
+    };
+    // the cell scalars of step 0
+    {
+        const double2 v0 = scal_pair(0);
+        if (sp_slot < 8) *reinterpret_cast<double2*>(&scs[0][r][2 * sp_slot]) = v0;
+    }
+    JStepIn buf[JPD + 1];
+    double hr[PH];
+#pragma unroll
+    for (int p = 0; p < PH; p++) hr[p] = load_agent(hsrc + (long long)(p + 1) * dq);  // columns 1..PH
+#pragma unroll
+    for (int p = 0; p < JPD; p++) load(p, buf[p]);
+    lds_barrier();
+    double a_in = 0.0, bo = 0.0;
+    bool dead = false;
+    const int tend = ((nsteps + JPD) / (JPD + 1)) * (JPD + 1);  // whole rounds; the extra steps are idle
+    for (int t0 = 0; t0 < tend; t0 += JPD + 1) {
+#pragma unroll
+        for (int u = 0; u <= JPD; u++) {
+            const int t = t0 + u;
+            double hv = hr[u % PH];
+            hr[u % PH] = load_agent(hsrc + (long long)(t + 1 + PH) * dq);
+            load(t + JPD, buf[(u + JPD) % (JPD + 1)]);
+            const JStepIn& in = buf[u];
+            const double* sct = scs[u % 3][r];
+            const int i = t - r + 1;
+            const bool act = rowok && i >= 1 && i <= ni;
+            double b_in = shift_up4(bo);
+            if (__builtin_expect(__ballot(mine && is_sent16(hv)) != 0ULL, 0))
+                hv = poll16(hsrc + (long long)(t + 1) * dq, hv, mine, spin_limit, d.iflag, dead);
+            if (r == 0) b_in = kok ? hv : 0.0;
+            double x = 0.0;
+            if (act && kok) {
+                x = sct[JS_AL] * (in.r + (a_in + b_in));
+                dqf[cidx(B, i, j)] = x;
+            }
+            xs[u & 1][r][k] = x;
+            const double pd = quad_sum(in.cp.y * x);
+            if ((L & 3) == 0) pq[u & 1][r][w] = pd;
+            if (sp_slot < 8) *reinterpret_cast<double2*>(&scs[(u + 1) % 3][r][2 * sp_slot]) = in.sp;
+            lds_barrier();
+            if (act) {
+                jvp_out<NY, C>(&xs[u & 1][r][0], &pq[u & 1][r][0], sct, kok ? k : 0, in.cp.x, 1.0, a_in, bo);
+            } else {
+                bo = 0.0;
+            }
+            if (has_next && r == 15 && act && kok) store_agent(hdst + (long long)i * dq, bo);
+        }
+    }
+}
+
+// backward sweep, Jacobian-free: x = x* - alpha (A-(i+1) x(i+1) dsci + B-(j+1) x(j+1) dscj),
+// then q += x (LU-SGS) or the (res, omega) partials (NR)
+template <int NY>
+__global__ void __launch_bounds__(NY + 3 <= 16 ? 256 : 320) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_sweep16j_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long long hstride, double* part,
+               long long spin_limit, const double2* __restrict__ jcp, const double* __restrict__ jsc) {
+    constexpr int dq = NY + 3, C = (dq + 3) / 4, DQP = 4 * C;
+    __shared__ double xs[2][16][DQP];
+    __shared__ double pq[2][16][C];
+    __shared__ __align__(16) double scs[3][16][JSC];
+    __shared__ double red[2][4];
+    const int s = nstrips - 1 - (int)blockIdx.x;
+    const int w = (int)(threadIdx.x >> 6), L = (int)(threadIdx.x & 63), r = L >> 2, k = 4 * w + (L & 3);
+    const Strip16 S = strips[s];
+    const DevBlock B = d.blk[S.b];
+    const int ni = B.ni, nj = B.nj, j = S.j0 + r;
+    const bool rowok = j <= nj, kok = k < dq;
+    const bool has_prev = S.prev >= 0;
+    const bool mine = r == 15 && kok;
+    const bool inner = inner_loop_scheme(d.p.scheme);
+    const double dqmax = d.p.dqmax;
+    const int nsteps = ni + 15;
+    const int sp_slot = 4 * w + (L & 3);
+    const int kk = kok ? k : 0;
+    const double* hsrc = hand + (long long)(S.next >= 0 ? S.next : nstrips) * hstride + (long long)HO * dq + kk;
+    double* hdst = hand + (long long)s * hstride + (long long)HO * dq + kk;
+    double* dqf = d.f[SHCFD_F_DQ] + (long long)kk * d.NC;
+    double* qf = d.f[SHCFD_F_Q] + (long long)kk * d.NC;
+    // backward step t of row r visits the cell of forward step ni + 14 - t
+    auto scal_pair = [&](int t) {
+        t = t < nsteps ? t : nsteps - 1;
+        return sp_slot < 8
+                   ? *reinterpret_cast<const double2*>(jsc + ((S.base + (ni + 14 - t)) * 16 + r) * JSC + 2 * sp_slot)
+                   : make_double2(0.0, 0.0);
+    };
+    auto load = [&](int t, JStepIn& in) {
+        const int tc = t < nsteps ? t : nsteps - 1;
+        in.cp = jcp[((S.base + (ni + 14 - tc)) * C + w) * 64 + L];
+        in.sp = scal_pair(t + 1);
+        const int i = ni - tc + 15 - r;
+        const int ic = i < 1 ? 1 : (i > ni ? ni : i);
+        const long long c = cidx(B, ic, rowok ? j : nj);
+        in.r = dqf[c];
+        in.q = qf[c];
+    };
+    {
+        const double2 v0 = scal_pair(0);
+        if (sp_slot < 8) *reinterpret_cast<double2*>(&scs[0][r][2 * sp_slot]) = v0;
+    }
+    JStepIn buf[JPD + 1];
+    double hr[PH];
+#pragma unroll
+    for (int p = 0; p < PH; p++) hr[p] = load_agent(hsrc + (long long)(ni - p) * dq);  // columns ni..ni-PH+1
+#pragma unroll
+    for (int p = 0; p < JPD; p++) load(p, buf[p]);
+    lds_barrier();
+    double a_in = 0.0, bo = 0.0, res = 0.0, omega = 1e300;
+    bool dead = false;
+    const int tend = ((nsteps + JPD) / (JPD + 1)) * (JPD + 1);
+    for (int t0 = 0; t0 < tend; t0 += JPD + 1) {
+#pragma unroll
+        for (int u = 0; u <= JPD; u++) {
+            const int t = t0 + u;
+            double hv = hr[u % PH];
+            hr[u % PH] = load_agent(hsrc + (long long)(ni - t - PH) * dq);
+            load(t + JPD, buf[(u + JPD) % (JPD + 1)]);
+            const JStepIn& in = buf[u];
+            const double* sct = scs[u % 3][r];
+            const int i = ni - t + 15 - r;
+            const bool act = rowok && i >= 1 && i <= ni;
+            double b_in = shift_down4(bo);
+            if (__builtin_expect(__ballot(mine && is_sent16(hv)) != 0ULL, 0))
+                hv = poll16(hsrc + (long long)(ni - t) * dq, hv, mine, spin_limit, d.iflag, dead);
+            if (r == 15) b_in = kok ? hv : 0.0;
+            double x = 0.0;
+            if (act && kok) {
+                const long long c = cidx(B, i, j);
+                x = in.r - sct[JS_AL] * (a_in + b_in);
+                if (inner) {
+                    if (k == NY + 2) res = res + x * x;
+                    if (k < NY && x < 0.0) omega = dmin(omega, (-in.q / x) * dqmax);
+                } else if (B.active) {
+                    qf[c] = in.q + x;
+                }
+                dqf[c] = x;
+            }
+            xs[u & 1][r][k] = x;
+            const double pd = quad_sum(in.cp.y * x);
+            if ((L & 3) == 0) pq[u & 1][r][w] = pd;
+            if (sp_slot < 8) *reinterpret_cast<double2*>(&scs[(u + 1) % 3][r][2 * sp_slot]) = in.sp;
+            lds_barrier();
+            if (act) {
+                jvp_out<NY, C>(&xs[u & 1][r][0], &pq[u & 1][r][0], sct, kok ? k : 0, in.cp.x, -1.0, a_in, bo);
+            } else {
+                bo = 0.0;
+            }
+            if (has_prev && r == 0 && act && kok) store_agent(hdst + (long long)i * dq, bo);
+        }
+    }
+    if (inner) {
+        for (int off = 32; off >= 1; off >>= 1) {
+            res = res + __shfl_xor(res, off);
+            omega = dmin(omega, __shfl_xor(omega, off));
+        }
+        if (L == 0) { red[0][w] = res; red[1][w] = omega; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double R = red[0][0], O = red[1][0];
+            for (int q = 1; q < C; q++) { R = R + red[0][q]; O = dmin(O, red[1][q]); }
+            part[2 * s] = R;
+            part[2 * s + 1] = O;
+        }
+    }
+}
+
 #define INST(NY)                                                                                              \
     template __global__ void k_sweep16_fwd<NY>(DevCtx, const Strip16*, int, double*, long long, long long,      \
                                                const double*, long long);                                     \
     template __global__ void k_sweep16_bwd<NY>(DevCtx, const Strip16*, int, double*, long long, double*,        \
-                                               long long, const double*, long long);
+                                               long long, const double*, long long);                          \
+    template __global__ void k_set_jd16<NY>(DevCtx, const Strip16*, double2*, double*);                         \
+    template __global__ void k_sweep16j_fwd<NY>(DevCtx, const Strip16*, int, double*, long long, long long,     \
+                                                const double2*, const double*);                               \
+    template __global__ void k_sweep16j_bwd<NY>(DevCtx, const Strip16*, int, double*, long long, double*,       \
+                                                long long, const double2*, const double*);
 SHCFD_FOR_EACH_NY(INST)
 #undef INST
 

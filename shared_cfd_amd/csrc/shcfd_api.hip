@@ -259,7 +259,8 @@ int shcfd_destroy(shcfd_ctx* ctx) {
     for (int fld = 0; fld < SHCFD_F_COUNT; fld++) if (ctx->f[fld]) hipFree(ctx->f[fld]);
     void* ptrs[] = {ctx->dblk, ctx->dtiles, ctx->dstrips, ctx->dprog, ctx->dbnd, ctx->dpart, ctx->scal,
                     ctx->iflag, ctx->dcut, ctx->dbc, ctx->dpool, ctx->dmech, ctx->dskew, ctx->dsbase,
-                    ctx->djac16, ctx->dstrips16, ctx->dbnd16, ctx->dnasa9, ctx->dhist, ctx->ddefer};
+                    ctx->djac16, ctx->dstrips16, ctx->dbnd16, ctx->dnasa9, ctx->dhist, ctx->ddefer,
+                    ctx->djcp16, ctx->djsc16};
     for (void* p : ptrs) if (p) hipFree(p);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -781,6 +782,7 @@ int shcfd_set_abpm(shcfd_ctx* ctx) {
     LAUNCHCHK();
     ctx->skew_active = 0;
     ctx->q16_active = 0;
+    ctx->jd16_active = 0;
     return SHCFD_OK;
 }
 
@@ -812,14 +814,63 @@ static bool skew_enabled(shcfd_ctx* ctx) {
     return true;
 }
 
-// the quad-row copy and hand-off state of the 16-row sweeps (nY > 1, every
-// implicit scheme but dual time, whose S scaling stays on the row-per-lane
-// sweeps); allocated on first use
+// the 16-row sweeps apply the split Jacobians in closed form (no operator
+// copy) for the schemes whose A+-, B+- are the unpreconditioned ones
+static bool jvp16_scheme(const shcfd_ctx* ctx) {
+    return ctx->use_jvp16 && (ctx->p.scheme == SHCFD_SCHEME_LUSGS || ctx->p.scheme == SHCFD_SCHEME_NR);
+}
+
+static bool q16_layout(shcfd_ctx* ctx);
+
+// the cell data of the Jacobian-free 16-row sweeps; allocated (and cleared,
+// so slots of no cell hold zeros) on first use
+static bool jd16_ready(shcfd_ctx* ctx) {
+    if (ctx->djcp16) return true;
+    const int C = (ctx->dimq + 3) / 4;
+    const size_t ncp = (size_t)ctx->nsl16 * C * 64, nsc = (size_t)ctx->nsl16 * 16 * 16;
+    if (hipMalloc((void**)&ctx->djcp16, sizeof(double2) * ncp) != hipSuccess ||
+        hipMalloc((void**)&ctx->djsc16, sizeof(double) * nsc) != hipSuccess ||
+        hipMemset(ctx->djcp16, 0, sizeof(double2) * ncp) != hipSuccess ||
+        hipMemset(ctx->djsc16, 0, sizeof(double) * nsc) != hipSuccess) {
+        hipFree(ctx->djcp16);
+        hipFree(ctx->djsc16);
+        ctx->djcp16 = nullptr;
+        ctx->djsc16 = nullptr;
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+
+// the 16-row sweeps (nY > 1, every implicit scheme but dual time, whose S
+// scaling stays on the row-per-lane sweeps): their strips and hand-off rows,
+// and the cell data (LU-SGS, NR) or the quad-row operator copy (the
+// preconditioned schemes, or with the closed form switched off); allocated
+// on first use
 static bool q16_enabled(shcfd_ctx* ctx) {
     if (!ctx->use_skew || ctx->ny == 1 || ctx->p.scheme == SHCFD_SCHEME_DUAL || ctx->p.dimension != SHCFD_DIM_PLANAR)
         return false;
+    if (!q16_layout(ctx)) return false;
+    if (jvp16_scheme(ctx)) {
+        if (jd16_ready(ctx)) return true;
+        ctx->use_jvp16 = 0;  // no room: the operator copy below
+    }
     if (ctx->djac16) return true;
     const int dq = ctx->dimq, C = (dq + 3) / 4;
+    if (hipMalloc((void**)&ctx->djac16, sizeof(double) * 2 * 2 * 64 * C * (dq + 2) * (size_t)ctx->nsl16) != hipSuccess ||
+        hipMemset(ctx->djac16, 0, sizeof(double) * 2 * 2 * 64 * C * (dq + 2) * (size_t)ctx->nsl16) != hipSuccess) {
+        hipFree(ctx->djac16);
+        ctx->djac16 = nullptr;
+        ctx->use_skew = 0;
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+
+static bool q16_layout(shcfd_ctx* ctx) {
+    if (ctx->dstrips16) return true;
+    const int dq = ctx->dimq;
     std::vector<Strip16> st;
     long long nsl = 0;
     int maxni = 0;
@@ -848,14 +899,12 @@ static bool q16_enabled(shcfd_ctx* ctx) {
         if (hipMalloc(p, bytes) != hipSuccess) return false;
         return hipMemset(*p, 0, bytes) == hipSuccess;
     };
-    bool ok = al((void**)&ctx->djac16, sizeof(double) * 2 * 2 * 64 * C * (dq + 2) * (size_t)nsl) &&
-              al((void**)&ctx->dstrips16, sizeof(Strip16) * ns) &&
+    bool ok = al((void**)&ctx->dstrips16, sizeof(Strip16) * ns) &&
               al((void**)&ctx->dbnd16, sizeof(double) * (size_t)(2 * bstride * (ns + 1))) &&
               hipMemcpy(ctx->dstrips16, st.data(), sizeof(Strip16) * ns, hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
-        void* ptrs[] = {ctx->djac16, ctx->dstrips16, ctx->dbnd16};
+        void* ptrs[] = {ctx->dstrips16, ctx->dbnd16};
         for (void* p : ptrs) if (p) hipFree(p);
-        ctx->djac16 = nullptr;
         ctx->dstrips16 = nullptr;
         ctx->dbnd16 = nullptr;
         ctx->use_skew = 0;
@@ -879,6 +928,13 @@ static int abpm_for_step(shcfd_ctx* ctx) {
             int maxni = 0;
             for (const auto& B : ctx->hblk) maxni = std::max(maxni, B.ni);
             dim3 grid((unsigned)ctx->nstrips16, (unsigned)((maxni + 15 + 15) / 16));
+            if (jvp16_scheme(ctx)) {
+                DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_set_jd16<NY>, grid, dim3(256), 0, ctx->stream, ctx->dev(),
+                                                        ctx->dstrips16, ctx->djcp16, ctx->djsc16));
+                LAUNCHCHK();
+                ctx->jd16_active = 1;
+                return SHCFD_OK;
+            }
             DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_set_abpm16<NY>, grid, dim3(256), 0, ctx->stream, ctx->dev(),
                                                     ctx->dstrips16, ctx->djac16, ctx->nsl16));
             LAUNCHCHK();
@@ -955,8 +1011,10 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
     DevCtx d = ctx->dev();
     int* prog_f = ctx->dprog;
     int* prog_b = ctx->dprog + ns;
-    const bool q16 = ctx->q16_active && ctx->djac16 && !fast_enabled(ctx);
+    const bool jd16 = ctx->jd16_active && ctx->djcp16 && !fast_enabled(ctx);
+    const bool q16 = jd16 || (ctx->q16_active && ctx->djac16 && !fast_enabled(ctx));
     ctx->q16_active = 0;
+    ctx->jd16_active = 0;
     const int C16 = (ctx->dimq + 3) / 4, ns16 = ctx->nstrips16;
     { ProfScope ps(ctx, PROF_RHS);
     DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_rhs<NY>, ctx->dprog, 2 * ns, ctx->dqdt_inline, ctx->abpm_inline)); }
@@ -974,13 +1032,23 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
         { ProfScope ps(ctx, PROF_SWEEP_FWD);
         hipLaunchKernelGGL(k_fill16, dim3(256), dim3(256), 0, ctx->stream, d, ctx->dstrips16, ns16, ctx->dbnd16,
                            ctx->bstride16, ctx->dimq);
-        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep16_fwd<NY>, dim3(ns16), blk, 0, ctx->stream, d, ctx->dstrips16,
-                                                ns16, hand_f, ctx->bstride16, ctx->spin_limit, ctx->djac16,
-                                                ctx->nsl16)) }
+        if (jd16)
+            DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep16j_fwd<NY>, dim3(ns16), blk, 0, ctx->stream, d,
+                                                    ctx->dstrips16, ns16, hand_f, ctx->bstride16, ctx->spin_limit,
+                                                    ctx->djcp16, ctx->djsc16))
+        else
+            DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep16_fwd<NY>, dim3(ns16), blk, 0, ctx->stream, d,
+                                                    ctx->dstrips16, ns16, hand_f, ctx->bstride16, ctx->spin_limit,
+                                                    ctx->djac16, ctx->nsl16)) }
         { ProfScope ps(ctx, PROF_SWEEP_BWD);
-        DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep16_bwd<NY>, dim3(ns16), blk, 0, ctx->stream, d, ctx->dstrips16,
-                                                ns16, hand_b, ctx->bstride16, ctx->dpart, ctx->spin_limit,
-                                                ctx->djac16, ctx->nsl16)) }
+        if (jd16)
+            DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep16j_bwd<NY>, dim3(ns16), blk, 0, ctx->stream, d,
+                                                    ctx->dstrips16, ns16, hand_b, ctx->bstride16, ctx->dpart,
+                                                    ctx->spin_limit, ctx->djcp16, ctx->djsc16))
+        else
+            DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep16_bwd<NY>, dim3(ns16), blk, 0, ctx->stream, d,
+                                                    ctx->dstrips16, ns16, hand_b, ctx->bstride16, ctx->dpart,
+                                                    ctx->spin_limit, ctx->djac16, ctx->nsl16)) }
         if (inner_loop_scheme(ctx->p.scheme)) {
             { ProfScope ps(ctx, PROF_REDUCE);
             hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, ctx->stream, d, ctx->dpart, ns16); }
@@ -1046,6 +1114,10 @@ int shcfd_set_option(shcfd_ctx* ctx, int option, int value) {
         return SHCFD_OK;
     case SHCFD_OPT_SKEW_SWEEP:
         ctx->use_skew = value ? 1 : 0;
+        drop_graph(ctx);
+        return SHCFD_OK;
+    case SHCFD_OPT_JVP_SWEEP:
+        ctx->use_jvp16 = value ? 1 : 0;
         drop_graph(ctx);
         return SHCFD_OK;
     case SHCFD_OPT_TILE_ROWS: {

@@ -190,6 +190,10 @@ struct Ctx {
     double* ddefer = nullptr;      // CHEMEQ2 cells handed to DVODE: (cell, t, n(ns), T) per record
     int defer_cap = 0;             // records: one per interior cell
     int q16_active = 0;            // the last set_ABpm of a step wrote the quad-row copy
+    int use_jvp16 = 1;             // LU-SGS / NR at nY > 1: Jacobian-free 16-row sweeps
+    int jd16_active = 0;           // the last set_ABpm of a step wrote their cell data
+    double2* djcp16 = nullptr;     // their per-lane (c1, DHi) pairs: nsl16 x C x 64
+    double* djsc16 = nullptr;      // their cell scalars: nsl16 x 16 rows x 16
     int skew_active = 0;           // the last set_ABpm of a step wrote the skewed copy
     double* dskew = nullptr;
     long long skew_nsl = 0;        // slots: sum over strips of 64 (ni + 63)

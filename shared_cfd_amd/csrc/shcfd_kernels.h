@@ -501,6 +501,13 @@ __global__ void k_reduce_res_omega(DevCtx, const double*, int);
 // quad-row sweeps for nY > 1 (k_sweep16.hip; operators from k_set_abpm16)
 template <int NY> __global__ void k_set_abpm16(DevCtx, const Strip16*, double*, long long);
 __global__ void k_fill16(DevCtx, const Strip16*, int, double*, long long, int);
+template <int NY> __global__ void k_set_jd16(DevCtx, const Strip16*, double2*, double*);
+template <int NY>
+__global__ void k_sweep16j_fwd(DevCtx, const Strip16*, int, double*, long long, long long, const double2*,
+                               const double*);
+template <int NY>
+__global__ void k_sweep16j_bwd(DevCtx, const Strip16*, int, double*, long long, double*, long long, const double2*,
+                               const double*);
 template <int NY>
 __global__ void k_sweep16_fwd(DevCtx, const Strip16*, int, double*, long long, long long, const double*, long long);
 template <int NY>

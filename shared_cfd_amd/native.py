@@ -38,7 +38,7 @@ EXPORTS = [
     "shcfd_nasa9_parse", "shcfd_set_nasa9", "shcfd_nasa9_stream", "shcfd_load_nasa9",
 ]
 ADJ_LAMBDA, ADJ_QBAR, ADJ_XBAR, ADJ_XBAR_SUM, ADJ_LAMBDA_LATER = 0, 1, 2, 3, 4
-OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT, OPT_SKEW_SWEEP, OPT_TILE_ROWS = 1, 2, 3, 4
+OPT_FAST_IMPLICIT, OPT_SPIN_LIMIT, OPT_SKEW_SWEEP, OPT_TILE_ROWS, OPT_JVP_SWEEP = 1, 2, 3, 4, 5
 
 # kernel classes of shcfd_profile_read (include/shcfd.h)
 KCLASS = ["PRIMITIVE", "BC", "HV", "TG", "TGV", "DT", "ABPM", "DQDT", "RHS", "SWEEP_FWD", "SWEEP_BWD",

@@ -173,23 +173,28 @@ def test_reacting_routines_and_steps(mechs, which, reaction):
 
 
 def test_skewed_jacobian_sweeps_match_field_sweeps(mechs):
-    """Whole steps at nY = 9 read the split Jacobians from the strip-skewed
-    copy; the per-cell arithmetic is unchanged, so the result is identical
-    to the sweeps over the Ap/Am/Bp/Bm fields."""
-    from shared_cfd_amd.native import OPT_SKEW_SWEEP
+    """Whole steps at nY = 9 over the quad-row copy of the split Jacobians:
+    the per-cell arithmetic is unchanged, so the result is identical to the
+    sweeps over the Ap/Am/Bp/Bm fields; the default closed-form products
+    agree to 1e-12."""
+    from shared_cfd_amd.native import OPT_JVP_SWEEP, OPT_SKEW_SWEEP
 
     lm, _ = mechs["syn"]
     case = C.mixing_layer(lm, ni=70, nj=130, seed=8, species=chem.SPECIES9, reaction=2)
     out = []
-    for skew in (1, 0):
+    for skew, jvp in ((1, 0), (0, 0), (1, 1)):
         s = Solver(case)
         s.set_option(OPT_SKEW_SWEEP, skew)
+        s.set_option(OPT_JVP_SWEEP, jvp)
         s.prepare()
         for k in range(3):
             s.step(k + 1, 0)
         out.append(s.download("Q", 0))
         s.close()
     assert np.array_equal(out[0], out[1])
+    # the default closed-form sweeps: the same algebra, other rounding
+    e = float(np.max(np.abs(out[2] - out[1])) / np.max(np.abs(out[1])))
+    assert 0.0 < e <= 1e-12, e
 
 
 def test_axisymmetric_reacting_steps(mechs):
