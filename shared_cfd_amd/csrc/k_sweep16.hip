@@ -344,6 +344,7 @@ constexpr int JPD = 11;   // prefetch distance of the cell data, in steps (JPD +
 // the scalar slots (3) and the LDS exchange buffers (2)
 static_assert((JPD + 1) % PH == 0 && (JPD + 1) % 6 == 0, "round length");
 constexpr int JSC = 16;   // cell scalars per (step, row): u v g H nxi nyi nxj nyj nuA/2 nuB/2 alpha dsci dscj
+constexpr int JST = 17;   // their row stride in LDS (odd: the 16 rows of a read fall in distinct banks)
 enum { JS_U = 0, JS_V, JS_G, JS_H, JS_NXI, JS_NYI, JS_NXJ, JS_NYJ, JS_HNA, JS_HNB, JS_AL, JS_DSCI, JS_DSCJ };
 
 template <int CTRL>
@@ -359,6 +360,36 @@ __device__ inline double quad_sum(double x) {
     return y + quad_perm<2 | (3 << 2) | (0 << 4) | (1 << 6)>(y);             // quad_perm [2,3,0,1]
 }
 
+// Buffer views: base in scalar registers, a per-lane byte offset fixed for
+// the sweep and a per-step byte offset in a scalar register, so an access
+// costs one instruction; a lane that must not store gives an offset beyond
+// the view (the hardware's range check drops it).
+typedef unsigned int jv_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int jv_u32x2 __attribute__((ext_vector_type(2)));
+constexpr int JV_DROP = 0x7FFFFF00;
+constexpr int JV_AGENT = 16;   // sc1: device-coherent access
+__device__ inline __amdgpu_buffer_rsrc_t jview(const void* base, long long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ inline double jv_d(unsigned lo, unsigned hi) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline double2 jv_ld2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    const jv_u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    return make_double2(jv_d(v.x, v.y), jv_d(v.z, v.w));
+}
+template <int AUX = 0>
+__device__ inline double jv_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    const jv_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX);
+    return jv_d(v.x, v.y);
+}
+template <int AUX = 0>
+__device__ inline void jv_st(__amdgpu_buffer_rsrc_t r, double x, int voff, int soff) {
+    const unsigned long long a = (unsigned long long)__double_as_longlong(x);
+    const jv_u32x2 v = {(unsigned)a, (unsigned)(a >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, AUX);
+}
+
 // per-lane inputs of one step of the Jacobian-free sweeps
 struct JStepIn {
     double2 cp;    // (c1, DHi) of the lane's component
@@ -367,31 +398,30 @@ struct JStepIn {
 };
 
 // the lane's outgoing terms dsci A+-(x) and dscj B+-(x) (sgn = +1 forward,
-// -1 backward) from its row's x (xa, in LDS order), the quad partial sums of
-// DHi x (pq) and the cell scalars (sc)
-template <int NY, int C>
-__device__ inline void jvp_out(const double* xa, const double* pq, const double* sc, int k, double c1, double sgn,
-                               double& a_out, double& b_out) {
-    double s = 0.0;
-#pragma unroll
-    for (int l = 0; l < NY; l++) s = s + xa[l];
-    double qd = pq[0];
-#pragma unroll
-    for (int w = 1; w < C; w++) qd = qd + pq[w];
-    const double X = xa[NY], V = xa[NY + 1], E = xa[NY + 2];
+// -1 backward) from its row's sums s = sum_k x_k and qd = sum_k DHi_k x_k,
+// the momentum and energy components X, V, E, its own component xk and the
+// cell scalars sc (LDS, row stride JST)
+template <int NY, bool FWD>
+__device__ inline void jvp_out(double s, double qd, double X, double V, double E, double xk, const double* sc, int k,
+                               double c1, double& a_out, double& b_out) {
+#pragma clang fp contract(fast)
     const double u = sc[JS_U], v = sc[JS_V], g = sc[JS_G];
     const double nxi = sc[JS_NXI], nyi = sc[JS_NYI], nxj = sc[JS_NXJ], nyj = sc[JS_NYJ];
     const double Ui = (nxi * u) + (nyi * v), Uj = (nxj * u) + (nyj * v);
     const double ei = ((nxi * X) + (nyi * V)) - (Ui * s);
     const double ej = ((nxj * X) + (nyj * V)) - (Uj * s);
-    const double dp = qd + g * (((((0.5 * ((u * u) + (v * v))) * s) - (u * X)) - (v * V)) + E);
-    const double xk = xa[k];
+    const double dp = qd + g * ((((0.5 * ((u * u) + (v * v))) * s) - ((u * X) + (v * V))) + E);
     const double cai = k < NY ? 0.0 : (k == NY ? nxi : (k == NY + 1 ? nyi : Ui));
     const double caj = k < NY ? 0.0 : (k == NY ? nxj : (k == NY + 1 ? nyj : Uj));
     const double oA = ((Ui * xk) + (c1 * ei)) + (cai * dp);
     const double oB = ((Uj * xk) + (c1 * ej)) + (caj * dp);
-    a_out = sc[JS_DSCI] * ((0.5 * oA) + sgn * (sc[JS_HNA] * xk));
-    b_out = sc[JS_DSCJ] * ((0.5 * oB) + sgn * (sc[JS_HNB] * xk));
+    if (FWD) {
+        a_out = sc[JS_DSCI] * ((0.5 * oA) + (sc[JS_HNA] * xk));
+        b_out = sc[JS_DSCJ] * ((0.5 * oB) + (sc[JS_HNB] * xk));
+    } else {
+        a_out = sc[JS_DSCI] * ((0.5 * oA) - (sc[JS_HNA] * xk));
+        b_out = sc[JS_DSCJ] * ((0.5 * oB) - (sc[JS_HNB] * xk));
+    }
 }
 }  // namespace
 
@@ -461,15 +491,23 @@ __global__ void __launch_bounds__(256) k_set_jd16(DevCtx d, const Strip16* strip
     o[JS_DSCJ] = dscj;
 }
 
+// One step of a Jacobian-free sweep, per lane: x of its component from the
+// incoming terms and alpha (read one step ahead); then x, the quad partial
+// sums of x_k and DHi_k x_k over the species lanes, and the next step's cell
+// scalars go to LDS; after the barrier the lane reads X, V, E, the partial
+// sums of its row and the scalars, and forms its outgoing terms.  LDS rows
+// have odd strides, so the 16 rows of a wave's read fall in distinct banks.
+
 // forward sweep, Jacobian-free: x = alpha (R + A+(i-1) x(i-1) dsci + B+(j-1) x(j-1) dscj)
 template <int NY>
 __global__ void __launch_bounds__(NY + 3 <= 16 ? 256 : 320) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_sweep16j_fwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long long hstride, long long spin_limit,
                const double2* __restrict__ jcp, const double* __restrict__ jsc) {
-    constexpr int dq = NY + 3, C = (dq + 3) / 4, DQP = 4 * C;
-    __shared__ double xs[2][16][DQP];
-    __shared__ double pq[2][16][C];
-    __shared__ __align__(16) double scs[3][16][JSC];
+    constexpr int dq = NY + 3, C = (dq + 3) / 4, XST = (4 * C) | 1, PST = 2 * C + 1;
+    constexpr int CP_STEP = C * 64 * 16, SC_STEP = 16 * JSC * 8;
+    __shared__ double xs[2][16][XST];
+    __shared__ double pq[2][16][PST];
+    __shared__ double scs[3][16][JST];
     const int s = blockIdx.x;
     const int w = (int)(threadIdx.x >> 6), L = (int)(threadIdx.x & 63), r = L >> 2, k = 4 * w + (L & 3);
     const Strip16 S = strips[s];
@@ -480,35 +518,42 @@ k_sweep16j_fwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long 
     const bool mine = r == 0 && kok;
     const int nsteps = ni + 15;
     const int sp_slot = 4 * w + (L & 3);   // the scalar pair this lane carries (< 8 only)
+    const bool carrier = sp_slot < 8;
     const int kk = kok ? k : 0;
-    const double* hsrc = hand + (long long)(S.prev >= 0 ? S.prev : nstrips) * hstride + (long long)HO * dq + kk;
-    double* hdst = hand + (long long)s * hstride + (long long)HO * dq + kk;
-    double* dqf = d.f[SHCFD_F_DQ] + (long long)kk * d.NC;
-    auto scal_pair = [&](int t) {
-        t = t < nsteps ? t : nsteps - 1;
-        return sp_slot < 8 ? *reinterpret_cast<const double2*>(jsc + ((S.base + t) * 16 + r) * JSC + 2 * sp_slot)
-                           : make_double2(0.0, 0.0);
-    };
+    const long long prow = S.prev >= 0 ? S.prev : nstrips;
+    const double* hpoll = hand + prow * hstride + (long long)HO * dq + kk;
+    const __amdgpu_buffer_rsrc_t vin = jview(hand + prow * hstride, hstride * 8);
+    const __amdgpu_buffer_rsrc_t vout = jview(hand + (long long)s * hstride, hstride * 8);
+    const int h_lane = (HO * dq + kk) * 8;                                     // column c at + c dq 8
+    const int h_store = (has_next && r == 15 && kok) ? h_lane - 14 * dq * 8 : JV_DROP;  // column t - 14
+    const __amdgpu_buffer_rsrc_t vcp = jview(jcp + S.base * C * 64, (long long)nsteps * CP_STEP);
+    const int cp_lane = (w * 64 + L) * 16;
+    const __amdgpu_buffer_rsrc_t vsc = jview(jsc + S.base * 16 * JSC, (long long)nsteps * SC_STEP);
+    const int sc_lane = (r * JSC + 2 * (carrier ? sp_slot : 0)) * 8;
+    const __amdgpu_buffer_rsrc_t vdq = jview(d.f[SHCFD_F_DQ] + (long long)kk * d.NC, (long long)d.NC * 8);
+    const int dq_lane = (int)((cidx(B, 0, rowok ? j : nj) + 1 - r) * 8);  // the cell of step t at + 8 t
     auto load = [&](int t, JStepIn& in) {
         const int tc = t < nsteps ? t : nsteps - 1;  // past the end: re-read the last step
-        in.cp = jcp[((S.base + tc) * C + w) * 64 + L];
-        in.sp = scal_pair(t + 1);
-        const int i = tc - r + 1;
-        const int ic = i < 1 ? 1 : (i > ni ? ni : i);
-        in.r = dqf[cidx(B, ic, rowok ? j : nj)];
+        const int tn = t + 1 < nsteps ? t + 1 : nsteps - 1;
+        in.cp = jv_ld2(vcp, cp_lane, tc * CP_STEP);
+        in.sp = jv_ld2(vsc, sc_lane, tn * SC_STEP);
+        in.r = jv_ld(vdq, dq_lane, tc * 8);
     };
-    // the cell scalars of step 0
     {
-        const double2 v0 = scal_pair(0);
-        if (sp_slot < 8) *reinterpret_cast<double2*>(&scs[0][r][2 * sp_slot]) = v0;
+        const double2 v0 = jv_ld2(vsc, sc_lane, 0);
+        if (carrier) {
+            scs[0][r][2 * sp_slot] = v0.x;
+            scs[0][r][2 * sp_slot + 1] = v0.y;
+        }
     }
     JStepIn buf[JPD + 1];
     double hr[PH];
 #pragma unroll
-    for (int p = 0; p < PH; p++) hr[p] = load_agent(hsrc + (long long)(p + 1) * dq);  // columns 1..PH
+    for (int p = 0; p < PH; p++) hr[p] = jv_ld<JV_AGENT>(vin, h_lane, (p + 1) * dq * 8);  // columns 1..PH
 #pragma unroll
     for (int p = 0; p < JPD; p++) load(p, buf[p]);
     lds_barrier();
+    double al = scs[0][r][JS_AL];
     double a_in = 0.0, bo = 0.0;
     bool dead = false;
     const int tend = ((nsteps + JPD) / (JPD + 1)) * (JPD + 1);  // whole rounds; the extra steps are idle
@@ -517,32 +562,45 @@ k_sweep16j_fwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long 
         for (int u = 0; u <= JPD; u++) {
             const int t = t0 + u;
             double hv = hr[u % PH];
-            hr[u % PH] = load_agent(hsrc + (long long)(t + 1 + PH) * dq);
+            hr[u % PH] = jv_ld<JV_AGENT>(vin, h_lane, (t + 1 + PH) * dq * 8);
             load(t + JPD, buf[(u + JPD) % (JPD + 1)]);
             const JStepIn& in = buf[u];
-            const double* sct = scs[u % 3][r];
             const int i = t - r + 1;
             const bool act = rowok && i >= 1 && i <= ni;
             double b_in = shift_up4(bo);
             if (__builtin_expect(__ballot(mine && is_sent16(hv)) != 0ULL, 0))
-                hv = poll16(hsrc + (long long)(t + 1) * dq, hv, mine, spin_limit, d.iflag, dead);
+                hv = poll16(hpoll + (long long)(t + 1) * dq, hv, mine, spin_limit, d.iflag, dead);
             if (r == 0) b_in = kok ? hv : 0.0;
-            double x = 0.0;
-            if (act && kok) {
-                x = sct[JS_AL] * (in.r + (a_in + b_in));
-                dqf[cidx(B, i, j)] = x;
-            }
+            const double xc = al * (in.r + (a_in + b_in));
+            const double x = (act && kok) ? xc : 0.0;
+            jv_st(vdq, x, (act && kok) ? dq_lane : JV_DROP, t * 8);
             xs[u & 1][r][k] = x;
+            const double ps = quad_sum(k < NY ? x : 0.0);
             const double pd = quad_sum(in.cp.y * x);
-            if ((L & 3) == 0) pq[u & 1][r][w] = pd;
-            if (sp_slot < 8) *reinterpret_cast<double2*>(&scs[(u + 1) % 3][r][2 * sp_slot]) = in.sp;
-            lds_barrier();
-            if (act) {
-                jvp_out<NY, C>(&xs[u & 1][r][0], &pq[u & 1][r][0], sct, kok ? k : 0, in.cp.x, 1.0, a_in, bo);
-            } else {
-                bo = 0.0;
+            if ((L & 3) == 0) {
+                pq[u & 1][r][2 * w] = ps;
+                pq[u & 1][r][2 * w + 1] = pd;
             }
-            if (has_next && r == 15 && act && kok) store_agent(hdst + (long long)i * dq, bo);
+            if (carrier) {
+                scs[(u + 1) % 3][r][2 * sp_slot] = in.sp.x;
+                scs[(u + 1) % 3][r][2 * sp_slot + 1] = in.sp.y;
+            }
+            lds_barrier();
+            const double* sct = scs[u % 3][r];
+            const double* xr = xs[u & 1][r];
+            const double* pr = pq[u & 1][r];
+            double sx = pr[0], qd = pr[1];
+#pragma unroll
+            for (int q = 1; q < C; q++) {
+                sx = sx + pr[2 * q];
+                qd = qd + pr[2 * q + 1];
+            }
+            double ao, bn;
+            jvp_out<NY, true>(sx, qd, xr[NY], xr[NY + 1], xr[NY + 2], x, sct, kk, in.cp.x, ao, bn);
+            al = scs[(u + 1) % 3][r][JS_AL];
+            a_in = act ? ao : a_in;
+            bo = act ? bn : 0.0;
+            jv_st<JV_AGENT>(vout, bo, act ? h_store : JV_DROP, t * dq * 8);
         }
     }
 }
@@ -553,10 +611,11 @@ template <int NY>
 __global__ void __launch_bounds__(NY + 3 <= 16 ? 256 : 320) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_sweep16j_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long long hstride, double* part,
                long long spin_limit, const double2* __restrict__ jcp, const double* __restrict__ jsc) {
-    constexpr int dq = NY + 3, C = (dq + 3) / 4, DQP = 4 * C;
-    __shared__ double xs[2][16][DQP];
-    __shared__ double pq[2][16][C];
-    __shared__ __align__(16) double scs[3][16][JSC];
+    constexpr int dq = NY + 3, C = (dq + 3) / 4, XST = (4 * C) | 1, PST = 2 * C + 1;
+    constexpr int CP_STEP = C * 64 * 16, SC_STEP = 16 * JSC * 8;
+    __shared__ double xs[2][16][XST];
+    __shared__ double pq[2][16][PST];
+    __shared__ double scs[3][16][JST];
     __shared__ double red[2][4];
     const int s = nstrips - 1 - (int)blockIdx.x;
     const int w = (int)(threadIdx.x >> 6), L = (int)(threadIdx.x & 63), r = L >> 2, k = 4 * w + (L & 3);
@@ -567,42 +626,49 @@ k_sweep16j_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long 
     const bool has_prev = S.prev >= 0;
     const bool mine = r == 15 && kok;
     const bool inner = inner_loop_scheme(d.p.scheme);
+    const bool upd = !inner && B.active;
     const double dqmax = d.p.dqmax;
     const int nsteps = ni + 15;
     const int sp_slot = 4 * w + (L & 3);
+    const bool carrier = sp_slot < 8;
     const int kk = kok ? k : 0;
-    const double* hsrc = hand + (long long)(S.next >= 0 ? S.next : nstrips) * hstride + (long long)HO * dq + kk;
-    double* hdst = hand + (long long)s * hstride + (long long)HO * dq + kk;
-    double* dqf = d.f[SHCFD_F_DQ] + (long long)kk * d.NC;
-    double* qf = d.f[SHCFD_F_Q] + (long long)kk * d.NC;
+    const long long nrow = S.next >= 0 ? S.next : nstrips;
+    const double* hpoll = hand + nrow * hstride + (long long)HO * dq + kk;
+    const __amdgpu_buffer_rsrc_t vin = jview(hand + nrow * hstride, hstride * 8);
+    const __amdgpu_buffer_rsrc_t vout = jview(hand + (long long)s * hstride, hstride * 8);
+    const int h_lane = (HO * dq + kk) * 8;
+    const int h_store = (has_prev && r == 0 && kok) ? h_lane : JV_DROP;   // column ni + 15 - t
     // backward step t of row r visits the cell of forward step ni + 14 - t
-    auto scal_pair = [&](int t) {
-        t = t < nsteps ? t : nsteps - 1;
-        return sp_slot < 8
-                   ? *reinterpret_cast<const double2*>(jsc + ((S.base + (ni + 14 - t)) * 16 + r) * JSC + 2 * sp_slot)
-                   : make_double2(0.0, 0.0);
-    };
+    const __amdgpu_buffer_rsrc_t vcp = jview(jcp + S.base * C * 64, (long long)nsteps * CP_STEP);
+    const int cp_lane = (w * 64 + L) * 16;
+    const __amdgpu_buffer_rsrc_t vsc = jview(jsc + S.base * 16 * JSC, (long long)nsteps * SC_STEP);
+    const int sc_lane = (r * JSC + 2 * (carrier ? sp_slot : 0)) * 8;
+    const __amdgpu_buffer_rsrc_t vdq = jview(d.f[SHCFD_F_DQ] + (long long)kk * d.NC, (long long)d.NC * 8);
+    const __amdgpu_buffer_rsrc_t vq = jview(d.f[SHCFD_F_Q] + (long long)kk * d.NC, (long long)d.NC * 8);
+    const int c_lane = (int)((cidx(B, 0, rowok ? j : nj) + 1 - r) * 8);   // the cell of step t at + 8 (ni + 14 - t)
     auto load = [&](int t, JStepIn& in) {
         const int tc = t < nsteps ? t : nsteps - 1;
-        in.cp = jcp[((S.base + (ni + 14 - tc)) * C + w) * 64 + L];
-        in.sp = scal_pair(t + 1);
-        const int i = ni - tc + 15 - r;
-        const int ic = i < 1 ? 1 : (i > ni ? ni : i);
-        const long long c = cidx(B, ic, rowok ? j : nj);
-        in.r = dqf[c];
-        in.q = qf[c];
+        const int tn = t + 1 < nsteps ? t + 1 : nsteps - 1;
+        in.cp = jv_ld2(vcp, cp_lane, (ni + 14 - tc) * CP_STEP);
+        in.sp = jv_ld2(vsc, sc_lane, (ni + 14 - tn) * SC_STEP);
+        in.r = jv_ld(vdq, c_lane, (ni + 14 - tc) * 8);
+        in.q = jv_ld(vq, c_lane, (ni + 14 - tc) * 8);
     };
     {
-        const double2 v0 = scal_pair(0);
-        if (sp_slot < 8) *reinterpret_cast<double2*>(&scs[0][r][2 * sp_slot]) = v0;
+        const double2 v0 = jv_ld2(vsc, sc_lane, (ni + 14) * SC_STEP);
+        if (carrier) {
+            scs[0][r][2 * sp_slot] = v0.x;
+            scs[0][r][2 * sp_slot + 1] = v0.y;
+        }
     }
     JStepIn buf[JPD + 1];
     double hr[PH];
 #pragma unroll
-    for (int p = 0; p < PH; p++) hr[p] = load_agent(hsrc + (long long)(ni - p) * dq);  // columns ni..ni-PH+1
+    for (int p = 0; p < PH; p++) hr[p] = jv_ld<JV_AGENT>(vin, h_lane, (ni - p) * dq * 8);  // columns ni..ni-PH+1
 #pragma unroll
     for (int p = 0; p < JPD; p++) load(p, buf[p]);
     lds_barrier();
+    double al = scs[0][r][JS_AL];
     double a_in = 0.0, bo = 0.0, res = 0.0, omega = 1e300;
     bool dead = false;
     const int tend = ((nsteps + JPD) / (JPD + 1)) * (JPD + 1);
@@ -611,39 +677,52 @@ k_sweep16j_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long 
         for (int u = 0; u <= JPD; u++) {
             const int t = t0 + u;
             double hv = hr[u % PH];
-            hr[u % PH] = load_agent(hsrc + (long long)(ni - t - PH) * dq);
+            hr[u % PH] = jv_ld<JV_AGENT>(vin, h_lane, (ni - t - PH) * dq * 8);
             load(t + JPD, buf[(u + JPD) % (JPD + 1)]);
             const JStepIn& in = buf[u];
-            const double* sct = scs[u % 3][r];
             const int i = ni - t + 15 - r;
             const bool act = rowok && i >= 1 && i <= ni;
             double b_in = shift_down4(bo);
             if (__builtin_expect(__ballot(mine && is_sent16(hv)) != 0ULL, 0))
-                hv = poll16(hsrc + (long long)(ni - t) * dq, hv, mine, spin_limit, d.iflag, dead);
+                hv = poll16(hpoll + (long long)(ni - t) * dq, hv, mine, spin_limit, d.iflag, dead);
             if (r == 15) b_in = kok ? hv : 0.0;
-            double x = 0.0;
-            if (act && kok) {
-                const long long c = cidx(B, i, j);
-                x = in.r - sct[JS_AL] * (a_in + b_in);
-                if (inner) {
-                    if (k == NY + 2) res = res + x * x;
-                    if (k < NY && x < 0.0) omega = dmin(omega, (-in.q / x) * dqmax);
-                } else if (B.active) {
-                    qf[c] = in.q + x;
-                }
-                dqf[c] = x;
+            const double xc = in.r - al * (a_in + b_in);
+            const bool live = act && kok;
+            const double x = live ? xc : 0.0;
+            if (inner && live) {
+                if (k == NY + 2) res = res + x * x;
+                if (k < NY && x < 0.0) omega = dmin(omega, (-in.q / x) * dqmax);
             }
+            const int so = (ni + 14 - t) * 8;
+            jv_st(vq, in.q + x, (live && upd) ? c_lane : JV_DROP, so);
+            jv_st(vdq, x, live ? c_lane : JV_DROP, so);
             xs[u & 1][r][k] = x;
+            const double ps = quad_sum(k < NY ? x : 0.0);
             const double pd = quad_sum(in.cp.y * x);
-            if ((L & 3) == 0) pq[u & 1][r][w] = pd;
-            if (sp_slot < 8) *reinterpret_cast<double2*>(&scs[(u + 1) % 3][r][2 * sp_slot]) = in.sp;
-            lds_barrier();
-            if (act) {
-                jvp_out<NY, C>(&xs[u & 1][r][0], &pq[u & 1][r][0], sct, kok ? k : 0, in.cp.x, -1.0, a_in, bo);
-            } else {
-                bo = 0.0;
+            if ((L & 3) == 0) {
+                pq[u & 1][r][2 * w] = ps;
+                pq[u & 1][r][2 * w + 1] = pd;
             }
-            if (has_prev && r == 0 && act && kok) store_agent(hdst + (long long)i * dq, bo);
+            if (carrier) {
+                scs[(u + 1) % 3][r][2 * sp_slot] = in.sp.x;
+                scs[(u + 1) % 3][r][2 * sp_slot + 1] = in.sp.y;
+            }
+            lds_barrier();
+            const double* sct = scs[u % 3][r];
+            const double* xr = xs[u & 1][r];
+            const double* pr = pq[u & 1][r];
+            double sx = pr[0], qd = pr[1];
+#pragma unroll
+            for (int q = 1; q < C; q++) {
+                sx = sx + pr[2 * q];
+                qd = qd + pr[2 * q + 1];
+            }
+            double ao, bn;
+            jvp_out<NY, false>(sx, qd, xr[NY], xr[NY + 1], xr[NY + 2], x, sct, kk, in.cp.x, ao, bn);
+            al = scs[(u + 1) % 3][r][JS_AL];
+            a_in = act ? ao : a_in;
+            bo = act ? bn : 0.0;
+            jv_st<JV_AGENT>(vout, bo, act ? h_store : JV_DROP, (ni + 15 - t) * dq * 8);
         }
     }
     if (inner) {
