@@ -298,6 +298,15 @@ int shcfd_residual_vjp(shcfd_ctx *ctx);
  * lambda, so XBAR is dJ/dX and the pool bars dJ/d(fixed states). */
 int shcfd_adjoint_solve(shcfd_ctx *ctx, int max_iter, double rtol, int check_every, double *res0, double *res,
                         int *iters);
+/* The same adjoint system solved by restarted GMRES(restart),
+ * right-preconditioned by the transposed LU-SGS sweeps, until the RMS of
+ * dR/dq^T lambda + dJ/dq falls below rtol times its value at the starting
+ * lambda (the current one) or max_iter Krylov steps were taken.  For the
+ * second-order (MUSCL) residual, whose exact linearisation the fixed-point
+ * iteration above need not converge on.  Allocates (restart + 5) adjoint
+ * fields for the call. */
+int shcfd_adjoint_solve_gmres(shcfd_ctx *ctx, int restart, int max_iter, double rtol, double *res0, double *res,
+                              int *iters);
 /* Unsteady adjoint of the dual-time (BDF2) steps of the NR or dual scheme
  * with a fixed time step: U^n = R(q^n) - Vol (a0 q^n + a1 q^(n-1) + a2 q^(n-2))/dt
  * = 0, with (a0, a1, a2) = (3/2, -2, 1/2) and (1, -1, 0) on the first step.

@@ -101,14 +101,25 @@ class AirfoilFlow:
         return float(F @ a_l), float(F @ a_d)
 
     # ---- adjoint ---------------------------------------------------------------
-    def gradient(self, w_l, w_d, rtol=1e-10, max_iter=200000, check_every=100) -> AdjointResult:
-        """d(w_l C_L + w_d C_D)/d(p, alpha) at the current (converged) state."""
+    def reset_adjoint(self):
+        """lambda = 0 (the next solve starts from zero instead of the last
+        solution)."""
+        self.solver.adjoint_upload_lambda(0, np.zeros((self.ac.nj + 4, self.ac.ni + 4, 4)))
+
+    def gradient(self, w_l, w_d, rtol=1e-10, max_iter=200000, check_every=100, gmres=0) -> AdjointResult:
+        """d(w_l C_L + w_d C_D)/d(p, alpha) at the current (converged) state.
+
+        gmres = m > 0 solves the adjoint system by GMRES(m) (max_iter Krylov
+        steps) instead of the fixed-point transposed LU-SGS iteration."""
         s = self.solver
         a_l, a_d = self.directions()
         a = w_l * a_l + w_d * a_d
         s.adjoint_objective(0, self.i0, self.i1, float(a[0]), float(a[1]))
         t0 = time.perf_counter()
-        res0, res, its = s.adjoint_solve(max_iter, rtol, check_every)
+        if gmres:
+            res0, res, its = s.adjoint_solve_gmres(gmres, max_iter, rtol)
+        else:
+            res0, res, its = s.adjoint_solve(max_iter, rtol, check_every)
         s.synchronize()
         secs = time.perf_counter() - t0
         ni, nj = self.ac.ni, self.ac.nj
@@ -147,7 +158,8 @@ class AirfoilCoefficients(torch.autograd.Function):
         g = grad.detach().cpu().numpy()
         flow = ctx.flow
         try:
-            r = flow.gradient(float(g[0]), float(g[1]), rtol=ctx.opts.get("rtol", 1e-10))
+            r = flow.gradient(float(g[0]), float(g[1]), rtol=ctx.opts.get("rtol", 1e-10),
+                              gmres=ctx.opts.get("gmres", 0), max_iter=ctx.opts.get("adj_max_iter", 200000))
         finally:
             flow.close()
         return torch.from_numpy(r.dJ_dp), torch.tensor(r.dJ_dalpha, dtype=torch.float64), None

@@ -88,6 +88,62 @@ void adjoint_release(Ctx* c) {
 
 using namespace shcfd;
 
+// ---- restarted GMRES over adjoint fields --------------------------------
+// Vectors are cell fields of 4 components (component k of cell c at k NC + c);
+// only interior cells take part, block by block.
+__global__ void __launch_bounds__(256) k_gm_axpby(DevCtx d, int b, double* y, const double* x, double a, double c) {
+    const DevBlock B = d.blk[b];
+    const long long n = (long long)B.ni * B.nj;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const long long cc = cidx(B, (int)(t % B.ni) + 1, (int)(t / B.ni) + 1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const long long e = k * d.NC + cc;
+            y[e] = (c == 0.0 ? 0.0 : c * y[e]) + (a == 0.0 ? 0.0 : a * x[e]);
+        }
+    }
+}
+
+// partial sums of <w, V_i> for i = blockIdx.y (V_i at V + i ld), one per
+// workgroup of the grid's x dimension: part[i gridDim.x + blockIdx.x]
+__global__ void __launch_bounds__(256) k_gm_dots(DevCtx d, int b, const double* w, const double* V, long long ld,
+                                                 double* part) {
+    __shared__ double red[256];
+    const DevBlock B = d.blk[b];
+    const double* v = V + blockIdx.y * ld;
+    const long long n = (long long)B.ni * B.nj;
+    double s = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const long long cc = cidx(B, (int)(t % B.ni) + 1, (int)(t / B.ni) + 1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) s += w[k * d.NC + cc] * v[k * d.NC + cc];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// w += sum_i coef[i] V_i (i < nv)
+__global__ void __launch_bounds__(256) k_gm_combine(DevCtx d, int b, double* w, const double* V, long long ld,
+                                                    const double* coef, int nv) {
+    const DevBlock B = d.blk[b];
+    const long long n = (long long)B.ni * B.nj;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const long long cc = cidx(B, (int)(t % B.ni) + 1, (int)(t / B.ni) + 1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const long long e = k * d.NC + cc;
+            double acc = w[e];
+            for (int i = 0; i < nv; i++) acc += coef[i] * V[i * ld + e];
+            w[e] = acc;
+        }
+    }
+}
+
 namespace {
 
 int supported(shcfd_ctx* ctx) {
@@ -206,6 +262,168 @@ int transposed_iterations(shcfd_ctx* ctx, int max_iter, double rtol, int check_e
     if (!rc) rc = shcfd_check_error(ctx);
     return rc;
 }
+
+// Restarted GMRES for (dR/dq)^T lambda = -dJ/dq, right-preconditioned by
+// the transposed LU-SGS sweeps P = M^-T: the fixed-point iteration above
+// converges only where every eigenvalue of I + P (dR/dq)^T lies inside the
+// unit circle, which the linearised second-order (MUSCL) scheme does not
+// guarantee; GMRES minimises the residual over the Krylov space instead.
+// Classical Gram-Schmidt run twice (one multi-dot kernel per pass).  The
+// residual reported is the RMS of dR/dq^T lambda + dJ/dq over the interior
+// cells, as in the fixed-point solve; on return QBAR, XBAR and the pool bars
+// belong to the final lambda.
+struct Gmres {
+    shcfd_ctx* ctx;
+    int m;
+    long long ld;           // doubles per vector (4 NC)
+    double* V = nullptr;    // m + 1 basis vectors
+    double* x = nullptr;    // the iterate
+    double* g = nullptr;    // dJ/dq
+    double* z = nullptr;    // P v
+    double* w = nullptr;
+    double* coef = nullptr; // device copy of combination coefficients (m + 1)
+    double* part = nullptr; // dot partials: (m + 1) x NP
+    static constexpr int NP = 64;
+    long long cells = 0;
+
+    int alloc() {
+        const size_t n = (size_t)ld * (m + 5) + (m + 1) + (size_t)(m + 1) * NP;
+        if (hipMalloc((void**)&V, sizeof(double) * n) != hipSuccess) return SHCFD_E_NOMEM;
+        if (hipMemset(V, 0, sizeof(double) * n) != hipSuccess) return SHCFD_E_HIP;
+        x = V + ld * (m + 1);
+        g = x + ld;
+        z = g + ld;
+        w = z + ld;
+        coef = w + ld;
+        part = coef + (m + 1);
+        for (int b = 0; b < ctx->nb; b++) cells += (long long)ctx->hblk[b].ni * ctx->hblk[b].nj;
+        return SHCFD_OK;
+    }
+    void release() { if (V) (void)hipFree(V); V = nullptr; }
+    unsigned grid(int b) const {
+        const long long n = (long long)ctx->hblk[b].ni * ctx->hblk[b].nj;
+        return (unsigned)std::min<long long>((n + 255) / 256, 1024);
+    }
+    // y = a x + c y
+    void axpby(double* y, const double* xx, double a, double c) {
+        for (int b = 0; b < ctx->nb; b++)
+            hipLaunchKernelGGL(k_gm_axpby, dim3(grid(b)), dim3(256), 0, ctx->stream, ctx->dev(), b, y, xx, a, c);
+    }
+    // h[i] = <v, V_i>, i < nv (fixed-order host sum of the partials)
+    int dots(const double* v, const double* basis, int nv, double* h) {
+        std::vector<double> acc(nv, 0.0), hp((size_t)nv * NP);
+        for (int b = 0; b < ctx->nb; b++) {
+            hipLaunchKernelGGL(k_gm_dots, dim3(NP, (unsigned)nv), dim3(256), 0, ctx->stream, ctx->dev(), b, v, basis,
+                               ld, part);
+            if (hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, ctx->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess)
+                return SHCFD_E_HIP;
+            for (int i = 0; i < nv; i++)
+                for (int q = 0; q < NP; q++) acc[i] += hp[(size_t)i * NP + q];
+        }
+        for (int i = 0; i < nv; i++) h[i] = acc[i];
+        return SHCFD_OK;
+    }
+    int norm(const double* v, double* out) {
+        int rc = dots(v, v, 1, out);
+        *out = std::sqrt(*out);
+        return rc;
+    }
+    // w += sum_i c[i] V_i
+    int combine(double* dst, const double* c, int nv) {
+        if (hipMemcpyAsync(coef, c, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+            return SHCFD_E_HIP;
+        for (int b = 0; b < ctx->nb; b++)
+            hipLaunchKernelGGL(k_gm_combine, dim3(grid(b)), dim3(256), 0, ctx->stream, ctx->dev(), b, dst, V, ld, coef,
+                               nv);
+        return SHCFD_OK;
+    }
+    // qbar = (dR/dq)^T lam + dJ/dq with lam = src
+    void vjp_of(const double* src) {
+        AdjView& a = ctx->adj->v;
+        (void)hipMemcpyAsync(a.lam, src, sizeof(double) * ld, hipMemcpyDeviceToDevice, ctx->stream);
+        vjp_async(ctx);
+    }
+
+    int solve(int max_iter, double rtol, double* res0, double* res, int* iters) {
+        AdjView& a = ctx->adj->v;
+        const double rms = 1.0 / std::sqrt((double)cells);
+        int rc = SHCFD_OK, it = 0;
+        // dJ/dq, then the residual of the starting lambda
+        (void)hipMemcpyAsync(x, a.lam, sizeof(double) * ld, hipMemcpyDeviceToDevice, ctx->stream);
+        (void)hipMemsetAsync(w, 0, sizeof(double) * ld, ctx->stream);
+        vjp_of(w);
+        (void)hipMemcpyAsync(g, a.qbar, sizeof(double) * ld, hipMemcpyDeviceToDevice, ctx->stream);
+        vjp_of(x);
+        double beta = 0.0, beta0 = 0.0;
+        axpby(V, a.qbar, -1.0, 0.0);   // r = -(A x + g)
+        if ((rc = norm(V, &beta0))) return rc;
+        beta = beta0;
+        if (res0) *res0 = beta0 * rms;
+        std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1), h(m + 1), h2(m + 1), y(m);
+        while (it < max_iter && beta > rtol * beta0 && std::isfinite(beta)) {
+            axpby(V, V, 1.0 / beta - 1.0, 1.0);   // V_0 = r / beta
+            std::fill(gv.begin(), gv.end(), 0.0);
+            gv[0] = beta;
+            int kdim = 0;
+            for (int j = 0; j < m && it < max_iter; j++) {
+                double* vj = V + (long long)j * ld;
+                double* vn = V + (long long)(j + 1) * ld;
+                (void)hipMemsetAsync(z, 0, sizeof(double) * ld, ctx->stream);
+                if ((rc = ctx->fast->adjoint_apply(*ctx, vj, z))) return rc;
+                vjp_of(z);
+                axpby(vn, a.qbar, 1.0, 0.0);
+                axpby(vn, g, -1.0, 1.0);   // A z
+                // classical Gram-Schmidt, twice
+                if ((rc = dots(vn, V, j + 1, h.data()))) return rc;
+                for (int i = 0; i <= j; i++) h2[i] = -h[i];
+                if ((rc = combine(vn, h2.data(), j + 1))) return rc;
+                if ((rc = dots(vn, V, j + 1, h2.data()))) return rc;
+                for (int i = 0; i <= j; i++) h[i] += h2[i], h2[i] = -h2[i];
+                if ((rc = combine(vn, h2.data(), j + 1))) return rc;
+                double hn = 0.0;
+                if ((rc = norm(vn, &hn))) return rc;
+                if (!std::isfinite(hn)) return SHCFD_E_NAN;
+                if (hn > 0.0) axpby(vn, vn, 1.0 / hn - 1.0, 1.0);
+                // Givens rotations of the new column of H
+                for (int i = 0; i < j; i++) {
+                    const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+                    h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+                    h[i] = t;
+                }
+                const double den = std::hypot(h[j], hn);
+                cs[j] = den > 0.0 ? h[j] / den : 1.0;
+                sn[j] = den > 0.0 ? hn / den : 0.0;
+                h[j] = den;
+                gv[j + 1] = -sn[j] * gv[j];
+                gv[j] = cs[j] * gv[j];
+                for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = h[i];
+                kdim = j + 1;
+                it++;
+                if (std::fabs(gv[j + 1]) <= rtol * beta0 || hn == 0.0) break;
+            }
+            // y = H^-1 gv (upper triangular), x += P (V y)
+            for (int i = kdim - 1; i >= 0; i--) {
+                double t = gv[i];
+                for (int l = i + 1; l < kdim; l++) t -= H[(size_t)i * m + l] * y[l];
+                y[i] = t / H[(size_t)i * m + i];
+            }
+            (void)hipMemsetAsync(w, 0, sizeof(double) * ld, ctx->stream);
+            if ((rc = combine(w, y.data(), kdim))) return rc;
+            if ((rc = ctx->fast->adjoint_apply(*ctx, w, x))) return rc;
+            // the true residual of the new iterate (also leaves the bars of x)
+            if ((rc = residual_state(ctx))) return rc;
+            vjp_of(x);
+            axpby(V, a.qbar, -1.0, 0.0);
+            if ((rc = norm(V, &beta))) return rc;
+        }
+        if (res) *res = beta * rms;
+        if (iters) *iters = it;
+        if (!std::isfinite(beta)) return SHCFD_E_NAN;
+        return shcfd_check_error(ctx);
+    }
+};
 
 // host <-> device copy of K cell-indexed components of one block in the
 // reference layout (k fastest, then i = -1..ni+2, then j = -1..nj+2)
@@ -335,6 +553,25 @@ int shcfd_adjoint_solve(shcfd_ctx* ctx, int max_iter, double rtol, int check_eve
     if (!rc) rc = shcfd_set_dt(ctx, 0, nullptr);
     if (!rc) rc = ctx->fast->adjoint_prep(*ctx);
     if (!rc) rc = transposed_iterations(ctx, max_iter, rtol, check_every, res0, res, iters);
+    return rc;
+}
+
+int shcfd_adjoint_solve_gmres(shcfd_ctx* ctx, int restart, int max_iter, double rtol, double* res0, double* res,
+                              int* iters) {
+    if (!ctx || restart < 1 || restart > 500 || max_iter < 0) return SHCFD_E_ARG;
+    int rc = ensure(ctx);
+    if (!rc && (ctx->adj->v.obj_block < 0 || ctx->p.scheme != SHCFD_SCHEME_LUSGS)) rc = SHCFD_E_STATE;
+    if (!rc && !fast_ready(ctx)) rc = SHCFD_E_STATE;
+    if (rc) return rc;
+    ctx->adj->v.unsteady = 0;
+    rc = residual_state(ctx);
+    if (!rc) rc = shcfd_set_dt(ctx, 0, nullptr);
+    if (!rc) rc = ctx->fast->adjoint_prep(*ctx);
+    if (rc) return rc;
+    Gmres G{ctx, restart, 4LL * ctx->NC};
+    rc = G.alloc();
+    if (!rc) rc = G.solve(max_iter, rtol, res0, res, iters);
+    G.release();
     return rc;
 }
 

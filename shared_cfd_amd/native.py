@@ -32,7 +32,7 @@ EXPORTS = [
     "shcfd_chem_boundary_state", "shcfd_chem_eval", "shcfd_chem_eval_noalloc", "shcfd_chem_iterations",
     "shcfd_set_block_nu", "shcfd_set_block_active", "shcfd_energy_rms_blocks", "shcfd_iterate", "shcfd_autotune",
     "shcfd_adjoint_objective", "shcfd_objective", "shcfd_adjoint_upload", "shcfd_adjoint_download",
-    "shcfd_adjoint_poolbar", "shcfd_residual_vjp", "shcfd_adjoint_solve",
+    "shcfd_adjoint_poolbar", "shcfd_residual_vjp", "shcfd_adjoint_solve", "shcfd_adjoint_solve_gmres",
     "shcfd_set_fixed_dt", "shcfd_snapshot_save", "shcfd_snapshot_load", "shcfd_snapshot_free",
     "shcfd_adjoint_unsteady_reset", "shcfd_adjoint_unsteady_step", "shcfd_adjoint_unsteady_poolbar",
     "shcfd_nasa9_parse", "shcfd_set_nasa9", "shcfd_nasa9_stream", "shcfd_load_nasa9",
@@ -95,6 +95,8 @@ def lib():
         L.shcfd_residual_vjp.argtypes = [vp]
         L.shcfd_adjoint_solve.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp,
                                           ctypes.POINTER(ctypes.c_int)]
+        L.shcfd_adjoint_solve_gmres.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, dp,
+                                                ctypes.POINTER(ctypes.c_int)]
         L.shcfd_set_fixed_dt.argtypes = [vp, ctypes.c_double]
         L.shcfd_snapshot_save.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.shcfd_snapshot_load.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -332,6 +334,14 @@ class Solver:
         r0, r, it = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_int(0)
         check(self.L.shcfd_adjoint_solve(self.ctx, max_iter, rtol, check_every, ctypes.byref(r0), ctypes.byref(r),
                                          ctypes.byref(it)), "adjoint_solve")
+        return r0.value, r.value, it.value
+
+    def adjoint_solve_gmres(self, restart=40, max_iter=2000, rtol=1e-10):
+        """Restarted GMRES, right-preconditioned by the transposed LU-SGS
+        sweeps: (initial RMS, final RMS, Krylov steps)."""
+        r0, r, it = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_int(0)
+        check(self.L.shcfd_adjoint_solve_gmres(self.ctx, restart, max_iter, rtol, ctypes.byref(r0), ctypes.byref(r),
+                                               ctypes.byref(it)), "adjoint_solve_gmres")
         return r0.value, r.value, it.value
 
     # ---- fixed time step, snapshots, unsteady adjoint ----

@@ -76,6 +76,13 @@ def test_gradients_match_central_differences():
     assert g.res <= 1e-12 * g.res0
     # the adjoint iteration costs about what the primal does
     assert g.iterations <= 2 * pr.steps
+    # restarted GMRES from lambda = 0 reaches the same solution in fewer
+    # preconditioner applications
+    f.reset_adjoint()
+    gm = f.gradient(1.0, 0.0, rtol=1e-12, max_iter=20000, gmres=40)
+    assert gm.res <= 1e-12 * gm.res0 and gm.iterations < g.iterations
+    assert np.max(np.abs(gm.dJ_dp - g.dJ_dp)) <= 1e-9 * np.max(np.abs(g.dJ_dp))
+    assert abs(gm.dJ_dalpha - g.dJ_dalpha) <= 1e-9 * abs(g.dJ_dalpha)
     f.close()
     h = 1e-7
 
