@@ -298,7 +298,8 @@ def bench_adjoint_unsteady(a, p0, mesh, nsteps=200, window=100, dt=0.002, inner=
     """Unsteady variant: the same airfoil started impulsively from the free
     stream, 200 BDF2 dual-time steps (NR scheme, fixed dt, 60 inner
     iterations each); J = mean C_L over the last 100 steps.  The reverse
-    sweep of unsteady adjoint steps (each to 1e-10 relative) runs once with
+    sweep of unsteady adjoint steps (each to 1e-10 of its right-hand side,
+    starting from the later step's lambda) runs once with
     every state kept on the device and once with checkpoints every 15 steps
     (segments recomputed); cost = reverse sweep time over forward run time."""
     import numpy as np

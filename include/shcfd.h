@@ -318,7 +318,10 @@ int shcfd_adjoint_solve_gmres(shcfd_ctx *ctx, int restart, int max_iter, double 
  * states) to the pool-bar sum.  reset zeroes lambda, the later lambda and
  * the sums.  Before each step the caller restores Q = q^n, QP = q^(n-1),
  * QPP = q^(n-2) (shcfd_snapshot_load), sets first_dual with shcfd_set_time
- * and the weights of J^n with shcfd_adjoint_objective (0, 0 for none). */
+ * and the weights of J^n with shcfd_adjoint_objective (0, 0 for none).
+ * Each step starts from lambda^(n+1) and stops once the RMS of its residual
+ * falls below rtol times the RMS of its right-hand side (the residual at
+ * lambda = 0), which step returns in res0. */
 int shcfd_adjoint_unsteady_reset(shcfd_ctx *ctx);
 int shcfd_adjoint_unsteady_step(shcfd_ctx *ctx, int max_iter, double rtol, int check_every, double *res0,
                                 double *res, int *iters);

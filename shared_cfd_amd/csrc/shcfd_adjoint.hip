@@ -237,18 +237,19 @@ int qbar_rms(shcfd_ctx* ctx, double* out) {
 }
 
 // lambda += M^-T (dR/dq^T lambda + g) from the current lambda until the RMS
-// of the bracket falls below rtol times its first value (the operators of M
-// are those of the last adjoint_prep); QBAR, XBAR and the pool bars belong
-// to the final lambda on return
+// of the bracket falls below rtol times ref (its first value when ref <= 0;
+// the operators of M are those of the last adjoint_prep); QBAR, XBAR and the
+// pool bars belong to the final lambda on return
 int transposed_iterations(shcfd_ctx* ctx, int max_iter, double rtol, int check_every, double* res0, double* res,
-                          int* iters) {
+                          int* iters, double ref = 0.0) {
     AdjView& a = ctx->adj->v;
     double r0 = 0.0, r = 0.0;
     int it = 0;
     vjp_async(ctx);
     int rc = qbar_rms(ctx, &r0);
     r = r0;
-    while (!rc && it < max_iter && r > rtol * r0) {
+    if (ref <= 0.0) ref = r0;
+    while (!rc && it < max_iter && r > rtol * ref) {
         rc = ctx->fast->adjoint_apply(*ctx, a.qbar, a.lam);
         if (rc) break;
         vjp_async(ctx);
@@ -256,7 +257,7 @@ int transposed_iterations(shcfd_ctx* ctx, int max_iter, double rtol, int check_e
         if (it % check_every == 0 || it == max_iter) rc = qbar_rms(ctx, &r);
     }
     if (!rc) rc = qbar_rms(ctx, &r);
-    if (res0) *res0 = r0;
+    if (res0) *res0 = ref;
     if (res) *res = r;
     if (iters) *iters = it;
     if (!rc) rc = shcfd_check_error(ctx);
@@ -612,7 +613,16 @@ int shcfd_adjoint_unsteady_step(shcfd_ctx* ctx, int max_iter, double rtol, int c
                            d, a, b);
     }
     rc = ctx->fast->adjoint_prep(*ctx);
-    if (!rc) rc = transposed_iterations(ctx, max_iter, rtol, check_every, res0, res, iters);
+    if (rc) return rc;
+    // the right-hand side: the residual at lambda = 0 (lambda^(n+1), the
+    // starting guess, is kept in the later lambda meanwhile)
+    const size_t fld = sizeof(double) * 4 * (size_t)ctx->NC;
+    double rhs = 0.0;
+    if (hipMemsetAsync(a.lam, 0, fld, st) != hipSuccess) return SHCFD_E_HIP;
+    vjp_async(ctx);
+    if ((rc = qbar_rms(ctx, &rhs))) return rc;
+    if (hipMemcpyAsync(a.lam, a.lam2, fld, hipMemcpyDeviceToDevice, st) != hipSuccess) return SHCFD_E_HIP;
+    rc = transposed_iterations(ctx, max_iter, rtol, check_every, res0, res, iters, rhs > 0.0 ? rhs : -1.0);
     if (rc) return rc;
     for (int b = 0; b < ctx->nb; b++) {
         const DevBlock& B = ctx->hblk[b];
