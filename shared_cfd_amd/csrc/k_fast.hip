@@ -434,6 +434,10 @@ k_fast_fwd(DevCtx d, const QStrip* strips, int nstrips, const double* R_sl, cons
     double hb[PH];
 #pragma unroll
     for (int q = 0; q < PH; q++) hb[q] = ld_agent(rin, vc, col_off(q + 1));
+    // drain before the loop: the wait counts at the loop head then follow the
+    // back edge alone (in-order vmcnt: with these loads still in flight on
+    // entry, every step would also wait for the stores of the last steps)
+    wait_all_vmem();
     double a = 0.0, bo = 0.0, vlo = 0.0;
     Chunk A, B;
     wg_barrier();  // chunk 0 in LDS
@@ -563,6 +567,7 @@ k_fast_bwd(DevCtx d, const QStrip* strips, int nstrips, const double* X_sl, cons
     double hb[PH];
 #pragma unroll
     for (int q = 0; q < PH; q++) hb[q] = ld_agent(rin, vc, col_off(top - q - (QR - 2)));
+    wait_all_vmem();  // as in the forward sweep
     double a = 0.0, bo = 0.0, zhi = 0.0;
     Chunk A, B;
     wg_barrier();
