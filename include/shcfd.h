@@ -263,9 +263,10 @@ int shcfd_snapshot_load(shcfd_ctx *ctx, int slot, int field);
 int shcfd_snapshot_free(shcfd_ctx *ctx);
 
 /* ---- discrete adjoint (no counterpart in the reference, which has no
- * adjoint; for shape and inflow-angle gradients of inviscid ideal-gas
- * solutions: nY = 1, optimised implicit path; steady: LU-SGS, unsteady:
- * the NR or dual scheme with a fixed time step) --------------------------- */
+ * adjoint; for shape and inflow-angle gradients of ideal-gas solutions,
+ * inviscid or laminar (viscous): nY = 1, planar, optimised implicit path;
+ * steady: LU-SGS, unsteady: the NR or dual scheme with a fixed time step;
+ * boundary kinds up to the curved slip wall) ------------------------------ */
 /* what of shcfd_adjoint_upload / shcfd_adjoint_download */
 enum {
     SHCFD_ADJ_LAMBDA = 0, /* adjoint state, dimq per cell (reference cell layout) */
@@ -277,8 +278,9 @@ enum {
                                   done (lambda^2 once step 1 is done)             */
 };
 /* objective J = sum over the j-min faces i in [i0, i1] of block (0-based
- * block, 1-based i) of dsj * (TGj_u * ax + TGj_v * ay): the pressure
- * (momentum) flux through a wall, so ax, ay = -e/(q_inf c) give the force
+ * block, 1-based i) of dsj * ((TGj - TGvj)_u * ax + (TGj - TGvj)_v * ay):
+ * the momentum flux through a wall (pressure, plus the viscous stress when
+ * the flow is viscous), so ax, ay = -e/(q_inf c) give the force
  * coefficient along e */
 int shcfd_adjoint_objective(shcfd_ctx *ctx, int block, int i0, int i1, double ax, double ay);
 /* evaluates the residual pipeline at the current state and J */

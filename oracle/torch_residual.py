@@ -1,5 +1,6 @@
-"""Float64 torch restatement of the steady inviscid ideal-gas residual, for
-checking the hand-written discrete-adjoint kernels with torch autograd.
+"""Float64 torch restatement of the steady ideal-gas residual (inviscid or
+laminar), for checking the hand-written discrete-adjoint kernels with torch
+autograd.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ (the checker), never by the
 product.  It restates, for one block with nY = 1:
@@ -8,11 +9,18 @@ product.  It restates, for one block with nY = 1:
 * primitive and ideal-gas thermo state (core/scheme.f90:1-29,
   therm_lib/ideal/thermal_model.raw.f90:7-33),
 * ghost fills: cut rows (cond/no-nV/condition.tail.f90:2-70, the cut_copro
-  row format) then side rows (zero gradient, curved slip wall, fixed state),
+  row format), side rows (zero gradient, curved slip wall, no-slip wall,
+  fixed state), then the corners (condition.tail.f90:331-356),
 * MUSCL face states with the reference limiter (high_order/muscl/muscl.f90:7-73),
 * the SLAU flux (core/scheme.f90:71-153),
-* the flux balance (LU-SGS/2d/time_lusgs_viscous.f90:12-22, Area = 0),
-* the wall-force objective J = sum dsj (TG_u a_x + TG_v a_y) over j-min faces.
+* when viscous: cell centres with extrapolated ghost centres and the face
+  metrics from them (viscosity/viscous/no-nV/2d/sch_viscous.f90:1-54), the
+  viscous flux (sch_viscous.f90:106-178) with the species enthalpy
+  vhi = kappa/(kappa-1) R T of the ideal gas (ghosts filled like w),
+* the flux balance (LU-SGS/2d/time_lusgs_viscous.f90:12-22, Area = 0) of
+  TG - TGv,
+* the wall-force objective J = sum dsj ((TG - TGv)_u a_x + (TG - TGv)_v a_y)
+  over j-min faces.
 
 Arrays are (j, i) ordered like the reference layout without ghosts for q and
 (i, j) for the nodes, matching Case.xh/Case.rh.  Its fluxes are pinned
@@ -41,7 +49,8 @@ def face_metrics(x, y):
 
 
 def ideal_w(q, kap, rg, nu=0.0):
-    """q (..., 4) -> w (..., 9): rho, u, v, p, Y, gamma, H, R, mu."""
+    """q (..., 4) -> w (..., 10): rho, u, v, p, Y, gamma, H, R, mu and the
+    species enthalpy vhi."""
     rho = q[..., 0]
     inv = 1.0 / rho
     u = q[..., 1] * inv
@@ -52,7 +61,8 @@ def ideal_w(q, kap, rg, nu=0.0):
     p = (rho * rg) * T
     H = (q[..., 3] + p) / rho
     one = torch.ones_like(rho)
-    return torch.stack([rho, u, v, p, Y, kap * one, H, rg * one, rho * nu], dim=-1)
+    vhi = ((kap / (kap - 1.0)) * rg) * T
+    return torch.stack([rho, u, v, p, Y, kap * one, H, rg * one, rho * nu, vhi], dim=-1)
 
 
 def _cut_cells(t, inc, copy):
@@ -76,7 +86,7 @@ def _cut_cells(t, inc, copy):
 
 
 def ghosted_w(w_int, ni, nj, cut_rows, bc_rows, pool, vni, vnj):
-    """Interior w (nj, ni, 9) -> w with 2 ghost layers (nj+4, ni+4, 9).
+    """Interior w (nj, ni, 10) -> w with 2 ghost layers (nj+4, ni+4, 10).
     Single block; cut rows first, then side rows (a later fill of the same
     ghost replaces an earlier one).  Built out of place for autograd."""
     W0 = torch.nn.functional.pad(w_int.movedim(-1, 0), (2, 2, 2, 2)).movedim(0, -1)
@@ -104,10 +114,12 @@ def ghosted_w(w_int, ni, nj, cut_rows, bc_rows, pool, vni, vnj):
             else:
                 ii, jj, g1, g2 = tpos, nj, (tpos, nj + 1), (tpos, nj + 2)
             if kind == BC_FIXED:
-                wv = pool[off:off + 9]
+                wv = pool[off:off + 10]
             else:
                 wv = W0[at(ii, jj)]
-                if kind == BC_SLIP_NORMAL:
+                if kind == BC_NOSLIP:
+                    wv = torch.cat([wv[:1], -wv[1:3], wv[3:]])
+                elif kind == BC_SLIP_NORMAL:
                     if side == SIDE_JMIN:
                         nx, ny = vnj[ii - 1, 0, 0], vnj[ii - 1, 0, 1]
                     elif side == SIDE_JMAX:
@@ -127,7 +139,78 @@ def ghosted_w(w_int, ni, nj, cut_rows, bc_rows, pool, vni, vnj):
     rows = torch.tensor([k[0] for k in keys])
     cols = torch.tensor([k[1] for k in keys])
     vals = torch.stack([fills[k] for k in keys])
-    return W0.index_put((rows, cols), vals)
+    W1 = W0.index_put((rows, cols), vals)
+    # corners: the mean of the corner cell and its two side ghosts
+    cr, cc, cv = [], [], []
+    for pi in (-1, 1):
+        i = 1 if pi < 0 else ni
+        for pj in (-1, 1):
+            j = 1 if pj < 0 else nj
+            cr.append(at(i + pi, j + pj)[0])
+            cc.append(at(i + pi, j + pj)[1])
+            cv.append((1.0 / 3.0) * ((W1[at(i, j)] + W1[at(i, j + pj)]) + W1[at(i + pi, j)]))
+    return W1.index_put((torch.tensor(cr), torch.tensor(cc)), torch.stack(cv))
+
+
+def centres(x, y):
+    """Cell centres (nj+2, ni+2) with one layer of ghost centres: the means
+    of the four nodes, then x(0) = 2 x(1) - x(2) on the i-sides and after
+    them the j-sides (dim/2d/geometry.f90:79-169)."""
+    def one(z):
+        c = 0.25 * (((z[:-1, :-1] + z[1:, :-1]) + z[:-1, 1:]) + z[1:, 1:])
+        c = c.transpose(0, 1)  # (nj, ni)
+        c = torch.cat([(2.0 * c[:, 0] - c[:, 1])[:, None], c, (2.0 * c[:, -1] - c[:, -2])[:, None]], dim=1)
+        return torch.cat([(2.0 * c[0] - c[1])[None], c, (2.0 * c[-1] - c[-2])[None]], dim=0)
+    return one(x), one(y)
+
+
+def geojac(xs, rs):
+    """Face metrics (g11, g21, g12, g22) from the centre coordinates of the
+    stencil (c, c1, a, b, e, f), DIR 0 layout (xi along c -> c1)."""
+    xc, x1, xa, xb, xe, xf = xs
+    rc, r1, ra, rb, re, rf = rs
+    dxdxi, dydxi = x1 - xc, r1 - rc
+    dxdeta = (((xa + xb) - xe) - xf) * 0.25
+    dydeta = (((ra + rb) - re) - rf) * 0.25
+    det = (dxdxi * dydeta) - (dydxi * dxdeta)
+    return dydeta / det, -dydxi / det, -dxdeta / det, dxdxi / det
+
+
+def viscous_flux(ws, g, n1, n2, dir_):
+    """Viscous flux (..., 4) from the w of the stencil cells (c, c1, a, b, e,
+    f), the face metrics and unit normal.  dir_ 0: c1 - c is the xi
+    difference; 1: the eta difference."""
+    wc, w1, wa, wb, we, wf = ws
+    T = lambda w: (w[..., 3] / w[..., 0]) / w[..., 7]
+    avg = lambda k: (wc[..., k] + w1[..., k]) * 0.5
+    rho, u, v, gamm, Rg, mu, lhi = avg(0), avg(1), avg(2), avg(5), avg(7), avg(8), avg(9)
+    dn = [w1[..., 1] - wc[..., 1], w1[..., 2] - wc[..., 2], T(w1) - T(wc), w1[..., 4] - wc[..., 4]]
+    cd = lambda a_, b_, e_, f_: (((a_ + b_) - e_) - f_) * 0.25
+    dc = [cd(wa[..., 1], wb[..., 1], we[..., 1], wf[..., 1]), cd(wa[..., 2], wb[..., 2], we[..., 2], wf[..., 2]),
+          cd(T(wa), T(wb), T(we), T(wf)), cd(wa[..., 4], wb[..., 4], we[..., 4], wf[..., 4])]
+    dxi, deta = (dn, dc) if dir_ == 0 else (dc, dn)
+    g11, g21, g12, g22 = g
+    D = mu / rho
+    k_heat = ((mu * gamm) / (gamm - 1.0)) * Rg
+    dudx = dxi[0] * g11 + deta[0] * g21
+    dvdx = dxi[1] * g11 + deta[1] * g21
+    dTdx = dxi[2] * g11 + deta[2] * g21
+    dudr = dxi[0] * g12 + deta[0] * g22
+    dvdr = dxi[1] * g12 + deta[1] * g22
+    dTdr = dxi[2] * g12 + deta[2] * g22
+    dYdx = dxi[3] * g11 + deta[3] * g21
+    dYdr = dxi[3] * g12 + deta[3] * g22
+    divu = dudx + dvdr
+    t_xx = -((2.0 / 3.0) * mu) * divu + (2.0 * mu) * dudx
+    t_rr = -((2.0 / 3.0) * mu) * divu + (2.0 * mu) * dvdr
+    t_xr = mu * (dvdx + dudr)
+    qx = -(k_heat * dTdx) - ((rho * D) * (dYdx * lhi))
+    qr = -(k_heat * dTdr) - ((rho * D) * (dYdr * lhi))
+    f0 = ((rho * D) * dYdx) * n1 + ((rho * D) * dYdr) * n2
+    f1 = t_xx * n1 + t_xr * n2
+    f2 = t_xr * n1 + t_rr * n2
+    f3 = (((u * t_xx) + (v * t_xr)) - qx) * n1 + (((u * t_xr) + (v * t_rr)) - qr) * n2
+    return torch.stack([f0, f1, f2, f3], dim=-1)
 
 
 def muscl_delta(Du, Dl, k_mus):
@@ -189,13 +272,13 @@ def slau(wl, wr, n1, n2):
 
 
 def residual(q, x, y, pool, case, obj=None):
-    """R (nj, ni, 4), TGi (nj, ni+1, 4), TGj (nj+1, ni, 4) and the objective
-    of one block.  q: (nj, ni, 4); x, y: (ni+1, nj+1); pool: bc pool tensor.
+    """R (nj, ni, 4), the net face fluxes TGi - TGvi (nj, ni+1, 4) and
+    TGj - TGvj (nj+1, ni, 4), and the objective of one block.  q: (nj, ni, 4); x, y: (ni+1, nj+1); pool: bc pool tensor.
     obj = (i0, i1, ax, ay) or None."""
     p = case.params
     ni, nj = case.ni[0], case.nj[0]
     dsi, vni, dsj, vnj = face_metrics(x, y)
-    w = ideal_w(q, p.kappa, p.rgas)
+    w = ideal_w(q, p.kappa, p.rgas, p.nu)
     W = ghosted_w(w, ni, nj, case.cut_rows, case.bc_rows, pool, vni, vnj)
     hp = p.phi_mus * 0.5
     # i-faces: rows j = 1..nj, faces i = 0..ni
@@ -209,6 +292,30 @@ def residual(q, x, y, pool, case, obj=None):
     TGj = slau(lj, rj, nj_[..., 0], nj_[..., 1])
     dsi_t = dsi.transpose(0, 1)[..., None]  # (nj, ni+1, 1)
     dsj_t = dsj.transpose(0, 1)[..., None]  # (nj+1, ni, 1)
+    if p.viscous:
+        XC, RC = centres(x, y)
+        # i-faces (fi, fj), fi = 0..ni, fj = 1..nj: cell (i, j) at W[j+1, i+1], centre at XC[j, i]
+        wsi = [W[2:nj + 2, 1:ni + 2], W[2:nj + 2, 2:ni + 3], W[3:nj + 3, 2:ni + 3], W[3:nj + 3, 1:ni + 2],
+               W[1:nj + 1, 2:ni + 3], W[1:nj + 1, 1:ni + 2]]
+        sl_i = [(slice(1, nj + 1), slice(0, ni + 1)), (slice(1, nj + 1), slice(1, ni + 2)),
+                (slice(2, nj + 2), slice(1, ni + 2)), (slice(2, nj + 2), slice(0, ni + 1)),
+                (slice(0, nj), slice(1, ni + 2)), (slice(0, nj), slice(0, ni + 1))]
+        gi = geojac([XC[s] for s in sl_i], [RC[s] for s in sl_i])
+        TGi = TGi - viscous_flux(wsi, gi, ni_[..., 0], ni_[..., 1], 0)
+        # j-faces (fi, fj), fi = 1..ni, fj = 0..nj; stencil (c, c1, a, b, e, f) with xi along a/b - e/f
+        wsj = [W[1:nj + 2, 2:ni + 2], W[2:nj + 3, 2:ni + 2], W[2:nj + 3, 3:ni + 3], W[1:nj + 2, 3:ni + 3],
+               W[2:nj + 3, 1:ni + 1], W[1:nj + 2, 1:ni + 1]]
+        sl_j = [(slice(0, nj + 1), slice(1, ni + 1)), (slice(1, nj + 2), slice(1, ni + 1)),
+                (slice(1, nj + 2), slice(2, ni + 2)), (slice(0, nj + 1), slice(2, ni + 2)),
+                (slice(1, nj + 2), slice(0, ni)), (slice(0, nj + 1), slice(0, ni))]
+        # geojac of a j-face: xi from (a + b - e - f)/4, eta from c1 - c
+        xs, rs = [XC[s] for s in sl_j], [RC[s] for s in sl_j]
+        dxdxi = (((xs[2] + xs[3]) - xs[4]) - xs[5]) * 0.25
+        dydxi = (((rs[2] + rs[3]) - rs[4]) - rs[5]) * 0.25
+        dxdeta, dydeta = xs[1] - xs[0], rs[1] - rs[0]
+        det = (dxdxi * dydeta) - (dydxi * dxdeta)
+        gj = (dydeta / det, -dydxi / det, -dxdeta / det, dxdxi / det)
+        TGj = TGj - viscous_flux(wsj, gj, nj_[..., 0], nj_[..., 1], 1)
     Fi = dsi_t * TGi
     Fj = dsj_t * TGj
     R = (Fi[:, :-1] - Fi[:, 1:]) + (Fj[:-1, :] - Fj[1:, :])

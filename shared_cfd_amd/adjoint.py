@@ -1,8 +1,8 @@
 """Shape and inflow-angle gradients of airfoil force coefficients through the
 discrete adjoint of the GPU solver, behind a torch.autograd.Function.
 
-The primal is the steady inviscid LU-SGS solution on the C-mesh of
-shared_cfd_amd.airfoil; the adjoint (include/shcfd.h, shcfd_adjoint_*) solves
+The primal is the steady LU-SGS solution, inviscid or laminar, on the C-mesh
+of shared_cfd_amd.airfoil; the adjoint (include/shcfd.h, shcfd_adjoint_*) solves
 (dR/dq)^T lambda = -(dJ/dq)^T with the transposed LU-SGS iteration and returns
 dJ/dX per node and dJ/d(free-stream state).  The chain to the design
 variables is exact and linear:

@@ -98,8 +98,10 @@ def _wall_line(ni, nwake, wake_len, t):
 
 def naca_cmesh(ni=512, nj=128, nwake=64, mach=0.8, alpha_deg=1.25, t=0.12, radius=15.0, wake_len=15.0,
                first_height=2e-3, nb=32, amplitudes=None, cfl=5.0, order=2, rho_inf=1.4, p_inf=1.0,
-               kappa=1.4, params=None) -> AirfoilCase:
-    """The inviscid NACA 4-digit (thickness ``t``) airfoil case.
+               kappa=1.4, params=None, reynolds=None) -> AirfoilCase:
+    """The NACA 4-digit (thickness ``t``) airfoil case: inviscid with a
+    curved slip wall, or laminar with a no-slip wall at the chord Reynolds
+    number ``reynolds`` (nu = V_inf * chord / reynolds).
 
     ``amplitudes`` (2*nb values, upper then lower surface bumps, in chords)
     deform the wall; the case starts from the free stream.  LU-SGS with local
@@ -154,21 +156,24 @@ def naca_cmesh(ni=512, nj=128, nwake=64, mach=0.8, alpha_deg=1.25, t=0.12, radiu
     qc = C.cons_from_prim(np.full((nj, ni), rho_inf), np.full((nj, ni), u_inf), np.full((nj, ni), v_inf),
                           np.full((nj, ni), p_inf), kappa)
     q = [C._fill_q(ni, nj, qc)]
-    pool = np.array(C.fixed_state_pool(rho_inf, u_inf, v_inf, p_inf, kappa, rgas, mu=0.0), dtype=np.float64)
+    nu = 0.0 if reynolds is None else mach * c_inf / float(reynolds)
+    pool = np.array(C.fixed_state_pool(rho_inf, u_inf, v_inf, p_inf, kappa, rgas, mu=rho_inf * nu), dtype=np.float64)
     rows = [
         [1, C.SIDE_IMIN, 1, nj, C.BC_EXTRAP, 0],
         [1, C.SIDE_IMAX, 1, nj, C.BC_EXTRAP, 0],
-        [1, C.SIDE_JMIN, nwake + 1, ni - nwake, C.BC_SLIP_NORMAL, 0],
+        [1, C.SIDE_JMIN, nwake + 1, ni - nwake, C.BC_SLIP_NORMAL if reynolds is None else C.BC_NOSLIP, 0],
         [1, C.SIDE_JMAX, 1, ni, C.BC_FIXED, 0],
     ]
     # wake: j-min cells i=1..nwake exchange with i=ni..ni-nwake+1
     cut = [[4, nwake, -1, 1, 1, 1, -1, 1, 1, ni, -1]]
     kw = dict(scheme=C.SCHEME_LUSGS, time_step=1, high_order=0 if order == 1 else 1, limiter=C.LIMITER_REFERENCE,
-              cfl=cfl, k_mus=-1.0 if order == 2 else 1.0 / 3.0, kappa=kappa, rgas=rgas, nu=0.0,
+              cfl=cfl, k_mus=-1.0 if order == 2 else 1.0 / 3.0, kappa=kappa, rgas=rgas, nu=nu,
+              viscous=0 if reynolds is None else 1,
               res_rate_warning=1e300, res_rate_error=1e300)
     kw.update(params or {})
     params = C.make_params(**kw)
     case = C.Case("naca_cmesh", [ni], [nj], 1, params, [Xn[0].copy()], [Xn[1].copy()], q,
                   np.array(rows, dtype=np.int32), pool, np.array(cut, dtype=np.int32),
-                  description=f"NACA 4-digit t={t:.2f} C-mesh {ni}x{nj}, M={mach}, alpha={alpha_deg} deg, inviscid")
+                  description=f"NACA 4-digit t={t:.2f} C-mesh {ni}x{nj}, M={mach}, alpha={alpha_deg} deg, "
+                              + ("inviscid" if reynolds is None else f"laminar Re={reynolds:g}"))
     return AirfoilCase(case, ni, nj, nwake, mach, alpha, rho_inf, p_inf, kappa, rgas, phi, X0, npar)

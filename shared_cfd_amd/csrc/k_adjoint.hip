@@ -1,8 +1,12 @@
-// k_adjoint.hip -- discrete-adjoint (reverse) kernels of the steady inviscid
+// k_adjoint.hip -- discrete-adjoint (reverse) kernels of the steady
 // ideal-gas residual
-//     R(c) = dsi(i-1) TGi(i-1) - dsi(i) TGi(i) + dsj(j-1) TGj(j-1) - dsj(j) TGj(j)
+//     R(c) = dsi(i-1) (TGi - TGvi)(i-1) - dsi(i) (TGi - TGvi)(i)
+//          + dsj(j-1) (TGj - TGvj)(j-1) - dsj(j) (TGj - TGvj)(j)
 // (LU-SGS/2d/time_lusgs_viscous.f90:12-22 with Area = 0) through the SLAU
-// flux (core/scheme.f90:71-153), MUSCL face states with the limiter
+// flux (core/scheme.f90:71-153), the viscous flux when the flow is viscous
+// (viscosity/viscous/no-nV/2d/sch_viscous.f90:1-178, with the corner ghosts
+// of cond/no-nV/condition.tail.f90:331-356 and the face metrics from the
+// cell centres), MUSCL face states with the limiter
 // (high_order/muscl/muscl.f90:7-73), the boundary ghost fills
 // (cond/no-nV/condition.tail.f90:2-70 and the physical side rows), the
 // primitive/thermo map (core/scheme.f90:1-29,
@@ -10,7 +14,7 @@
 // (dim/2d/geometry.f90:79-169).
 //
 // Given lambda (one vector per interior cell) and an objective
-//     J = sum over wall faces of dsj (TG_u a_x + TG_v a_y),
+//     J = sum over wall faces of dsj ((TG - TGv)_u a_x + (TG - TGv)_v a_y),
 // one pass computes lambda^T dR/dq + dJ/dq per cell, lambda^T dR/dX + dJ/dX
 // per node and the derivative with respect to the fixed boundary states.
 // Local Jacobians come from forward-mode dual evaluations of the same
@@ -49,18 +53,21 @@ __global__ void __launch_bounds__(128) k_adj_face(DevCtx d, AdjView a, int b) {
     const int ftg = DIR == 0 ? SHCFD_F_TGI : SHCFD_F_TGJ;
     const int fl = DIR == 0 ? SHCFD_F_WHLI : SHCFD_F_WHLJ, fr = DIR == 0 ? SHCFD_F_WHRI : SHCFD_F_WHRJ;
     const double ds = FLD(d, fds, 0, c0);
+    // the face's net flux TG - TGv (TGv zero when inviscid)
+    const int ftv = DIR == 0 ? SHCFD_F_TGVI : SHCFD_F_TGVJ;
+    auto net = [&](int k) { return FLD(d, ftg, k, c0) - (d.p.viscous ? FLD(d, ftv, k, c0) : 0.0); };
     double tgb[4], dsb = 0.0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const double dl = (in1 ? a.lam[k * d.NC + c1] : 0.0) - (in0 ? a.lam[k * d.NC + c0] : 0.0);
         tgb[k] = ds * dl;
-        dsb += FLD(d, ftg, k, c0) * dl;
+        dsb += net(k) * dl;
     }
     // objective: momentum flux through the wall faces of the objective segment
     if (DIR == 1 && j == 0 && b == a.obj_block && i >= a.obj_i0 && i <= a.obj_i1) {
         tgb[1] += ds * a.obj_ax;
         tgb[2] += ds * a.obj_ay;
-        dsb += FLD(d, ftg, 1, c0) * a.obj_ax + FLD(d, ftg, 2, c0) * a.obj_ay;
+        dsb += net(1) * a.obj_ax + net(2) * a.obj_ay;
     }
     using T = Dual<2 * NFV + 2>;
     T wl[D1::dw], wr[D1::dw], tg[4];
@@ -107,6 +114,168 @@ __global__ void __launch_bounds__(128) k_adj_face(DevCtx d, AdjView a, int b) {
 }
 template __global__ void k_adj_face<0>(DevCtx, AdjView, int);
 template __global__ void k_adj_face<1>(DevCtx, AdjView, int);
+
+// ---------------------------------------------------------------------------
+// viscous faces (ideal gas, nY = 1, planar).  R carries -ds TGv, so the bars
+// of TGv are minus those TG gets.  Per face:
+//  - the flux's dependence on its 14 intermediates (viscous_in_t) and 6
+//    metrics (geojac, unit normal), in dual numbers through viscous_out_t;
+//  - the intermediates' dependence on the w vectors of the 6 stencil cells
+//    (face averages, xi/eta differences, T = (p/rho)/R), applied by hand;
+//  - the metrics' dependence on the stencil's centre coordinates (geojac_t
+//    in dual numbers).
+// Stored per face and stencil position, gathered per cell by k_adj_muscl
+// and k_adj_centres.  With nY = 1 the species-diffusion terms multiply the
+// differences of Y = rho/rho, zero up to rounding, so the face enthalpies
+// lhi are held constant.
+// ---------------------------------------------------------------------------
+constexpr int NVW = 8;  // w components of the viscous flux: rho, u, v, p, Y, gamma, R, mu
+// slot of w component k among the NVW (-1: H, which the flux does not use)
+__device__ inline int vw_slot(int k) { return k < D1::ih ? k : (k == D1::ih ? -1 : k - 1); }
+// face (fi, fj) = (i + DFI[s], j + DFJ[s]) holds cell (i, j) at stencil position s
+__constant__ int VDFI[2][6] = {{0, -1, -1, 0, -1, 0}, {0, 0, -1, -1, 1, 1}};
+__constant__ int VDFJ[2][6] = {{0, 0, -1, -1, 1, 1}, {0, -1, -1, 0, -1, 0}};
+
+template <int DIR>
+__global__ void __launch_bounds__(128) k_adj_vface(DevCtx d, AdjView a, int b) {
+    const DevBlock B = d.blk[b];
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + (DIR == 0 ? 0 : 1);
+    const int j = (int)blockIdx.y + (DIR == 0 ? 1 : 0);
+    if (DIR == 0 ? (i > B.ni) : (i > B.ni || j > B.nj)) return;
+    const long long c0 = cidx(B, i, j);
+    long long st[6];
+    viscous_stencil<DIR>(c0, B.pitch, st);
+    const int i1 = DIR == 0 ? i + 1 : i, j1 = DIR == 0 ? j : j + 1;
+    const bool in0 = interior(B, i, j), in1 = interior(B, i1, j1);
+    const double ds = FLD(d, DIR == 0 ? SHCFD_F_DSI : SHCFD_F_DSJ, 0, c0);
+    double tb[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        tb[k] = -ds * ((in1 ? a.lam[k * d.NC + st[1]] : 0.0) - (in0 ? a.lam[k * d.NC + c0] : 0.0));
+    if (DIR == 1 && j == 0 && b == a.obj_block && i >= a.obj_i0 && i <= a.obj_i1) {
+        tb[1] -= ds * a.obj_ax;
+        tb[2] -= ds * a.obj_ay;
+    }
+    double ws[6][D1::dw];
+    const double* wp[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+#pragma unroll
+        for (int k = 0; k < D1::dw; k++) ws[s][k] = FLD(d, SHCFD_F_W, k, st[s]);
+        wp[s] = ws[s];
+    }
+    ViscIn<1, double> z0;
+    viscous_in_t<1, DIR, double>(wp, z0);
+    const int fg = DIR == 0 ? SHCFD_F_GEOJACI : SHCFD_F_GEOJACJ, fvn = DIR == 0 ? SHCFD_F_VNI : SHCFD_F_VNJ;
+    double zb[20];
+    {
+        using T = Dual<20>;
+        ViscIn<1, T> z;
+        z.rho = T::var(z0.rho, 0);
+        z.u = T::var(z0.u, 1);
+        z.v = T::var(z0.v, 2);
+        z.gamm = T::var(z0.gamm, 3);
+        z.Rg = T::var(z0.Rg, 4);
+        z.mu = T::var(z0.mu, 5);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            z.dxi[q] = T::var(z0.dxi[q], 6 + q);
+            z.deta[q] = T::var(z0.deta[q], 10 + q);
+        }
+        T g[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) g[q] = T::var(FLD(d, fg, q, c0), 14 + q);
+        const T n1 = T::var(FLD(d, fvn, 0, c0), 18), n2 = T::var(FLD(d, fvn, 1, c0), 19);
+        const T lhi[1] = {T((FLD(d, SHCFD_F_VHI, 0, c0) + FLD(d, SHCFD_F_VHI, 0, st[1])) * 0.5)};
+        T out[4];
+        viscous_out_t<1, T>(z, lhi, g, n1, n2, false, T(1.0), out);
+#pragma unroll
+        for (int m = 0; m < 20; m++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) acc += tb[k] * out[k].d[m];
+            zb[m] = acc;
+        }
+    }
+    double* nbar = DIR == 0 ? a.nbar_i : a.nbar_j;
+    nbar[c0] += zb[18];
+    nbar[d.NC + c0] += zb[19];
+    // the c1 - c differences are the xi ones for DIR 0, the eta ones for DIR 1
+    const int on = DIR == 0 ? 6 : 10, oc = DIR == 0 ? 10 : 6;
+    double* vw = DIR == 0 ? a.vw_i : a.vw_j;
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        double wb[NVW];
+#pragma unroll
+        for (int m = 0; m < NVW; m++) wb[m] = 0.0;
+        if (s < 2) {  // face averages of c and c1
+            wb[0] = 0.5 * zb[0];
+            wb[1] = 0.5 * zb[1];
+            wb[2] = 0.5 * zb[2];
+            wb[5] = 0.5 * zb[3];
+            wb[6] = 0.5 * zb[4];
+            wb[7] = 0.5 * zb[5];
+        }
+        const double fn = s == 0 ? -1.0 : (s == 1 ? 1.0 : 0.0);
+        const double fc = s < 2 ? 0.0 : (s < 4 ? 0.25 : -0.25);
+        const double ub = fn * zb[on] + fc * zb[oc], vb = fn * zb[on + 1] + fc * zb[oc + 1];
+        const double Tb = fn * zb[on + 2] + fc * zb[oc + 2], Yb = fn * zb[on + 3] + fc * zb[oc + 3];
+        wb[1] += ub;
+        wb[2] += vb;
+        wb[4] += Yb;
+        // T = (p / rho) / R
+        const double rho = ws[s][IW_RHO], R = ws[s][D1::ir];
+        const double Tv = (ws[s][IW_P] / rho) / R;
+        wb[3] += Tb / (rho * R);
+        wb[0] -= Tb * Tv / rho;
+        wb[6] -= Tb * Tv / R;
+#pragma unroll
+        for (int m = 0; m < NVW; m++) vw[(s * NVW + m) * d.NC + c0] = wb[m];
+    }
+    // metric bars -> centre-coordinate bars of the stencil
+    using G = Dual<12>;
+    G xc[6], rc[6], gg[4];
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        xc[s] = G::var(FLD(d, SHCFD_F_X, 0, st[s]), s);
+        rc[s] = G::var(FLD(d, SHCFD_F_R, 0, st[s]), 6 + s);
+    }
+    geojac_t<DIR, G>(xc, rc, gg);
+    double* vg = DIR == 0 ? a.vg_i : a.vg_j;
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        double xb = 0.0, rb = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            xb += zb[14 + q] * gg[q].d[s];
+            rb += zb[14 + q] * gg[q].d[6 + s];
+        }
+        vg[(2 * s) * d.NC + c0] = xb;
+        vg[(2 * s + 1) * d.NC + c0] = rb;
+    }
+}
+template __global__ void k_adj_vface<0>(DevCtx, AdjView, int);
+template __global__ void k_adj_vface<1>(DevCtx, AdjView, int);
+
+// sum over the viscous faces holding cell (i, j) of their stored bars
+// (component slot m of nper per position)
+__device__ inline double viscous_gather(const DevCtx& d, const DevBlock& B, const double* fi_arr,
+                                        const double* fj_arr, int nper, int m, int i, int j) {
+    double s = 0.0;
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+        const int fi = i + VDFI[0][p], fj = j + VDFJ[0][p];
+        if (fi >= 0 && fi <= B.ni && fj >= 1 && fj <= B.nj)
+            s += fi_arr[(long long)(p * nper + m) * d.NC + cidx(B, fi, fj)];
+    }
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+        const int fi = i + VDFI[1][p], fj = j + VDFJ[1][p];
+        if (fi >= 1 && fi <= B.ni && fj >= 0 && fj <= B.nj)
+            s += fj_arr[(long long)(p * nper + m) * d.NC + cidx(B, fi, fj)];
+    }
+    return s;
+}
 
 // ---------------------------------------------------------------------------
 // MUSCL: face-state bars -> w bars of the stencil cells (gather per cell,
@@ -175,7 +344,68 @@ __global__ void __launch_bounds__(128) k_adj_muscl(DevCtx d, AdjView a, int b) {
                 if (o >= 0 && o <= 2) s += br * m.r[o];
             }
         }
+        if (d.p.viscous && i >= 0 && i <= B.ni + 1 && j >= 0 && j <= B.nj + 1 && vw_slot(k) >= 0)
+            s += viscous_gather(d, B, a.vw_i, a.vw_j, NVW, vw_slot(k), i, j);
         a.wbar[k * d.NC + c] = s;
+    }
+}
+
+// corner ghosts (set_corners, cond/no-nV/condition.tail.f90:331-356: the
+// mean of the corner cell and its two side ghosts), reversed before the
+// side rows; one thread per block
+__global__ void k_adj_bc_corner(DevCtx d, AdjView a) {
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= d.nb) return;
+    const DevBlock B = d.blk[b];
+    for (int q = 0; q < 4; q++) {
+        const int pi = (q & 1) ? 1 : -1, pj = (q & 2) ? 1 : -1;
+        const int i = pi < 0 ? 1 : B.ni, j = pj < 0 ? 1 : B.nj;
+        const long long cc = cidx(B, i + pi, j + pj), c0 = cidx(B, i, j), cj = cidx(B, i, j + pj),
+                        ci = cidx(B, i + pi, j);
+        for (int k = 0; k < D1::dw; k++) {
+            const double x = a.wbar[k * d.NC + cc] * (1.0 / 3.0);
+            a.wbar[k * d.NC + cc] = 0.0;
+            a.wbar[k * d.NC + c0] += x;
+            a.wbar[k * d.NC + cj] += x;
+            a.wbar[k * d.NC + ci] += x;
+        }
+    }
+}
+
+// centre-coordinate bars per cell (ghost centres included) from the faces
+__global__ void __launch_bounds__(128) k_adj_centres(DevCtx d, AdjView a, int b) {
+    const DevBlock B = d.blk[b];
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)blockIdx.y;
+    if (i > B.ni + 1 || j > B.nj + 1) return;
+    const long long c = cidx(B, i, j);
+    a.cbar[c] = viscous_gather(d, B, a.vg_i, a.vg_j, 2, 0, i, j);
+    a.cbar[d.NC + c] = viscous_gather(d, B, a.vg_i, a.vg_j, 2, 1, i, j);
+}
+
+// ghost centres (k_geom_ghost: x(0) = 2 x(1) - x(2) per side; pass 1 the
+// i-sides, then pass 2 the j-sides over i = 0..ni+1) reversed: pass 2 first
+__global__ void k_adj_ghost_centres(DevCtx d, AdjView a, int b, int pass) {
+    const DevBlock B = d.blk[b];
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    auto fold = [&](long long g, long long c1, long long c2) {
+        for (int q = 0; q < 2; q++) {
+            double* cb = a.cbar + q * d.NC;
+            const double x = cb[g];
+            cb[g] = 0.0;
+            cb[c1] += 2.0 * x;
+            cb[c2] -= x;
+        }
+    };
+    if (pass == 2) {
+        const int i = t;
+        if (i > B.ni + 1) return;
+        fold(cidx(B, i, 0), cidx(B, i, 1), cidx(B, i, 2));
+        fold(cidx(B, i, B.nj + 1), cidx(B, i, B.nj), cidx(B, i, B.nj - 1));
+    } else {
+        const int j = t + 1;
+        if (j > B.nj) return;
+        fold(cidx(B, 0, j), cidx(B, 1, j), cidx(B, 2, j));
+        fold(cidx(B, B.ni + 1, j), cidx(B, B.ni, j), cidx(B, B.ni - 1, j));
     }
 }
 
@@ -184,7 +414,8 @@ __global__ void __launch_bounds__(128) k_adj_muscl(DevCtx d, AdjView a, int b) {
 // per position along the side, one workgroup so the fixed-state sums reduce
 // in a fixed order)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_adj_bc_side(DevCtx d, AdjView a, const int* row, double* poolbar) {
+__global__ void __launch_bounds__(256) k_adj_bc_side(DevCtx d, AdjView a, const int* row, const double* pool,
+                                                     double* poolbar) {
     __shared__ double red[256];
     const int b = row[0] - 1, side = row[1], kind = row[4], off = row[5];
     const DevBlock B = d.blk[b];
@@ -222,6 +453,20 @@ __global__ void __launch_bounds__(256) k_adj_bc_side(DevCtx d, AdjView a, const 
             gb[IW_V] = -gb[IW_V];
         } else if (kind == SHCFD_BC_OUTFLOW_P) {
             gb[IW_P] = 0.0;
+        } else if (kind == SHCFD_BC_INFLOW_TUV) {
+            // ghost (p/(R T), u, v, p, 1, kappa, cp T + (u^2 + v^2)/2, R, nu p/(R T)) with (T, u, v) from
+            // the pool and p from the cell
+            const double* pv = pool + off;
+            const double Tq = pv[0], uq = pv[1], vq = pv[2], rg = d.p.rgas, kap = d.p.kappa;
+            const double p = FLD(d, SHCFD_F_W, IW_P, ci), rt = rg * Tq;
+            const double rb = gb[IW_RHO] + B.nu * gb[D1::imu];  // bar of rho_ghost (mu = nu rho)
+            pacc[0] += -rb * p / (rt * Tq) + gb[D1::ih] * (kap / (kap - 1.0)) * rg;
+            pacc[1] += gb[IW_U] + gb[D1::ih] * uq;
+            pacc[2] += gb[IW_V] + gb[D1::ih] * vq;
+            const double pb = gb[IW_P] + rb / rt;
+#pragma unroll
+            for (int k = 0; k < D1::dw; k++) gb[k] = 0.0;
+            gb[IW_P] = pb;
         } else if (kind == SHCFD_BC_SLIP_NORMAL) {
             long long cf;
             double* nbar;
@@ -243,8 +488,9 @@ __global__ void __launch_bounds__(256) k_adj_bc_side(DevCtx d, AdjView a, const 
 #pragma unroll
         for (int k = 0; k < D1::dw; k++) a.wbar[k * d.NC + ci] += gb[k];
     }
-    if (kind != SHCFD_BC_FIXED) return;
-    for (int k = 0; k < D1::dw; k++) {
+    if (kind != SHCFD_BC_FIXED && kind != SHCFD_BC_INFLOW_TUV) return;
+    const int npv = kind == SHCFD_BC_FIXED ? D1::dw : 3;  // pool entries with bars
+    for (int k = 0; k < npv; k++) {
         red[threadIdx.x] = pacc[k];
         __syncthreads();
         for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -441,6 +687,17 @@ __global__ void __launch_bounds__(128) k_adj_nodes(DevCtx d, AdjView a, int b) {
                       gx, gy);
         sx -= gx; sy -= gy;
     }
+    if (d.p.viscous) {
+        // cell centres are the means of their four nodes: node (i, j) is a
+        // corner of cells (i..i+1, j..j+1)
+        for (int cj = j; cj <= j + 1; cj++)
+            for (int ci = i; ci <= i + 1; ci++)
+                if (interior(B, ci, cj)) {
+                    const long long cc = cidx(B, ci, cj);
+                    sx += 0.25 * a.cbar[cc];
+                    sy += 0.25 * a.cbar[d.NC + cc];
+                }
+    }
     const long long c = cidx(B, i, j);
     a.xbar[c] = sx;
     a.ybar[c] = sy;
@@ -453,7 +710,9 @@ __global__ void __launch_bounds__(256) k_adj_objective(DevCtx d, AdjView a, doub
     double s = 0.0;
     for (int i = a.obj_i0 + (int)threadIdx.x; i <= a.obj_i1; i += blockDim.x) {
         const long long c = cidx(B, i, 0);
-        s += FLD(d, SHCFD_F_DSJ, 0, c) * (FLD(d, SHCFD_F_TGJ, 1, c) * a.obj_ax + FLD(d, SHCFD_F_TGJ, 2, c) * a.obj_ay);
+        const double tu = FLD(d, SHCFD_F_TGJ, 1, c) - (d.p.viscous ? FLD(d, SHCFD_F_TGVJ, 1, c) : 0.0);
+        const double tv = FLD(d, SHCFD_F_TGJ, 2, c) - (d.p.viscous ? FLD(d, SHCFD_F_TGVJ, 2, c) : 0.0);
+        s += FLD(d, SHCFD_F_DSJ, 0, c) * (tu * a.obj_ax + tv * a.obj_ay);
     }
     red[threadIdx.x] = s;
     __syncthreads();

@@ -114,26 +114,26 @@ __global__ void __launch_bounds__(256) k_geojac(DevCtx d, const Tile* tiles) {
     const int i = t.i, j = t.j, ni = B.ni, nj = B.nj;
     const long long c = cidx(B, i, j);
     if (i <= ni && j >= 1 && j <= nj) {
-        double dxdxi = XC(i + 1, j) - XC(i, j);
-        double dydxi = RC(i + 1, j) - RC(i, j);
-        double dxdeta = (((XC(i + 1, j + 1) + XC(i, j + 1)) - XC(i + 1, j - 1)) - XC(i, j - 1)) * 0.25;
-        double dydeta = (((RC(i + 1, j + 1) + RC(i, j + 1)) - RC(i + 1, j - 1)) - RC(i, j - 1)) * 0.25;
-        double det = (dxdxi * dydeta) - (dydxi * dxdeta);
-        FLD(d, SHCFD_F_GEOJACI, 0, c) = dydeta / det;
-        FLD(d, SHCFD_F_GEOJACI, 2, c) = -dxdeta / det;
-        FLD(d, SHCFD_F_GEOJACI, 1, c) = -dydxi / det;
-        FLD(d, SHCFD_F_GEOJACI, 3, c) = dxdxi / det;
+        long long st[6];
+        viscous_stencil<0>(c, B.pitch, st);
+        double xc[6], rc[6], g[4];
+        for (int s = 0; s < 6; s++) {
+            xc[s] = FLD(d, SHCFD_F_X, 0, st[s]);
+            rc[s] = FLD(d, SHCFD_F_R, 0, st[s]);
+        }
+        geojac_t<0, double>(xc, rc, g);
+        for (int k = 0; k < 4; k++) FLD(d, SHCFD_F_GEOJACI, k, c) = g[k];
     }
     if (i >= 1 && i <= ni && j <= nj) {
-        double dxdxi = (((XC(i + 1, j + 1) + XC(i + 1, j)) - XC(i - 1, j + 1)) - XC(i - 1, j)) * 0.25;
-        double dydxi = (((RC(i + 1, j + 1) + RC(i + 1, j)) - RC(i - 1, j + 1)) - RC(i - 1, j)) * 0.25;
-        double dxdeta = XC(i, j + 1) - XC(i, j);
-        double dydeta = RC(i, j + 1) - RC(i, j);
-        double det = (dxdxi * dydeta) - (dydxi * dxdeta);
-        FLD(d, SHCFD_F_GEOJACJ, 0, c) = dydeta / det;
-        FLD(d, SHCFD_F_GEOJACJ, 2, c) = -dxdeta / det;
-        FLD(d, SHCFD_F_GEOJACJ, 1, c) = -dydxi / det;
-        FLD(d, SHCFD_F_GEOJACJ, 3, c) = dxdxi / det;
+        long long st[6];
+        viscous_stencil<1>(c, B.pitch, st);
+        double xc[6], rc[6], g[4];
+        for (int s = 0; s < 6; s++) {
+            xc[s] = FLD(d, SHCFD_F_X, 0, st[s]);
+            rc[s] = FLD(d, SHCFD_F_R, 0, st[s]);
+        }
+        geojac_t<1, double>(xc, rc, g);
+        for (int k = 0; k < 4; k++) FLD(d, SHCFD_F_GEOJACJ, k, c) = g[k];
     }
     if (d.p.dimension == SHCFD_DIM_AXISYMMETRIC && i >= 1 && i <= ni && j >= 1 && j <= nj) {
         // cell-centred metrics of the hoop stress (viscous/no-nV/q2d/sch_viscous.f90:10-28)

@@ -28,6 +28,16 @@ struct AdjView {
     double* lam2;      // lambda of the later step (4 per interior cell)
     double* xacc;      // node bars summed over the steps
     double* yacc;
+    // viscous (allocated when the flow is viscous, else null): per face and
+    // stencil position s (c, c1, a, b, e, f), the w bars (8 components per
+    // position, at (8 s + m) NC + face) and the centre-coordinate bars (at
+    // (2 s + {0: x, 1: r}) NC + face) of the face's viscous flux; and the
+    // centre-coordinate bars per cell (x at c, r at NC + c)
+    double* vw_i;
+    double* vw_j;
+    double* vg_i;
+    double* vg_j;
+    double* cbar;
 };
 
 struct Adjoint {

@@ -1,6 +1,6 @@
-// shcfd_adjoint.hip -- C-ABI of the discrete adjoint (inviscid, ideal gas, on
-// the optimised implicit path: steady LU-SGS, and the BDF2 dual-time steps of
-// the NR scheme with a fixed time step).  The reference has no adjoint; these
+// shcfd_adjoint.hip -- C-ABI of the discrete adjoint (ideal gas, inviscid or
+// viscous, on the optimised implicit path: steady LU-SGS, and the BDF2
+// dual-time steps of the NR scheme with a fixed time step).  The reference has no adjoint; these
 // entry points drive the reverse kernels of k_adjoint.hip and the transposed
 // sweeps of k_fast.hip.
 #include <cmath>
@@ -19,8 +19,12 @@ namespace shcfd {
 
 // k_adjoint.hip
 template <int DIR> __global__ void k_adj_face(DevCtx, AdjView, int);
+template <int DIR> __global__ void k_adj_vface(DevCtx, AdjView, int);
 __global__ void k_adj_muscl(DevCtx, AdjView, int);
-__global__ void k_adj_bc_side(DevCtx, AdjView, const int*, double*);
+__global__ void k_adj_bc_corner(DevCtx, AdjView);
+__global__ void k_adj_centres(DevCtx, AdjView, int);
+__global__ void k_adj_ghost_centres(DevCtx, AdjView, int, int);
+__global__ void k_adj_bc_side(DevCtx, AdjView, const int*, const double*, double*);
 __global__ void k_adj_bc_cut(DevCtx, AdjView, const int*);
 __global__ void k_adj_cell(DevCtx, AdjView, int);
 __global__ void k_adj_nodes(DevCtx, AdjView, int);
@@ -36,7 +40,8 @@ __global__ void k_adj_set_dt(DevCtx d, double dt) { d.scal[SCAL_DT_GRBL] = dt; }
 
 int Adjoint::init(Ctx& c) {
     const long long NC = c.NC;
-    const long long n = NC * (4 + 4 + 9 + 4 * 9 + 2 + 2 + 1 + 1 + 1 + 1 + 4 + 4 + 1 + 1);
+    const bool vis = c.p.viscous != 0;
+    const long long n = NC * (4 + 4 + 9 + 4 * 9 + 2 + 2 + 1 + 1 + 1 + 1 + 4 + 4 + 1 + 1 + (vis ? 2 * 48 + 2 * 12 + 2 : 0));
     const size_t npb = sizeof(double) * (c.npool > 0 ? c.npool : 1);
     npart = 256;
     if (hipMalloc((void**)&mem, sizeof(double) * n) != hipSuccess) return SHCFD_E_NOMEM;
@@ -63,6 +68,11 @@ int Adjoint::init(Ctx& c) {
     v.lam2 = take(4);
     v.xacc = take(1);
     v.yacc = take(1);
+    v.vw_i = vis ? take(48) : nullptr;
+    v.vw_j = vis ? take(48) : nullptr;
+    v.vg_i = vis ? take(12) : nullptr;
+    v.vg_j = vis ? take(12) : nullptr;
+    v.cbar = vis ? take(2) : nullptr;
     v.unsteady = 0;
     v.obj_block = -1;
     ready = true;
@@ -147,7 +157,7 @@ __global__ void __launch_bounds__(256) k_gm_combine(DevCtx d, int b, double* w, 
 namespace {
 
 int supported(shcfd_ctx* ctx) {
-    if (ctx->ny != 1 || ctx->p.viscous || ctx->p.thermal != SHCFD_THERMAL_IDEAL ||
+    if (ctx->ny != 1 || ctx->p.thermal != SHCFD_THERMAL_IDEAL ||
         ctx->p.dimension != SHCFD_DIM_PLANAR ||
         (ctx->p.scheme != SHCFD_SCHEME_LUSGS && ctx->p.scheme != SHCFD_SCHEME_NR &&
          ctx->p.scheme != SHCFD_SCHEME_DUAL))
@@ -161,6 +171,8 @@ int supported(shcfd_ctx* ctx) {
 int ensure(shcfd_ctx* ctx) {
     int rc = supported(ctx);
     if (rc) return rc;
+    // the viscous arrays exist only if the flow was viscous at set-up
+    if (ctx->adj && ctx->p.viscous && !ctx->adj->v.vw_i) adjoint_release(ctx);
     if (!ctx->adj) {
         ctx->adj = new (std::nothrow) Adjoint();
         if (!ctx->adj) return SHCFD_E_NOMEM;
@@ -170,13 +182,14 @@ int ensure(shcfd_ctx* ctx) {
     return SHCFD_OK;
 }
 
-// residual pipeline at the current state (face states, fluxes, time steps)
+// residual pipeline at the current state (face states, fluxes)
 int residual_state(shcfd_ctx* ctx) {
     int rc = shcfd_set_w(ctx);
     if (!rc) rc = shcfd_set_thermo_prop(ctx, 0);
     if (!rc) rc = shcfd_set_bc(ctx, 0);
     if (!rc) rc = shcfd_set_hv(ctx);
     if (!rc) rc = shcfd_set_tg(ctx);
+    if (!rc && ctx->p.viscous) rc = shcfd_set_tgv(ctx);
     return rc;
 }
 
@@ -185,12 +198,17 @@ void vjp_async(shcfd_ctx* ctx) {
     DevCtx d = ctx->dev();
     AdjView& a = ctx->adj->v;
     hipStream_t st = ctx->stream;
+    const bool vis = ctx->p.viscous != 0;
     for (int b = 0; b < ctx->nb; b++) {
         const DevBlock& B = ctx->hblk[b];
-        hipLaunchKernelGGL(k_adj_face<0>, dim3((unsigned)((B.ni + 1 + 127) / 128), (unsigned)B.nj), dim3(128), 0, st,
-                           d, a, b);
-        hipLaunchKernelGGL(k_adj_face<1>, dim3((unsigned)((B.ni + 127) / 128), (unsigned)(B.nj + 1)), dim3(128), 0, st,
-                           d, a, b);
+        const dim3 gi((unsigned)((B.ni + 1 + 127) / 128), (unsigned)B.nj), gj((unsigned)((B.ni + 127) / 128),
+                                                                                (unsigned)(B.nj + 1));
+        hipLaunchKernelGGL(k_adj_face<0>, gi, dim3(128), 0, st, d, a, b);
+        hipLaunchKernelGGL(k_adj_face<1>, gj, dim3(128), 0, st, d, a, b);
+        if (vis) {  // after k_adj_face: they add to its normal bars
+            hipLaunchKernelGGL(k_adj_vface<0>, gi, dim3(128), 0, st, d, a, b);
+            hipLaunchKernelGGL(k_adj_vface<1>, gj, dim3(128), 0, st, d, a, b);
+        }
     }
     for (int b = 0; b < ctx->nb; b++) {
         const DevBlock& B = ctx->hblk[b];
@@ -198,10 +216,11 @@ void vjp_async(shcfd_ctx* ctx) {
                            d, a, b);
     }
     hipMemsetAsync(ctx->adj->poolbar, 0, sizeof(double) * (ctx->npool > 0 ? ctx->npool : 1), st);
-    // reverse of the forward fill order: corners (unused by inviscid fluxes),
-    // side rows, cut rows
+    // reverse of the forward fill order: corners (read by the viscous
+    // fluxes only), side rows, cut rows
+    if (vis) hipLaunchKernelGGL(k_adj_bc_corner, dim3((unsigned)((ctx->nb + 63) / 64)), dim3(64), 0, st, d, a);
     for (int r = ctx->nbc - 1; r >= 0; r--)
-        hipLaunchKernelGGL(k_adj_bc_side, dim3(1), dim3(256), 0, st, d, a, ctx->dbc + SHCFD_BC_ROW * r,
+        hipLaunchKernelGGL(k_adj_bc_side, dim3(1), dim3(256), 0, st, d, a, ctx->dbc + SHCFD_BC_ROW * r, ctx->dpool,
                            ctx->adj->poolbar);
     for (int r = ctx->ncut - 1; r >= 0; r--) {
         const int w = ctx->hcut[SHCFD_CUT_ROW * r + 1];
@@ -212,6 +231,13 @@ void vjp_async(shcfd_ctx* ctx) {
     for (int b = 0; b < ctx->nb; b++) {
         const DevBlock& B = ctx->hblk[b];
         hipLaunchKernelGGL(k_adj_cell, dim3((unsigned)((B.ni + 127) / 128), (unsigned)B.nj), dim3(128), 0, st, d, a, b);
+        if (vis) {  // centre-coordinate bars, ghost centres folded back, before the nodes
+            hipLaunchKernelGGL(k_adj_centres, dim3((unsigned)((B.ni + 2 + 127) / 128), (unsigned)(B.nj + 2)), dim3(128),
+                               0, st, d, a, b);
+            hipLaunchKernelGGL(k_adj_ghost_centres, dim3((unsigned)((B.ni + 2 + 127) / 128)), dim3(128), 0, st, d, a, b,
+                               2);
+            hipLaunchKernelGGL(k_adj_ghost_centres, dim3((unsigned)((B.nj + 127) / 128)), dim3(128), 0, st, d, a, b, 1);
+        }
         hipLaunchKernelGGL(k_adj_nodes, dim3((unsigned)((B.ni + 1 + 127) / 128), (unsigned)(B.nj + 1)), dim3(128), 0,
                            st, d, a, b);
     }

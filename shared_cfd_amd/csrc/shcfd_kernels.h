@@ -112,90 +112,141 @@ __device__ inline void slau_face(const double* wl, const double* wr, double n1, 
 // Axisymmetric: the dilatation gains v/r with the face-averaged v over the
 // centre radius of cell (i,j), the lower-index cell of the face, as
 // viscous/no-nV/q2d/sch_viscous.f90:173-174 and :262-263 write it.
-template <int NY, int DIR>
-__device__ inline void viscous_face(const DevCtx& d, const DevBlock& B, int i, int j, double* out) {
+// Generic in the scalar type, in two parts: viscous_in_t forms the face
+// averages and the xi/eta differences of (u, v, T, Y) from the w vectors of
+// the stencil cells w[0..5] = c, c1 (the face's two cells), a, b, e, f --
+//   DIR 0: a = (i+1,j+1), b = (i,j+1), e = (i+1,j-1), f = (i,j-1);
+//   DIR 1: a = (i+1,j+1), b = (i+1,j), e = (i-1,j+1), f = (i-1,j) --
+// (xi differences c1 - c and eta differences (a + b - e - f)/4 for DIR 0,
+// the other way round for DIR 1); viscous_out_t forms the flux from them,
+// the face-averaged species enthalpies lhi, the face metrics g (geojac), the
+// unit normal (n1, n2) and, axisymmetric, the centre radius rc of c.
+template <int NY, typename T>
+struct ViscIn {
+    T rho, u, v, gamm, Rg, mu;  // face averages
+    T dxi[3 + NY], deta[3 + NY];  // differences of (u, v, T, Y_k) along xi and eta
+};
+template <int NY, int DIR, typename T>
+__device__ inline void viscous_in_t(const T* const* w, ViscIn<NY, T>& z) {
     constexpr int dq = NY + 3, ig = dq + 1, ir = dq + 3, imu = dq + 4;
-    const long long P = B.pitch;
-    const long long c = cidx(B, i, j);
-    const long long c1 = DIR == 0 ? c + 1 : c + P;
-    auto Wf = [&](int k, long long cc) { return FLD(d, SHCFD_F_W, k, cc); };
-    auto Td = [&](long long cc) { return (Wf(IW_P, cc) / Wf(IW_RHO, cc)) / Wf(ir, cc); };
+    const T *wc = w[0], *w1 = w[1], *wa = w[2], *wb = w[3], *we = w[4], *wf = w[5];
+    auto Td = [&](const T* x) { return (x[IW_P] / x[IW_RHO]) / x[ir]; };
+    z.rho = (wc[IW_RHO] + w1[IW_RHO]) * 0.5;
+    z.u = (wc[IW_U] + w1[IW_U]) * 0.5;
+    z.v = (wc[IW_V] + w1[IW_V]) * 0.5;
+    z.gamm = (wc[ig] + w1[ig]) * 0.5;
+    z.Rg = (wc[ir] + w1[ir]) * 0.5;
+    z.mu = (wc[imu] + w1[imu]) * 0.5;
+    T* dn = DIR == 0 ? z.dxi : z.deta;   // c1 - c
+    T* dc = DIR == 0 ? z.deta : z.dxi;   // (a + b - e - f) / 4
+    dn[0] = w1[IW_U] - wc[IW_U];
+    dn[1] = w1[IW_V] - wc[IW_V];
+    dn[2] = Td(w1) - Td(wc);
+#pragma unroll
+    for (int k = 0; k < NY; k++) dn[3 + k] = w1[IW_Y(k)] - wc[IW_Y(k)];
+    dc[0] = (((wa[IW_U] + wb[IW_U]) - we[IW_U]) - wf[IW_U]) * 0.25;
+    dc[1] = (((wa[IW_V] + wb[IW_V]) - we[IW_V]) - wf[IW_V]) * 0.25;
+    dc[2] = (((Td(wa) + Td(wb)) - Td(we)) - Td(wf)) * 0.25;
+#pragma unroll
+    for (int k = 0; k < NY; k++) dc[3 + k] = (((wa[IW_Y(k)] + wb[IW_Y(k)]) - we[IW_Y(k)]) - wf[IW_Y(k)]) * 0.25;
+}
+template <int NY, typename T>
+__device__ inline void viscous_out_t(const ViscIn<NY, T>& z, const T* lhi, const T* g, const T& n1, const T& n2,
+                                     bool axi, const T& rc, T* out) {
     const double Sc = 1.0, Pr = 1.0;
-    double rho = (Wf(IW_RHO, c) + Wf(IW_RHO, c1)) * 0.5;
-    double u = (Wf(IW_U, c) + Wf(IW_U, c1)) * 0.5;
-    double v = (Wf(IW_V, c) + Wf(IW_V, c1)) * 0.5;
-    double gamm = (Wf(ig, c) + Wf(ig, c1)) * 0.5;
-    double Rg = (Wf(ir, c) + Wf(ir, c1)) * 0.5;
-    double mu = (Wf(imu, c) + Wf(imu, c1)) * 0.5;
-    double lhi[NY], dwdxi[NY], dwdeta[NY], dwdx[NY], dwdr[NY];
-#pragma unroll
-    for (int k = 0; k < NY; k++) lhi[k] = (FLD(d, SHCFD_F_VHI, k, c) + FLD(d, SHCFD_F_VHI, k, c1)) * 0.5;
-    double D = (mu / rho) / Sc;
-    double k_heat = (((mu * gamm) / (gamm - 1.0)) * Rg) / Pr;
-    double dudxi, dvdxi, dTdxi, dudeta, dvdeta, dTdeta;
-    const double* gj;
-    int fg;
-    if (DIR == 0) {
-        // neighbours (i+1,j+1),(i,j+1),(i+1,j-1),(i,j-1)
-        long long a = c + 1 + P, bq = c + P, e = c + 1 - P, f = c - P;
-        dudxi = Wf(IW_U, c1) - Wf(IW_U, c);
-        dvdxi = Wf(IW_V, c1) - Wf(IW_V, c);
-        dTdxi = Td(c1) - Td(c);
-#pragma unroll
-        for (int k = 0; k < NY; k++) dwdxi[k] = Wf(IW_Y(k), c1) - Wf(IW_Y(k), c);
-        dudeta = (((Wf(IW_U, a) + Wf(IW_U, bq)) - Wf(IW_U, e)) - Wf(IW_U, f)) * 0.25;
-        dvdeta = (((Wf(IW_V, a) + Wf(IW_V, bq)) - Wf(IW_V, e)) - Wf(IW_V, f)) * 0.25;
-        dTdeta = (((Td(a) + Td(bq)) - Td(e)) - Td(f)) * 0.25;
-#pragma unroll
-        for (int k = 0; k < NY; k++)
-            dwdeta[k] = (((Wf(IW_Y(k), a) + Wf(IW_Y(k), bq)) - Wf(IW_Y(k), e)) - Wf(IW_Y(k), f)) * 0.25;
-        fg = SHCFD_F_GEOJACI;
-    } else {
-        // neighbours (i+1,j+1),(i+1,j),(i-1,j+1),(i-1,j)
-        long long a = c + 1 + P, bq = c + 1, e = c - 1 + P, f = c - 1;
-        dudxi = (((Wf(IW_U, a) + Wf(IW_U, bq)) - Wf(IW_U, e)) - Wf(IW_U, f)) * 0.25;
-        dvdxi = (((Wf(IW_V, a) + Wf(IW_V, bq)) - Wf(IW_V, e)) - Wf(IW_V, f)) * 0.25;
-        dTdxi = (((Td(a) + Td(bq)) - Td(e)) - Td(f)) * 0.25;
-#pragma unroll
-        for (int k = 0; k < NY; k++)
-            dwdxi[k] = (((Wf(IW_Y(k), a) + Wf(IW_Y(k), bq)) - Wf(IW_Y(k), e)) - Wf(IW_Y(k), f)) * 0.25;
-        dudeta = Wf(IW_U, c1) - Wf(IW_U, c);
-        dvdeta = Wf(IW_V, c1) - Wf(IW_V, c);
-        dTdeta = Td(c1) - Td(c);
-#pragma unroll
-        for (int k = 0; k < NY; k++) dwdeta[k] = Wf(IW_Y(k), c1) - Wf(IW_Y(k), c);
-        fg = SHCFD_F_GEOJACJ;
-    }
-    (void)gj;
-    double g11 = FLD(d, fg, 0, c), g21 = FLD(d, fg, 1, c), g12 = FLD(d, fg, 2, c), g22 = FLD(d, fg, 3, c);
-    double dudx = (dudxi * g11) + (dudeta * g21);
-    double dvdx = (dvdxi * g11) + (dvdeta * g21);
-    double dTdx = (dTdxi * g11) + (dTdeta * g21);
-    double dudr = (dudxi * g12) + (dudeta * g22);
-    double dvdr = (dvdxi * g12) + (dvdeta * g22);
-    double dTdr = (dTdxi * g12) + (dTdeta * g22);
-    double hidwdx = 0.0, hidwdr = 0.0;
+    const T rho = z.rho, u = z.u, v = z.v, gamm = z.gamm, Rg = z.Rg, mu = z.mu;
+    T D = (mu / rho) / Sc;
+    T k_heat = (((mu * gamm) / (gamm - 1.0)) * Rg) / Pr;
+    const T g11 = g[0], g21 = g[1], g12 = g[2], g22 = g[3];
+    T dudx = (z.dxi[0] * g11) + (z.deta[0] * g21);
+    T dvdx = (z.dxi[1] * g11) + (z.deta[1] * g21);
+    T dTdx = (z.dxi[2] * g11) + (z.deta[2] * g21);
+    T dudr = (z.dxi[0] * g12) + (z.deta[0] * g22);
+    T dvdr = (z.dxi[1] * g12) + (z.deta[1] * g22);
+    T dTdr = (z.dxi[2] * g12) + (z.deta[2] * g22);
+    T dwdx[NY], dwdr[NY];
+    T hidwdx = T(0.0), hidwdr = T(0.0);
 #pragma unroll
     for (int k = 0; k < NY; k++) {
-        dwdx[k] = (dwdxi[k] * g11) + (dwdeta[k] * g21);
+        dwdx[k] = (z.dxi[3 + k] * g11) + (z.deta[3 + k] * g21);
         hidwdx = hidwdx + (dwdx[k] * lhi[k]);
-        dwdr[k] = (dwdxi[k] * g12) + (dwdeta[k] * g22);
+        dwdr[k] = (z.dxi[3 + k] * g12) + (z.deta[3 + k] * g22);
         hidwdr = hidwdr + (dwdr[k] * lhi[k]);
     }
-    double divu = dudx + dvdr;
-    if (d.p.dimension == SHCFD_DIM_AXISYMMETRIC) divu = divu + (v / FLD(d, SHCFD_F_R, 0, c));
-    double t_xx = -(((2.0 / 3.0) * mu) * divu) + ((2.0 * mu) * dudx);
-    double t_rr = -(((2.0 / 3.0) * mu) * divu) + ((2.0 * mu) * dvdr);
-    double t_xr = mu * (dvdx + dudr);
-    double qx = -(k_heat * dTdx) - ((rho * D) * hidwdx);
-    double qr = -(k_heat * dTdr) - ((rho * D) * hidwdr);
-    const int fvn = DIR == 0 ? SHCFD_F_VNI : SHCFD_F_VNJ;
-    double n1 = FLD(d, fvn, 0, c), n2 = FLD(d, fvn, 1, c);
+    T divu = dudx + dvdr;
+    if (axi) divu = divu + (v / rc);
+    T t_xx = -(((2.0 / 3.0) * mu) * divu) + ((2.0 * mu) * dudx);
+    T t_rr = -(((2.0 / 3.0) * mu) * divu) + ((2.0 * mu) * dvdr);
+    T t_xr = mu * (dvdx + dudr);
+    T qx = -(k_heat * dTdx) - ((rho * D) * hidwdx);
+    T qr = -(k_heat * dTdr) - ((rho * D) * hidwdr);
 #pragma unroll
     for (int k = 0; k < NY; k++) out[k] = (((rho * D) * dwdx[k]) * n1) + (((rho * D) * dwdr[k]) * n2);
     out[NY] = (t_xx * n1) + (t_xr * n2);
     out[NY + 1] = (t_xr * n1) + (t_rr * n2);
     out[NY + 2] = ((((u * t_xx) + (v * t_xr)) - qx) * n1) + ((((u * t_xr) + (v * t_rr)) - qr) * n2);
+}
+
+// face metrics (geojaci / geojacj) from the centre coordinates of the same
+// stencil (c, c1, a, b, e, f): viscosity/viscous/no-nV/2d/sch_viscous.f90:1-54
+template <int DIR, typename T>
+__device__ inline void geojac_t(const T* xc, const T* rc, T* g) {
+    T dxdxi, dydxi, dxdeta, dydeta;
+    if (DIR == 0) {
+        dxdxi = xc[1] - xc[0];
+        dydxi = rc[1] - rc[0];
+        dxdeta = (((xc[2] + xc[3]) - xc[4]) - xc[5]) * 0.25;
+        dydeta = (((rc[2] + rc[3]) - rc[4]) - rc[5]) * 0.25;
+    } else {
+        dxdxi = (((xc[2] + xc[3]) - xc[4]) - xc[5]) * 0.25;
+        dydxi = (((rc[2] + rc[3]) - rc[4]) - rc[5]) * 0.25;
+        dxdeta = xc[1] - xc[0];
+        dydeta = rc[1] - rc[0];
+    }
+    T det = (dxdxi * dydeta) - (dydxi * dxdeta);
+    g[0] = dydeta / det;
+    g[2] = -dxdeta / det;
+    g[1] = -dydxi / det;
+    g[3] = dxdxi / det;
+}
+
+// stencil cells of a viscous face in viscous_in_t's order (c, c1, a, b, e, f)
+template <int DIR>
+__device__ inline void viscous_stencil(long long c, long long P, long long* s) {
+    s[0] = c;
+    s[1] = DIR == 0 ? c + 1 : c + P;
+    s[2] = c + 1 + P;
+    s[3] = DIR == 0 ? c + P : c + 1;
+    s[4] = DIR == 0 ? c + 1 - P : c - 1 + P;
+    s[5] = DIR == 0 ? c - P : c - 1;
+}
+
+// the primal (double) instance, reading the fields
+template <int NY, int DIR>
+__device__ inline void viscous_face(const DevCtx& d, const DevBlock& B, int i, int j, double* out) {
+    constexpr int dw = NY + 8;
+    const long long c = cidx(B, i, j);
+    long long st[6];
+    viscous_stencil<DIR>(c, B.pitch, st);
+    double ws[6][dw];
+    const double* wp[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+#pragma unroll
+        for (int k = 0; k < dw; k++) ws[s][k] = FLD(d, SHCFD_F_W, k, st[s]);
+        wp[s] = ws[s];
+    }
+    double lhi[NY];
+#pragma unroll
+    for (int k = 0; k < NY; k++) lhi[k] = (FLD(d, SHCFD_F_VHI, k, c) + FLD(d, SHCFD_F_VHI, k, st[1])) * 0.5;
+    const int fg = DIR == 0 ? SHCFD_F_GEOJACI : SHCFD_F_GEOJACJ;
+    const double g[4] = {FLD(d, fg, 0, c), FLD(d, fg, 1, c), FLD(d, fg, 2, c), FLD(d, fg, 3, c)};
+    const int fvn = DIR == 0 ? SHCFD_F_VNI : SHCFD_F_VNJ;
+    const bool axi = d.p.dimension == SHCFD_DIM_AXISYMMETRIC;
+    ViscIn<NY, double> z;
+    viscous_in_t<NY, DIR, double>(wp, z);
+    viscous_out_t<NY, double>(z, lhi, g, FLD(d, fvn, 0, c), FLD(d, fvn, 1, c), axi,
+                              axi ? FLD(d, SHCFD_F_R, 0, c) : 1.0, out);
 }
 
 // radial-momentum source Sq + Svq of an axisymmetric cell: the pressure

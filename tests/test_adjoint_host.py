@@ -1,6 +1,8 @@
 """CPU checks of the adjoint's reference: the float64 torch residual
 (oracle/torch_residual.py) against the C oracle's fluxes and face metrics on
-a perturbed C-mesh state, and its own autograd against finite differences."""
+a perturbed C-mesh state, inviscid and laminar (no-slip wall, viscous
+fluxes from the cell-centre metrics, corner ghosts), and its own autograd
+against finite differences."""
 import numpy as np
 import pytest
 import torch
@@ -10,8 +12,8 @@ from oracle import torch_residual as TR
 from shared_cfd_amd import airfoil as A
 
 
-def perturbed_case(seed=0, ni=48, nj=12, nwake=8, order=2):
-    ac = A.naca_cmesh(ni=ni, nj=nj, nwake=nwake, mach=0.5, order=order)
+def perturbed_case(seed=0, ni=48, nj=12, nwake=8, order=2, reynolds=None):
+    ac = A.naca_cmesh(ni=ni, nj=nj, nwake=nwake, mach=0.5, order=order, reynolds=reynolds)
     rng = np.random.default_rng(seed)
     q = ac.case.q0[0].copy()
     q[2:nj + 2, 2:ni + 2] *= 1.0 + 0.02 * rng.standard_normal((nj, ni, 4))
@@ -29,29 +31,37 @@ def torch_inputs(ac):
     return q, x, y, pool
 
 
-@pytest.mark.parametrize("order", [2, 1])
-def test_torch_fluxes_match_the_oracle(order):
-    ac = perturbed_case(order=order)
+@pytest.mark.parametrize("order,reynolds", [(2, None), (1, None), (2, 200.0), (1, 200.0)])
+def test_torch_fluxes_match_the_oracle(order, reynolds):
+    ac = perturbed_case(order=order, reynolds=reynolds)
     if order == 1:
         ac.case.params.high_order = 0
     c = ac.case
     ni, nj = c.ni[0], c.nj[0]
     o = Oracle(c)
     o.set_w(); o.set_thermo_prop(0); o.set_bc(0); o.set_hv(); o.set_tg()
+    if reynolds:
+        o.set_tgv()
     q, x, y, pool = torch_inputs(ac)
     with torch.no_grad():
         R, TGi, TGj, _ = TR.residual(q, x, y, pool, c)
         dsi, vni, dsj, vnj = TR.face_metrics(x, y)
     tgi_o = o.field("TGI", 0)[2:nj + 2, 1:ni + 2]
     tgj_o = o.field("TGJ", 0)[1:nj + 2, 2:ni + 2]
+    if reynolds:
+        # the viscous part is a sizeable share of the net flux
+        tvi, tvj = o.field("TGVI", 0)[2:nj + 2, 1:ni + 2], o.field("TGVJ", 0)[1:nj + 2, 2:ni + 2]
+        assert np.max(np.abs(tvj[..., 1:3])) > 1e-3 * np.max(np.abs(tgj_o[..., 1:3]))
+        tgi_o, tgj_o = tgi_o - tvi, tgj_o - tvj
     assert np.max(np.abs(TGi.numpy() - tgi_o)) <= 1e-12 * np.max(np.abs(tgi_o))
     assert np.max(np.abs(TGj.numpy() - tgj_o)) <= 1e-12 * np.max(np.abs(tgj_o))
     assert np.allclose(dsi.numpy().T, o.field("DSI", 0)[2:nj + 2, 1:ni + 2, 0], rtol=1e-14, atol=0)
     assert np.allclose(vnj.numpy().transpose(1, 0, 2), o.field("VNJ", 0)[1:nj + 2, 2:ni + 2], rtol=1e-13, atol=1e-15)
 
 
-def test_torch_autograd_matches_finite_differences():
-    ac = perturbed_case(seed=1, ni=32, nj=8, nwake=6)
+@pytest.mark.parametrize("reynolds", [None, 200.0])
+def test_torch_autograd_matches_finite_differences(reynolds):
+    ac = perturbed_case(seed=1, ni=32, nj=8, nwake=6, reynolds=reynolds)
     c = ac.case
     i0, i1 = ac.wall_range()
     obj = (i0, i1, 0.3, -0.7)

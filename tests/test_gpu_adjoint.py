@@ -1,7 +1,7 @@
 """Discrete adjoint on the GPU: the hand-written reverse kernels against
 torch autograd of the float64 reference residual (oracle/torch_residual.py),
-the transposed LU-SGS solve, and shape/angle gradients against central
-finite differences of converged solutions."""
+inviscid and laminar, the transposed LU-SGS and GMRES solves, and shape/angle
+gradients against central finite differences of converged solutions."""
 import numpy as np
 import pytest
 import torch
@@ -13,8 +13,8 @@ from shared_cfd_amd.native import ADJ_QBAR, ADJ_XBAR, Solver
 pytestmark = pytest.mark.gpu
 
 
-def perturbed(seed, ni=48, nj=12, nwake=8, order=2):
-    ac = A.naca_cmesh(ni=ni, nj=nj, nwake=nwake, mach=0.5, order=order)
+def perturbed(seed, ni=48, nj=12, nwake=8, order=2, reynolds=None):
+    ac = A.naca_cmesh(ni=ni, nj=nj, nwake=nwake, mach=0.5, order=order, reynolds=reynolds)
     if order == 1:
         ac.case.params.high_order = 0
     rng = np.random.default_rng(seed)
@@ -24,9 +24,11 @@ def perturbed(seed, ni=48, nj=12, nwake=8, order=2):
     return ac
 
 
-@pytest.mark.parametrize("order", [2, 1])
-def test_vjp_matches_torch_autograd(order):
-    ac = perturbed(11, order=order)
+@pytest.mark.parametrize("order,reynolds", [(2, None), (1, None), (2, 200.0), (1, 200.0)])
+def test_vjp_matches_torch_autograd(order, reynolds):
+    """Laminar cases add the viscous fluxes (no-slip wall, corner ghosts,
+    face metrics from the cell centres) to the reverse pass."""
+    ac = perturbed(11, order=order, reynolds=reynolds)
     c = ac.case
     ni, nj = c.ni[0], c.nj[0]
     i0, i1 = ac.wall_range()
@@ -99,6 +101,37 @@ def test_gradients_match_central_differences():
         pm[k] -= h
         fd = (cl(pp) - cl(pm)) / (2 * h)
         assert abs(fd - g.dJ_dp[k]) <= 1e-6 * abs(fd), (k, fd, g.dJ_dp[k])
+    fd = (cl(p0, h) - cl(p0, -h)) / (2 * h)
+    assert abs(fd - g.dJ_dalpha) <= 1e-6 * abs(fd), (fd, g.dJ_dalpha)
+
+
+def test_laminar_gradients_match_central_differences():
+    """Laminar flow (Re 500 on the chord, no-slip wall): dC_L/dp and
+    dC_L/dalpha through the viscous reverse kernels against central
+    differences (h = 1e-7) of converged solutions."""
+    from shared_cfd_amd.adjoint import AirfoilFlow
+    mesh = dict(ni=96, nj=24, nwake=12, mach=0.5, cfl=1.0, order=1, reynolds=500.0, first_height=1e-2)
+    p0 = np.zeros(64)
+    f = AirfoilFlow(amplitudes=p0, **mesh)
+    pr = f.converge(rtol=1e-12, max_steps=300000)
+    assert pr.rms <= 1e-12 * pr.rms0
+    g = f.gradient(1.0, 0.0, rtol=1e-12, max_iter=300000)
+    assert g.res <= 1e-12 * g.res0
+    f.close()
+    h = 1e-7
+
+    def cl(p, dalpha=0.0):
+        ff = AirfoilFlow(amplitudes=p, alpha_deg=1.25 + np.rad2deg(dalpha), **mesh)
+        ff.converge(rtol=1e-12, max_steps=300000)
+        v = ff.coefficients()[0]
+        ff.close()
+        return v
+
+    pp, pm = p0.copy(), p0.copy()
+    pp[20] += h
+    pm[20] -= h
+    fd = (cl(pp) - cl(pm)) / (2 * h)
+    assert abs(fd - g.dJ_dp[20]) <= 1e-6 * abs(fd), (fd, g.dJ_dp[20])
     fd = (cl(p0, h) - cl(p0, -h)) / (2 * h)
     assert abs(fd - g.dJ_dalpha) <= 1e-6 * abs(fd), (fd, g.dJ_dalpha)
 
