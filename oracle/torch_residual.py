@@ -10,7 +10,8 @@ product.  It restates, for one block with nY = 1:
   therm_lib/ideal/thermal_model.raw.f90:7-33),
 * ghost fills: cut rows (cond/no-nV/condition.tail.f90:2-70, the cut_copro
   row format), side rows (zero gradient, curved slip wall, no-slip wall,
-  fixed state), then the corners (condition.tail.f90:331-356),
+  fixed state, inflow of given T, u, v), then the corners
+  (condition.tail.f90:331-356),
 * MUSCL face states with the reference limiter (high_order/muscl/muscl.f90:7-73),
 * the SLAU flux (core/scheme.f90:71-153),
 * when viscous: cell centres with extrapolated ghost centres and the face
@@ -85,7 +86,7 @@ def _cut_cells(t, inc, copy):
     raise ValueError(geo)
 
 
-def ghosted_w(w_int, ni, nj, cut_rows, bc_rows, pool, vni, vnj):
+def ghosted_w(w_int, ni, nj, cut_rows, bc_rows, pool, vni, vnj, kap=1.4, rg=1.0, nu=0.0):
     """Interior w (nj, ni, 10) -> w with 2 ghost layers (nj+4, ni+4, 10).
     Single block; cut rows first, then side rows (a later fill of the same
     ghost replaces an earlier one).  Built out of place for autograd."""
@@ -115,6 +116,15 @@ def ghosted_w(w_int, ni, nj, cut_rows, bc_rows, pool, vni, vnj):
                 ii, jj, g1, g2 = tpos, nj, (tpos, nj + 1), (tpos, nj + 2)
             if kind == BC_FIXED:
                 wv = pool[off:off + 10]
+            elif kind == BC_INFLOW_TUV:
+                # T, u, v given, p from the cell (ideal gas)
+                T, u, v = pool[off], pool[off + 1], pool[off + 2]
+                p_in = W0[at(ii, jj)][3]
+                rho = p_in / (rg * T)
+                cp = (kap / (kap - 1.0)) * rg
+                one = torch.ones_like(rho)
+                wv = torch.stack([rho, u * one, v * one, p_in, one, kap * one, cp * T + 0.5 * (u * u + v * v),
+                                  rg * one, nu * rho, cp * T * one])
             else:
                 wv = W0[at(ii, jj)]
                 if kind == BC_NOSLIP:
@@ -279,7 +289,7 @@ def residual(q, x, y, pool, case, obj=None):
     ni, nj = case.ni[0], case.nj[0]
     dsi, vni, dsj, vnj = face_metrics(x, y)
     w = ideal_w(q, p.kappa, p.rgas, p.nu)
-    W = ghosted_w(w, ni, nj, case.cut_rows, case.bc_rows, pool, vni, vnj)
+    W = ghosted_w(w, ni, nj, case.cut_rows, case.bc_rows, pool, vni, vnj, p.kappa, p.rgas, p.nu)
     hp = p.phi_mus * 0.5
     # i-faces: rows j = 1..nj, faces i = 0..ni
     Wi = W[2:nj + 2]

@@ -13,8 +13,18 @@ from shared_cfd_amd.native import ADJ_QBAR, ADJ_XBAR, Solver
 pytestmark = pytest.mark.gpu
 
 
-def perturbed(seed, ni=48, nj=12, nwake=8, order=2, reynolds=None):
+def perturbed(seed, ni=48, nj=12, nwake=8, order=2, reynolds=None, tuv=False):
     ac = A.naca_cmesh(ni=ni, nj=nj, nwake=nwake, mach=0.5, order=order, reynolds=reynolds)
+    if tuv:
+        # far field as an inflow of given (T, u, v) with p from the cell
+        from shared_cfd_amd import cases as C
+        pool = ac.case.bc_pool
+        T_inf = pool[3] / (pool[0] * pool[7])
+        ac.case.bc_pool = np.concatenate([pool, [T_inf, pool[1] * 1.01, pool[2] + 0.01]])
+        rows = ac.case.bc_rows.copy()
+        k = [r for r in range(len(rows)) if rows[r][1] == C.SIDE_JMAX][0]
+        rows[k][4], rows[k][5] = C.BC_INFLOW_TUV, len(pool)
+        ac.case.bc_rows = rows
     if order == 1:
         ac.case.params.high_order = 0
     rng = np.random.default_rng(seed)
@@ -24,11 +34,13 @@ def perturbed(seed, ni=48, nj=12, nwake=8, order=2, reynolds=None):
     return ac
 
 
-@pytest.mark.parametrize("order,reynolds", [(2, None), (1, None), (2, 200.0), (1, 200.0)])
-def test_vjp_matches_torch_autograd(order, reynolds):
+@pytest.mark.parametrize("order,reynolds,tuv", [(2, None, False), (1, None, False), (2, 200.0, False),
+                                              (1, 200.0, False), (2, 200.0, True)])
+def test_vjp_matches_torch_autograd(order, reynolds, tuv):
     """Laminar cases add the viscous fluxes (no-slip wall, corner ghosts,
-    face metrics from the cell centres) to the reverse pass."""
-    ac = perturbed(11, order=order, reynolds=reynolds)
+    face metrics from the cell centres) to the reverse pass; tuv makes the
+    far field an inflow of given T, u, v (pool bars of those three)."""
+    ac = perturbed(11, order=order, reynolds=reynolds, tuv=tuv)
     c = ac.case
     ni, nj = c.ni[0], c.nj[0]
     i0, i1 = ac.wall_range()
@@ -60,8 +72,13 @@ def test_vjp_matches_torch_autograd(order, reynolds):
     scale = max(np.max(np.abs(gx)), np.max(np.abs(gy)))
     assert np.max(np.abs(xb[..., 0] - gx)) <= 1e-10 * scale
     assert np.max(np.abs(xb[..., 1] - gy)) <= 1e-10 * scale
-    gp = pool.grad.numpy()[:9]
-    assert np.max(np.abs(pbar[:9] - gp)) <= 1e-10 * max(np.max(np.abs(gp)), 1e-300)
+    # every pool entry but the fixed state's species enthalpy (entry 9), which
+    # the reverse pass holds constant (it multiplies the gradient of Y = 1)
+    keep = [k for k in range(c.bc_pool.size) if k != 9]
+    gp = pool.grad.numpy()[keep]
+    assert np.max(np.abs(pbar[keep] - gp)) <= 1e-10 * max(np.max(np.abs(gp)), 1e-300)
+    if tuv:
+        assert np.all(np.abs(gp[-3:]) > 0)
 
 
 def test_gradients_match_central_differences():
