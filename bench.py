@@ -425,6 +425,9 @@ def main(argv=None):
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of this workload "
                                   "(profiles/forward_step_258k_traffic.json), not measured in this run",
+                "traffic_note": ("the forward sweep launch also forms the backward operators (HA, HB: 256 B per "
+                                 "cell written) on the CUs its strips leave idle; traffic counts both, the "
+                                 "algorithmic bytes only the sweep") if dominant == "SWEEP_FWD" else None,
                 "kernel": dominant, "avg_launch_us": round(avg_s * 1e6, 3), "launches_timed": n_dom,
                 "alg_bytes_per_launch": bytes_launch,
                 "share_of_step": {k: round(v[0] / sum(x[0] for x in prof_all.values()), 4) for k, v in prof_all.items()}}
