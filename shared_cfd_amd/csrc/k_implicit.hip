@@ -405,7 +405,20 @@ __global__ void __launch_bounds__(256) k_update(DevCtx d, const Tile* tiles) {
     for (int k = 0; k < NY + 3; k++) FLD(d, SHCFD_F_Q, k, c) = FLD(d, SHCFD_F_Q, k, c) + om * FLD(d, SHCFD_F_DQ, k, c);
 }
 
+// q += dq over the active blocks' cells (LU-SGS after the Jacobian-free
+// backward sweep: the same addition time_lusgs_viscous.f90 makes per cell)
+template <int NY>
+__global__ void __launch_bounds__(256) k_add_dq(DevCtx d, const Tile* tiles) {
+    TileCoord t = tile_coord(tiles);
+    const DevBlock B = d.blk[t.b];
+    if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj || !B.active) return;
+    const long long c = cidx(B, t.i, t.j);
+#pragma unroll
+    for (int k = 0; k < NY + 3; k++) FLD(d, SHCFD_F_Q, k, c) = FLD(d, SHCFD_F_Q, k, c) + FLD(d, SHCFD_F_DQ, k, c);
+}
+
 #define INST(NY)                                                                                               \
+    template __global__ void k_add_dq<NY>(DevCtx, const Tile*);                                                \
     template __global__ void k_rhs<NY>(DevCtx, const Tile*, int*, int, int, int);                                        \
     template __global__ void k_sweep_fwd_plain<NY, false>(DevCtx, const Strip*, int*, double*, long long, long long, \
                                                           const double*, long long, const long long*);          \

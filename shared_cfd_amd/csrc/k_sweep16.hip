@@ -530,8 +530,11 @@ k_sweep16j_fwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long 
     const int cp_lane = (w * 64 + L) * 16;
     const __amdgpu_buffer_rsrc_t vsc = jview(jsc + S.base * 16 * JSC, (long long)nsteps * SC_STEP);
     const int sc_lane = (r * JSC + 2 * (carrier ? sp_slot : 0)) * 8;
-    const __amdgpu_buffer_rsrc_t vdq = jview(d.f[SHCFD_F_DQ] + (long long)kk * d.NC, (long long)d.NC * 8);
-    const int dq_lane = (int)((cidx(B, 0, rowok ? j : nj) + 1 - r) * 8);  // the cell of step t at + 8 t
+    // one view of all dq components (uniform across the wave: a per-lane base
+    // would make every access a loop over the lanes' distinct bases); the
+    // lane's component and cell in the lane offset
+    const __amdgpu_buffer_rsrc_t vdq = jview(d.f[SHCFD_F_DQ], (long long)dq * d.NC * 8);
+    const int dq_lane = (int)((kk * d.NC + cidx(B, 0, rowok ? j : nj) + 1 - r) * 8);  // the cell of step t at + 8 t
     auto load = [&](int t, JStepIn& in) {
         const int tc = t < nsteps ? t : nsteps - 1;  // past the end: re-read the last step
         const int tn = t + 1 < nsteps ? t + 1 : nsteps - 1;
@@ -606,7 +609,9 @@ k_sweep16j_fwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long 
 }
 
 // backward sweep, Jacobian-free: x = x* - alpha (A-(i+1) x(i+1) dsci + B-(j+1) x(j+1) dscj),
-// then q += x (LU-SGS) or the (res, omega) partials (NR)
+// and the (res, omega) partials (NR); LU-SGS's q += x is k_add_dq's pass
+// after the sweep (the scattered q accesses of each step, 16 rows x 4
+// components of 8 bytes, cost more than the pass)
 template <int NY>
 __global__ void __launch_bounds__(NY + 3 <= 16 ? 256 : 320) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_sweep16j_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long long hstride, double* part,
@@ -626,7 +631,6 @@ k_sweep16j_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long 
     const bool has_prev = S.prev >= 0;
     const bool mine = r == 15 && kok;
     const bool inner = inner_loop_scheme(d.p.scheme);
-    const bool upd = !inner && B.active;
     const double dqmax = d.p.dqmax;
     const int nsteps = ni + 15;
     const int sp_slot = 4 * w + (L & 3);
@@ -643,16 +647,18 @@ k_sweep16j_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long 
     const int cp_lane = (w * 64 + L) * 16;
     const __amdgpu_buffer_rsrc_t vsc = jview(jsc + S.base * 16 * JSC, (long long)nsteps * SC_STEP);
     const int sc_lane = (r * JSC + 2 * (carrier ? sp_slot : 0)) * 8;
-    const __amdgpu_buffer_rsrc_t vdq = jview(d.f[SHCFD_F_DQ] + (long long)kk * d.NC, (long long)d.NC * 8);
-    const __amdgpu_buffer_rsrc_t vq = jview(d.f[SHCFD_F_Q] + (long long)kk * d.NC, (long long)d.NC * 8);
-    const int c_lane = (int)((cidx(B, 0, rowok ? j : nj) + 1 - r) * 8);   // the cell of step t at + 8 (ni + 14 - t)
+    // views of all dq and q components, as in the forward sweep (q read for
+    // the omega limit of the inner-loop schemes only)
+    const __amdgpu_buffer_rsrc_t vdq = jview(d.f[SHCFD_F_DQ], (long long)dq * d.NC * 8);
+    const __amdgpu_buffer_rsrc_t vq = jview(d.f[SHCFD_F_Q], (long long)dq * d.NC * 8);
+    const int c_lane = (int)((kk * d.NC + cidx(B, 0, rowok ? j : nj) + 1 - r) * 8);  // the cell of step t at + 8 (ni + 14 - t)
     auto load = [&](int t, JStepIn& in) {
         const int tc = t < nsteps ? t : nsteps - 1;
         const int tn = t + 1 < nsteps ? t + 1 : nsteps - 1;
         in.cp = jv_ld2(vcp, cp_lane, (ni + 14 - tc) * CP_STEP);
         in.sp = jv_ld2(vsc, sc_lane, (ni + 14 - tn) * SC_STEP);
         in.r = jv_ld(vdq, c_lane, (ni + 14 - tc) * 8);
-        in.q = jv_ld(vq, c_lane, (ni + 14 - tc) * 8);
+        in.q = jv_ld(vq, inner ? c_lane : JV_DROP, (ni + 14 - tc) * 8);  // omega's only
     };
     {
         const double2 v0 = jv_ld2(vsc, sc_lane, (ni + 14) * SC_STEP);
@@ -693,9 +699,7 @@ k_sweep16j_bwd(DevCtx d, const Strip16* strips, int nstrips, double* hand, long 
                 if (k == NY + 2) res = res + x * x;
                 if (k < NY && x < 0.0) omega = dmin(omega, (-in.q / x) * dqmax);
             }
-            const int so = (ni + 14 - t) * 8;
-            jv_st(vq, in.q + x, (live && upd) ? c_lane : JV_DROP, so);
-            jv_st(vdq, x, live ? c_lane : JV_DROP, so);
+            jv_st(vdq, x, live ? c_lane : JV_DROP, (ni + 14 - t) * 8);
             xs[u & 1][r][k] = x;
             const double ps = quad_sum(k < NY ? x : 0.0);
             const double pd = quad_sum(in.cp.y * x);

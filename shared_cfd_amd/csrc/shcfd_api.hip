@@ -817,7 +817,11 @@ static bool skew_enabled(shcfd_ctx* ctx) {
 // the 16-row sweeps apply the split Jacobians in closed form (no operator
 // copy) for the schemes whose A+-, B+- are the unpreconditioned ones
 static bool jvp16_scheme(const shcfd_ctx* ctx) {
-    return ctx->use_jvp16 && (ctx->p.scheme == SHCFD_SCHEME_LUSGS || ctx->p.scheme == SHCFD_SCHEME_NR);
+    // the Jacobian-free sweeps address the dq and q fields through one buffer
+    // view of all their components, whose size must stay below their drop
+    // offset (2^31 bytes)
+    const bool fits = (long long)(ctx->ny + 3) * ctx->NC * 8 < 0x7FF00000LL;
+    return ctx->use_jvp16 && fits && (ctx->p.scheme == SHCFD_SCHEME_LUSGS || ctx->p.scheme == SHCFD_SCHEME_NR);
 }
 
 static bool q16_layout(shcfd_ctx* ctx);
@@ -1049,6 +1053,10 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
             DISPATCH_NY(ctx->ny, hipLaunchKernelGGL(k_sweep16_bwd<NY>, dim3(ns16), blk, 0, ctx->stream, d,
                                                     ctx->dstrips16, ns16, hand_b, ctx->bstride16, ctx->dpart,
                                                     ctx->spin_limit, ctx->djac16, ctx->nsl16)) }
+        if (jd16 && !inner_loop_scheme(ctx->p.scheme)) {
+            ProfScope ps(ctx, PROF_UPDATE);
+            DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_add_dq<NY>));
+        }
         if (inner_loop_scheme(ctx->p.scheme)) {
             { ProfScope ps(ctx, PROF_REDUCE);
             hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, ctx->stream, d, ctx->dpart, ns16); }

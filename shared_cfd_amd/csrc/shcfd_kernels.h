@@ -548,6 +548,7 @@ template <int NY, bool SKEW>
 __global__ void k_sweep_bwd_plain(DevCtx, const Strip*, int, int*, double*, long long, double*, long long,
                                   const double*, long long, const long long*);
 template <int NY> __global__ void k_update(DevCtx, const Tile*);
+template <int NY> __global__ void k_add_dq(DevCtx, const Tile*);
 __global__ void k_reduce_res_omega(DevCtx, const double*, int);
 // quad-row sweeps for nY > 1 (k_sweep16.hip; operators from k_set_abpm16)
 template <int NY> __global__ void k_set_abpm16(DevCtx, const Strip16*, double*, long long);
