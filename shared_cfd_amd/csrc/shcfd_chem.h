@@ -924,9 +924,12 @@ SHCFD_HD int point_implicit(const shcfd_mech* m, double T, double (&vrho)[NS], d
             return SHCFD_E_CHEM;
         }
         it = it + 1;
-        double f[N], J[N][N], dJ[N];
-        fex<NS>(m, y, f);
-        jex<NS>(m, y, J, dJ);
+        // F(y) is the dn/dt Jex forms beside the Jacobian: the rates of Fex
+        // (FJ.f90:152-300) in the same order, equal to Fex's result in every
+        // bit for a positive total concentration (y is floored at N_EPS), so
+        // each rate is evaluated once per iteration
+        double f[N], J[N][N];
+        jex<NS>(m, y, J, f);
 #pragma unroll
         for (int a = 0; a < N; a++) {
 #pragma unroll
@@ -1052,9 +1055,8 @@ __device__ int point_implicit_lds(const shcfd_mech* m, double T, double (&vrho)[
             return SHCFD_E_CHEM;
         }
         it = it + 1;
-        double f[N], dJ[N];
-        fex<NS>(m, y, f);
-        jex_impl<NS>(m, y, J, dJ);
+        double f[N];
+        jex_impl<NS>(m, y, J, f);  // F(y) beside J, as in point_implicit
         for (int a = 0; a < N; a++) {
             for (int b = 0; b < N; b++) J.set(a, b, -(dt * J.get(a, b)));
             J.set(a, a, J.get(a, a) + 1.0);

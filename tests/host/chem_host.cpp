@@ -1,5 +1,6 @@
-// Host build of the library's per-cell chemistry integrators (shcfd_chem.h,
-// shcfd_dvode.h): CHEMEQ2 with its DVODE hand-over and the DVODE reaction,
+// Host build of the library's per-cell chemistry (shcfd_chem.h, shcfd_dvode.h):
+// Fex beside the dn/dt of Jex, and the integrators CHEMEQ2 with its DVODE
+// hand-over and the DVODE reaction,
 // with the 0-D harness's rows (in: vrho(ns), T; out: vrho(ns); status:
 // attempts or DVODE steps, or -code).  The GPU tests check that the device
 // build of the same code gives the same bits.  Test infrastructure only.
@@ -26,7 +27,32 @@ void rows(int op, const shcfd_mech* m, int n, const double* in, double* out, int
     }
 }
 
+// f of Fex and the dn/dt that Jex forms beside the Jacobian, per state (in:
+// n(ns) mol/cm^3, T)
+template <int NS>
+void fj_rows(const shcfd_mech* m, int n, const double* in, double* f, double* dndt) {
+    for (int q = 0; q < n; q++) {
+        double y[NS + 1], a[NS + 1], b[NS + 1], J[NS + 1][NS + 1];
+        for (int k = 0; k <= NS; k++) y[k] = in[(long long)q * (NS + 1) + k];
+        shcfd::chem::fex<NS>(m, y, a);
+        shcfd::chem::jex<NS>(m, y, J, b);
+        for (int k = 0; k <= NS; k++) {
+            f[(long long)q * (NS + 1) + k] = a[k];
+            dndt[(long long)q * (NS + 1) + k] = b[k];
+        }
+    }
+}
+
 }  // namespace
+
+extern "C" int host_fex_jex(const shcfd_mech* m, int n, const double* in, double* f, double* dndt) {
+    switch (m->ns) {
+    case 8: fj_rows<8>(m, n, in, f, dndt); return 0;
+    case 9: fj_rows<9>(m, n, in, f, dndt); return 0;
+    case 12: fj_rows<12>(m, n, in, f, dndt); return 0;
+    default: return SHCFD_E_ARG;
+    }
+}
 
 extern "C" int host_chem_eval(const shcfd_mech* m, int op, int n, const double* in, double* out, int* status,
                               double dt) {
