@@ -46,12 +46,12 @@ def alg_bytes_per_cell(kclass: str, dimq: int = 4, geo: int = 13) -> float:
 
 # Algorithmic FP64 operations of one point-implicit Newton iteration for one
 # cell (add/mul = 1, FMA = 2; exp/log counted apart), for ns species and nr
-# reactions: NASA-7 sums with derivatives 5 x 7 x 2 x ns (Jex) + 3 x 7 x 2 x ns
-# (Fex); rates ~17 per reaction (twice: Fex and Jex); Jacobian scatter
-# nr x (ns+1) x 2 x 3; LU 2/3 (ns+1)^3 and solve 2 (ns+1)^2.
+# reactions: NASA-7 sums with derivatives 5 x 7 x 2 x ns; rates ~17 per
+# reaction (once: F(y) is the dn/dt Jex forms beside the Jacobian); Jacobian
+# scatter nr x (ns+1) x 2 x 3; LU 2/3 (ns+1)^3 and solve 2 (ns+1)^2.
 def chem_flops_per_iteration(ns: int, nr: int) -> float:
     n1 = ns + 1
-    return 5 * 7 * 2 * ns + 3 * 7 * 2 * ns + 2 * 17 * nr + nr * n1 * 2 * 3 + (2.0 / 3.0) * n1 ** 3 + 2 * n1 ** 2
+    return 5 * 7 * 2 * ns + 17 * nr + nr * n1 * 2 * 3 + (2.0 / 3.0) * n1 ** 3 + 2 * n1 ** 2
 
 
 FP64_PEAK_TFLOPS = 78.6
