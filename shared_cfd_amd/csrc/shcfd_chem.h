@@ -966,37 +966,54 @@ SHCFD_HD int point_implicit(const shcfd_mech* m, double T, double (&vrho)[NS], d
 }
 
 // lu_solve with the matrix in per-lane LDS: the same pivot choice, swaps and
-// operation order as lu_solve, with runtime row indices
+// operation order as lu_solve, with runtime row indices.  Column k, once
+// pivoted and scaled, stays in registers for the updates of the columns
+// right of it (the loop over k is unrolled, so the copy is indexed at
+// compile time); every element still receives the same operations in the
+// same order.
 template <int N>
 __device__ int lu_solve_lds(LdsMat<N>& A, double (&b)[N]) {
     int info = 0;
     int piv[N];
+#pragma unroll
     for (int k = 0; k < N - 1; k++) {
+        double col[N];
+#pragma unroll
+        for (int i = k; i < N; i++) col[i] = A.get(i, k);
         int l = k;
-        double amax = A.get(k, k) < 0.0 ? -A.get(k, k) : A.get(k, k);
+        double amax = col[k] < 0.0 ? -col[k] : col[k];
+#pragma unroll
         for (int i = k + 1; i < N; i++) {
-            const double ai = A.get(i, k);
-            const double a = ai < 0.0 ? -ai : ai;
+            const double a = col[i] < 0.0 ? -col[i] : col[i];
             if (a > amax) {
                 amax = a;
                 l = i;
             }
         }
         piv[k] = l;
-        const double pv = A.get(l, k);
+        double pv = col[k];
+#pragma unroll
+        for (int i = k + 1; i < N; i++)
+            if (i == l) pv = col[i];
         if (pv == 0.0) {
             info = k + 1;
             continue;
         }
-        if (l != k) A.set(l, k, A.get(k, k));
-        A.set(k, k, pv);
+#pragma unroll
+        for (int i = k + 1; i < N; i++)
+            if (i == l) col[i] = col[k];
+        col[k] = pv;
         const double t = -1.0 / pv;
-        for (int i = k + 1; i < N; i++) A.set(i, k, t * A.get(i, k));
+#pragma unroll
+        for (int i = k + 1; i < N; i++) col[i] = t * col[i];
+#pragma unroll
+        for (int i = k; i < N; i++) A.set(i, k, col[i]);
         for (int j = k + 1; j < N; j++) {
             const double tj = A.get(l, j);
             if (l != k) A.set(l, j, A.get(k, j));
             A.set(k, j, tj);
-            for (int i = k + 1; i < N; i++) A.set(i, j, A.get(i, j) + tj * A.get(i, k));
+#pragma unroll
+            for (int i = k + 1; i < N; i++) A.set(i, j, A.get(i, j) + tj * col[i]);
         }
     }
     piv[N - 1] = N - 1;
