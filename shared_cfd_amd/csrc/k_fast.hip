@@ -798,6 +798,13 @@ int FastPath::init(Ctx& c) {
     }
     nstrips = (int)hs.size();
     nwork = (int)hw.size();
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        ncu = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+                  ? prop.multiProcessorCount
+                  : 0;
+    }
     nslots = base;
     // columns -HOFF .. maxcols + PH + HM of every hand-off row
     hstride = 4LL * (HOFF + maxcols + PH + HM + 2);
@@ -832,14 +839,17 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
     double* hand_f = hand;
     double* hand_b = hand + hstride * (nstrips + 1);
     // Workgroups of the forward launch past the strips form (HA, HB) on the
-    // CUs the strips leave idle.  The launch asks for 28 KB of dynamic LDS
-    // beside the sweep's 54 KB, so that no CU holds two of its workgroups:
-    // an operator workgroup next to a strip would take issue slots and
-    // vector-memory bandwidth from the strip's waves (measured on the 258k-cell
-    // forward step: 195 us per forward sweep with them side by side, 178 us
-    // apart, 167 us with (HA, HB) formed in prep instead, where prep costs
-    // 14 us more).
-    constexpr unsigned FWD_PAD_LDS = 28672;
+    // CUs the strips leave idle.  When the strips leave CUs idle, the launch
+    // asks for 28 KB of dynamic LDS beside the sweep's 54 KB, so that no CU
+    // holds two of its workgroups: an operator workgroup next to a strip
+    // would take issue slots and vector-memory bandwidth from the strip's
+    // waves (measured on the 258k-cell forward step: 195 us per forward
+    // sweep with them side by side, 178 us apart, 167 us with (HA, HB)
+    // formed in prep instead, where prep costs 14 us more).  With more
+    // strips than CUs (batches of many blocks) no CU is idle, and the pad
+    // would only halve the strips resident per CU (256 plate/channel blocks:
+    // 2.91 ms per forward launch with it, 2.59 ms without).
+    const unsigned FWD_PAD_LDS = (ncu > 0 && nstrips > ncu) ? 0u : 28672u;
     const int nhwg = std::min(nwork, 512);
     pb(&c, PROF_PREP);
     hipLaunchKernelGGL(k_fast_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, R, G, hand, nstrips, hstride);

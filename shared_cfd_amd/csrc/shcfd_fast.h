@@ -9,6 +9,7 @@ struct QStrip;
 struct FastPath {
     bool ready = false;
     int nstrips = 0, nwork = 0;
+    int ncu = 0;  // compute units of the device (hipDeviceProp_t::multiProcessorCount)
     long long nslots = 0, hstride = 0;
     QStrip* strips = nullptr;
     int2* work = nullptr;
