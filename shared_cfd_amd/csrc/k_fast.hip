@@ -118,10 +118,11 @@ __device__ inline long long op_index(const QStrip& S, int dd, int L, int p, int 
     return (S.base / 64 + dd) * 512 + p * 128 + 2 * L + e;
 }
 
-// the folded operators of one cell in its skewed slots: (GA, GB) into G
-// (forward sweep, with R) or (HA, HB) into H (backward sweep)
+// the folded operators of one cell in its skewed slots, from one cell
+// Jacobian: R and (GA, GB) into G for the forward sweep when G is given,
+// (HA, HB) into H for the backward sweep when H is given
 __device__ inline void prep_cell(const DevCtx& d, const QStrip& S, const DevBlock& B, int dd, int r, double* R_sl,
-                                 double* G, double* H, bool fwd) {
+                                 double* G, double* H) {
     int i, j;
     if (!slot_cell(S, B, dd, r, i, j)) return;
     const long long c = cidx(B, i, j);
@@ -132,25 +133,26 @@ __device__ inline void prep_cell(const DevCtx& d, const QStrip& S, const DevBloc
     post_matrix(J, dual, Sm);
     const double dsci = FLD(d, SHCFD_F_DSCI, 0, c), dscj = FLD(d, SHCFD_F_DSCJ, 0, c);
     const double fa = dsci * J.alpha, fb = dscj * J.alpha;
-    if (fwd) {
+    if (G) {
         // R (component c in lane 4r+c)
 #pragma unroll
         for (int k = 0; k < 4; k++) R_sl[slot1(S.base, dd, 4 * r + k)] = FLD(d, SHCFD_F_DQ, k, c);
     }
-    // rows of the two operators: lane 4r+k holds row k (4 doubles)
-    const double* mats[2] = {fwd ? J.Ap : J.Am, fwd ? J.Bp : J.Bm};
+    // rows of the operators: lane 4r+k holds row k (4 doubles)
+    const double* mats[4] = {J.Ap, J.Bp, J.Am, J.Bm};
     const double fs[2] = {fa, fb};
-    double* o = fwd ? G : H;
 #pragma unroll
-    for (int m = 0; m < 2; m++) {
+    for (int m = 0; m < 4; m++) {
+        double* o = m < 2 ? G : H;
+        if (!o) continue;
         mm4(mats[m], Sm, T);
         // doubles (2p', 2p'+1) of a lane's row are adjacent: one 16-byte store
 #pragma unroll
         for (int k = 0; k < 4; k++)
 #pragma unroll
             for (int lp = 0; lp < 2; lp++)
-                *reinterpret_cast<double2*>(&o[op_index(S, dd, 4 * r + k, 2 * m + lp, 0)]) =
-                    make_double2(fs[m] * T[k + 4 * (2 * lp)], fs[m] * T[k + 4 * (2 * lp + 1)]);
+                *reinterpret_cast<double2*>(&o[op_index(S, dd, 4 * r + k, 2 * (m & 1) + lp, 0)]) =
+                    make_double2(fs[m & 1] * T[k + 4 * (2 * lp)], fs[m & 1] * T[k + 4 * (2 * lp + 1)]);
     }
 }
 
@@ -170,16 +172,17 @@ __device__ inline void hand_refill(const DevCtx& d, const QStrip* strips, double
 
 // prep of the forward sweep: R and (GA, GB) per cell, the hand-off rows.
 // Workgroup = 16 steps x 16 rows of one quad strip; thread = one cell.  The
-// backward operators (HA, HB) are formed by the forward sweep's launch on
-// the CUs its strips leave idle (k_fast_fwd).
+// backward operators (HA, HB) are formed here too when H is given, else by
+// the forward sweep's launch on the CUs its strips leave idle (k_fast_fwd).
 __global__ void __launch_bounds__(256) k_fast_prep(DevCtx d, const QStrip* strips, const int2* work, double* R_sl,
-                                                   double* G, double* hand, int nstrips, long long hstride) {
+                                                   double* G, double* H, double* hand, int nstrips,
+                                                   long long hstride) {
     hand_refill(d, strips, hand, nstrips, hstride);
     const int2 wk = work[blockIdx.x];
     const QStrip S = strips[wk.x];
     const DevBlock B = d.blk[S.b];
     const int dd = wk.y + (int)(threadIdx.x >> 4), r = (int)(threadIdx.x & 15);
-    prep_cell(d, S, B, dd, r, R_sl, G, nullptr, true);
+    prep_cell(d, S, B, dd, r, R_sl, G, H);
 }
 
 // ---------------------------------------------------------------------------
@@ -396,7 +399,7 @@ k_fast_fwd(DevCtx d, const QStrip* strips, int nstrips, const double* R_sl, cons
 #pragma unroll 1
             for (int h = 0; h < 2; h++) {
                 const int t = (int)threadIdx.x + 128 * h;
-                prep_cell(d, S, B, wk.y + (t >> 4), t & 15, nullptr, nullptr, H, false);
+                prep_cell(d, S, B, wk.y + (t >> 4), t & 15, nullptr, nullptr, H);
             }
         }
         return;
@@ -838,21 +841,22 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
     hipStream_t st = c.stream;
     double* hand_f = hand;
     double* hand_b = hand + hstride * (nstrips + 1);
-    // Workgroups of the forward launch past the strips form (HA, HB) on the
-    // CUs the strips leave idle.  When the strips leave CUs idle, the launch
-    // asks for 28 KB of dynamic LDS beside the sweep's 54 KB, so that no CU
-    // holds two of its workgroups: an operator workgroup next to a strip
-    // would take issue slots and vector-memory bandwidth from the strip's
-    // waves (measured on the 258k-cell forward step: 195 us per forward
-    // sweep with them side by side, 178 us apart, 167 us with (HA, HB)
-    // formed in prep instead, where prep costs 14 us more).  With more
-    // strips than CUs (batches of many blocks) no CU is idle, and the pad
-    // would only halve the strips resident per CU (256 plate/channel blocks:
-    // 2.91 ms per forward launch with it, 2.59 ms without).
-    const unsigned FWD_PAD_LDS = (ncu > 0 && nstrips > ncu) ? 0u : 28672u;
-    const int nhwg = std::min(nwork, 512);
+    // When the strips leave CUs idle, workgroups of the forward launch past
+    // the strips form (HA, HB) there, and the launch asks for 28 KB of
+    // dynamic LDS beside the sweep's 54 KB so that no CU holds two of its
+    // workgroups: an operator workgroup next to a strip would take issue
+    // slots and vector-memory bandwidth from the strip's waves (measured on
+    // the 258k-cell forward step: 195 us per forward sweep with them side
+    // by side, 178 us apart, 167 us with (HA, HB) formed in prep instead,
+    // where prep costs 14 us more).  With more strips than CUs (batches of
+    // many blocks) no CU is idle: prep forms (HA, HB) from the same cell
+    // Jacobians as (GA, GB), and the forward launch holds the strips alone.
+    const bool busy = ncu > 0 && nstrips > ncu;
+    const unsigned FWD_PAD_LDS = busy ? 0u : 28672u;
+    const int nhwg = busy ? 0 : std::min(nwork, 512);
     pb(&c, PROF_PREP);
-    hipLaunchKernelGGL(k_fast_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, R, G, hand, nstrips, hstride);
+    hipLaunchKernelGGL(k_fast_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, R, G, busy ? H : nullptr, hand,
+                       nstrips, hstride);
     pe(&c, PROF_PREP);
     pb(&c, PROF_SWEEP_FWD);
     hipLaunchKernelGGL(k_fast_fwd, dim3(nstrips + nhwg), dim3(128), FWD_PAD_LDS, st, d, strips, nstrips, R, G, V,
