@@ -206,7 +206,9 @@ def bench_batched(a):
     cells = case.ncells
     s = Solver(case)
     s.prepare()
-    choice = s.autotune(4)
+    # --no-autotune keeps the default schedule (fast sweeps, graph of steps):
+    # a kernel trace then holds the measured steps alone
+    choice = None if a.no_autotune else s.autotune(4)
     s.iterate(1, 8, -1)  # warm-up with the tuned schedule (captures the graph)
     s.synchronize()
     t0 = time.perf_counter()
@@ -231,7 +233,7 @@ def bench_batched(a):
         seq_time += time.perf_counter() - tb0
         seq_cells += sub.ncells * a.seq_steps
         p.close()
-    seq = seq_cells / seq_time / 1e6
+    seq = seq_cells / seq_time / 1e6 if seq_cells else None
     return {
         "metric": "FP64 Mcell-updates/s (batched independent blocks)",
         "value": round(batched, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": 8,
@@ -239,8 +241,8 @@ def bench_batched(a):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": case.description, "cells": cells, "blocks": case.nblocks, "autotuned": choice},
         "sequential_plain": {"value": round(seq, 3), "unit": "Mcell-updates/s",
-                             "instances": min(a.seq_instances, case.nblocks), "steps": a.seq_steps},
-        "speedup_vs_sequential": round(batched / seq, 2),
+                             "instances": min(a.seq_instances, case.nblocks), "steps": a.seq_steps} if seq else None,
+        "speedup_vs_sequential": round(batched / seq, 2) if seq else None,
     }
 
 
@@ -328,6 +330,7 @@ def main(argv=None):
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seq-instances", type=int, default=256)
     ap.add_argument("--seq-steps", type=int, default=5)
+    ap.add_argument("--no-autotune", action="store_true", help="batched: default schedule, no autotuner trials")
     ap.add_argument("--workload", choices=["step", "reacting", "batched", "adjoint"], default="step",
                     help="step: Mach-3 forward step (the headline metric); reacting: mixing layer with "
                          "chemistry; batched: 256 independent plate/channel blocks; adjoint: airfoil shape "

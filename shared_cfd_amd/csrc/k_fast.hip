@@ -120,9 +120,13 @@ __device__ inline long long op_index(const QStrip& S, int dd, int L, int p, int 
 
 // the folded operators of one cell in its skewed slots, from one cell
 // Jacobian: R and (GA, GB) into G for the forward sweep when G is given,
-// (HA, HB) into H for the backward sweep when H is given
+// (HA, HB) into H for the backward sweep when H is given, and with S = I
+// the backward sweep's diagonal terms into Dn when Dn is given: A- = A+ -
+// nuA I and B- = B+ - nuB I, so HA = GA - dsci alpha nuA I and HB = GB -
+// dscj alpha nuB I; Dn holds (-dsci alpha nuA, -dscj alpha nuB) per row and
+// step, 256 bytes per step
 __device__ inline void prep_cell(const DevCtx& d, const QStrip& S, const DevBlock& B, int dd, int r, double* R_sl,
-                                 double* G, double* H) {
+                                 double* G, double* H, double* Dn = nullptr) {
     int i, j;
     if (!slot_cell(S, B, dd, r, i, j)) return;
     const long long c = cidx(B, i, j);
@@ -138,6 +142,9 @@ __device__ inline void prep_cell(const DevCtx& d, const QStrip& S, const DevBloc
 #pragma unroll
         for (int k = 0; k < 4; k++) R_sl[slot1(S.base, dd, 4 * r + k)] = FLD(d, SHCFD_F_DQ, k, c);
     }
+    if (Dn)
+        *reinterpret_cast<double2*>(&Dn[(S.base / 64 + dd) * 32 + 2 * r]) =
+            make_double2(-(fa * J.nuA), -(fb * J.nuB));
     // rows of the operators: lane 4r+k holds row k (4 doubles)
     const double* mats[4] = {J.Ap, J.Bp, J.Am, J.Bm};
     const double fs[2] = {fa, fb};
@@ -172,17 +179,18 @@ __device__ inline void hand_refill(const DevCtx& d, const QStrip* strips, double
 
 // prep of the forward sweep: R and (GA, GB) per cell, the hand-off rows.
 // Workgroup = 16 steps x 16 rows of one quad strip; thread = one cell.  The
-// backward operators (HA, HB) are formed here too when H is given, else by
-// the forward sweep's launch on the CUs its strips leave idle (k_fast_fwd).
+// backward operators (HA, HB) are formed here too when H is given, or their
+// diagonal terms when Dn is given (S = I), else by the forward sweep's launch
+// on the CUs its strips leave idle (k_fast_fwd).
 __global__ void __launch_bounds__(256) k_fast_prep(DevCtx d, const QStrip* strips, const int2* work, double* R_sl,
-                                                   double* G, double* H, double* hand, int nstrips,
+                                                   double* G, double* H, double* Dn, double* hand, int nstrips,
                                                    long long hstride) {
     hand_refill(d, strips, hand, nstrips, hstride);
     const int2 wk = work[blockIdx.x];
     const QStrip S = strips[wk.x];
     const DevBlock B = d.blk[S.b];
     const int dd = wk.y + (int)(threadIdx.x >> 4), r = (int)(threadIdx.x & 15);
-    prep_cell(d, S, B, dd, r, R_sl, G, H);
+    prep_cell(d, S, B, dd, r, R_sl, G, H, Dn);
 }
 
 // ---------------------------------------------------------------------------
@@ -256,6 +264,7 @@ __device__ inline void st_agent(__amdgpu_buffer_rsrc_t r, double x, int voff, in
 // 32 bytes
 __device__ inline int op_off(int dd) { return (dd + CH) * 4096; }
 __device__ inline int pair_off(int dd) { return (dd + CH) * 512; }
+__device__ inline int dn_off(int dd) { return (dd + CH) * 256; }
 __device__ inline int col_off(int col) { return (col + HOFF) * 32; }
 
 // Sweep schedule.  A sweep workgroup has two wavefronts on one CU: wave 0
@@ -274,8 +283,10 @@ __device__ inline int col_off(int col) { return (col + HOFF) * 32; }
 constexpr int CS = 4;                                   // steps per LDS chunk
 constexpr int NSLOT = 3;                                // LDS chunk slots
 constexpr int SLOT_G = CS * 4096;                       // operator bytes per slot
-constexpr int SLOT_BYTES = SLOT_G + (CS / 2) * 1024;    // + R/X step pairs
-constexpr int LD_PER_CHUNK = 4 * CS + CS / 2;           // LDS-DMA loads per chunk
+constexpr int SLOT_DN = SLOT_G + (CS / 2) * 1024;       // + R/X step pairs
+constexpr int SLOT_BYTES = SLOT_DN + CS * 256;          // + diagonal terms (backward sweep with S = I)
+constexpr int LD_PER_CHUNK = 4 * CS + CS / 2;           // LDS-DMA loads per chunk, + 1 with Dn
+static_assert(CS * 256 == 1024, "one LDS-DMA load carries a chunk's diagonal terms");
 
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -294,8 +305,9 @@ __device__ inline void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xC07F); }
 
 // loader wave: chunk j = aligned block m of CS steps (steps CS*m .. CS*m+CS-1)
 // into slot j % NSLOT; the caller orders blocks (ascending or descending)
-__device__ inline void load_block(__amdgpu_buffer_rsrc_t rop, __amdgpu_buffer_rsrc_t rpair, char* lds, int j,
-                                  int m, int vl) {
+template <bool DN>
+__device__ inline void load_block(__amdgpu_buffer_rsrc_t rop, __amdgpu_buffer_rsrc_t rpair, __amdgpu_buffer_rsrc_t rdn,
+                                  char* lds, int j, int m, int vl) {
     char* slot = lds + (j % NSLOT) * SLOT_BYTES;
 #pragma unroll
     for (int t = 0; t < CS; t++)
@@ -303,24 +315,27 @@ __device__ inline void load_block(__amdgpu_buffer_rsrc_t rop, __amdgpu_buffer_rs
         for (int p = 0; p < 4; p++) lds_dma(rop, slot + t * 4096 + p * 1024, vl, op_off(CS * m + t) + 1024 * p);
 #pragma unroll
     for (int q = 0; q < CS / 2; q++) lds_dma(rpair, slot + SLOT_G + q * 1024, vl, pair_off(CS * m + 2 * q));
+    if constexpr (DN) lds_dma(rdn, slot + SLOT_DN, vl, dn_off(CS * m));
 }
 
-// loader side of the chunk protocol: nck chunks, block of chunk j = blk(j)
-template <typename BLK>
-__device__ inline void loader(__amdgpu_buffer_rsrc_t rop, __amdgpu_buffer_rsrc_t rpair, char* lds, int nck, int vl,
-                              BLK blk) {
-    load_block(rop, rpair, lds, 0, blk(0), vl);
-    load_block(rop, rpair, lds, 1, blk(1), vl);
-    load_block(rop, rpair, lds, 2, blk(2), vl);
-    wait_vmcnt<2 * LD_PER_CHUNK>();  // chunk 0 in LDS
+// loader side of the chunk protocol: nck chunks, block of chunk j = blk(j);
+// DN: the chunks also carry the backward sweep's diagonal terms
+template <bool DN, typename BLK>
+__device__ inline void loader(__amdgpu_buffer_rsrc_t rop, __amdgpu_buffer_rsrc_t rpair, __amdgpu_buffer_rsrc_t rdn,
+                              char* lds, int nck, int vl, BLK blk) {
+    constexpr int LD = LD_PER_CHUNK + (DN ? 1 : 0);
+    load_block<DN>(rop, rpair, rdn, lds, 0, blk(0), vl);
+    load_block<DN>(rop, rpair, rdn, lds, 1, blk(1), vl);
+    load_block<DN>(rop, rpair, rdn, lds, 2, blk(2), vl);
+    wait_vmcnt<2 * LD>();  // chunk 0 in LDS
     wg_barrier();
-    wait_vmcnt<LD_PER_CHUNK>();      // chunk 1
+    wait_vmcnt<LD>();      // chunk 1
     wg_barrier();
     for (int j = 0; j < nck; j++) {
         // chunk j+3 takes the slot of chunk j, read before the last barrier;
         // the blocks past the strip's end fall in its guard steps
-        load_block(rop, rpair, lds, j + 3, blk(j + 3), vl);
-        wait_vmcnt<LD_PER_CHUNK>();  // chunk j+2
+        load_block<DN>(rop, rpair, rdn, lds, j + 3, blk(j + 3), vl);
+        wait_vmcnt<LD>();  // chunk j+2
         wg_barrier();
     }
 }
@@ -329,15 +344,24 @@ __device__ inline void loader(__amdgpu_buffer_rsrc_t rop, __amdgpu_buffer_rsrc_t
 struct Chunk {
     double2 g[CS][4];
     double2 r[CS / 2];
+    double2 dn[CS];  // diagonal terms of the lane's row (DN only)
 };
+template <bool DN = false>
 __device__ inline void read_chunk(Chunk& c, const char* lds, int j, int vl) {
-    const char* slot = lds + (j % NSLOT) * SLOT_BYTES + vl;
+    const char* base = lds + (j % NSLOT) * SLOT_BYTES;
+    const char* slot = base + vl;
 #pragma unroll
     for (int t = 0; t < CS; t++)
 #pragma unroll
         for (int p = 0; p < 4; p++) c.g[t][p] = *reinterpret_cast<const double2*>(slot + t * 4096 + p * 1024);
 #pragma unroll
     for (int q = 0; q < CS / 2; q++) c.r[q] = *reinterpret_cast<const double2*>(slot + SLOT_G + q * 1024);
+    if constexpr (DN) {
+        // row r = lane/4 of step t: 16 bytes at t*256 + 16 r (four lanes share it)
+        const char* dn = base + SLOT_DN + 4 * (vl / 16 & ~3);
+#pragma unroll
+        for (int t = 0; t < CS; t++) c.dn[t] = *reinterpret_cast<const double2*>(dn + t * 256);
+    }
 }
 
 constexpr int PH = 4;   // hand-off prefetch depth (steps); divides 2*CS
@@ -414,7 +438,7 @@ k_fast_fwd(DevCtx d, const QStrip* strips, int nstrips, const double* R_sl, cons
     if (threadIdx.x >= 64) {
         const __amdgpu_buffer_rsrc_t rg = view(G + (S.base / 64 - CH) * 512, nstep * 512);
         const __amdgpu_buffer_rsrc_t rr = view(R_sl + S.base - 64 * CH, nstep * 64);
-        loader(rg, rr, sweep_lds, nck, vl, [](int j) { return j; });
+        loader<false>(rg, rr, rr, sweep_lds, nck, vl, [](int j) { return j; });
         return;
     }
     const bool cons = L < 4;  // lanes of row 0: they take the hand-off
@@ -504,7 +528,10 @@ __global__ void __launch_bounds__(256) k_fast_mid(DevCtx d, const QStrip* strips
 // backward sweep: workgroup g handles strip (n-1-g), so the strip it waits
 // for (the one above) has a lower workgroup id
 // ---------------------------------------------------------------------------
-// one backward step: step dd, slot step t of chunk c, ring position u
+// one backward step: step dd, slot step t of chunk c, ring position u.  DN:
+// the operators are (GA, GB) and the step subtracts the diagonal terms
+// (HA z = GA z - dsci alpha nuA z, the term entering as the first addend)
+template <bool DN>
 __device__ inline void bwd_step(const Chunk& c, int t, int u, int dd, double& a, double& bo, double& zhi,
                                 double (&hb)[PH], bool cons, const double* hin, __amdgpu_buffer_rsrc_t rin,
                                 __amdgpu_buffer_rsrc_t rz, __amdgpu_buffer_rsrc_t rout, int vl, int vc, int vo,
@@ -527,15 +554,23 @@ __device__ inline void bwd_step(const Chunk& c, int t, int u, int dd, double& a,
     else
         st_pair(rz, make_double2(z, zhi), vl, pair_off(dd));
     const double z0 = quad_bcast<0>(z), z1 = quad_bcast<1>(z), z2 = quad_bcast<2>(z), z3 = quad_bcast<3>(z);
-    a = fma(ha23.y, z3, fma(ha23.x, z2, fma(ha01.y, z1, ha01.x * z0)));
-    bo = fma(hb23.y, z3, fma(hb23.x, z2, fma(hb01.y, z1, hb01.x * z0)));
+    if constexpr (DN) {
+        const double pa = c.dn[t].x * z, pb = c.dn[t].y * z;
+        a = fma(ha23.y, z3, fma(ha23.x, z2, fma(ha01.y, z1, fma(ha01.x, z0, pa))));
+        bo = fma(hb23.y, z3, fma(hb23.x, z2, fma(hb01.y, z1, fma(hb01.x, z0, pb))));
+    } else {
+        a = fma(ha23.y, z3, fma(ha23.x, z2, fma(ha01.y, z1, ha01.x * z0)));
+        bo = fma(hb23.y, z3, fma(hb23.x, z2, fma(hb01.y, z1, hb01.x * z0)));
+    }
     // row 0 hands column dd+1 to the strip above
     st_agent(rout, bo, vo, col_off(dd + 1));
 }
 
+// DN: H holds (GA, GB) and Dn the diagonal terms (schemes with S = I)
+template <bool DN>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1)))
-k_fast_bwd(DevCtx d, const QStrip* strips, int nstrips, const double* X_sl, const double* H, double* Z_sl,
-           double* hand, long long hstride, long long spin_limit) {
+k_fast_bwd(DevCtx d, const QStrip* strips, int nstrips, const double* X_sl, const double* H, const double* Dn,
+           double* Z_sl, double* hand, long long hstride, long long spin_limit) {
     const int s = nstrips - 1 - (int)blockIdx.x;
     const int L = threadIdx.x & 63;
     const QStrip S = strips[s];
@@ -545,7 +580,8 @@ k_fast_bwd(DevCtx d, const QStrip* strips, int nstrips, const double* X_sl, cons
     if (threadIdx.x >= 64) {
         const __amdgpu_buffer_rsrc_t rh = view(H + (S.base / 64 - CH) * 512, nstep * 512);
         const __amdgpu_buffer_rsrc_t rx = view(X_sl + S.base - 64 * CH, nstep * 64);
-        loader(rh, rx, sweep_lds, nck, vl, [nck](int j) { return nck - 1 - j; });
+        const __amdgpu_buffer_rsrc_t rd = DN ? view(Dn + (S.base / 64 - CH) * 32, nstep * 32) : rx;
+        loader<DN>(rh, rx, rd, sweep_lds, nck, vl, [nck](int j) { return nck - 1 - j; });
         return;
     }
     const bool cons = L >= 60;  // lanes of row 15: they take the hand-off
@@ -574,22 +610,22 @@ k_fast_bwd(DevCtx d, const QStrip* strips, int nstrips, const double* X_sl, cons
     double a = 0.0, bo = 0.0, zhi = 0.0;
     Chunk A, B;
     wg_barrier();
-    read_chunk(A, sweep_lds, 0, vl);
+    read_chunk<DN>(A, sweep_lds, 0, vl);
     wait_lgkm0();
     wg_barrier();
     for (int j = 0; j < nck; j += 2) {
-        read_chunk(B, sweep_lds, j + 1, vl);
+        read_chunk<DN>(B, sweep_lds, j + 1, vl);
         const int m0 = nck - 1 - j;  // block of chunk j
 #pragma unroll
         for (int u = 0; u < CS; u++)
-            bwd_step(A, CS - 1 - u, u, CS * m0 + CS - 1 - u, a, bo, zhi, hb, cons, hin, rin, rz, rout, vl, vc, vo,
+            bwd_step<DN>(A, CS - 1 - u, u, CS * m0 + CS - 1 - u, a, bo, zhi, hb, cons, hin, rin, rz, rout, vl, vc, vo,
                      spin_limit, d.iflag);
         wait_lgkm0();
         wg_barrier();
-        read_chunk(A, sweep_lds, j + 2, vl);
+        read_chunk<DN>(A, sweep_lds, j + 2, vl);
 #pragma unroll
         for (int u = 0; u < CS; u++)
-            bwd_step(B, CS - 1 - u, CS + u, CS * (m0 - 1) + CS - 1 - u, a, bo, zhi, hb, cons, hin, rin, rz, rout, vl,
+            bwd_step<DN>(B, CS - 1 - u, CS + u, CS * (m0 - 1) + CS - 1 - u, a, bo, zhi, hb, cons, hin, rin, rz, rout, vl,
                      vc, vo, spin_limit, d.iflag);
         wait_lgkm0();
         wg_barrier();
@@ -819,6 +855,7 @@ int FastPath::init(Ctx& c) {
               al((void**)&R, sizeof(double) * nslots) && al((void**)&V, sizeof(double) * nslots) &&
               al((void**)&X, sizeof(double) * nslots) && al((void**)&Z, sizeof(double) * nslots) &&
               al((void**)&G, sizeof(double) * 8 * nslots) && al((void**)&H, sizeof(double) * 8 * nslots) &&
+              al((void**)&Dn, sizeof(double) * (nslots / 2)) &&
               al((void**)&hand, sizeof(double) * 2 * hstride * (nstrips + 1)) &&
               al((void**)&part, sizeof(double) * 2 * nwork);
     if (!ok) return SHCFD_E_NOMEM;
@@ -829,7 +866,7 @@ int FastPath::init(Ctx& c) {
 }
 
 void FastPath::release() {
-    void* ptrs[] = {strips, work, R, V, X, Z, G, H, hand, part, aR, aV, aZ, aG, aH, ahand};
+    void* ptrs[] = {strips, work, R, V, X, Z, G, H, Dn, hand, part, aR, aV, aZ, aG, aH, ahand};
     for (void* p : ptrs)
         if (p) hipFree(p);
     *this = FastPath();
@@ -841,22 +878,27 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
     hipStream_t st = c.stream;
     double* hand_f = hand;
     double* hand_b = hand + hstride * (nstrips + 1);
-    // When the strips leave CUs idle, workgroups of the forward launch past
-    // the strips form (HA, HB) there, and the launch asks for 28 KB of
-    // dynamic LDS beside the sweep's 54 KB so that no CU holds two of its
-    // workgroups: an operator workgroup next to a strip would take issue
-    // slots and vector-memory bandwidth from the strip's waves (measured on
-    // the 258k-cell forward step: 195 us per forward sweep with them side
-    // by side, 178 us apart, 167 us with (HA, HB) formed in prep instead,
-    // where prep costs 14 us more).  With more strips than CUs (batches of
-    // many blocks) no CU is idle: prep forms (HA, HB) from the same cell
-    // Jacobians as (GA, GB), and the forward launch holds the strips alone.
+    // Without the dual-time post-scaling (S = I) the backward operators are
+    // the forward ones less diagonal terms: the backward sweep streams (GA,
+    // GB) and the 16 bytes of those terms per cell that prep writes, instead
+    // of 256 bytes of (HA, HB).  In dual time, when the strips leave CUs idle,
+    // workgroups of the forward launch past the strips form (HA, HB) there,
+    // and the launch asks for 28 KB of dynamic LDS beside the sweep's 57 KB
+    // so that no CU holds two of its workgroups: an operator workgroup next
+    // to a strip would take issue slots and vector-memory bandwidth from the
+    // strip's waves (measured on the 258k-cell forward step: 195 us per
+    // forward sweep with them side by side, 178 us apart, 167 us with (HA,
+    // HB) formed in prep instead, where prep costs 14 us more).  With more
+    // strips than CUs (batches of many blocks) no CU is idle: prep forms (HA,
+    // HB) from the same cell Jacobians as (GA, GB), and the forward launch
+    // holds the strips alone.
+    const bool diag = !dual;
     const bool busy = ncu > 0 && nstrips > ncu;
     const unsigned FWD_PAD_LDS = busy ? 0u : 28672u;
-    const int nhwg = busy ? 0 : std::min(nwork, 512);
+    const int nhwg = (busy || diag) ? 0 : std::min(nwork, 512);
     pb(&c, PROF_PREP);
-    hipLaunchKernelGGL(k_fast_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, R, G, busy ? H : nullptr, hand,
-                       nstrips, hstride);
+    hipLaunchKernelGGL(k_fast_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, R, G,
+                       (busy && !diag) ? H : nullptr, diag ? Dn : nullptr, hand, nstrips, hstride);
     pe(&c, PROF_PREP);
     pb(&c, PROF_SWEEP_FWD);
     hipLaunchKernelGGL(k_fast_fwd, dim3(nstrips + nhwg), dim3(128), FWD_PAD_LDS, st, d, strips, nstrips, R, G, V,
@@ -870,8 +912,12 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
         Xin = X;
     }
     pb(&c, PROF_SWEEP_BWD);
-    hipLaunchKernelGGL(k_fast_bwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, Xin, H, Z, hand_b, hstride,
-                       c.spin_limit);
+    if (diag)
+        hipLaunchKernelGGL(k_fast_bwd<true>, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, Xin, G, Dn, Z,
+                           hand_b, hstride, c.spin_limit);
+    else
+        hipLaunchKernelGGL(k_fast_bwd<false>, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, Xin, H, nullptr, Z,
+                           hand_b, hstride, c.spin_limit);
     pe(&c, PROF_SWEEP_BWD);
     pb(&c, PROF_UPDATE);
     hipLaunchKernelGGL(k_fast_post, dim3(nwork), dim3(256), 0, st, d, strips, work, Z, part);
@@ -918,7 +964,7 @@ int FastPath::adjoint_apply(Ctx& c, const double* r_cell, double* lam) {
     hipLaunchKernelGGL(k_fast_hand_fill, dim3(256), dim3(256), 0, st, d, strips, ahand, nstrips, hstride);
     hipLaunchKernelGGL(k_fast_fwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, aR, aG, aV, ahand, hstride,
                        c.spin_limit, work, 0, nullptr);
-    hipLaunchKernelGGL(k_fast_bwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, aV, aH, aZ,
+    hipLaunchKernelGGL(k_fast_bwd<false>, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, aV, aH, nullptr, aZ,
                        ahand + hstride * (nstrips + 1), hstride, c.spin_limit);
     hipLaunchKernelGGL(k_fast_adj_post, dim3(nwork), dim3(256), 0, st, d, strips, work, aZ, lam);
     return hipGetLastError() == hipSuccess ? SHCFD_OK : SHCFD_E_HIP;

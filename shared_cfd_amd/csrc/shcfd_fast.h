@@ -15,6 +15,7 @@ struct FastPath {
     int2* work = nullptr;
     double *R = nullptr, *V = nullptr, *X = nullptr, *Z = nullptr;
     double *G = nullptr, *H = nullptr;  // (GA, GB) and (HA, HB), split-half layout
+    double* Dn = nullptr;               // diagonal terms of the backward sweep with S = I
     double* hand = nullptr;
     double* part = nullptr;
     // transposed sweeps of the discrete adjoint (allocated on first use)

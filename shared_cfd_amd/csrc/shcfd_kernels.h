@@ -307,6 +307,7 @@ struct CellJac {
     static constexpr int dq = NY + 3;
     double Ap[dq * dq], Am[dq * dq], Bp[dq * dq], Bm[dq * dq];
     double alpha, phi, dtm;
+    double nuA, nuB;  // spectral radii of the split: A+ - A- = nuA I, B+ - B- = nuB I
     double pre1[dq], pre2[dq], ps[4];
     bool precond;
 };
@@ -485,6 +486,8 @@ __device__ inline void cell_jacobian(const DevCtx& d, long long c, long long P, 
     }
 #undef AM_
 #undef BM_
+    J.nuA = nuA;
+    J.nuB = nuB;
     if (scheme == SHCFD_SCHEME_NR || scheme == SHCFD_SCHEME_PRECON)  // BDF2 diagonal: sch_NR.raw.f90:134-136
         J.alpha = 1.0 / ((((3.0 * vol) / (2.0 * dtv)) + (dsci * nuA)) + (dscj * nuB));
     else if (scheme != SHCFD_SCHEME_DUAL)
