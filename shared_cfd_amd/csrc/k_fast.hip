@@ -189,7 +189,9 @@ __global__ void __launch_bounds__(256) k_fast_prep(DevCtx d, const QStrip* strip
     const int2 wk = work[blockIdx.x];
     const QStrip S = strips[wk.x];
     const DevBlock B = d.blk[S.b];
-    const int dd = wk.y + (int)(threadIdx.x >> 4), r = (int)(threadIdx.x & 15);
+    // a wave takes 4 rows x 16 steps: each load of a cell field then reads
+    // 4 runs of 16 consecutive cells
+    const int dd = wk.y + (int)(threadIdx.x & 15), r = (int)(threadIdx.x >> 4);
     prep_cell(d, S, B, dd, r, R_sl, G, H, Dn);
 }
 
@@ -423,7 +425,7 @@ k_fast_fwd(DevCtx d, const QStrip* strips, int nstrips, const double* R_sl, cons
 #pragma unroll 1
             for (int h = 0; h < 2; h++) {
                 const int t = (int)threadIdx.x + 128 * h;
-                prep_cell(d, S, B, wk.y + (t >> 4), t & 15, nullptr, nullptr, H);
+                prep_cell(d, S, B, wk.y + (t & 15), t >> 4, nullptr, nullptr, H);
             }
         }
         return;
