@@ -812,33 +812,6 @@ namespace shcfd {
 
 int FastPath::init(Ctx& c) {
     if (c.ny != 1) return SHCFD_E_STATE;
-    std::vector<QStrip> hs;
-    std::vector<int2> hw;
-    long long base = 0;
-    int maxcols = 0;
-    for (int b = 0; b < c.nb; b++) {
-        const DevBlock& B = c.hblk[b];
-        int ns = (B.nj + QR - 1) / QR;
-        int first = (int)hs.size();
-        const int nd = B.ni + QR - 1;
-        const int nch = (nd + CH - 1) / CH;
-        maxcols = std::max(maxcols, nch * CH);
-        for (int k = 0; k < ns; k++) {
-            QStrip S{};
-            S.b = b;
-            S.j0 = 1 + QR * k;
-            S.prev = k > 0 ? first + k - 1 : -1;
-            S.next = k < ns - 1 ? first + k + 1 : -1;
-            S.nch = nch;
-            // one chunk of guard steps before and after the padded steps
-            S.base = base + 64LL * CH;
-            for (int d0 = 0; d0 < nd; d0 += 16) hw.push_back(make_int2((int)hs.size(), d0));
-            base += 64LL * CH * (nch + 2);
-            hs.push_back(S);
-        }
-    }
-    nstrips = (int)hs.size();
-    nwork = (int)hw.size();
     {
         int dev = 0;
         hipDeviceProp_t prop;
@@ -846,6 +819,55 @@ int FastPath::init(Ctx& c) {
                   ? prop.multiProcessorCount
                   : 0;
     }
+    // strips (b, k) of every block; launch order: block by block, or, with
+    // more strips than CUs, strip k of every block before strip k+1 of any,
+    // so that a strip is launched well after the one it waits for instead
+    // of holding a CU through the pipeline fill of its block.  Either way a
+    // strip's neighbours in the sweep order have lower workgroup ids in the
+    // sweep that waits for them.
+    std::vector<int> nsb(c.nb), first(c.nb);
+    int total = 0, maxns = 0;
+    for (int b = 0; b < c.nb; b++) {
+        nsb[b] = (c.hblk[b].nj + QR - 1) / QR;
+        total += nsb[b];
+        maxns = std::max(maxns, nsb[b]);
+    }
+    std::vector<int2> order;  // (b, k) in launch order
+    if (ncu > 0 && total > ncu) {
+        for (int k = 0; k < maxns; k++)
+            for (int b = 0; b < c.nb; b++)
+                if (k < nsb[b]) order.push_back(make_int2(b, k));
+    } else {
+        for (int b = 0; b < c.nb; b++)
+            for (int k = 0; k < nsb[b]; k++) order.push_back(make_int2(b, k));
+    }
+    std::vector<int> idx(total);  // strip index of (b, k) at first[b] + k
+    for (int b = 0, o = 0; b < c.nb; o += nsb[b], b++) first[b] = o;
+    for (int s = 0; s < total; s++) idx[first[order[s].x] + order[s].y] = s;
+    std::vector<QStrip> hs;
+    std::vector<int2> hw;
+    long long base = 0;
+    int maxcols = 0;
+    for (int s = 0; s < total; s++) {
+        const int b = order[s].x, k = order[s].y;
+        const DevBlock& B = c.hblk[b];
+        const int nd = B.ni + QR - 1;
+        const int nch = (nd + CH - 1) / CH;
+        maxcols = std::max(maxcols, nch * CH);
+        QStrip S{};
+        S.b = b;
+        S.j0 = 1 + QR * k;
+        S.prev = k > 0 ? idx[first[b] + k - 1] : -1;
+        S.next = k < nsb[b] - 1 ? idx[first[b] + k + 1] : -1;
+        S.nch = nch;
+        // one chunk of guard steps before and after the padded steps
+        S.base = base + 64LL * CH;
+        for (int d0 = 0; d0 < nd; d0 += 16) hw.push_back(make_int2((int)hs.size(), d0));
+        base += 64LL * CH * (nch + 2);
+        hs.push_back(S);
+    }
+    nstrips = (int)hs.size();
+    nwork = (int)hw.size();
     nslots = base;
     // columns -HOFF .. maxcols + PH + HM of every hand-off row
     hstride = 4LL * (HOFF + maxcols + PH + HM + 2);
