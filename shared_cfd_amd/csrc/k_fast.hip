@@ -23,7 +23,7 @@
 // Each strip's steps are padded to whole chunks of CH steps, with one chunk
 // of guard steps on either side, so the sweeps prefetch and store without a
 // bounds test.  Slots that hold no cell keep zero coefficients (the arrays
-// are zeroed once and k_fast_prep writes only cells), which makes every
+// are zeroed once and k_fast_rhs_prep writes only cells), which makes every
 // contribution out of them exactly zero: the sweeps need no per-cell mask.
 // Arrays with one double per lane and step (R, V, X, Z) interleave pairs of
 // steps, (dd even, dd odd) per lane, so one 16-byte access covers two steps.
@@ -118,30 +118,20 @@ __device__ inline long long op_index(const QStrip& S, int dd, int L, int p, int 
     return (S.base / 64 + dd) * 512 + p * 128 + 2 * L + e;
 }
 
-// the folded operators of one cell in its skewed slots, from one cell
-// Jacobian: R and (GA, GB) into G for the forward sweep when G is given,
-// (HA, HB) into H for the backward sweep when H is given, and with S = I
-// the backward sweep's diagonal terms into Dn when Dn is given: A- = A+ -
-// nuA I and B- = B+ - nuB I, so HA = GA - dsci alpha nuA I and HB = GB -
-// dscj alpha nuB I; Dn holds (-dsci alpha nuA, -dscj alpha nuB) per row and
-// step, 256 bytes per step
-__device__ inline void prep_cell(const DevCtx& d, const QStrip& S, const DevBlock& B, int dd, int r, double* R_sl,
-                                 double* G, double* H, double* Dn = nullptr) {
-    int i, j;
-    if (!slot_cell(S, B, dd, r, i, j)) return;
-    const long long c = cidx(B, i, j);
+// the folded operators of one cell c (slot step dd, row r) in its skewed
+// slots from its cell Jacobian J: (GA, GB) into G for the forward sweep when
+// G is given, (HA, HB) into H for the backward sweep when H is given, and
+// with S = I the backward sweep's diagonal terms into Dn when Dn is given:
+// A- = A+ - nuA I and B- = B+ - nuB I, so HA = GA - dsci alpha nuA I and
+// HB = GB - dscj alpha nuB I; Dn holds (-dsci alpha nuA, -dscj alpha nuB) per
+// row and step, 256 bytes per step
+__device__ inline void prep_ops(const DevCtx& d, const QStrip& S, int dd, int r, long long c, const CellJac<1>& J,
+                                double* G, double* H, double* Dn) {
     const bool dual = d.p.scheme == SHCFD_SCHEME_DUAL;
-    CellJac<1> J;
-    cell_jacobian<1>(d, c, B.pitch, J);
     double Sm[16], T[16];
     post_matrix(J, dual, Sm);
     const double dsci = FLD(d, SHCFD_F_DSCI, 0, c), dscj = FLD(d, SHCFD_F_DSCJ, 0, c);
     const double fa = dsci * J.alpha, fb = dscj * J.alpha;
-    if (G) {
-        // R (component c in lane 4r+c)
-#pragma unroll
-        for (int k = 0; k < 4; k++) R_sl[slot1(S.base, dd, 4 * r + k)] = FLD(d, SHCFD_F_DQ, k, c);
-    }
     if (Dn)
         *reinterpret_cast<double2*>(&Dn[(S.base / 64 + dd) * 32 + 2 * r]) =
             make_double2(-(fa * J.nuA), -(fb * J.nuB));
@@ -163,6 +153,16 @@ __device__ inline void prep_cell(const DevCtx& d, const QStrip& S, const DevBloc
     }
 }
 
+// one cell's backward operators (HA, HB) from a new cell Jacobian
+__device__ inline void prep_cell_h(const DevCtx& d, const QStrip& S, const DevBlock& B, int dd, int r, double* H) {
+    int i, j;
+    if (!slot_cell(S, B, dd, r, i, j)) return;
+    const long long c = cidx(B, i, j);
+    CellJac<1> J;
+    cell_jacobian<1>(d, c, B.pitch, J);
+    prep_ops(d, S, dd, r, c, J, nullptr, H, nullptr);
+}
+
 // refill of the hand-off rows of both sweeps: the sentinel for the columns
 // 1..ni of each strip's block, zero elsewhere (grid-stride over all rows)
 __device__ inline void hand_refill(const DevCtx& d, const QStrip* strips, double* hand, int nstrips, long long hstride) {
@@ -177,28 +177,53 @@ __device__ inline void hand_refill(const DevCtx& d, const QStrip* strips, double
     }
 }
 
-// prep of the forward sweep: R and (GA, GB) per cell, the hand-off rows.
-// Workgroup = 16 steps x 16 rows of one quad strip; thread = one cell.  The
-// backward operators (HA, HB) are formed here too when H is given, or their
-// diagonal terms when Dn is given (S = I), else by the forward sweep's launch
-// on the CUs its strips leave idle (k_fast_fwd).
-__global__ void __launch_bounds__(256) k_fast_prep(DevCtx d, const QStrip* strips, const int2* work, double* R_sl,
-                                                   double* G, double* H, double* Dn, double* hand, int nstrips,
-                                                   long long hstride) {
+// The right-hand side and prep of the optimised update in one pass (k_rhs
+// followed by the operators' prep): per cell one cell Jacobian gives
+// set_ABpm's scalars (stored when abpm_inline, as k_rhs stores them), the
+// right-hand side of the scheme goes straight into R, and the folded
+// operators follow; the hand-off rows are refilled.  The natural DQ field is
+// left to k_fast_post, which stores the update there.  (HA, HB) are formed
+// here too when H is given, or their diagonal terms when Dn is given (S =
+// I), else by the forward sweep's launch on the CUs its strips leave idle
+// (k_fast_fwd).  Workgroup = one work item (16 steps x 16 rows of one quad
+// strip); a wave takes 4 rows x 16 steps, so each load of a cell field reads
+// 4 runs of 16 consecutive cells.
+__global__ void __launch_bounds__(256) k_fast_rhs_prep(DevCtx d, const QStrip* strips, const int2* work,
+                                                       int dqdt_inline, int abpm_inline, double* R_sl, double* G,
+                                                       double* H, double* Dn, double* hand, int nstrips,
+                                                       long long hstride) {
     hand_refill(d, strips, hand, nstrips, hstride);
     const int2 wk = work[blockIdx.x];
     const QStrip S = strips[wk.x];
     const DevBlock B = d.blk[S.b];
-    // a wave takes 4 rows x 16 steps: each load of a cell field then reads
-    // 4 runs of 16 consecutive cells
     const int dd = wk.y + (int)(threadIdx.x & 15), r = (int)(threadIdx.x >> 4);
-    prep_cell(d, S, B, dd, r, R_sl, G, H, Dn);
+    int i, j;
+    if (!slot_cell(S, B, dd, r, i, j)) return;
+    const long long c = cidx(B, i, j);
+    CellJac<1> J;
+    cell_jacobian<1>(d, c, B.pitch, J);
+    double s[4], x[4];
+    if (abpm_inline) {
+        store_abpm_scalars<1>(d, c, J);
+        flux_balance<1>(d, B, c, s);
+        rhs_of_cell<1>(
+            d, c, s, dqdt_inline != 0, [&](int k) { return J.pre1[k]; }, [&](int k) { return J.pre2[k]; },
+            [&]() { return J.ps[1]; }, x);
+    } else {
+        flux_balance<1>(d, B, c, s);
+        rhs_of_cell<1>(
+            d, c, s, dqdt_inline != 0, [&](int k) { return FLD(d, SHCFD_F_PRE1, k, c); },
+            [&](int k) { return FLD(d, SHCFD_F_PRE2, k, c); }, [&]() { return FLD(d, SHCFD_F_PRESCA, 1, c); }, x);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) R_sl[slot1(S.base, dd, 4 * r + k)] = x[k];
+    prep_ops(d, S, dd, r, c, J, G, H, Dn);
 }
 
 // ---------------------------------------------------------------------------
 // Hand-off between quad strips.  The producer's boundary row stores its
 // outgoing contribution with an agent-coherent 64-bit store into a slot that
-// holds a signalling-NaN sentinel (filled by k_fast_prep before every
+// holds a signalling-NaN sentinel (filled by k_fast_rhs_prep before every
 // sweep); the consumer polls the slot until the sentinel is gone.  No flag
 // and no store-completion wait is needed, so the producer never drains its
 // load pipeline.  Arithmetic never produces this NaN payload.
@@ -425,7 +450,7 @@ k_fast_fwd(DevCtx d, const QStrip* strips, int nstrips, const double* R_sl, cons
 #pragma unroll 1
             for (int h = 0; h < 2; h++) {
                 const int t = (int)threadIdx.x + 128 * h;
-                prep_cell(d, S, B, wk.y + (t & 15), t >> 4, nullptr, nullptr, H);
+                prep_cell_h(d, S, B, wk.y + (t & 15), t >> 4, H);
             }
         }
         return;
@@ -795,7 +820,7 @@ __global__ void __launch_bounds__(256) k_fast_adj_post(DevCtx d, const QStrip* s
     for (int k = 0; k < 4; k++) lam[k * d.NC + c] += Z_sl[slot1(S.base, dd, 4 * r + k)];
 }
 
-// refill of the hand-off rows alone (the adjoint sweeps do not run k_fast_prep)
+// refill of the hand-off rows alone (the adjoint sweeps do not run k_fast_rhs_prep)
 __global__ void __launch_bounds__(256) k_fast_hand_fill(DevCtx d, const QStrip* strips, double* hand, int nstrips,
                                                         long long hstride) {
     hand_refill(d, strips, hand, nstrips, hstride);
@@ -896,7 +921,8 @@ void FastPath::release() {
     *this = FastPath();
 }
 
-int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*pe)(Ctx*, int)) {
+int FastPath::implicit(Ctx& c, int step_internal, int dqdt_inline, int abpm_inline, void (*pb)(Ctx*, int),
+                       void (*pe)(Ctx*, int)) {
     DevCtx d = c.dev();
     const bool dual = c.p.scheme == SHCFD_SCHEME_DUAL;
     hipStream_t st = c.stream;
@@ -921,7 +947,7 @@ int FastPath::implicit(Ctx& c, int step_internal, void (*pb)(Ctx*, int), void (*
     const unsigned FWD_PAD_LDS = busy ? 0u : 28672u;
     const int nhwg = (busy || diag) ? 0 : std::min(nwork, 512);
     pb(&c, PROF_PREP);
-    hipLaunchKernelGGL(k_fast_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, R, G,
+    hipLaunchKernelGGL(k_fast_rhs_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, dqdt_inline, abpm_inline, R, G,
                        (busy && !diag) ? H : nullptr, diag ? Dn : nullptr, hand, nstrips, hstride);
     pe(&c, PROF_PREP);
     pb(&c, PROF_SWEEP_FWD);

@@ -29,10 +29,10 @@ namespace shcfd {
 // progress flags.  With dqdt_inline set (whole steps of the inner-loop
 // schemes) the cell's dq/dt of set_dqdt (sch_dual.raw.f90:211-240) is formed
 // here, with set_dqdt's expressions, stored and used, instead of in a pass
-// of its own.  With abpm_inline set (whole steps on the optimised sweeps,
-// which need no stored Jacobians) the cell's set_ABpm scalars are formed and
-// stored here first, by the same code as the set_ABpm pass, which then does
-// not run.
+// of its own.  With abpm_inline set (the set_ABpm pass was left out, as for
+// the optimised sweeps, whose prep pass k_fast_rhs_prep forms the scalars and
+// this right-hand side itself) the cell's set_ABpm scalars are formed and
+// stored here first, by the same code as the set_ABpm pass.
 // ---------------------------------------------------------------------------
 template <int NY>
 __global__ void __launch_bounds__(256) k_rhs(DevCtx d, const Tile* tiles, int* prog, int nprog, int dqdt_inline,
@@ -46,49 +46,13 @@ __global__ void __launch_bounds__(256) k_rhs(DevCtx d, const Tile* tiles, int* p
     if (t.i < 1 || t.i > B.ni || t.j < 1 || t.j > B.nj) return;
     if (abpm_inline) set_abpm_cell<NY, 0>(d, B, t.i, t.j, nullptr, 0, 0);
     const long long c = cidx(B, t.i, t.j);
-    double s[dq];
+    double s[dq], x[dq];
     flux_balance<NY>(d, B, c, s);
-    const int scheme = d.p.scheme;
-    if (dqdt_inline) {
-        const double dtv = (d.p.time_step == 0) ? d.scal[SCAL_DT_GRBL] : FLD(d, SHCFD_F_DT, 0, c);
-        const bool first = d.scal[SCAL_FIRST_DUAL] != 0.0;
+    rhs_of_cell<NY>(
+        d, c, s, dqdt_inline != 0, [&](int k) { return FLD(d, SHCFD_F_PRE1, k, c); },
+        [&](int k) { return FLD(d, SHCFD_F_PRE2, k, c); }, [&]() { return FLD(d, SHCFD_F_PRESCA, 1, c); }, x);
 #pragma unroll
-        for (int k = 0; k < dq; k++) {
-            const double q = FLD(d, SHCFD_F_Q, k, c), qp = FLD(d, SHCFD_F_QP, k, c);
-            FLD(d, SHCFD_F_DQDT, k, c) =
-                first ? (q - qp) / dtv : (((3.0 * q) - (4.0 * qp)) + FLD(d, SHCFD_F_QPP, k, c)) / (2.0 * dtv);
-        }
-    }
-    if (scheme == SHCFD_SCHEME_DUAL) {
-        double vol = FLD(d, SHCFD_F_VOL, 0, c);
-        double x[dq];
-#pragma unroll
-        for (int k = 0; k < dq; k++) x[k] = s[k] - vol * FLD(d, SHCFD_F_DQDT, k, c);
-        double dt = 0.0;
-#pragma unroll
-        for (int k = 0; k < dq; k++) dt = dt + FLD(d, SHCFD_F_PRE2, k, c) * x[k];
-        double ps2 = FLD(d, SHCFD_F_PRESCA, 1, c);
-#pragma unroll
-        for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = x[k] + (ps2 * dt) * FLD(d, SHCFD_F_PRE1, k, c);
-    } else if (scheme == SHCFD_SCHEME_PRECON_LUSGS || scheme == SHCFD_SCHEME_PRECON) {
-        const bool inner = scheme == SHCFD_SCHEME_PRECON;
-        const double vol = inner ? FLD(d, SHCFD_F_VOL, 0, c) : 0.0;
-        double dt = 0.0;
-#pragma unroll
-        for (int k = 0; k < dq; k++) dt = dt + FLD(d, SHCFD_F_PRE2, k, c) * s[k];
-#pragma unroll
-        for (int k = 0; k < dq; k++) {
-            const double x = s[k] + FLD(d, SHCFD_F_PRE1, k, c) * dt;
-            FLD(d, SHCFD_F_DQ, k, c) = inner ? x - vol * FLD(d, SHCFD_F_DQDT, k, c) : x;
-        }
-    } else if (scheme == SHCFD_SCHEME_NR) {
-        const double vol = FLD(d, SHCFD_F_VOL, 0, c);
-#pragma unroll
-        for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = s[k] - vol * FLD(d, SHCFD_F_DQDT, k, c);
-    } else {
-#pragma unroll
-        for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = s[k];
-    }
+    for (int k = 0; k < dq; k++) FLD(d, SHCFD_F_DQ, k, c) = x[k];
 }
 
 template <int DQN>

@@ -1020,15 +1020,18 @@ static int implicit_async(shcfd_ctx* ctx, int step_internal) {
     ctx->q16_active = 0;
     ctx->jd16_active = 0;
     const int C16 = (ctx->dimq + 3) / 4, ns16 = ctx->nstrips16;
+    if (fast_enabled(ctx)) {
+        // the optimised update forms the right-hand side in its prep pass
+        int rc = ctx->fast->implicit(*ctx, step_internal, ctx->dqdt_inline, ctx->abpm_inline, prof_begin, prof_end);
+        ctx->dqdt_inline = 0;
+        ctx->abpm_inline = 0;
+        LAUNCHCHK();
+        return rc;
+    }
     { ProfScope ps(ctx, PROF_RHS);
     DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_rhs<NY>, ctx->dprog, 2 * ns, ctx->dqdt_inline, ctx->abpm_inline)); }
     ctx->dqdt_inline = 0;
     ctx->abpm_inline = 0;
-    if (fast_enabled(ctx)) {
-        int rc = ctx->fast->implicit(*ctx, step_internal, prof_begin, prof_end);
-        LAUNCHCHK();
-        return rc;
-    }
     if (q16) {
         const dim3 blk((unsigned)(64 * C16));
         double* hand_f = ctx->dbnd16;

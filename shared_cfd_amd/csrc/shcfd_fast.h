@@ -22,7 +22,10 @@ struct FastPath {
     double *aR = nullptr, *aV = nullptr, *aZ = nullptr, *aG = nullptr, *aH = nullptr, *ahand = nullptr;
     int init(Ctx& c);
     void release();
-    int implicit(Ctx& c, int step_internal, void (*prof_begin)(Ctx*, int), void (*prof_end)(Ctx*, int));
+    // right-hand side (with dq/dt and set_ABpm's scalars formed when asked),
+    // operators, both sweeps and the update
+    int implicit(Ctx& c, int step_internal, int dqdt_inline, int abpm_inline, void (*prof_begin)(Ctx*, int),
+                 void (*prof_end)(Ctx*, int));
     int adjoint_prep(Ctx& c);                                      // transposed operators at the current state
     int adjoint_apply(Ctx& c, const double* r_cell, double* lam);  // lam += M^-T r
 };

@@ -586,6 +586,91 @@ __global__ void k_geojac(DevCtx, const Tile*);
 // k_nasa9.hip
 __global__ void k_thermo_cold(DevCtx d, const Tile* tiles, const shcfd_nasa9* __restrict__ m);
 
+// right-hand side of the implicit update of one cell from its flux balance s:
+// the dq/dt term (formed and stored here when dqdt_inline), then the scheme's
+// form (time_dual_viscous.f90:37-43, time_precon_viscous.f90 of both
+// preconditioned schemes, time_NR_viscous.f90, time_lusgs_viscous.f90);
+// pre1(k), pre2(k) and ps2() give set_ABpm's scalars of the cell
+template <int NY, class P1, class P2, class PS>
+__device__ inline void rhs_of_cell(const DevCtx& d, long long c, const double* s, bool dqdt_inline, P1 pre1, P2 pre2,
+                                   PS ps2, double* x) {
+    constexpr int dq = NY + 3;
+    const int scheme = d.p.scheme;
+    if (dqdt_inline) {
+        const double dtv = (d.p.time_step == 0) ? d.scal[SCAL_DT_GRBL] : FLD(d, SHCFD_F_DT, 0, c);
+        const bool first = d.scal[SCAL_FIRST_DUAL] != 0.0;
+#pragma unroll
+        for (int k = 0; k < dq; k++) {
+            const double q = FLD(d, SHCFD_F_Q, k, c), qp = FLD(d, SHCFD_F_QP, k, c);
+            FLD(d, SHCFD_F_DQDT, k, c) =
+                first ? (q - qp) / dtv : (((3.0 * q) - (4.0 * qp)) + FLD(d, SHCFD_F_QPP, k, c)) / (2.0 * dtv);
+        }
+    }
+    if (scheme == SHCFD_SCHEME_DUAL) {
+        double vol = FLD(d, SHCFD_F_VOL, 0, c);
+        double y[dq];
+#pragma unroll
+        for (int k = 0; k < dq; k++) y[k] = s[k] - vol * FLD(d, SHCFD_F_DQDT, k, c);
+        double dt = 0.0;
+#pragma unroll
+        for (int k = 0; k < dq; k++) dt = dt + pre2(k) * y[k];
+        const double p2 = ps2();
+#pragma unroll
+        for (int k = 0; k < dq; k++) x[k] = y[k] + (p2 * dt) * pre1(k);
+    } else if (scheme == SHCFD_SCHEME_PRECON_LUSGS || scheme == SHCFD_SCHEME_PRECON) {
+        const bool inner = scheme == SHCFD_SCHEME_PRECON;
+        const double vol = inner ? FLD(d, SHCFD_F_VOL, 0, c) : 0.0;
+        double dt = 0.0;
+#pragma unroll
+        for (int k = 0; k < dq; k++) dt = dt + pre2(k) * s[k];
+#pragma unroll
+        for (int k = 0; k < dq; k++) {
+            const double y = s[k] + pre1(k) * dt;
+            x[k] = inner ? y - vol * FLD(d, SHCFD_F_DQDT, k, c) : y;
+        }
+    } else if (scheme == SHCFD_SCHEME_NR) {
+        const double vol = FLD(d, SHCFD_F_VOL, 0, c);
+#pragma unroll
+        for (int k = 0; k < dq; k++) x[k] = s[k] - vol * FLD(d, SHCFD_F_DQDT, k, c);
+    } else {
+#pragma unroll
+        for (int k = 0; k < dq; k++) x[k] = s[k];
+    }
+}
+
+// set_ABpm's per-cell scalars: alpha, the implicit pressure source's dp/dq
+// (axisymmetric), the preconditioning vectors and scalars, the local dt of the
+// preconditioned schemes
+template <int NY>
+__device__ inline void store_abpm_scalars(const DevCtx& d, long long c, const CellJac<NY>& J) {
+    constexpr int dq = NY + 3;
+    FLD(d, SHCFD_F_ALPHA, 0, c) = J.alpha;
+    if (d.p.dimension == SHCFD_DIM_AXISYMMETRIC) {
+        // dp/dq of the implicit pressure source, kept in pre2 (unused by the
+        // axisymmetric schemes): LU-SGS/q2d/sch_lusgs.raw.f90:43-49,
+        // NR/q2d/sch_NR.raw.f90:43-49
+        const double u = FLD(d, SHCFD_F_W, IW_U, c), v = FLD(d, SHCFD_F_W, IW_V, c);
+        const double gmmad = FLD(d, SHCFD_F_W, dq + 1, c) - 1.0;
+        const double Vs = (u * u) + (v * v);
+        for (int k = 0; k < NY; k++) FLD(d, SHCFD_F_PRE2, k, c) = ((0.5 * gmmad) * Vs) + FLD(d, SHCFD_F_DHI, k, c);
+        FLD(d, SHCFD_F_PRE2, NY, c) = -(gmmad * u);
+        FLD(d, SHCFD_F_PRE2, NY + 1, c) = -(gmmad * v);
+        FLD(d, SHCFD_F_PRE2, NY + 2, c) = gmmad;
+    }
+    if (J.precond) {
+        FLD(d, SHCFD_F_PHI, 0, c) = J.phi;
+        for (int k = 0; k < dq; k++) {
+            FLD(d, SHCFD_F_PRE1, k, c) = J.pre1[k];
+            FLD(d, SHCFD_F_PRE2, k, c) = J.pre2[k];
+        }
+        if (d.p.scheme == SHCFD_SCHEME_DUAL) {
+            for (int k = 0; k < 4; k++) FLD(d, SHCFD_F_PRESCA, k, c) = J.ps[k];
+        } else {
+            FLD(d, SHCFD_F_DT, 0, c) = J.dtm;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // set_ABpm of one cell (LU-SGS, preconLU-SGS, NR, precon, dual): the stored
 // scalars (alpha; phi, pre1, pre2 and pre_sca or the local dt of the
@@ -644,32 +729,7 @@ __device__ inline void set_abpm_cell(const DevCtx& d, const DevBlock& B, int i, 
             }
         }
     }
-    FLD(d, SHCFD_F_ALPHA, 0, c) = J.alpha;
-    if (d.p.dimension == SHCFD_DIM_AXISYMMETRIC) {
-        // dp/dq of the implicit pressure source, kept in pre2 (unused by the
-        // axisymmetric schemes): LU-SGS/q2d/sch_lusgs.raw.f90:43-49,
-        // NR/q2d/sch_NR.raw.f90:43-49
-        const double u = FLD(d, SHCFD_F_W, IW_U, c), v = FLD(d, SHCFD_F_W, IW_V, c);
-        const double gmmad = FLD(d, SHCFD_F_W, dq + 1, c) - 1.0;
-        const double Vs = (u * u) + (v * v);
-        for (int k = 0; k < NY; k++) FLD(d, SHCFD_F_PRE2, k, c) = ((0.5 * gmmad) * Vs) + FLD(d, SHCFD_F_DHI, k, c);
-        FLD(d, SHCFD_F_PRE2, NY, c) = -(gmmad * u);
-        FLD(d, SHCFD_F_PRE2, NY + 1, c) = -(gmmad * v);
-        FLD(d, SHCFD_F_PRE2, NY + 2, c) = gmmad;
-    }
-    if (J.precond) {
-        FLD(d, SHCFD_F_PHI, 0, c) = J.phi;
-        for (int k = 0; k < dq; k++) {
-            FLD(d, SHCFD_F_PRE1, k, c) = J.pre1[k];
-            FLD(d, SHCFD_F_PRE2, k, c) = J.pre2[k];
-        }
-        if (d.p.scheme == SHCFD_SCHEME_DUAL) {
-            for (int k = 0; k < 4; k++) FLD(d, SHCFD_F_PRESCA, k, c) = J.ps[k];
-        } else {
-            FLD(d, SHCFD_F_DT, 0, c) = J.dtm;
-        }
-    }
+    store_abpm_scalars<NY>(d, c, J);
 }
-
 
 }  // namespace shcfd
