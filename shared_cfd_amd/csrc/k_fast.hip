@@ -669,10 +669,13 @@ __global__ void __launch_bounds__(256) k_fast_post(DevCtx d, const QStrip* strip
     const int2 wk = work[blockIdx.x];
     const QStrip S = strips[wk.x];
     const DevBlock B = d.blk[S.b];
-    const int r = threadIdx.x & 15;
-    const int dd = wk.y + (threadIdx.x >> 4);
     const bool dual = d.p.scheme == SHCFD_SCHEME_DUAL;
     const bool inner = inner_loop_scheme(d.p.scheme);
+    // without the (res, omega) partials a wave takes 4 rows x 16 steps, as in
+    // k_fast_rhs_prep; with them it keeps 16 rows x 4 steps, the layout their
+    // summation order follows
+    const int r = inner ? (int)(threadIdx.x & 15) : (int)(threadIdx.x >> 4);
+    const int dd = wk.y + (inner ? (int)(threadIdx.x >> 4) : (int)(threadIdx.x & 15));
     double res = 0.0, om = 1e300;
     int i, j;
     if (slot_cell(S, B, dd, r, i, j)) {
