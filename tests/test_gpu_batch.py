@@ -118,3 +118,36 @@ def test_tile_rows_do_not_change_cell_results():
     for o in out[1:]:
         for x, y in zip(o, out[0]):
             assert np.array_equal(x, y)
+
+
+def test_dual_batch_with_more_strips_than_cus_matches_per_routine_sweeps():
+    """Dual time on a batch with more quad strips than the GPU has compute
+    units: the optimised update then forms the backward operators in its
+    right-hand-side pass instead of on idle CUs, and numbers the strips of
+    all blocks interleaved (strip k of every block before strip k+1 of any).
+    Two physical steps agree with the per-routine sweeps: the increments to
+    1e-11 of their size (the optimised path rounds differently)."""
+    case = C.batched_plates(n=48, seed=5)
+    nstrips = sum((nj + 15) // 16 for nj in case.nj)
+    assert nstrips > 256, nstrips  # an MI355X has 256 CUs
+    p = case.params
+    p.scheme = C.SCHEME_DUAL
+    p.time_step = 0  # dual time steps every cell with the global dt
+    p.num_internal_loop = 4
+    p.cfl_tau, p.omega_max, p.dqmax = 5.0, 1.0, 0.5
+    p.res_rate_warning = p.res_rate_error = 1e300
+    f, s = Solver(case, fast=True), Solver(case, fast=False)
+    f.prepare()
+    s.prepare()
+    q0 = _q(s, case)
+    for k in range(2):
+        rf, rs = f.step(k + 1, 0), s.step(k + 1, 0)
+        assert np.isfinite(rf).all() and np.isfinite(rs).all()
+    # the increments of the two steps, relative to their size per block
+    worst = 0.0
+    for qf, qs, qi in zip(_q(f, case), _q(s, case), q0):
+        dq = np.max(np.abs(qs - qi))
+        assert dq > 0.0
+        worst = max(worst, float(np.max(np.abs(qf - qs)) / dq))
+    print(f"dual batch, {nstrips} strips: max rel {worst:.3e} of the increments")
+    assert worst <= 1e-11
