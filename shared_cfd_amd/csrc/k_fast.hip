@@ -163,6 +163,15 @@ __device__ inline void prep_cell_h(const DevCtx& d, const QStrip& S, const DevBl
     prep_ops(d, S, dd, r, c, J, nullptr, H, nullptr);
 }
 
+// XCD-aware work order.  The dispatcher hands a launch's workgroups to the
+// 8 XCDs in turn (workgroup b runs on XCD b % 8).  Giving XCD x the
+// contiguous work items x*per .. x*per + per - 1 keeps neighbouring items,
+// which share cache lines at their edges, in one L2.  A launch then has
+// 8*per workgroups; those past the last item do nothing but the refill.
+constexpr int NXCD = 8;
+__host__ __device__ inline int xcd_groups(int n) { return NXCD * ((n + NXCD - 1) / NXCD); }
+__device__ inline int xcd_item(int b, int n) { return (b % NXCD) * ((n + NXCD - 1) / NXCD) + b / NXCD; }
+
 // refill of the hand-off rows of both sweeps: the sentinel for the columns
 // 1..ni of each strip's block, zero elsewhere (grid-stride over all rows)
 __device__ inline void hand_refill(const DevCtx& d, const QStrip* strips, double* hand, int nstrips, long long hstride) {
@@ -186,14 +195,16 @@ __device__ inline void hand_refill(const DevCtx& d, const QStrip* strips, double
 // here too when H is given, or their diagonal terms when Dn is given (S =
 // I), else by the forward sweep's launch on the CUs its strips leave idle
 // (k_fast_fwd).  Workgroup = one work item (16 steps x 16 rows of one quad
-// strip); a wave takes 4 rows x 16 steps, so each load of a cell field reads
+// strip, in XCD-aware order); a wave takes 4 rows x 16 steps, so each load of a cell field reads
 // 4 runs of 16 consecutive cells.
-__global__ void __launch_bounds__(256) k_fast_rhs_prep(DevCtx d, const QStrip* strips, const int2* work,
+__global__ void __launch_bounds__(256) k_fast_rhs_prep(DevCtx d, const QStrip* strips, const int2* work, int nwork,
                                                        int dqdt_inline, int abpm_inline, double* R_sl, double* G,
                                                        double* H, double* Dn, double* hand, int nstrips,
                                                        long long hstride) {
     hand_refill(d, strips, hand, nstrips, hstride);
-    const int2 wk = work[blockIdx.x];
+    const int w = xcd_item((int)blockIdx.x, nwork);
+    if (w >= nwork) return;
+    const int2 wk = work[w];
     const QStrip S = strips[wk.x];
     const DevBlock B = d.blk[S.b];
     const int dd = wk.y + (int)(threadIdx.x & 15), r = (int)(threadIdx.x >> 4);
@@ -520,13 +531,16 @@ k_fast_fwd(DevCtx d, const QStrip* strips, int nstrips, const double* R_sl, cons
 // between the sweeps (dual only): x2/alpha from v, in the skewed layout
 // (time_dual_viscous.f90:44-60 applied to Dq* = alpha*S*v)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_fast_mid(DevCtx d, const QStrip* strips, const int2* work, const double* V_sl,
-                                                  double* X_sl) {
-    const int2 wk = work[blockIdx.x];
+__global__ void __launch_bounds__(256) k_fast_mid(DevCtx d, const QStrip* strips, const int2* work, int nwork,
+                                                  const double* V_sl, double* X_sl) {
+    // work items in XCD-aware order; a wave takes 4 rows x 16 steps
+    const int w = xcd_item((int)blockIdx.x, nwork);
+    if (w >= nwork) return;
+    const int2 wk = work[w];
     const QStrip S = strips[wk.x];
     const DevBlock B = d.blk[S.b];
-    const int r = threadIdx.x & 15;
-    const int dd = wk.y + (threadIdx.x >> 4);
+    const int r = (int)(threadIdx.x >> 4);
+    const int dd = wk.y + (int)(threadIdx.x & 15);
     int i, j;
     if (!slot_cell(S, B, dd, r, i, j)) return;
     const long long c = cidx(B, i, j);
@@ -663,10 +677,14 @@ k_fast_bwd(DevCtx d, const QStrip* strips, int nstrips, const double* X_sl, cons
 // post: Dq = alpha*S*z into the natural DQ field; inner-loop schemes: (res,
 // omega) partials per workgroup; otherwise q += Dq
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_fast_post(DevCtx d, const QStrip* strips, const int2* work, const double* Z_sl,
-                                                   double* part) {
+__global__ void __launch_bounds__(256) k_fast_post(DevCtx d, const QStrip* strips, const int2* work, int nwork,
+                                                   const double* Z_sl, double* part) {
     __shared__ double shr[4], sho[4];
-    const int2 wk = work[blockIdx.x];
+    // work items in XCD-aware order; the partials are stored per item, so
+    // their summation order does not depend on it
+    const int wi = xcd_item((int)blockIdx.x, nwork);
+    if (wi >= nwork) return;
+    const int2 wk = work[wi];
     const QStrip S = strips[wk.x];
     const DevBlock B = d.blk[S.b];
     const bool dual = d.p.scheme == SHCFD_SCHEME_DUAL;
@@ -718,8 +736,8 @@ __global__ void __launch_bounds__(256) k_fast_post(DevCtx d, const QStrip* strip
     if ((threadIdx.x & 63) == 0) { shr[w] = res; sho[w] = om; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = ((shr[0] + shr[1]) + (shr[2] + shr[3]));
-        part[2 * blockIdx.x + 1] = dmin(dmin(sho[0], sho[1]), dmin(sho[2], sho[3]));
+        part[2 * wi] = ((shr[0] + shr[1]) + (shr[2] + shr[3]));
+        part[2 * wi + 1] = dmin(dmin(sho[0], sho[1]), dmin(sho[2], sho[3]));
     }
 }
 
@@ -950,7 +968,8 @@ int FastPath::implicit(Ctx& c, int step_internal, int dqdt_inline, int abpm_inli
     const unsigned FWD_PAD_LDS = busy ? 0u : 28672u;
     const int nhwg = (busy || diag) ? 0 : std::min(nwork, 512);
     pb(&c, PROF_PREP);
-    hipLaunchKernelGGL(k_fast_rhs_prep, dim3(nwork), dim3(256), 0, st, d, strips, work, dqdt_inline, abpm_inline, R, G,
+    hipLaunchKernelGGL(k_fast_rhs_prep, dim3(xcd_groups(nwork)), dim3(256), 0, st, d, strips, work, nwork, dqdt_inline,
+                       abpm_inline, R, G,
                        (busy && !diag) ? H : nullptr, diag ? Dn : nullptr, hand, nstrips, hstride);
     pe(&c, PROF_PREP);
     pb(&c, PROF_SWEEP_FWD);
@@ -960,7 +979,7 @@ int FastPath::implicit(Ctx& c, int step_internal, int dqdt_inline, int abpm_inli
     const double* Xin = V;
     if (dual) {
         pb(&c, PROF_PREP);
-        hipLaunchKernelGGL(k_fast_mid, dim3(nwork), dim3(256), 0, st, d, strips, work, V, X);
+        hipLaunchKernelGGL(k_fast_mid, dim3(xcd_groups(nwork)), dim3(256), 0, st, d, strips, work, nwork, V, X);
         pe(&c, PROF_PREP);
         Xin = X;
     }
@@ -973,7 +992,7 @@ int FastPath::implicit(Ctx& c, int step_internal, int dqdt_inline, int abpm_inli
                            hand_b, hstride, c.spin_limit);
     pe(&c, PROF_SWEEP_BWD);
     pb(&c, PROF_UPDATE);
-    hipLaunchKernelGGL(k_fast_post, dim3(nwork), dim3(256), 0, st, d, strips, work, Z, part);
+    hipLaunchKernelGGL(k_fast_post, dim3(xcd_groups(nwork)), dim3(256), 0, st, d, strips, work, nwork, Z, part);
     if (inner_loop_scheme(c.p.scheme)) {
         hipLaunchKernelGGL(k_reduce_res_omega, dim3(1), dim3(256), 0, st, d, part, nwork);
         if (step_internal == 1) hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, st, d, (int)SCAL_RES1, (int)SCAL_RES);

@@ -481,7 +481,7 @@ __global__ void __launch_bounds__(256) k_set_dt(DevCtx d, const Tile* tiles, dou
         FLD(d, SHCFD_F_DT, 0, c) = dt;
     }
     double m = block_reduce_min(dt, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = m;
+    if (threadIdx.x == 0) part[tiles[blockIdx.x].slot] = m;
 }
 
 // final fixed-order reduction of per-tile partials (one workgroup)
@@ -517,7 +517,7 @@ __global__ void __launch_bounds__(256) k_rms_part(DevCtx d, const Tile* tiles, d
         v = e * e;
     }
     double s = block_reduce_sum(v, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (threadIdx.x == 0) part[tiles[blockIdx.x].slot] = s;
 }
 
 // ---------------------------------------------------------------------------

@@ -116,8 +116,25 @@ static void build_tiles(shcfd::Ctx* ctx, int rows) {
     for (int b = 0; b < ctx->nb; b++) {
         const DevBlock& B = ctx->hblk[b];
         for (int j0 = 0; j0 <= B.nj + 1; j0 += rows)
-            for (int i0 = 0; i0 <= B.ni + 1; i0 += TI) ctx->htiles.push_back(Tile{b, i0, j0, 0});
+            for (int i0 = 0; i0 <= B.ni + 1; i0 += TI)
+                ctx->htiles.push_back(Tile{b, i0, j0, (int)ctx->htiles.size()});
     }
+}
+
+// The tiles as the device list holds them: in XCD-aware launch order.  The
+// dispatcher hands a launch's workgroups to the 8 XCDs in turn (workgroup p
+// runs on XCD p % 8), so position p = 8k + x holds tile x*per + k of the
+// block order: each XCD takes a contiguous range of tiles, and the rows a
+// stencil shares between neighbouring tiles stay in one L2.  The per-tile
+// partials are stored at each tile's slot, so their order is unchanged.
+static std::vector<Tile> tiles_in_launch_order(const std::vector<Tile>& tiles) {
+    const int n = (int)tiles.size(), per = (n + 7) / 8;
+    std::vector<Tile> out;
+    out.reserve(tiles.size());
+    for (int k = 0; k < per; k++)
+        for (int x = 0; x < 8; x++)
+            if (x * per + k < n) out.push_back(tiles[x * per + k]);
+    return out;
 }
 
 extern "C" {
@@ -237,7 +254,10 @@ int shcfd_create(int nblocks, const int* ni, const int* nj, int ny, const shcfd_
         ok = al((void**)&ctx->f[fld], sizeof(double) * (size_t)field_ncomp(fld, ny) * (size_t)ctx->NC);
     if (!ok) { shcfd_destroy(ctx); return SHCFD_E_NOMEM; }
     hipMemcpy(ctx->dblk, ctx->hblk.data(), sizeof(DevBlock) * nblocks, hipMemcpyHostToDevice);
-    hipMemcpy(ctx->dtiles, ctx->htiles.data(), sizeof(Tile) * ctx->htiles.size(), hipMemcpyHostToDevice);
+    {
+        const std::vector<Tile> lt = tiles_in_launch_order(ctx->htiles);
+        hipMemcpy(ctx->dtiles, lt.data(), sizeof(Tile) * lt.size(), hipMemcpyHostToDevice);
+    }
     hipMemcpy(ctx->dstrips, ctx->hstrips.data(), sizeof(Strip) * ctx->hstrips.size(), hipMemcpyHostToDevice);
     double one = 1.0;
     hipMemcpy(ctx->scal + SCAL_FIRST_DUAL, &one, sizeof(double), hipMemcpyHostToDevice);
@@ -1144,7 +1164,8 @@ int shcfd_set_option(shcfd_ctx* ctx, int option, int value) {
             hipFree(nt);
             return SHCFD_E_NOMEM;
         }
-        HIPCHK(hipMemcpy(nt, ctx->htiles.data(), sizeof(Tile) * ctx->htiles.size(), hipMemcpyHostToDevice));
+        const std::vector<Tile> lt = tiles_in_launch_order(ctx->htiles);
+        HIPCHK(hipMemcpy(nt, lt.data(), sizeof(Tile) * lt.size(), hipMemcpyHostToDevice));
         hipFree(ctx->dtiles);
         hipFree(ctx->dpart);
         ctx->dtiles = nt;

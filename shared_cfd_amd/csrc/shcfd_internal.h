@@ -78,7 +78,8 @@ constexpr int TI = 64;
 constexpr int TJ = 4;
 
 struct Tile {
-    int b, i0, j0, pad;
+    int b, i0, j0;
+    int slot;  // the tile's position in block order: its slot in per-tile partial arrays
 };
 
 // One wavefront-strip of the LU-SGS sweeps: 64 consecutive rows of a block.
