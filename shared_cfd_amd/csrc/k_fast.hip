@@ -763,13 +763,16 @@ __device__ inline void put_transposed(double* o, const QStrip& S, int dd, int r,
             o[op_index(S, dd, 4 * r + k, pbase + (l >> 1), l & 1)] = present ? -(fs * M[l + 4 * k]) : 0.0;
 }
 
-__global__ void __launch_bounds__(256) k_fast_adj_prep(DevCtx d, const QStrip* strips, const int2* work, double* G,
-                                                       double* H) {
-    const int2 wk = work[blockIdx.x];
+__global__ void __launch_bounds__(256) k_fast_adj_prep(DevCtx d, const QStrip* strips, const int2* work, int nwork,
+                                                       double* G, double* H) {
+    // work items in XCD-aware order; a wave takes 4 rows x 16 steps
+    const int w = xcd_item((int)blockIdx.x, nwork);
+    if (w >= nwork) return;
+    const int2 wk = work[w];
     const QStrip S = strips[wk.x];
     const DevBlock B = d.blk[S.b];
-    const int r = threadIdx.x & 15;
-    const int dd = wk.y + (threadIdx.x >> 4);
+    const int r = (int)(threadIdx.x >> 4);
+    const int dd = wk.y + (int)(threadIdx.x & 15);
     int i, j;
     if (!slot_cell(S, B, dd, r, i, j)) return;
     const long long c = cidx(B, i, j), P = B.pitch;
@@ -811,13 +814,16 @@ __global__ void __launch_bounds__(256) k_fast_adj_prep(DevCtx d, const QStrip* s
 }
 
 // right-hand side of the transposed sweep: alpha r into the skewed layout
-__global__ void __launch_bounds__(256) k_fast_adj_rhs(DevCtx d, const QStrip* strips, const int2* work,
+__global__ void __launch_bounds__(256) k_fast_adj_rhs(DevCtx d, const QStrip* strips, const int2* work, int nwork,
                                                       const double* r_cell, double* R_sl) {
-    const int2 wk = work[blockIdx.x];
+    // work items in XCD-aware order; a wave takes 4 rows x 16 steps
+    const int w = xcd_item((int)blockIdx.x, nwork);
+    if (w >= nwork) return;
+    const int2 wk = work[w];
     const QStrip S = strips[wk.x];
     const DevBlock B = d.blk[S.b];
-    const int r = threadIdx.x & 15;
-    const int dd = wk.y + (threadIdx.x >> 4);
+    const int r = (int)(threadIdx.x >> 4);
+    const int dd = wk.y + (int)(threadIdx.x & 15);
     int i, j;
     if (!slot_cell(S, B, dd, r, i, j)) return;
     const long long c = cidx(B, i, j);
@@ -827,13 +833,16 @@ __global__ void __launch_bounds__(256) k_fast_adj_rhs(DevCtx d, const QStrip* st
 }
 
 // lambda += x (no alpha: the transposed product ends with the forward sweep's L^T)
-__global__ void __launch_bounds__(256) k_fast_adj_post(DevCtx d, const QStrip* strips, const int2* work,
+__global__ void __launch_bounds__(256) k_fast_adj_post(DevCtx d, const QStrip* strips, const int2* work, int nwork,
                                                        const double* Z_sl, double* lam) {
-    const int2 wk = work[blockIdx.x];
+    // work items in XCD-aware order; a wave takes 4 rows x 16 steps
+    const int w = xcd_item((int)blockIdx.x, nwork);
+    if (w >= nwork) return;
+    const int2 wk = work[w];
     const QStrip S = strips[wk.x];
     const DevBlock B = d.blk[S.b];
-    const int r = threadIdx.x & 15;
-    const int dd = wk.y + (threadIdx.x >> 4);
+    const int r = (int)(threadIdx.x >> 4);
+    const int dd = wk.y + (int)(threadIdx.x & 15);
     int i, j;
     if (!slot_cell(S, B, dd, r, i, j)) return;
     const long long c = cidx(B, i, j);
@@ -1025,20 +1034,21 @@ int FastPath::adjoint_prep(Ctx& c) {
     // pseudo-time preconditioning the primal inner loop used.
     DevCtx dp = c.dev();
     if (dp.p.scheme == SHCFD_SCHEME_DUAL) dp.p.scheme = SHCFD_SCHEME_NR;
-    hipLaunchKernelGGL(k_fast_adj_prep, dim3(nwork), dim3(256), 0, c.stream, dp, strips, work, aG, aH);
+    hipLaunchKernelGGL(k_fast_adj_prep, dim3(xcd_groups(nwork)), dim3(256), 0, c.stream, dp, strips, work, nwork, aG,
+                       aH);
     return hipGetLastError() == hipSuccess ? SHCFD_OK : SHCFD_E_HIP;
 }
 
 int FastPath::adjoint_apply(Ctx& c, const double* r_cell, double* lam) {
     DevCtx d = c.dev();
     hipStream_t st = c.stream;
-    hipLaunchKernelGGL(k_fast_adj_rhs, dim3(nwork), dim3(256), 0, st, d, strips, work, r_cell, aR);
+    hipLaunchKernelGGL(k_fast_adj_rhs, dim3(xcd_groups(nwork)), dim3(256), 0, st, d, strips, work, nwork, r_cell, aR);
     hipLaunchKernelGGL(k_fast_hand_fill, dim3(256), dim3(256), 0, st, d, strips, ahand, nstrips, hstride);
     hipLaunchKernelGGL(k_fast_fwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, aR, aG, aV, ahand, hstride,
                        c.spin_limit, work, 0, nullptr);
     hipLaunchKernelGGL(k_fast_bwd<false>, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, aV, aH, nullptr, aZ,
                        ahand + hstride * (nstrips + 1), hstride, c.spin_limit);
-    hipLaunchKernelGGL(k_fast_adj_post, dim3(nwork), dim3(256), 0, st, d, strips, work, aZ, lam);
+    hipLaunchKernelGGL(k_fast_adj_post, dim3(xcd_groups(nwork)), dim3(256), 0, st, d, strips, work, nwork, aZ, lam);
     return hipGetLastError() == hipSuccess ? SHCFD_OK : SHCFD_E_HIP;
 }
 
