@@ -414,9 +414,9 @@ __global__ void k_adj_ghost_centres(DevCtx d, AdjView a, int b, int pass) {
 // per position along the side, one workgroup so the fixed-state sums reduce
 // in a fixed order)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_adj_bc_side(DevCtx d, AdjView a, const int* row, const double* pool,
-                                                     double* poolbar) {
-    __shared__ double red[256];
+// one side row, by all threads of the workgroup (red: 256 doubles of LDS)
+__device__ inline void adj_bc_row(const DevCtx& d, AdjView& a, const int* row, const double* pool, double* poolbar,
+                                  double* red) {
     const int b = row[0] - 1, side = row[1], kind = row[4], off = row[5];
     const DevBlock B = d.blk[b];
     const int n = (side <= SHCFD_SIDE_IMAX) ? B.nj : B.ni;
@@ -498,6 +498,20 @@ __global__ void __launch_bounds__(256) k_adj_bc_side(DevCtx d, AdjView a, const 
             __syncthreads();
         }
         if (threadIdx.x == 0) poolbar[off + k] += red[0];
+        __syncthreads();
+    }
+}
+
+// all side rows in one workgroup, last row first (the reverse of the forward
+// fill): rows that meet at a block corner add into the same interior cell,
+// so each row starts after a barrier.  The pool bars are cleared first.
+__global__ void __launch_bounds__(256) k_adj_bc_side(DevCtx d, AdjView a, const int* bc, int nbc, const double* pool,
+                                                     double* poolbar, int npool) {
+    __shared__ double red[256];
+    for (int k = (int)threadIdx.x; k < npool; k += (int)blockDim.x) poolbar[k] = 0.0;
+    __syncthreads();
+    for (int r = nbc - 1; r >= 0; r--) {
+        adj_bc_row(d, a, bc + SHCFD_BC_ROW * r, pool, poolbar, red);
         __syncthreads();
     }
 }

@@ -24,7 +24,7 @@ __global__ void k_adj_muscl(DevCtx, AdjView, int);
 __global__ void k_adj_bc_corner(DevCtx, AdjView);
 __global__ void k_adj_centres(DevCtx, AdjView, int);
 __global__ void k_adj_ghost_centres(DevCtx, AdjView, int, int);
-__global__ void k_adj_bc_side(DevCtx, AdjView, const int*, const double*, double*);
+__global__ void k_adj_bc_side(DevCtx, AdjView, const int*, int, const double*, double*, int);
 __global__ void k_adj_bc_cut(DevCtx, AdjView, const int*);
 __global__ void k_adj_cell(DevCtx, AdjView, int);
 __global__ void k_adj_nodes(DevCtx, AdjView, int);
@@ -215,13 +215,12 @@ void vjp_async(shcfd_ctx* ctx) {
         hipLaunchKernelGGL(k_adj_muscl, dim3((unsigned)((B.ni + 4 + 127) / 128), (unsigned)(B.nj + 4)), dim3(128), 0, st,
                            d, a, b);
     }
-    hipMemsetAsync(ctx->adj->poolbar, 0, sizeof(double) * (ctx->npool > 0 ? ctx->npool : 1), st);
     // reverse of the forward fill order: corners (read by the viscous
-    // fluxes only), side rows, cut rows
+    // fluxes only), side rows (one workgroup, which clears the pool bars
+    // first), cut rows
     if (vis) hipLaunchKernelGGL(k_adj_bc_corner, dim3((unsigned)((ctx->nb + 63) / 64)), dim3(64), 0, st, d, a);
-    for (int r = ctx->nbc - 1; r >= 0; r--)
-        hipLaunchKernelGGL(k_adj_bc_side, dim3(1), dim3(256), 0, st, d, a, ctx->dbc + SHCFD_BC_ROW * r, ctx->dpool,
-                           ctx->adj->poolbar);
+    hipLaunchKernelGGL(k_adj_bc_side, dim3(1), dim3(256), 0, st, d, a, ctx->dbc, ctx->nbc, ctx->dpool,
+                       ctx->adj->poolbar, ctx->npool > 0 ? ctx->npool : 1);
     for (int r = ctx->ncut - 1; r >= 0; r--) {
         const int w = ctx->hcut[SHCFD_CUT_ROW * r + 1];
         if (w > 0)
