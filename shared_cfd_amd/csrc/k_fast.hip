@@ -815,7 +815,10 @@ __global__ void __launch_bounds__(256) k_fast_adj_prep(DevCtx d, const QStrip* s
 
 // right-hand side of the transposed sweep: alpha r into the skewed layout
 __global__ void __launch_bounds__(256) k_fast_adj_rhs(DevCtx d, const QStrip* strips, const int2* work, int nwork,
-                                                      const double* r_cell, double* R_sl) {
+                                                      const double* r_cell, double* R_sl, double* hand, int nstrips,
+                                                      long long hstride) {
+    // the transposed sweeps' hand-off rows are refilled here too
+    hand_refill(d, strips, hand, nstrips, hstride);
     // work items in XCD-aware order; a wave takes 4 rows x 16 steps
     const int w = xcd_item((int)blockIdx.x, nwork);
     if (w >= nwork) return;
@@ -848,12 +851,6 @@ __global__ void __launch_bounds__(256) k_fast_adj_post(DevCtx d, const QStrip* s
     const long long c = cidx(B, i, j);
 #pragma unroll
     for (int k = 0; k < 4; k++) lam[k * d.NC + c] += Z_sl[slot1(S.base, dd, 4 * r + k)];
-}
-
-// refill of the hand-off rows alone (the adjoint sweeps do not run k_fast_rhs_prep)
-__global__ void __launch_bounds__(256) k_fast_hand_fill(DevCtx d, const QStrip* strips, double* hand, int nstrips,
-                                                        long long hstride) {
-    hand_refill(d, strips, hand, nstrips, hstride);
 }
 
 }  // namespace shcfd
@@ -1042,8 +1039,8 @@ int FastPath::adjoint_prep(Ctx& c) {
 int FastPath::adjoint_apply(Ctx& c, const double* r_cell, double* lam) {
     DevCtx d = c.dev();
     hipStream_t st = c.stream;
-    hipLaunchKernelGGL(k_fast_adj_rhs, dim3(xcd_groups(nwork)), dim3(256), 0, st, d, strips, work, nwork, r_cell, aR);
-    hipLaunchKernelGGL(k_fast_hand_fill, dim3(256), dim3(256), 0, st, d, strips, ahand, nstrips, hstride);
+    hipLaunchKernelGGL(k_fast_adj_rhs, dim3(xcd_groups(nwork)), dim3(256), 0, st, d, strips, work, nwork, r_cell, aR,
+                       ahand, nstrips, hstride);
     hipLaunchKernelGGL(k_fast_fwd, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, aR, aG, aV, ahand, hstride,
                        c.spin_limit, work, 0, nullptr);
     hipLaunchKernelGGL(k_fast_bwd<false>, dim3(nstrips), dim3(128), 0, st, d, strips, nstrips, aV, aH, nullptr, aZ,
