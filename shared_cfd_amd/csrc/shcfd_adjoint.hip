@@ -193,8 +193,12 @@ int residual_state(shcfd_ctx* ctx) {
     return rc;
 }
 
-// one reverse pass with the current lambda (launches only)
-void vjp_async(shcfd_ctx* ctx) {
+void metric_bars_async(shcfd_ctx* ctx);
+
+// one reverse pass with the current lambda (launches only): QBAR =
+// (dR/dq)^T lam + dJ/dq, the pool bars and, when metrics, the node bars of
+// the metrics
+void vjp_async(shcfd_ctx* ctx, bool metrics = true) {
     DevCtx d = ctx->dev();
     AdjView& a = ctx->adj->v;
     hipStream_t st = ctx->stream;
@@ -230,6 +234,20 @@ void vjp_async(shcfd_ctx* ctx) {
     for (int b = 0; b < ctx->nb; b++) {
         const DevBlock& B = ctx->hblk[b];
         hipLaunchKernelGGL(k_adj_cell, dim3((unsigned)((B.ni + 127) / 128), (unsigned)B.nj), dim3(128), 0, st, d, a, b);
+    }
+    if (metrics) metric_bars_async(ctx);
+}
+
+// the node bars of the metrics (XBAR, and with viscous fluxes the cell-centre
+// bars folded into them) from the face and centre bars of the last product;
+// only the shape gradient reads them
+void metric_bars_async(shcfd_ctx* ctx) {
+    DevCtx d = ctx->dev();
+    AdjView& a = ctx->adj->v;
+    hipStream_t st = ctx->stream;
+    const bool vis = ctx->p.viscous != 0;
+    for (int b = 0; b < ctx->nb; b++) {
+        const DevBlock& B = ctx->hblk[b];
         if (vis) {  // centre-coordinate bars, ghost centres folded back, before the nodes
             hipLaunchKernelGGL(k_adj_centres, dim3((unsigned)((B.ni + 2 + 127) / 128), (unsigned)(B.nj + 2)), dim3(128),
                                0, st, d, a, b);
@@ -277,10 +295,11 @@ int transposed_iterations(shcfd_ctx* ctx, int max_iter, double rtol, int check_e
     while (!rc && it < max_iter && r > rtol * ref) {
         rc = ctx->fast->adjoint_apply(*ctx, a.qbar, a.lam);
         if (rc) break;
-        vjp_async(ctx);
+        vjp_async(ctx, false);  // the node bars once, after the loop
         it++;
         if (it % check_every == 0 || it == max_iter) rc = qbar_rms(ctx, &r);
     }
+    if (!rc && it > 0) metric_bars_async(ctx);  // of the final lambda
     if (!rc) rc = qbar_rms(ctx, &r);
     if (res0) *res0 = ref;
     if (res) *res = r;
