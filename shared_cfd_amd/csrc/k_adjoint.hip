@@ -502,11 +502,28 @@ __device__ inline void adj_bc_row(const DevCtx& d, AdjView& a, const int* row, c
     }
 }
 
-// all side rows in one workgroup, last row first (the reverse of the forward
-// fill): rows that meet at a block corner add into the same interior cell,
+// one cut row (t), positions over the workgroup's threads; each position's
+// copies in the reverse of the forward fill
+__device__ inline void adj_cut_row(const DevCtx& d, AdjView& a, const int* t) {
+    for (int inc = (int)threadIdx.x; inc < t[1]; inc += (int)blockDim.x)
+        for (int copy = 3; copy >= 0; copy--) {
+            int bd, di, dj, bs, si, sj;
+            if (!cut_map(t, inc, copy, bd, di, dj, bs, si, sj)) break;
+            const long long g = cidx(d.blk[bd], di, dj), src = cidx(d.blk[bs], si, sj);
+#pragma unroll
+            for (int k = 0; k < D1::dw; k++) {
+                a.wbar[k * d.NC + src] += a.wbar[k * d.NC + g];
+                a.wbar[k * d.NC + g] = 0.0;
+            }
+        }
+}
+
+// the boundary rows of the reverse pass in one workgroup, in the reverse of
+// the forward fill: the side rows, last row first, then the cut rows, last
+// first.  Rows that meet at a block corner add into the same interior cell,
 // so each row starts after a barrier.  The pool bars are cleared first.
-__global__ void __launch_bounds__(256) k_adj_bc_side(DevCtx d, AdjView a, const int* bc, int nbc, const double* pool,
-                                                     double* poolbar, int npool) {
+__global__ void __launch_bounds__(256) k_adj_bc_rows(DevCtx d, AdjView a, const int* bc, int nbc, const double* pool,
+                                                     double* poolbar, int npool, const int* cut, int ncut) {
     __shared__ double red[256];
     for (int k = (int)threadIdx.x; k < npool; k += (int)blockDim.x) poolbar[k] = 0.0;
     __syncthreads();
@@ -514,21 +531,9 @@ __global__ void __launch_bounds__(256) k_adj_bc_side(DevCtx d, AdjView a, const 
         adj_bc_row(d, a, bc + SHCFD_BC_ROW * r, pool, poolbar, red);
         __syncthreads();
     }
-}
-
-// one cut row: thread per position, copies in the order of the forward fill
-__global__ void k_adj_bc_cut(DevCtx d, AdjView a, const int* t) {
-    const int inc = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (inc >= t[1]) return;
-    for (int copy = 3; copy >= 0; copy--) {
-        int bd, di, dj, bs, si, sj;
-        if (!cut_map(t, inc, copy, bd, di, dj, bs, si, sj)) return;
-        const long long g = cidx(d.blk[bd], di, dj), src = cidx(d.blk[bs], si, sj);
-#pragma unroll
-        for (int k = 0; k < D1::dw; k++) {
-            a.wbar[k * d.NC + src] += a.wbar[k * d.NC + g];
-            a.wbar[k * d.NC + g] = 0.0;
-        }
+    for (int r = ncut - 1; r >= 0; r--) {
+        adj_cut_row(d, a, cut + SHCFD_CUT_ROW * r);
+        __syncthreads();
     }
 }
 

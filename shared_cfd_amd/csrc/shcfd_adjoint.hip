@@ -24,8 +24,7 @@ __global__ void k_adj_muscl(DevCtx, AdjView, int);
 __global__ void k_adj_bc_corner(DevCtx, AdjView);
 __global__ void k_adj_centres(DevCtx, AdjView, int);
 __global__ void k_adj_ghost_centres(DevCtx, AdjView, int, int);
-__global__ void k_adj_bc_side(DevCtx, AdjView, const int*, int, const double*, double*, int);
-__global__ void k_adj_bc_cut(DevCtx, AdjView, const int*);
+__global__ void k_adj_bc_rows(DevCtx, AdjView, const int*, int, const double*, double*, int, const int*, int);
 __global__ void k_adj_cell(DevCtx, AdjView, int);
 __global__ void k_adj_nodes(DevCtx, AdjView, int);
 __global__ void k_adj_objective(DevCtx, AdjView, double*);
@@ -220,17 +219,11 @@ void vjp_async(shcfd_ctx* ctx, bool metrics = true) {
                            d, a, b);
     }
     // reverse of the forward fill order: corners (read by the viscous
-    // fluxes only), side rows (one workgroup, which clears the pool bars
-    // first), cut rows
+    // fluxes only), then side rows and cut rows in one workgroup (which
+    // clears the pool bars first)
     if (vis) hipLaunchKernelGGL(k_adj_bc_corner, dim3((unsigned)((ctx->nb + 63) / 64)), dim3(64), 0, st, d, a);
-    hipLaunchKernelGGL(k_adj_bc_side, dim3(1), dim3(256), 0, st, d, a, ctx->dbc, ctx->nbc, ctx->dpool,
-                       ctx->adj->poolbar, ctx->npool > 0 ? ctx->npool : 1);
-    for (int r = ctx->ncut - 1; r >= 0; r--) {
-        const int w = ctx->hcut[SHCFD_CUT_ROW * r + 1];
-        if (w > 0)
-            hipLaunchKernelGGL(k_adj_bc_cut, dim3((unsigned)((w + 127) / 128)), dim3(128), 0, st, d, a,
-                               ctx->dcut + SHCFD_CUT_ROW * r);
-    }
+    hipLaunchKernelGGL(k_adj_bc_rows, dim3(1), dim3(256), 0, st, d, a, ctx->dbc, ctx->nbc, ctx->dpool,
+                       ctx->adj->poolbar, ctx->npool > 0 ? ctx->npool : 1, ctx->dcut, ctx->ncut);
     for (int b = 0; b < ctx->nb; b++) {
         const DevBlock& B = ctx->hblk[b];
         hipLaunchKernelGGL(k_adj_cell, dim3((unsigned)((B.ni + 127) / 128), (unsigned)B.nj), dim3(128), 0, st, d, a, b);
