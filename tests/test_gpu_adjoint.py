@@ -40,7 +40,17 @@ def test_vjp_matches_torch_autograd(order, reynolds, tuv):
     """Laminar cases add the viscous fluxes (no-slip wall, corner ghosts,
     face metrics from the cell centres) to the reverse pass; tuv makes the
     far field an inflow of given T, u, v (pool bars of those three)."""
-    ac = perturbed(11, order=order, reynolds=reynolds, tuv=tuv)
+    _vjp_check(perturbed(11, order=order, reynolds=reynolds, tuv=tuv), tuv)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_vjp_on_the_512x128_c_mesh(order):
+    """The reverse pass on the adjoint workload's own mesh (512x128 cells,
+    64 wake cells), where every reverse kernel runs over many workgroups."""
+    _vjp_check(perturbed(11, ni=512, nj=128, nwake=64, order=order), False)
+
+
+def _vjp_check(ac, tuv):
     c = ac.case
     ni, nj = c.ni[0], c.nj[0]
     i0, i1 = ac.wall_range()
