@@ -59,10 +59,11 @@ __global__ void __launch_bounds__(256) k_thermo_ideal(DevCtx d, const Tile* tile
 
 // set_w followed by the ideal-gas set_thermo_prop inside a whole step: one
 // pass over the cells, the same expressions as k_set_w and k_thermo_ideal.
-// apply_update: first the previous inner iteration's q += omega Dq, with
+// PRIM_COPY_QP: the step's copy_qp (QP = q, as k_copy_q) of the q read here;
+// PRIM_APPLY_UPDATE: then the previous inner iteration's q += omega Dq, with
 // k_update's expression (time_dual_viscous.f90:113-124)
 template <int NY>
-__global__ void __launch_bounds__(256) k_primitive_ideal(DevCtx d, const Tile* tiles, int apply_update) {
+__global__ void __launch_bounds__(256) k_primitive_ideal(DevCtx d, const Tile* tiles, int flags) {
     using D = Dims<NY>;
     TileCoord t = tile_coord(tiles);
     const DevBlock B = d.blk[t.b];
@@ -71,7 +72,11 @@ __global__ void __launch_bounds__(256) k_primitive_ideal(DevCtx d, const Tile* t
     double q[D::dq];
 #pragma unroll
     for (int k = 0; k < D::dq; k++) q[k] = FLD(d, SHCFD_F_Q, k, c);
-    if (apply_update && B.active) {
+    if (flags & PRIM_COPY_QP) {  // copy_qp of the step start: the q read here
+#pragma unroll
+        for (int k = 0; k < D::dq; k++) FLD(d, SHCFD_F_QP, k, c) = q[k];
+    }
+    if ((flags & PRIM_APPLY_UPDATE) && B.active) {
         const double om = dmin(d.p.omega_max, d.scal[SCAL_OMEGA]);
 #pragma unroll
         for (int k = 0; k < D::dq; k++) {

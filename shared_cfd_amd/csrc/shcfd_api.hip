@@ -1251,14 +1251,20 @@ static int apply_pending_update(shcfd_ctx* ctx) {
 
 // set_w + set_thermo_prop (ideal gas: one fused pass, which also applies a
 // deferred inner-iteration update)
-static int part_primitive(shcfd_ctx* ctx) {
+// copy_qp: the step opens with QP = Q, which the ideal-gas pass stores as it
+// reads q
+static int part_primitive(shcfd_ctx* ctx, bool copy_qp = false) {
     if (ctx->p.thermal == SHCFD_THERMAL_IDEAL) {
-        const int apply = ctx->update_pending;
+        const int flags = (ctx->update_pending ? PRIM_APPLY_UPDATE : 0) | (copy_qp ? PRIM_COPY_QP : 0);
         ctx->update_pending = 0;
         ProfScope ps(ctx, PROF_PRIMITIVE);
-        DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_primitive_ideal<NY>, apply));
+        DISPATCH_NY(ctx->ny, TILE_LAUNCH(k_primitive_ideal<NY>, flags));
         LAUNCHCHK();
         return SHCFD_OK;
+    }
+    if (copy_qp) {
+        const int rc = shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q);
+        if (rc) return rc;
     }
     int rc = apply_pending_update(ctx);
     if (rc) return rc;
@@ -1299,30 +1305,26 @@ static int enqueue_step(shcfd_ctx* ctx, int step) {
             TRY(shcfd_set_thermo_prop(ctx, step));
             TRY(shcfd_set_dt(ctx, step, nullptr));
             TRY(shcfd_proceed_reaction(ctx));
-            TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
-            TRY(part_primitive(ctx));
+            TRY(part_primitive(ctx, true));  // QP = Q, then set_w
             TRY(part_main(ctx));
             TRY(abpm_for_step(ctx));
             TRY(implicit_async(ctx, 0));
             break;
         }
-        TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
-        TRY(part_primitive(ctx));
+        TRY(part_primitive(ctx, true));  // QP = Q, then set_w
         TRY(shcfd_set_dt(ctx, step, nullptr));
         TRY(part_main(ctx));
         TRY(abpm_for_step(ctx));
         TRY(implicit_async(ctx, 0));
         break;
     case SHCFD_SCHEME_PRECON_LUSGS:
-        TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
-        TRY(part_primitive(ctx));
+        TRY(part_primitive(ctx, true));  // QP = Q, then set_w
         TRY(part_main(ctx));
         TRY(abpm_for_step(ctx));
         TRY(implicit_async(ctx, 0));
         break;
     case SHCFD_SCHEME_EULER:
-        TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
-        TRY(part_primitive(ctx));
+        TRY(part_primitive(ctx, true));  // QP = Q, then set_w
         TRY(shcfd_set_dt(ctx, step, nullptr));
         TRY(part_main(ctx));
         TRY(shcfd_calc_next_step(ctx));
@@ -1333,11 +1335,9 @@ static int enqueue_step(shcfd_ctx* ctx, int step) {
             TRY(shcfd_set_thermo_prop(ctx, step));
             TRY(shcfd_set_dt(ctx, step, nullptr));
             TRY(shcfd_proceed_reaction(ctx));
-            TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
-            TRY(part_primitive(ctx));
+            TRY(part_primitive(ctx, true));  // QP = Q, then set_w
         } else {
-            TRY(shcfd_copy_state(ctx, SHCFD_F_QP, SHCFD_F_Q));
-            TRY(part_primitive(ctx));
+            TRY(part_primitive(ctx, true));  // QP = Q, then set_w
             TRY(shcfd_set_dt(ctx, step, nullptr));
         }
         TRY(part_main(ctx));

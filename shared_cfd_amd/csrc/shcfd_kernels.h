@@ -526,6 +526,9 @@ template <int NY> __global__ void k_bc_side(DevCtx, const int*, int, const doubl
 template <int NY> __global__ void k_bc_corner(DevCtx);
 template <int NY> __global__ void k_set_hv(DevCtx, const Tile*);
 template <int NY> __global__ void k_set_tg(DevCtx, const Tile*);
+// flags of k_primitive_ideal: apply the deferred update q += omega dq first;
+// store the q it reads into QP (the copy_qp that opens a step)
+constexpr int PRIM_APPLY_UPDATE = 1, PRIM_COPY_QP = 2;
 template <int NY> __global__ void k_primitive_ideal(DevCtx, const Tile*, int);
 template <int NY> __global__ void k_set_hv_tg(DevCtx, const Tile*);
 template <int NY> __global__ void k_set_tgv(DevCtx, const Tile*);
